@@ -1,0 +1,1181 @@
+/*
+ * psoracle.c -- FP64 CPU oracle for the mesh -> FFT -> P_l(k) path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP library and the
+ * timed CPU baseline.  The product library never links or calls this file.
+ *
+ * A from-scratch C99 restatement of the algorithm of cheng-zhao/powspec.
+ * Arithmetic that decides integer outcomes (cell indices, k-bin indices) and
+ * the per-mode FP64 expressions keep the reference's operation order, so that
+ * this file reproduces the reference's bits where the reference itself is
+ * deterministic.  Compile with -std=c99 and without -ffast-math or FMA
+ * contraction, like the reference (Makefile:1-25).
+ *
+ * The FFT is this file's own (FFTW3 is absent from the image): an exact-
+ * twiddle radix-2 FFT for power-of-two lengths and a direct DFT otherwise,
+ * with FFTW's r2c/c2r layout and sign conventions (src/fftw_define.h:47-64).
+ */
+#include "psoracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+#define PSO_BOX_CEIL 10 /* src/define.h:95 */
+
+/* Row-major mesh index, z fastest (src/define.h:134). */
+#define MIDX(Ng, i, j, k) ((((size_t)(i)) * (Ng) + (j)) * (Ng) + (k))
+
+static double wall_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int pso_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* Atomic FP64 add when running multi-threaded (the reference uses
+ * `omp atomic`, src/genr_mesh.c:20-25). */
+static inline void add_cell(double *rho, size_t idx, double v) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+  rho[idx] += v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mass assignment: src/genr_mesh.c:50-412.                                 */
+/* Per axis the scheme yields `np` cell indices (already wrapped the way the */
+/* reference wraps them) and weights; the weight of the particle is folded   */
+/* into the x-axis factors, and each cell receives (wx*wy)*wz.              */
+/* ------------------------------------------------------------------------ */
+
+/* Grid coordinate, evaluated as ((pos - bmin) * Ng) / L like the reference
+ * (src/genr_mesh.c:57-59). */
+static inline double grid_coord(double pos, double bmin, int Ng, double L) {
+  return (pos - bmin) * Ng / L;
+}
+
+static inline int up1(int c, int Ng) { return (c == Ng - 1) ? 0 : c + 1; }
+static inline int dn1(int c, int Ng) { return (c == 0) ? Ng - 1 : c - 1; }
+
+/* NGP, src/genr_mesh.c:57-69: round half up, wrapping the upper neighbour. */
+static void axis_ngp(double x, int Ng, int *c, double *w) {
+  int x0 = (int)x;
+  if (x - x0 >= 0.5) x0 = up1(x0, Ng);
+  c[0] = x0;
+  w[0] = 1;
+}
+
+/* CIC, src/genr_mesh.c:91-108. */
+static void axis_cic(double x, int Ng, int *c, double *w) {
+  int x0 = (int)x;
+  double w1 = x - x0;
+  double w0 = 1 - w1;
+  c[0] = x0;
+  c[1] = up1(x0, Ng);
+  w[0] = w0;
+  w[1] = w1;
+}
+
+/* TSC, src/genr_mesh.c:151-174. */
+static void axis_tsc(double x, int Ng, int *c, double *w) {
+  int x0 = (int)x;
+  double d = x - x0;
+  int a, b, e;
+  double w0, w1;
+  if (d < 0.5) {
+    b = x0;
+    a = dn1(b, Ng);
+    e = up1(b, Ng);
+    w0 = 0.5 - d;
+    w0 *= w0 * 0.5;
+    w1 = 0.75 - d * d;
+  } else {
+    a = x0;
+    b = up1(x0, Ng);
+    e = up1(b, Ng);
+    d = 1 - d;
+    w0 = 0.5 + d;
+    w0 *= w0 * 0.5;
+    w1 = 0.75 - d * d;
+  }
+  c[0] = a;
+  c[1] = b;
+  c[2] = e;
+  w[0] = w0;
+  w[1] = w1;
+  w[2] = 1 - w0 - w1;
+}
+
+/* PCS, src/genr_mesh.c:255-271; weights are 6x the spline values, the 1/216
+ * of all three axes is applied with the particle weight. */
+static void axis_pcs(double x, int Ng, int *c, double *w) {
+  int x1 = (int)x;
+  c[1] = x1;
+  c[0] = dn1(x1, Ng);
+  c[2] = up1(x1, Ng);
+  c[3] = up1(c[2], Ng);
+  double d = x - x1;
+  double d2 = d * d;
+  double w3 = d2 * d;
+  double w2 = 1 + 3 * (d + d2 - w3);
+  double w1 = 4 - 6 * d2 + 3 * w3;
+  w[3] = w3;
+  w[2] = w2;
+  w[1] = w1;
+  w[0] = 6 - w1 - w2 - w3;
+}
+
+void pso_assign(int scheme, const double *data, size_t n, int Ng,
+    const double bmin[3], const double L[3], double *rho) {
+  const int np = (scheme == PSO_ASSIGN_NGP) ? 1 :
+                 (scheme == PSO_ASSIGN_CIC) ? 2 :
+                 (scheme == PSO_ASSIGN_TSC) ? 3 : 4;
+  /* 1/6^3 for the three axes (src/genr_mesh.c:273-274). */
+  const double inv216 = 1.0 / 216;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (size_t p = 0; p < n; p++) {
+    const double *r = data + 4 * p;
+    int c[3][4];
+    double w[3][4];
+    for (int a = 0; a < 3; a++) {
+      double x = grid_coord(r[a], bmin[a], Ng, L[a]);
+      switch (scheme) {
+        case PSO_ASSIGN_NGP: axis_ngp(x, Ng, c[a], w[a]); break;
+        case PSO_ASSIGN_CIC: axis_cic(x, Ng, c[a], w[a]); break;
+        case PSO_ASSIGN_TSC: axis_tsc(x, Ng, c[a], w[a]); break;
+        default: axis_pcs(x, Ng, c[a], w[a]); break;
+      }
+    }
+    if (scheme == PSO_ASSIGN_PCS) {
+      double nrm = r[3] * inv216;
+      for (int q = 0; q < np; q++) w[0][q] *= nrm;
+    } else if (scheme != PSO_ASSIGN_NGP) {
+      for (int q = 0; q < np; q++) w[0][q] *= r[3];
+    }
+    if (scheme == PSO_ASSIGN_NGP) {
+      add_cell(rho, MIDX(Ng, c[0][0], c[1][0], c[2][0]), r[3]);
+      continue;
+    }
+    for (int a = 0; a < np; a++)
+      for (int b = 0; b < np; b++)
+        for (int e = 0; e < np; e++)
+          add_cell(rho, MIDX(Ng, c[0][a], c[1][b], c[2][e]),
+              w[0][a] * w[1][b] * w[2][e]);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Box definition: src/genr_mesh.c:427-578; periodic shift :590-602.        */
+/* ------------------------------------------------------------------------ */
+
+void pso_coord_bound(const double *data, size_t n, double min[3],
+    double max[3]) {
+  for (size_t p = 0; p < n; p++) {
+    for (int a = 0; a < 3; a++) {
+      double v = data[4 * p + a];
+      if (min[a] > v) min[a] = v;
+      if (max[a] < v) max[a] = v;
+    }
+  }
+}
+
+int pso_def_box(const double min[3], const double max[3], int issim,
+    const double *bsize, const double bpad[3], double size[3],
+    double bmin[3]) {
+  for (int i = 0; i < 3; i++) {
+    if (min[i] > max[i]) return PSO_ERR_CATA;
+    if (issim) {
+      if (min[i] < 0) return PSO_ERR_CATA;
+      if (max[i] >= bsize[i]) return PSO_ERR_CATA;
+    }
+  }
+  if (issim) {
+    for (int i = 0; i < 3; i++) {
+      size[i] = bsize[i];
+      bmin[i] = 0;
+    }
+    return 0;
+  }
+  if (bsize) {
+    for (int i = 0; i < 3; i++) {
+      if (max[i] - min[i] > bsize[i]) return PSO_ERR_CATA;
+      bmin[i] = (max[i] + min[i] - bsize[i]) * 0.5;
+      size[i] = bsize[i];
+    }
+  } else {
+    for (int i = 0; i < 3; i++) {
+      size[i] = ceil((max[i] - min[i]) * (1 + bpad[i]) / PSO_BOX_CEIL) *
+                PSO_BOX_CEIL;
+      bmin[i] = (max[i] + min[i] - size[i]) * 0.5;
+    }
+  }
+  return 0;
+}
+
+void pso_shift_cat(double *data, size_t n, const double smin[3],
+    const double L[3]) {
+  for (size_t p = 0; p < n; p++) {
+    for (int a = 0; a < 3; a++) {
+      if (data[4 * p + a] >= smin[a] + L[a]) data[4 * p + a] -= L[a];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Alias correction: src/multipole.c:46-100 (Jing 2005).                    */
+/* Quirk kept: the series branch is taken whenever pik <= 0.01, which       */
+/* includes every negative frequency.                                       */
+/* ------------------------------------------------------------------------ */
+double pso_alias_corr(int intlace, int assign, double pik) {
+  double fac;
+  if (intlace) {
+    if (pik > 0.01) {
+      fac = pik / sin(pik);
+    } else {
+      /* x/sin x = 1 + x^2/6 + 7x^4/360 + 31x^6/15120 */
+      double t = pik * pik;
+      fac = 1 + (1.0 / 6) * t + (7.0 / 360) * t * t + (31.0 / 15120) * t * t * t;
+    }
+    fac *= fac;
+    switch (assign) {
+      case PSO_ASSIGN_NGP: return fac;
+      case PSO_ASSIGN_CIC: return fac * fac;
+      case PSO_ASSIGN_TSC: return fac * fac * fac;
+      case PSO_ASSIGN_PCS: fac *= fac; return fac * fac;
+      default: return 1;
+    }
+  }
+  if (pik > 0.01) {
+    fac = sin(pik);
+  } else {
+    /* sin x = x - x^3/6 + x^5/120 - x^7/5040 */
+    double t = pik * pik;
+    fac = (1 - (1.0 / 6) * t + (1.0 / 120) * t * t - (1.0 / 5040) * t * t * t) *
+          pik;
+  }
+  fac *= fac;
+  switch (assign) {
+    case PSO_ASSIGN_NGP: return 1;
+    case PSO_ASSIGN_CIC: return 1 / (1 - (2.0 / 3) * fac);
+    case PSO_ASSIGN_TSC: return 1 / (1 - fac + (2.0 / 15) * fac * fac);
+    case PSO_ASSIGN_PCS:
+      return 1 / (1 - (4.0 / 3) * fac + 0.4 * fac * fac -
+                  (4.0 / 315) * fac * fac * fac);
+    default: return 1;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-bins: src/multipole.c:334-394.                                         */
+/* ------------------------------------------------------------------------ */
+int pso_kbins(int Ng, const double L[3], int logscale, double kmin,
+    double kmax, double dk, int cap, double *kedge, double *kcen) {
+  double bmax = (L[0] > L[1]) ? L[0] : L[1];
+  if (bmax < L[2]) bmax = L[2];
+  double kny = M_PI * Ng / bmax;
+  if (logscale) kny = log10(kny);
+  double kx = kny;
+  /* POWSPEC_KMAX_ISSET is `kmax > 0` (src/define.h:68), tested after the
+   * log10 conversion, so log-scale kmax < 1 is ignored like in the
+   * reference. */
+  if (kmax > 0 && kx > kmax) kx = kmax;
+  double nb = round((kx - kmin) / dk);
+  if (nb >= 2147483647.0) return PSO_ERR_PK;
+  int nbin = (int)nb;
+  if (kmin + dk * nbin > kny) nbin -= 1;
+  if (nbin < 1) return PSO_ERR_PK;
+  if (kedge && nbin <= cap) {
+    for (int i = 0; i <= nbin; i++) kedge[i] = kmin + dk * i;
+    if (kcen)
+      for (int i = 0; i < nbin; i++) kcen[i] = (kedge[i] + kedge[i + 1]) * 0.5;
+  }
+  return nbin;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-mode helpers shared by the precompute and the binning loops.         */
+/* ------------------------------------------------------------------------ */
+
+/* Signed frequency index of a full (non-halved) axis. */
+static inline int sidx(int i, int Ng) { return (i <= (Ng >> 1)) ? i : i - Ng; }
+
+/* Bin of |k| given kmod^2 (linear: sqrt; log: half log10), or -1 if outside,
+ * following src/multipole.c:147-158. */
+static inline int mode_bin(double kmod2, int logscale, int nbin,
+    const double *kedge, double dk, double *kval) {
+  double kmod = logscale ? 0.5 * log10(kmod2) : sqrt(kmod2);
+  *kval = kmod;
+  if (kmod < kedge[0] || kmod >= kedge[nbin]) return -1;
+  int kb = (int)((kmod - kedge[0]) / dk);
+  if (kb < 0 || kb >= nbin) return -1;
+  return kb;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Alias table, mode counts, mean k: src/multipole.c:111-257.               */
+/* ------------------------------------------------------------------------ */
+void pso_precomp(int Ng, const double L[3], int intlace, int assign,
+    int logscale, int nbin, const double *kedge, double dk, double *alias,
+    uint64_t *cnt, double *km) {
+  const int Ngk = Ng / 2 + 1;
+  const double fac = M_PI / Ng;
+  double vec[3];
+  for (int a = 0; a < 3; a++) vec[a] = 2 * M_PI / L[a];
+  for (int b = 0; b < nbin; b++) {
+    cnt[b] = 0;
+    km[b] = 0;
+  }
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    uint64_t *pc = calloc(nbin, sizeof(uint64_t));
+    double *pk = calloc(nbin, sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      double ki = sidx(i, Ng);
+      double ali = pso_alias_corr(intlace, assign, ki * fac);
+      ki *= vec[0];
+      double kmi = ki * ki;
+      for (int j = 0; j < Ng; j++) {
+        double kj = sidx(j, Ng);
+        double alj = pso_alias_corr(intlace, assign, kj * fac);
+        kj *= vec[1];
+        double kmj = kmi + kj * kj;
+        size_t base = ((size_t)i * Ng + j) * Ngk;
+        for (int k = 0; k < Ngk; k++) {
+          /* k = 0 and (even Ng) k = Ng/2 are single planes; the others
+           * stand for the omitted negative half as well. */
+          int single = (k == 0) || ((Ng & 1) == 0 && k == (Ng >> 1));
+          double kmod2, alk = 1;
+          if (k == 0) {
+            kmod2 = kmj;
+          } else {
+            alk = pso_alias_corr(intlace, assign, k * fac);
+            double kk = k * vec[2];
+            kmod2 = kmj + kk * kk;
+          }
+          double kval;
+          int kb = mode_bin(kmod2, logscale, nbin, kedge, dk, &kval);
+          if (kb < 0) {
+            alias[base + k] = 0;
+            continue;
+          }
+          alias[base + k] = (k == 0) ? ali * alj : ali * alj * alk;
+          if (single) {
+            pk[kb] += kval;
+            pc[kb] += 1;
+          } else {
+            pk[kb] += 2 * kval;
+            pc[kb] += 2;
+          }
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (int b = 0; b < nbin; b++) {
+      km[b] += pk[b];
+      cnt[b] += pc[b];
+    }
+    free(pc);
+    free(pk);
+  }
+  for (int b = 0; b < nbin; b++)
+    if (cnt[b]) km[b] /= cnt[b];
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT.  Complex arrays are interleaved (re, im) doubles.                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int n;
+  int pow2;
+  double *tw;   /* n entries of exp(-2 pi i m / n) */
+  int *rev;     /* bit reversal (pow2 only) */
+} fftplan1;
+
+static void plan1_init(fftplan1 *p, int n) {
+  p->n = n;
+  p->pow2 = (n > 0) && ((n & (n - 1)) == 0);
+  p->tw = malloc(sizeof(double) * 2 * (size_t)n);
+  for (int m = 0; m < n; m++) {
+    /* exact angle per entry, no recurrence */
+    double ang = 2.0 * M_PI * (double)m / (double)n;
+    p->tw[2 * m] = cos(ang);
+    p->tw[2 * m + 1] = sin(ang);
+  }
+  p->rev = NULL;
+  if (p->pow2) {
+    int lg = 0;
+    while ((1 << lg) < n) lg++;
+    p->rev = malloc(sizeof(int) * (size_t)n);
+    for (int m = 0; m < n; m++) {
+      int r = 0;
+      for (int b = 0; b < lg; b++)
+        if (m & (1 << b)) r |= 1 << (lg - 1 - b);
+      p->rev[m] = r;
+    }
+  }
+}
+
+static void plan1_free(fftplan1 *p) {
+  free(p->tw);
+  free(p->rev);
+}
+
+/* In-place 1-D DFT of length p->n; sign -1 forward, +1 backward; no
+ * normalisation.  work: 2n doubles (direct DFT only). */
+static void fft1(const fftplan1 *p, double *a, int sign, double *work) {
+  const int n = p->n;
+  if (n == 1) return;
+  if (p->pow2) {
+    for (int m = 0; m < n; m++) {
+      int r = p->rev[m];
+      if (r > m) {
+        double t0 = a[2 * m], t1 = a[2 * m + 1];
+        a[2 * m] = a[2 * r];
+        a[2 * m + 1] = a[2 * r + 1];
+        a[2 * r] = t0;
+        a[2 * r + 1] = t1;
+      }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+      int half = len >> 1;
+      int step = n / len;
+      for (int s = 0; s < n; s += len) {
+        for (int q = 0; q < half; q++) {
+          double wr = p->tw[2 * (q * step)];
+          double wi = sign * p->tw[2 * (q * step) + 1];
+          double *u = a + 2 * (s + q);
+          double *v = a + 2 * (s + q + half);
+          double tr = v[0] * wr - v[1] * wi;
+          double ti = v[0] * wi + v[1] * wr;
+          v[0] = u[0] - tr;
+          v[1] = u[1] - ti;
+          u[0] += tr;
+          u[1] += ti;
+        }
+      }
+    }
+    return;
+  }
+  /* direct DFT for other lengths */
+  for (int k = 0; k < n; k++) {
+    double sr = 0, si = 0;
+    for (int m = 0; m < n; m++) {
+      size_t e = ((size_t)k * m) % n;
+      double wr = p->tw[2 * e], wi = sign * p->tw[2 * e + 1];
+      sr += a[2 * m] * wr - a[2 * m + 1] * wi;
+      si += a[2 * m] * wi + a[2 * m + 1] * wr;
+    }
+    work[2 * k] = sr;
+    work[2 * k + 1] = si;
+  }
+  memcpy(a, work, sizeof(double) * 2 * n);
+}
+
+/* Transform the y and x axes of a half-complex array [Ng][Ng][Ngk]. */
+static void fft_xy(const fftplan1 *p, int Ng, double *c, int sign) {
+  const int Ngk = Ng / 2 + 1;
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *col = malloc(sizeof(double) * 2 * Ng);
+    double *work = malloc(sizeof(double) * 2 * Ng);
+    /* y axis */
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      for (int k = 0; k < Ngk; k++) {
+        for (int j = 0; j < Ng; j++) {
+          size_t e = (((size_t)i * Ng + j) * Ngk + k) * 2;
+          col[2 * j] = c[e];
+          col[2 * j + 1] = c[e + 1];
+        }
+        fft1(p, col, sign, work);
+        for (int j = 0; j < Ng; j++) {
+          size_t e = (((size_t)i * Ng + j) * Ngk + k) * 2;
+          c[e] = col[2 * j];
+          c[e + 1] = col[2 * j + 1];
+        }
+      }
+    }
+    /* x axis */
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int j = 0; j < Ng; j++) {
+      for (int k = 0; k < Ngk; k++) {
+        for (int i = 0; i < Ng; i++) {
+          size_t e = (((size_t)i * Ng + j) * Ngk + k) * 2;
+          col[2 * i] = c[e];
+          col[2 * i + 1] = c[e + 1];
+        }
+        fft1(p, col, sign, work);
+        for (int i = 0; i < Ng; i++) {
+          size_t e = (((size_t)i * Ng + j) * Ngk + k) * 2;
+          c[e] = col[2 * i];
+          c[e + 1] = col[2 * i + 1];
+        }
+      }
+    }
+    free(col);
+    free(work);
+  }
+}
+
+void pso_fft_r2c(int Ng, const double *in, double *out) {
+  const int Ngk = Ng / 2 + 1;
+  fftplan1 p;
+  plan1_init(&p, Ng);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *row = malloc(sizeof(double) * 2 * Ng);
+    double *work = malloc(sizeof(double) * 2 * Ng);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (size_t r = 0; r < (size_t)Ng * Ng; r++) {
+      for (int k = 0; k < Ng; k++) {
+        row[2 * k] = in[r * Ng + k];
+        row[2 * k + 1] = 0;
+      }
+      fft1(&p, row, -1, work);
+      memcpy(out + r * Ngk * 2, row, sizeof(double) * 2 * Ngk);
+    }
+    free(row);
+    free(work);
+  }
+  fft_xy(&p, Ng, out, -1);
+  plan1_free(&p);
+}
+
+void pso_fft_c2r(int Ng, const double *in, double *out) {
+  const int Ngk = Ng / 2 + 1;
+  const size_t nc = (size_t)Ng * Ng * Ngk;
+  double *c = malloc(sizeof(double) * 2 * nc);
+  memcpy(c, in, sizeof(double) * 2 * nc);
+  fftplan1 p;
+  plan1_init(&p, Ng);
+  /* complex inverse along x and y, then c2r along z; the c2r step ignores
+   * the imaginary parts of the z-DC and z-Nyquist entries (taking the real
+   * part of the Hermitian completion), as FFTW's c2r does */
+  fft_xy(&p, Ng, c, +1);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *row = malloc(sizeof(double) * 2 * Ng);
+    double *work = malloc(sizeof(double) * 2 * Ng);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (size_t r = 0; r < (size_t)Ng * Ng; r++) {
+      const double *h = c + r * Ngk * 2;
+      for (int k = 0; k < Ngk; k++) {
+        row[2 * k] = h[2 * k];
+        row[2 * k + 1] = h[2 * k + 1];
+      }
+      for (int k = Ngk; k < Ng; k++) {
+        row[2 * k] = h[2 * (Ng - k)];
+        row[2 * k + 1] = -h[2 * (Ng - k) + 1];
+      }
+      fft1(&p, row, +1, work);
+      for (int k = 0; k < Ng; k++) out[r * Ng + k] = row[2 * k];
+    }
+    free(row);
+    free(work);
+  }
+  plan1_free(&p);
+  free(c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interlace combine: src/multipole.c:461-487.                              */
+/* ------------------------------------------------------------------------ */
+void pso_interlace(int Ng, double *F0, const double *F1) {
+  const int Ngk = Ng / 2 + 1;
+  const double fac = M_PI / Ng;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (int i = 0; i < Ng; i++) {
+    double ki = fac * sidx(i, Ng);
+    for (int j = 0; j < Ng; j++) {
+      double kj = fac * sidx(j, Ng);
+      for (int k = 0; k < Ngk; k++) {
+        size_t e = (((size_t)i * Ng + j) * Ngk + k) * 2;
+        double kk = fac * k;
+        double ks = ki + kj + kk;
+        double cs = cos(ks), sn = sin(ks);
+        double r0 = F0[e], i0 = F0[e + 1];
+        double r1 = F1[e], i1 = F1[e + 1];
+        F0[e] = 0.5 * (r0 + cs * r1 - sn * i1);
+        F0[e + 1] = 0.5 * (i0 + sn * r1 + cs * i1);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Legendre polynomials (math/legpoly.h:47-61), Bonnet recursion.           */
+/* ------------------------------------------------------------------------ */
+double pso_legpoly(int ell, double x) {
+  if (ell < 0 || ell > PSO_MAX_ELL) return 0;
+  if (ell == 0) return 1;
+  double p0 = 1, p1 = x;
+  for (int n = 1; n < ell; n++) {
+    double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Real-form spherical harmonics (math/spherical.h:34-52 for the convention: */
+/* m>0 -> cos(m phi), m<0 -> sin(|m| phi), no Condon-Shortley phase),        */
+/* evaluated from the associated Legendre recursion with the normalisation   */
+/* sqrt(2 (2l+1)/(4 pi) (l-|m|)!/(l+|m|)!) for m != 0.                      */
+/* ------------------------------------------------------------------------ */
+double pso_ylm(int ell, int m, double ct, double st, double cp, double sp) {
+  if (ell < 0 || ell > PSO_MAX_ELL || m < -ell || m > ell) return 1;
+  int am = (m < 0) ? -m : m;
+  /* P_am^am = (2am-1)!! sin^am */
+  double pmm = 1;
+  for (int i = 1; i <= am; i++) pmm *= (2 * i - 1) * st;
+  double plm = pmm;
+  if (ell > am) {
+    double pm1 = ct * (2 * am + 1) * pmm;
+    for (int l = am + 2; l <= ell; l++) {
+      double pl = ((2 * l - 1) * ct * pm1 - (l + am - 1) * pmm) / (l - am);
+      pmm = pm1;
+      pm1 = pl;
+    }
+    plm = pm1;
+  }
+  double ratio = 1; /* (l-am)!/(l+am)! */
+  for (int i = ell - am + 1; i <= ell + am; i++) ratio /= i;
+  double K = sqrt((2 * ell + 1) / (4 * M_PI) * ratio);
+  if (m == 0) return K * plm;
+  double c = 1, s = 0;
+  for (int i = 0; i < am; i++) {
+    double c2 = c * cp - s * sp;
+    double s2 = s * cp + c * sp;
+    c = c2;
+    s = s2;
+  }
+  return sqrt(2.0) * K * plm * ((m > 0) ? c : s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Binning.                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* src/multipole.c:778-898 (linear) and 911-1031 (log). */
+void pso_count_sim(int Ng, const double L[3], int logscale, int nbin,
+    const double *kedge, double dk, const double los[3], int nl,
+    const int *poles, const double *alias, const double *F1,
+    const double *F2, double *pl, double *lcnt) {
+  const int Ngk = Ng / 2 + 1;
+  double vec[3];
+  for (int a = 0; a < 3; a++) vec[a] = 2 * M_PI / L[a];
+  memset(lcnt, 0, sizeof(double) * nl * nbin);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *ppl = calloc((size_t)nl * nbin, sizeof(double));
+    double *plc = calloc((size_t)nl * nbin, sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      double ki = vec[0] * sidx(i, Ng);
+      double kmi = ki * ki;
+      for (int j = 0; j < Ng; j++) {
+        double kj = vec[1] * sidx(j, Ng);
+        double kmj = kmi + kj * kj;
+        size_t base = ((size_t)i * Ng + j) * Ngk;
+        for (int k = 0; k < Ngk; k++) {
+          size_t e = base + k;
+          if (!alias[e]) continue;
+          int single = (k == 0) || ((Ng & 1) == 0 && k == (Ng >> 1));
+          double kmod2, kk = 0, kmod;
+          int kb;
+          if (k == 0) {
+            if (!kmj) continue; /* skip the DC mode (src/multipole.c:806) */
+            kmod2 = kmj;
+          } else {
+            kk = vec[2] * k;
+            kmod2 = kmj + kk * kk;
+          }
+          kmod = sqrt(kmod2);
+          if (logscale)
+            kb = (int)((0.5 * log10(kmod2) - kedge[0]) / dk);
+          else
+            kb = (int)((kmod - kedge[0]) / dk);
+          double mu = (k == 0) ? (ki * los[0] + kj * los[1]) / kmod :
+              (ki * los[0] + kj * los[1] + kk * los[2]) / kmod;
+          double p = (F1[2 * e] * F2[2 * e] + F1[2 * e + 1] * F2[2 * e + 1]) *
+                     alias[e];
+          double num = 1;
+          if (!single) {
+            p *= 2;
+            num = 2;
+          }
+          for (int l = 0; l < nl; l++) {
+            /* the two half-spaces cancel for odd l */
+            if (!single && (poles[l] & 1)) continue;
+            double leg = pso_legpoly(poles[l], mu);
+            ppl[l * nbin + kb] += p * leg;
+            plc[l * nbin + kb] += num * leg;
+          }
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (int q = 0; q < nl * nbin; q++) {
+      pl[q] += ppl[q];
+      lcnt[q] += plc[q];
+    }
+    free(ppl);
+    free(plc);
+  }
+}
+
+/* src/multipole.c:579-660 (linear) and 673-754 (log).  DC is not skipped. */
+void pso_count_survey(int Ng, const double L[3], int logscale, int nbin,
+    const double *kedge, double dk, const double *alias, const double *F1,
+    const double *F2, double *pl) {
+  const int Ngk = Ng / 2 + 1;
+  double vec[3];
+  for (int a = 0; a < 3; a++) vec[a] = 2 * M_PI / L[a];
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+  {
+    double *ppl = calloc(nbin, sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      double ki = vec[0] * sidx(i, Ng);
+      double kmi = ki * ki;
+      for (int j = 0; j < Ng; j++) {
+        double kj = vec[1] * sidx(j, Ng);
+        double kmj = kmi + kj * kj;
+        size_t base = ((size_t)i * Ng + j) * Ngk;
+        for (int k = 0; k < Ngk; k++) {
+          size_t e = base + k;
+          if (!alias[e]) continue;
+          int single = (k == 0) || ((Ng & 1) == 0 && k == (Ng >> 1));
+          double kmod2;
+          if (k == 0) {
+            kmod2 = kmj;
+          } else {
+            double kk = vec[2] * k;
+            kmod2 = kmj + kk * kk;
+          }
+          double kmod = logscale ? 0.5 * log10(kmod2) : sqrt(kmod2);
+          int kb = (int)((kmod - kedge[0]) / dk);
+          double p = (F1[2 * e] * F2[2 * e] + F1[2 * e + 1] * F2[2 * e + 1]) *
+                     alias[e];
+          if (!single) p *= 2;
+          ppl[kb] += p;
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    for (int q = 0; q < nbin; q++) pl[q] += ppl[q];
+    free(ppl);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multipole fields of the survey estimator: src/mp_template.c:48-149.      */
+/* ------------------------------------------------------------------------ */
+void pso_dens_kl(int ell, int Ng, const double L[3], const double smin_g[3],
+    const double *Fr, const double *alias, double *Frl, double *Fka,
+    double *Fkl) {
+  const int Ngk = Ng / 2 + 1;
+  const size_t nc = (size_t)Ng * Ng * Ngk;
+  double fac[3];
+  for (int a = 0; a < 3; a++) fac[a] = 1 / L[a];
+  memset(Fkl, 0, sizeof(double) * 2 * nc);
+  for (int m = -ell; m <= ell; m++) {
+    /* weight the real field by Y_lm of the position direction */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      double ri = (i + smin_g[0]) * L[0];
+      double r1 = ri * ri;
+      for (int j = 0; j < Ng; j++) {
+        size_t row = ((size_t)i * Ng + j) * Ng;
+        if (row == 0) {
+          /* the first row is copied unweighted (src/mp_template.c:75-78) */
+          memcpy(Frl, Fr, sizeof(double) * Ng);
+          continue;
+        }
+        double rj = (j + smin_g[1]) * L[1];
+        double r2 = r1 + rj * rj;
+        for (int k = 0; k < Ng; k++) {
+          double rk = (k + smin_g[2]) * L[2];
+          double r3 = sqrt(r2 + rk * rk);
+          double r22 = sqrt(r2);
+          double ct = rk / r3, st = r22 / r3;
+          double cp = ri / r22, sp = rj / r22;
+          Frl[row + k] = Fr[row + k] * pso_ylm(ell, m, ct, st, cp, sp);
+        }
+      }
+    }
+    pso_fft_r2c(Ng, Frl, Fka);
+    /* accumulate Y_lm(k direction) * Fka */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = 0; i < Ng; i++) {
+      double ki = fac[0] * sidx(i, Ng);
+      double k1 = ki * ki;
+      for (int j = 0; j < Ng; j++) {
+        double kj = fac[1] * sidx(j, Ng);
+        double k2 = k1 + kj * kj;
+        size_t base = ((size_t)i * Ng + j) * Ngk;
+        for (int k = 0; k < Ngk; k++) {
+          size_t e = base + k;
+          if (!alias[e]) continue;
+          double kk = fac[2] * k;
+          double k3 = sqrt(k2 + kk * kk);
+          if (k2 == 0 || k3 == 0) {
+            /* kx = ky = 0: added unweighted (src/mp_template.c:121-124) */
+            Fkl[2 * e] += Fka[2 * e];
+            Fkl[2 * e + 1] += Fka[2 * e + 1];
+          } else {
+            double k22 = sqrt(k2);
+            double sh = pso_ylm(ell, m, kk / k3, k22 / k3, ki / k22, kj / k22);
+            Fkl[2 * e] += Fka[2 * e] * sh;
+            Fkl[2 * e + 1] += Fka[2 * e + 1] * sh;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The whole path: genr_mesh + powspec.                                     */
+/* ------------------------------------------------------------------------ */
+
+static double g_timings[4];
+
+void pso_last_timings(double t[4]) {
+  for (int i = 0; i < 4; i++) t[i] = g_timings[i];
+}
+
+static void fill_zero(double *a, size_t n) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (size_t q = 0; q < n; q++) a[q] = 0;
+}
+
+int pso_run(const pso_conf *conf, pso_cata *cat, pso_result *res) {
+  const double t0 = wall_now();
+  double t_fft = 0, t_bin = 0, t_assign = 0;
+  if (!conf || !cat || !res) return PSO_ERR_ARG;
+  const int Ng = conf->gsize;
+  const int Ngk = Ng / 2 + 1;
+  const size_t Ntot = (size_t)Ng * Ng * Ng;
+  const size_t Nc = (size_t)Ng * Ng * Ngk;
+  const int num = cat->num;
+  const int issim = conf->issim, intlace = conf->intlace;
+
+  /* ---- genr_mesh: def_box (src/genr_mesh.c:509-578) ---- */
+  double cmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double cmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int c = 0; c < num; c++) {
+    pso_coord_bound(cat->data[c], cat->ndata[c], cmin, cmax);
+    if (!issim) pso_coord_bound(cat->rand[c], cat->nrand[c], cmin, cmax);
+  }
+  double L[3], bmin[3];
+  int err = pso_def_box(cmin, cmax, issim,
+      (issim || conf->has_bsize) ? conf->bsize : NULL, conf->bpad, L, bmin);
+  if (err) return PSO_ERR_MESH;
+
+  /* ---- mesh buffers (src/genr_mesh.c:650-723) ---- */
+  const int need_lpole = conf->poles[conf->npole - 1] != 0;
+  double *Fr[2] = {NULL, NULL}, *Frl[2] = {NULL, NULL};
+  double *Fk0[2] = {NULL, NULL}, *Fkl[2] = {NULL, NULL};
+  double *Fka = NULL;
+  double *alias = malloc(sizeof(double) * Nc);
+  int ok = alias != NULL;
+  for (int c = 0; c < num && ok; c++) {
+    Fr[c] = malloc(sizeof(double) * Ntot);
+    Fk0[c] = malloc(sizeof(double) * 2 * Nc);
+    ok = Fr[c] && Fk0[c];
+    if (ok && (!issim || intlace)) ok = (Frl[c] = malloc(sizeof(double) * Ntot)) != NULL;
+    if (ok && (intlace || need_lpole)) ok = (Fkl[c] = malloc(sizeof(double) * 2 * Nc)) != NULL;
+  }
+  if (ok && !issim && need_lpole) ok = (Fka = malloc(sizeof(double) * 2 * Nc)) != NULL;
+  if (!ok) {
+    err = PSO_ERR_MEMORY;
+    goto cleanup;
+  }
+
+  /* ---- gen_dens (src/genr_mesh.c:793-858) ---- */
+  double smin[3];
+  for (int c = 0; c < num; c++) {
+    fill_zero(Fr[c], Ntot);
+    pso_assign(conf->assign, cat->data[c], cat->ndata[c], Ng, bmin, L, Fr[c]);
+    if (!issim) {
+      fill_zero(Frl[c], Ntot);
+      pso_assign(conf->assign, cat->rand[c], cat->nrand[c], Ng, bmin, L, Frl[c]);
+      const double al = cat->alpha[c];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+      for (size_t q = 0; q < Ntot; q++) Fr[c][q] -= Frl[c][q] * al;
+      if (intlace) memcpy(Fk0[c], Fr[c], sizeof(double) * Ntot);
+    }
+    if (intlace) {
+      for (int a = 0; a < 3; a++) smin[a] = bmin[a] - 0.5 * L[a] / Ng;
+      int shift = 0;
+      for (int a = 0; a < 3; a++)
+        if (smin[a] + L[a] <= cmax[a]) {
+          shift = 1;
+          break;
+        }
+      if (shift) pso_shift_cat(cat->data[c], cat->ndata[c], smin, L);
+      fill_zero(Frl[c], Ntot);
+      pso_assign(conf->assign, cat->data[c], cat->ndata[c], Ng, smin, L, Frl[c]);
+      if (!issim) {
+        if (shift) pso_shift_cat(cat->rand[c], cat->nrand[c], smin, L);
+        fill_zero(Fr[c], Ntot);
+        pso_assign(conf->assign, cat->rand[c], cat->nrand[c], Ng, smin, L, Fr[c]);
+        const double al = cat->alpha[c];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (size_t q = 0; q < Ntot; q++) Frl[c][q] -= Fr[c][q] * al;
+        memcpy(Fr[c], Fk0[c], sizeof(double) * Ntot);
+      }
+    }
+  }
+  /* shot noise and normalisation of simulation boxes (:903-910) */
+  if (issim) {
+    for (int c = 0; c < num; c++) {
+      double vol = L[0] * L[1] * L[2];
+      cat->shot[c] = vol / cat->wdata[c];
+      cat->norm[c] = cat->wdata[c] * cat->wdata[c] / vol;
+    }
+  }
+  /* grid-unit box corner for the Y_lm weights (:912-914) */
+  double smin_g[3];
+  for (int a = 0; a < 3; a++) smin_g[a] = bmin[a] * Ng / L[a];
+  t_assign = wall_now() - t0;
+
+  /* ---- powspec_init (src/multipole.c:306-421) ---- */
+  int nbin = pso_kbins(Ng, L, conf->logscale, conf->kmin, conf->kmax,
+      conf->kbin, res->cap, res->kedge, res->k);
+  if (nbin < 0) {
+    err = nbin;
+    goto cleanup;
+  }
+  if (nbin > res->cap) {
+    err = PSO_ERR_ARG;
+    goto cleanup;
+  }
+  const int nl = conf->npole;
+  res->nbin = nbin;
+  res->nl = nl;
+  for (int a = 0; a < 3; a++) {
+    res->bsize[a] = L[a];
+    res->bmin[a] = bmin[a];
+    res->cmax[a] = cmax[a];
+  }
+  for (int q = 0; q < nl * nbin; q++) {
+    if (res->pl[0]) res->pl[0][q] = 0;
+    if (res->pl[1]) res->pl[1][q] = 0;
+    if (res->xpl) res->xpl[q] = 0;
+  }
+
+  /* ---- powspec_precomp (src/multipole.c:111-257) ---- */
+  double tb = wall_now();
+  pso_precomp(Ng, L, intlace, conf->assign, conf->logscale, nbin, res->kedge,
+      conf->kbin, alias, res->cnt, res->km);
+  t_bin += wall_now() - tb;
+
+  /* ---- dens_k0 (src/multipole.c:435-505) ---- */
+  double tf = wall_now();
+  for (int c = 0; c < num; c++) {
+    pso_fft_r2c(Ng, Fr[c], Fk0[c]);
+    if (intlace) {
+      pso_fft_r2c(Ng, Frl[c], Fkl[c]);
+      pso_interlace(Ng, Fk0[c], Fkl[c]);
+      pso_fft_c2r(Ng, Fk0[c], Fr[c]);
+      const double inv = (double)Ntot;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+      for (size_t q = 0; q < Ntot; q++) Fr[c][q] /= inv;
+    }
+  }
+  t_fft += wall_now() - tf;
+
+  /* ---- count_mode (src/multipole.c:1044-1162) via powspec :1221-1245 ---- */
+  if (issim) {
+    double *lc = res->lcnt;
+    tb = wall_now();
+    for (int c = 0; c < num; c++) {
+      if (!conf->isauto[c]) continue;
+      double *pl = res->pl[c];
+      pso_count_sim(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+          conf->los, nl, conf->poles, alias, Fk0[c], Fk0[c], pl, lc);
+      for (int l = 0; l < nl; l++)
+        for (int b = 0; b < nbin; b++) {
+          double *v = pl + l * nbin + b;
+          if (res->cnt[b]) {
+            *v /= cat->norm[c] * res->cnt[b];
+            *v -= cat->shot[c] * lc[b + l * nbin] / res->cnt[b];
+          }
+          *v *= 2 * conf->poles[l] + 1;
+        }
+    }
+    if (conf->iscross) {
+      pso_count_sim(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+          conf->los, nl, conf->poles, alias, Fk0[0], Fk0[1], res->xpl, lc);
+      for (int l = 0; l < nl; l++)
+        for (int b = 0; b < nbin; b++) {
+          double *v = res->xpl + l * nbin + b;
+          if (res->cnt[b]) *v /= sqrt(cat->norm[0] * cat->norm[1]) * res->cnt[b];
+          *v *= 2 * conf->poles[l] + 1;
+        }
+    }
+    t_bin += wall_now() - tb;
+  } else {
+    tb = wall_now();
+    if (conf->poles[0] == 0) {
+      for (int c = 0; c < num; c++) {
+        if (!conf->isauto[c]) continue;
+        double *pl = res->pl[c];
+        pso_count_survey(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+            alias, Fk0[c], Fk0[c], pl);
+        for (int b = 0; b < nbin; b++)
+          if (res->cnt[b]) {
+            pl[b] = pl[b] / res->cnt[b] - cat->shot[c];
+            pl[b] /= cat->norm[c];
+          }
+      }
+      if (conf->iscross) {
+        pso_count_survey(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+            alias, Fk0[0], Fk0[1], res->xpl);
+        for (int b = 0; b < nbin; b++)
+          if (res->cnt[b]) {
+            double nrm = sqrt(cat->norm[0] * cat->norm[1]) * res->cnt[b];
+            res->xpl[b] /= nrm;
+          }
+      }
+    }
+    t_bin += wall_now() - tb;
+    /* the other poles; the first pole is skipped when it is not 0
+     * (src/multipole.c:1237) */
+    for (int n = 1; n < nl; n++) {
+      int ell = conf->poles[n];
+      tf = wall_now();
+      for (int c = 0; c < num; c++)
+        pso_dens_kl(ell, Ng, L, smin_g, Fr[c], alias, Frl[c], Fka, Fkl[c]);
+      t_fft += wall_now() - tf;
+      tb = wall_now();
+      int pid = 0;
+      for (int q = 0; q < nl; q++)
+        if (conf->poles[q] == ell) {
+          pid = q;
+          break;
+        }
+      for (int c = 0; c < num; c++) {
+        if (!conf->isauto[c]) continue;
+        double *pl = res->pl[c] + pid * nbin;
+        pso_count_survey(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+            alias, Fk0[c], Fkl[c], pl);
+        for (int b = 0; b < nbin; b++)
+          if (res->cnt[b]) {
+            double nrm = cat->norm[c] * res->cnt[b];
+            pl[b] *= 4 * M_PI / nrm;
+          }
+      }
+      if (conf->iscross) {
+        double *pl = res->xpl + pid * nbin;
+        pso_count_survey(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+            alias, Fk0[0], Fkl[1], pl);
+        pso_count_survey(Ng, L, conf->logscale, nbin, res->kedge, conf->kbin,
+            alias, Fk0[1], Fkl[0], pl);
+        for (int b = 0; b < nbin; b++)
+          if (res->cnt[b]) {
+            double nrm = sqrt(cat->norm[0] * cat->norm[1]) * res->cnt[b];
+            pl[b] *= 2 * M_PI / nrm;
+          }
+      }
+      t_bin += wall_now() - tb;
+    }
+  }
+
+  /* ---- log-bin rescale, reproduced as written (src/multipole.c:1267-1273):
+   * km and the lower edges are exponentiated from the already-rescaled k ---- */
+  if (conf->logscale) {
+    for (int b = 0; b < nbin; b++) {
+      res->k[b] = pow(10, res->k[b]);
+      res->km[b] = pow(10, res->k[b]);
+      res->kedge[b] = pow(10, res->k[b]);
+    }
+    res->kedge[nbin] = pow(10, res->kedge[nbin]);
+  }
+  err = 0;
+
+cleanup:
+  for (int c = 0; c < 2; c++) {
+    free(Fr[c]);
+    free(Frl[c]);
+    free(Fk0[c]);
+    free(Fkl[c]);
+  }
+  free(Fka);
+  free(alias);
+  g_timings[0] = t_assign;
+  g_timings[1] = t_fft;
+  g_timings[2] = t_bin;
+  g_timings[3] = wall_now() - t0;
+  return err;
+}

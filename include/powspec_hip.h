@@ -1,0 +1,202 @@
+/*
+ * powspec_hip.h -- C-ABI of the MI355X (gfx950) mesh -> FFT -> P_l(k) library.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in layer: psh_genr_mesh / psh_powspec / psh_mesh_destroy /
+ *     psh_powspec_destroy take the place of the reference's genr_mesh and
+ *     powspec (src/genr_mesh.h:86, src/multipole.h:78, called from
+ *     src/powspec.c:47,55).  psh_data and psh_cata have the exact memory
+ *     layout of the reference's DATA and CATA (src/read_cata.h:42-59), so a
+ *     CATA* can be passed as a psh_cata*.  psh_conf carries the CONF fields the
+ *     path reads (src/load_conf.h:40-93).  NULL is returned on error, with a
+ *     message on stderr, like the reference.
+ *
+ *  2. Context layer: psh_ctx_* functions with integer status codes, device-
+ *     resident particle input, batched mocks and stage timings.  The drop-in
+ *     layer is built on it.
+ *
+ * All pointers are plain host or device pointers; no framework types.
+ * Status codes follow the reference's families (src/define.h:112-124).
+ */
+#ifndef POWSPEC_HIP_H
+#define POWSPEC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define PSH_OK 0
+#define PSH_ERR_MEMORY (-1)      /* POWSPEC_ERR_MEMORY (src/define.h:112)  */
+#define PSH_ERR_ARG (-2)         /* POWSPEC_ERR_ARG    (src/define.h:113)  */
+#define PSH_ERR_CATA (-11)       /* POWSPEC_ERR_CATA   (src/define.h:119)  */
+#define PSH_ERR_MESH (-13)       /* POWSPEC_ERR_MESH   (src/define.h:121)  */
+#define PSH_ERR_PK (-14)         /* POWSPEC_ERR_PK     (src/define.h:122)  */
+#define PSH_ERR_DEVICE (-20)     /* HIP runtime failure (no reference code) */
+#define PSH_ERR_UNSUPPORTED (-21)/* e.g. GRID_SIZE not a power of two       */
+
+#define PSH_ASSIGN_NGP 0         /* powspec_assign_t (src/genr_mesh.h:40-45) */
+#define PSH_ASSIGN_CIC 1
+#define PSH_ASSIGN_TSC 2
+#define PSH_ASSIGN_PCS 3
+
+#define PSH_MAX_ELL 6            /* POWSPEC_MAX_ELL (src/define.h:96) */
+
+/* ---- data types ------------------------------------------------------------ */
+
+/* Same layout as the reference's DATA (src/read_cata.h:42-45). */
+typedef struct {
+  double x[3];
+  double w;
+} psh_data;
+
+/* Same layout as the reference's CATA (src/read_cata.h:48-59). */
+typedef struct {
+  int num;
+  psh_data **data;
+  psh_data **rand;
+  size_t *ndata;
+  size_t *nrand;
+  double *wdata;
+  double *wrand;
+  double *alpha;
+  double *shot;
+  double *norm;
+} psh_cata;
+
+/* The CONF fields the path reads (src/load_conf.h:40-93), with kmin/kmax as
+ * the reference holds them after load_conf: log10-converted when logscale is
+ * set (src/load_conf.c:1351-1355); kmax <= 0 means unset.  poles must be
+ * sorted and unique (src/load_conf.c:1056-1074). */
+typedef struct {
+  int ndata;              /* number of data catalogues: 1 or 2          */
+  int issim;              /* CUBIC_SIM                                   */
+  int gsize;              /* GRID_SIZE                                   */
+  int assign;             /* PARTICLE_ASSIGN                             */
+  int intlace;            /* GRID_INTERLACE                              */
+  int npole;              /* number of entries of poles                  */
+  int poles[PSH_MAX_ELL + 1];
+  int has_bsize;          /* BOX_SIZE given (reference: bsize != NULL)   */
+  double bsize[3];        /* BOX_SIZE                                    */
+  double bpad[3];         /* BOX_PAD                                     */
+  double los[3];          /* LINE_OF_SIGHT                               */
+  int logscale;           /* LOG_SCALE                                   */
+  double kmin, kmax, kbin;/* KMIN, KMAX, BIN_SIZE                        */
+  int isauto[2];
+  int iscross;
+  int verbose;            /* VERBOSE                                     */
+} psh_conf;
+
+/* Result of one catalogue set, equivalent to the reference's PK
+ * (src/multipole.h:38-61).  Arrays are owned by the library (drop-in layer)
+ * or by the caller (context layer, sized with psh_ctx_nbin). */
+typedef struct {
+  int nbin;
+  int nl;
+  int poles[PSH_MAX_ELL + 1];
+  double *k;          /* [nbin]      bin centres                      */
+  double *kedge;      /* [nbin + 1]  bin edges                        */
+  double *km;         /* [nbin]      mean |k| of the modes in the bin */
+  uint64_t *cnt;      /* [nbin]      number of modes                  */
+  double *lcnt;       /* [nl*nbin]   sum of mult*L_l(mu) (boxes)      */
+  double *pl[2];      /* [nl*nbin]   auto multipoles, l-major         */
+  double *xpl;        /* [nl*nbin]   cross multipoles                 */
+} psh_pk;
+
+/* Mesh metadata kept after psh_powspec, as the reference's MESH keeps
+ * Ng, box and scheme for the output writer (src/save_res.c:70-84). */
+typedef struct psh_mesh psh_mesh;
+
+/* ---- drop-in layer ------------------------------------------------------- */
+
+/* Replaces genr_mesh (src/genr_mesh.c:874-926): defines the box, assigns the
+ * catalogues to device meshes, sets shot/norm for simulation boxes.  Like the
+ * reference it frees cat->data[i] and cat->rand[i] (which must come from
+ * malloc) and sets cat->data and cat->rand to NULL.  NULL on error. */
+psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat);
+
+/* Replaces powspec (src/multipole.c:1179-1278): FFTs, interlacing, binning,
+ * normalisation.  Releases the device meshes; the psh_mesh keeps its
+ * metadata.  NULL on error. */
+psh_pk *psh_powspec(const psh_conf *conf, const psh_cata *cat, psh_mesh *mesh);
+
+/* Replace mesh_destroy (src/genr_mesh.c:615-640) and powspec_destroy
+ * (src/multipole.c:270-295). */
+void psh_mesh_destroy(psh_mesh *mesh);
+void psh_powspec_destroy(psh_pk *pk);
+
+/* Mesh metadata accessors (for the output writer, src/save_res.c:70-84). */
+int psh_mesh_info(const psh_mesh *mesh, int *Ng, double bsize[3],
+    double bmin[3], int *assign, int *intlace);
+
+/* ---- context layer -------------------------------------------------------- */
+
+typedef struct psh_ctx psh_ctx;
+
+/* Create a context for one configuration on the current HIP device.
+ * nmock_max: number of independent mocks a batched call may hold (>= 1). */
+int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out);
+int psh_ctx_destroy(psh_ctx *ctx);
+
+/* Fix the box (size and lower corner) explicitly, e.g. for simulation boxes
+ * or precomputed survey boxes; computes the k-bin geometry. */
+int psh_ctx_set_box(psh_ctx *ctx, const double bsize[3], const double bmin[3]);
+
+/* Number of k bins of the current box (after psh_ctx_set_box or a run). */
+int psh_ctx_nbin(const psh_ctx *ctx);
+
+/* Run the whole path on host catalogues (same semantics as
+ * psh_genr_mesh + psh_powspec, but the caller keeps ownership of the
+ * particles, which are copied to the device and not modified).
+ * out arrays must hold psh_ctx_nbin() bins (call after the box is known:
+ * for surveys without BOX_SIZE use psh_ctx_define_box first). */
+int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out);
+
+/* Define the box from host catalogues (src/genr_mesh.c:509-578) without
+ * running the path. */
+int psh_ctx_define_box(psh_ctx *ctx, const psh_cata *cat);
+
+/* Device-resident simulation-box run: d_data points to n psh_data records in
+ * device memory (coordinates in [0, BOX_SIZE)); weight_sum = sum of w.
+ * Auto power spectrum of one catalogue.  The box must be set. */
+int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n,
+    double weight_sum, psh_pk *out);
+
+/* Batched mocks: nmock catalogues concatenated in d_data (device) with
+ * offsets[nmock+1] (host) and weight_sum[nmock] (host); out[nmock].
+ * graph != 0 replays the launch sequence as a captured hipGraph. */
+int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
+    const size_t *offsets, const double *weight_sum, int nmock, int graph,
+    psh_pk *out);
+
+/* Per-stage device time of the last run, milliseconds, measured with HIP
+ * events on the context's stream: [0] assign, [1] fft (z,y passes),
+ * [2] last FFT pass fused with binning, [3] total. */
+int psh_ctx_timings(const psh_ctx *ctx, double ms[4]);
+
+/* Enable/disable per-stage event timing (adds event records between
+ * stages; off by default). */
+int psh_ctx_set_profiling(psh_ctx *ctx, int on);
+
+/* Text of the last error on this context (or of the last drop-in call when
+ * ctx is NULL). */
+const char *psh_last_error(const psh_ctx *ctx);
+
+/* ---- device helpers (for hosts without their own HIP runtime bindings) ---- */
+int psh_device_count(void);
+int psh_device_malloc(void **ptr, size_t bytes);
+int psh_device_free(void *ptr);
+int psh_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int psh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int psh_device_synchronize(void);
+/* Name of the device architecture (e.g. "gfx950"). */
+int psh_device_arch(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

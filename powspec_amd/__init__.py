@@ -1,0 +1,316 @@
+"""Python bindings (ctypes) of the gfx950 power-spectrum library.
+
+The product is the C-ABI library ``libpowspec_hip.so`` (include/powspec_hip.h);
+this module only marshals arguments for tests, benchmarks and Python hosts.
+It never falls back to a CPU implementation: when the library is missing or
+cannot load, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpowspec_hip.so")
+
+NGP, CIC, TSC, PCS = 0, 1, 2, 3
+MAX_ELL = 6
+OK = 0
+ERR_MEMORY, ERR_ARG, ERR_CATA, ERR_MESH, ERR_PK = -1, -2, -11, -13, -14
+ERR_DEVICE, ERR_UNSUPPORTED = -20, -21
+
+
+class Conf(C.Structure):
+    """psh_conf (include/powspec_hip.h)."""
+
+    _fields_ = [
+        ("ndata", C.c_int), ("issim", C.c_int), ("gsize", C.c_int),
+        ("assign", C.c_int), ("intlace", C.c_int), ("npole", C.c_int),
+        ("poles", C.c_int * (MAX_ELL + 1)), ("has_bsize", C.c_int),
+        ("bsize", C.c_double * 3), ("bpad", C.c_double * 3),
+        ("los", C.c_double * 3), ("logscale", C.c_int),
+        ("kmin", C.c_double), ("kmax", C.c_double), ("kbin", C.c_double),
+        ("isauto", C.c_int * 2), ("iscross", C.c_int), ("verbose", C.c_int),
+    ]
+
+
+class Pk(C.Structure):
+    """psh_pk (include/powspec_hip.h)."""
+
+    _fields_ = [
+        ("nbin", C.c_int), ("nl", C.c_int), ("poles", C.c_int * (MAX_ELL + 1)),
+        ("k", C.c_void_p), ("kedge", C.c_void_p), ("km", C.c_void_p),
+        ("cnt", C.c_void_p), ("lcnt", C.c_void_p), ("pl", C.c_void_p * 2),
+        ("xpl", C.c_void_p),
+    ]
+
+
+class Cata(C.Structure):
+    """psh_cata: same layout as the reference's CATA."""
+
+    _fields_ = [
+        ("num", C.c_int), ("data", C.c_void_p), ("rand", C.c_void_p),
+        ("ndata", C.c_void_p), ("nrand", C.c_void_p), ("wdata", C.c_void_p),
+        ("wrand", C.c_void_p), ("alpha", C.c_void_p), ("shot", C.c_void_p),
+        ("norm", C.c_void_p),
+    ]
+
+
+EXPORTS = [
+    "psh_genr_mesh", "psh_powspec", "psh_mesh_destroy", "psh_powspec_destroy",
+    "psh_mesh_info", "psh_ctx_create", "psh_ctx_destroy", "psh_ctx_set_box",
+    "psh_ctx_nbin", "psh_ctx_run_host", "psh_ctx_define_box", "psh_ctx_run_device",
+    "psh_ctx_run_batch", "psh_ctx_timings", "psh_ctx_set_profiling", "psh_last_error",
+    "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
+    "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch",
+]
+
+_lib = None
+
+
+def load():
+    """Load libpowspec_hip.so (raises when it is missing: no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i, d = C.c_void_p, C.c_size_t, C.c_int, C.c_double
+    L.psh_ctx_create.argtypes = [C.POINTER(Conf), i, C.POINTER(vp)]
+    L.psh_ctx_destroy.argtypes = [vp]
+    L.psh_ctx_set_box.argtypes = [vp, C.POINTER(d), C.POINTER(d)]
+    L.psh_ctx_nbin.argtypes = [vp]
+    L.psh_ctx_run_host.argtypes = [vp, C.POINTER(Cata), C.POINTER(Pk)]
+    L.psh_ctx_define_box.argtypes = [vp, C.POINTER(Cata)]
+    L.psh_ctx_run_device.argtypes = [vp, vp, sz, d, C.POINTER(Pk)]
+    L.psh_ctx_run_batch.argtypes = [vp, vp, C.POINTER(sz), C.POINTER(d), i, i, C.POINTER(Pk)]
+    L.psh_ctx_timings.argtypes = [vp, C.POINTER(d)]
+    L.psh_ctx_set_profiling.argtypes = [vp, i]
+    L.psh_last_error.argtypes = [vp]
+    L.psh_last_error.restype = C.c_char_p
+    L.psh_device_malloc.argtypes = [C.POINTER(vp), sz]
+    L.psh_device_free.argtypes = [vp]
+    L.psh_memcpy_h2d.argtypes = [vp, vp, sz]
+    L.psh_memcpy_d2h.argtypes = [vp, vp, sz]
+    L.psh_device_arch.argtypes = [C.c_char_p, i]
+    L.psh_genr_mesh.argtypes = [C.POINTER(Conf), C.POINTER(Cata)]
+    L.psh_genr_mesh.restype = vp
+    L.psh_powspec.argtypes = [C.POINTER(Conf), C.POINTER(Cata), vp]
+    L.psh_powspec.restype = C.POINTER(Pk)
+    L.psh_mesh_destroy.argtypes = [vp]
+    L.psh_powspec_destroy.argtypes = [C.POINTER(Pk)]
+    L.psh_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(d), C.POINTER(d),
+                                C.POINTER(i), C.POINTER(i)]
+    _lib = L
+    return L
+
+
+def device_count() -> int:
+    return int(load().psh_device_count())
+
+
+def device_arch() -> str:
+    buf = C.create_string_buffer(64)
+    if load().psh_device_arch(buf, 64) != 0:
+        return ""
+    return buf.value.decode()
+
+
+class PshError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def make_conf(**kw) -> Conf:
+    """Conf from keyword arguments (names as in psh_conf)."""
+    c = Conf()
+    poles = kw.pop("poles", [0, 2, 4])
+    c.npole = len(poles)
+    for n, p in enumerate(poles):
+        c.poles[n] = p
+    c.ndata = kw.pop("ndata", 1)
+    c.issim = int(kw.pop("issim", True))
+    c.gsize = kw.pop("gsize")
+    c.assign = kw.pop("assign", CIC)
+    c.intlace = int(kw.pop("intlace", False))
+    bsize = kw.pop("bsize", None)
+    c.has_bsize = int(bsize is not None)
+    if bsize is not None:
+        c.bsize[:] = list(np.broadcast_to(np.asarray(bsize, float), (3,)))
+    c.bpad[:] = list(np.broadcast_to(np.asarray(kw.pop("bpad", 0.02), float), (3,)))
+    c.los[:] = list(np.broadcast_to(np.asarray(kw.pop("los", [0, 0, 1]), float), (3,)))
+    c.logscale = int(kw.pop("logscale", False))
+    c.kmin = kw.pop("kmin", 0.0)
+    c.kmax = kw.pop("kmax", -1.0)
+    c.kbin = kw.pop("kbin")
+    ia = kw.pop("isauto", [True, False])
+    c.isauto[0], c.isauto[1] = int(ia[0]), int(ia[1])
+    c.iscross = int(kw.pop("iscross", False))
+    c.verbose = int(kw.pop("verbose", False))
+    if kw:
+        raise TypeError(f"unknown conf keys {sorted(kw)}")
+    return c
+
+
+class DeviceArray:
+    """A raw device allocation owned by the library's allocator."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        if load().psh_device_malloc(C.byref(self.ptr), max(1, self.nbytes)) != 0:
+            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        if a.nbytes and load().psh_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes) != 0:
+            raise RuntimeError("host to device copy failed")
+        return d
+
+    def free(self):
+        if self.ptr:
+            load().psh_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _PkArrays:
+    def __init__(self, nbin, nl):
+        self.k = np.zeros(nbin)
+        self.kedge = np.zeros(nbin + 1)
+        self.km = np.zeros(nbin)
+        self.cnt = np.zeros(nbin, np.uint64)
+        self.lcnt = np.zeros(nl * nbin)
+        self.pl = [np.zeros(nl * nbin), np.zeros(nl * nbin)]
+        self.pk = Pk()
+        self.pk.k = self.k.ctypes.data
+        self.pk.kedge = self.kedge.ctypes.data
+        self.pk.km = self.km.ctypes.data
+        self.pk.cnt = self.cnt.ctypes.data
+        self.pk.lcnt = self.lcnt.ctypes.data
+        self.pk.pl[0] = self.pl[0].ctypes.data
+        self.pk.pl[1] = self.pl[1].ctypes.data
+        self.nbin, self.nl = nbin, nl
+
+    def result(self):
+        nb, nl = self.nbin, self.nl
+        return {"nbin": nb, "k": self.k.copy(), "kedge": self.kedge.copy(),
+                "km": self.km.copy(), "cnt": self.cnt.copy(),
+                "lcnt": self.lcnt.reshape(nl, nb).copy(),
+                "pl": [p.reshape(nl, nb).copy() for p in self.pl]}
+
+
+class Context:
+    """A psh_ctx: one configuration on the current device."""
+
+    def __init__(self, conf: Conf, nmock_max: int = 1):
+        self.L = load()
+        self.conf = conf
+        self.h = C.c_void_p()
+        rc = self.L.psh_ctx_create(C.byref(conf), nmock_max, C.byref(self.h))
+        if rc != 0:
+            raise PshError(rc, self.L.psh_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc != 0:
+            raise PshError(rc, self.L.psh_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.psh_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on=True):
+        self._check(self.L.psh_ctx_set_profiling(self.h, int(on)))
+
+    def timings(self):
+        t = (C.c_double * 4)()
+        self._check(self.L.psh_ctx_timings(self.h, t))
+        return list(t)
+
+    def set_box(self, bsize, bmin=(0.0, 0.0, 0.0)):
+        b = (C.c_double * 3)(*np.broadcast_to(np.asarray(bsize, float), (3,)))
+        m = (C.c_double * 3)(*bmin)
+        self._check(self.L.psh_ctx_set_box(self.h, b, m))
+
+    def nbin(self):
+        return int(self.L.psh_ctx_nbin(self.h))
+
+    def run_device(self, d_data: DeviceArray, n: int, weight_sum: float):
+        nb = self.nbin()
+        arr = _PkArrays(nb, self.conf.npole)
+        self._check(self.L.psh_ctx_run_device(self.h, d_data.ptr, n, float(weight_sum),
+                                              C.byref(arr.pk)))
+        return arr.result()
+
+    def run_batch(self, d_data: DeviceArray, offsets, weight_sums, graph=False):
+        nm = len(offsets) - 1
+        nb = self.nbin()
+        arrs = [_PkArrays(nb, self.conf.npole) for _ in range(nm)]
+        pks = (Pk * nm)(*[a.pk for a in arrs])
+        off = (C.c_size_t * (nm + 1))(*[int(o) for o in offsets])
+        ws = (C.c_double * nm)(*[float(w) for w in weight_sums])
+        self._check(self.L.psh_ctx_run_batch(self.h, d_data.ptr, off, ws, nm, int(graph), pks))
+        out = []
+        for m, a in enumerate(arrs):
+            # the library wrote through the copies in pks; re-read the arrays
+            out.append(a.result())
+        return out
+
+    def run_host(self, data, rand=None, sums=None):
+        """data/rand: lists of (n,4) float64 arrays; sums: per catalogue dict
+        with wdata, wrand, alpha, shot, norm (surveys)."""
+        num = len(data)
+        keep = []
+        dptrs = (C.c_void_p * 2)()
+        rptrs = (C.c_void_p * 2)()
+        nd = (C.c_size_t * 2)()
+        nr = (C.c_size_t * 2)()
+        vals = {k: (C.c_double * 2)() for k in ("wdata", "wrand", "alpha", "shot", "norm")}
+        for c in range(num):
+            d = np.ascontiguousarray(data[c], np.float64)
+            keep.append(d)
+            dptrs[c] = d.ctypes.data
+            nd[c] = len(d)
+            vals["wdata"][c] = float(d[:, 3].sum()) if len(d) else 0.0
+            if rand is not None:
+                r = np.ascontiguousarray(rand[c], np.float64)
+                keep.append(r)
+                rptrs[c] = r.ctypes.data
+                nr[c] = len(r)
+            if sums is not None:
+                for k in vals:
+                    if k in sums[c]:
+                        vals[k][c] = sums[c][k]
+        cat = Cata()
+        cat.num = num
+        cat.data = C.cast(dptrs, C.c_void_p)
+        cat.rand = C.cast(rptrs, C.c_void_p)
+        cat.ndata = C.cast(nd, C.c_void_p)
+        cat.nrand = C.cast(nr, C.c_void_p)
+        for k, v in vals.items():
+            setattr(cat, k, C.cast(v, C.c_void_p))
+        self._check(self.L.psh_ctx_define_box(self.h, C.byref(cat)))
+        arr = _PkArrays(self.nbin(), self.conf.npole)
+        self._check(self.L.psh_ctx_run_host(self.h, C.byref(cat), C.byref(arr.pk)))
+        res = arr.result()
+        res["shot"] = [vals["shot"][0], vals["shot"][1]]
+        res["norm"] = [vals["norm"][0], vals["norm"][1]]
+        return res

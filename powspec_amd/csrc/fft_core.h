@@ -1,0 +1,317 @@
+// fft_core.h -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A workgroup of T threads transforms C lines of N complex points.  Each
+// Stockham pass takes radix-R butterflies (R in 2,4,8,16) on elements
+// j + r*N/R, multiplies by the twiddles exp(-2 pi i r (j mod Ns) / (Ns R)),
+// and writes element (j / Ns) * Ns * R + (j mod Ns) + r * Ns; after the last
+// pass the line is in natural order (Govindaraju et al. 2008).  Between passes
+// the values are exchanged through LDS one component at a time (real parts,
+// then imaginary parts), which halves the LDS footprint of a tile.
+//
+// Lane mapping: COLMAP (strided columns in HBM) puts consecutive lanes on
+// consecutive lines, so that 16 lanes read one 128-byte row segment; ROWMAP
+// (contiguous rows) puts consecutive lanes on consecutive elements of a line.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pshfft {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <bool INV>
+__device__ __forceinline__ float2 twmul(float2 a, float2 w) {
+  return INV ? cmulc(a, w) : cmul(a, w);
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 rot(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
+constexpr int npasses(int N) { return (ilog2(N) + 3) / 4; }
+constexpr int radix_bits(int N, int p) {
+  return ilog2(N) / npasses(N) + (p < ilog2(N) % npasses(N) ? 1 : 0);
+}
+constexpr int radix(int N, int p) { return 1 << radix_bits(N, p); }
+constexpr int ns_before(int N, int p) {
+  return p == 0 ? 1 : ns_before(N, p - 1) * radix(N, p - 1);
+}
+
+// exp(-2 pi i m / 16) for a compile-time m (after unrolling)
+__device__ __forceinline__ float2 w16(int m) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f;
+  constexpr float h = 0.70710678118654752f;
+  switch (m & 15) {
+    case 0: return make_float2(1.f, 0.f);
+    case 1: return make_float2(c1, -s1);
+    case 2: return make_float2(h, -h);
+    case 3: return make_float2(s1, -c1);
+    case 4: return make_float2(0.f, -1.f);
+    case 5: return make_float2(-s1, -c1);
+    case 6: return make_float2(-h, -h);
+    case 7: return make_float2(-c1, -s1);
+    case 8: return make_float2(-1.f, 0.f);
+    case 9: return make_float2(-c1, s1);
+    case 10: return make_float2(-h, h);
+    case 11: return make_float2(-s1, c1);
+    case 12: return make_float2(0.f, 1.f);
+    case 13: return make_float2(s1, c1);
+    case 14: return make_float2(h, h);
+    default: return make_float2(c1, s1);
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2 &a, float2 &b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &x0, float2 &x1, float2 &x2,
+                                     float2 &x3) {
+  float2 a = cadd(x0, x2), b = csub(x0, x2);
+  float2 c = cadd(x1, x3), d = rot<INV>(csub(x1, x3));
+  x0 = cadd(a, c);
+  x2 = csub(a, c);
+  x1 = cadd(b, d);
+  x3 = csub(b, d);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 *v);
+
+template <>
+__device__ __forceinline__ void dft<2, false>(float2 *v) { dft2<false>(v[0], v[1]); }
+template <>
+__device__ __forceinline__ void dft<2, true>(float2 *v) { dft2<true>(v[0], v[1]); }
+template <>
+__device__ __forceinline__ void dft<4, false>(float2 *v) {
+  dft4<false>(v[0], v[1], v[2], v[3]);
+}
+template <>
+__device__ __forceinline__ void dft<4, true>(float2 *v) {
+  dft4<true>(v[0], v[1], v[2], v[3]);
+}
+
+// 8 = 4 x 2: X[k1 + 4 k2] = sum_n2 W8^(n2 k1) W2^(n2 k2) DFT4(x[2 n1 + n2])
+template <bool INV>
+__device__ __forceinline__ void dft8(float2 *v) {
+  float2 a0 = v[0], a1 = v[2], a2 = v[4], a3 = v[6];
+  float2 b0 = v[1], b1 = v[3], b2 = v[5], b3 = v[7];
+  dft4<INV>(a0, a1, a2, a3);
+  dft4<INV>(b0, b1, b2, b3);
+  b1 = twmul<INV>(b1, w16(2));
+  b2 = rot<INV>(b2);
+  b3 = twmul<INV>(b3, w16(6));
+  v[0] = cadd(a0, b0);
+  v[4] = csub(a0, b0);
+  v[1] = cadd(a1, b1);
+  v[5] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[6] = csub(a2, b2);
+  v[3] = cadd(a3, b3);
+  v[7] = csub(a3, b3);
+}
+template <>
+__device__ __forceinline__ void dft<8, false>(float2 *v) { dft8<false>(v); }
+template <>
+__device__ __forceinline__ void dft<8, true>(float2 *v) { dft8<true>(v); }
+
+// 16 = 4 x 4
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 *v) {
+  float2 y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+    y[n2][0] = v[n2];
+    y[n2][1] = v[4 + n2];
+    y[n2][2] = v[8 + n2];
+    y[n2][3] = v[12 + n2];
+    dft4<INV>(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < 4; n2++)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; k1++) y[n2][k1] = twmul<INV>(y[n2][k1], w16(n2 * k1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    float2 a = y[0][k1], b = y[1][k1], c = y[2][k1], d = y[3][k1];
+    dft4<INV>(a, b, c, d);
+    v[k1] = a;
+    v[k1 + 4] = b;
+    v[k1 + 8] = c;
+    v[k1 + 12] = d;
+  }
+}
+template <>
+__device__ __forceinline__ void dft<16, false>(float2 *v) { dft16<false>(v); }
+template <>
+__device__ __forceinline__ void dft<16, true>(float2 *v) { dft16<true>(v); }
+
+// LDS index of element n of line c (one float per element, padded)
+template <int N>
+struct Lds {
+  static constexpr int LN = N + (N >> 4) + 1;
+  __device__ static __forceinline__ int idx(int c, int n) {
+    return c * LN + n + (n >> 4);
+  }
+};
+
+// Per-pass thread geometry.
+template <int N, int C, int T, int P, bool COLMAP>
+struct PassGeom {
+  static constexpr int R = radix(N, P);
+  static constexpr int NS = ns_before(N, P);
+  static constexpr int NBF = N / R;           // butterflies per line
+  static constexpr int NBT = C * NBF;         // butterflies in the tile
+  static constexpr int NB = (NBT + T - 1) / T;// butterflies per thread
+  __device__ static __forceinline__ bool map(int q, int &c, int &j) {
+    int b = (int)threadIdx.x + q * T;
+    if (COLMAP) {
+      c = b % C;
+      j = b / C;
+    } else {
+      c = b / NBF;
+      j = b % NBF;
+    }
+    return b < NBT;
+  }
+};
+
+// Values per thread (same for all passes of a plan).
+template <int N, int C, int T>
+struct Plan {
+  static constexpr int P = npasses(N);
+  static constexpr int V = ((C * N + T - 1) / T) > 0 ? (C * N + T - 1) / T : 1;
+};
+
+// Load the inputs of pass P through `ld(c, n) -> float2`.
+template <int N, int C, int T, int P, bool COLMAP, class LD>
+__device__ __forceinline__ void pass_load(float2 *v, LD ld) {
+  using G = PassGeom<N, C, T, P, COLMAP>;
+#pragma unroll
+  for (int q = 0; q < G::NB; q++) {
+    int c, j;
+    bool ok = G::map(q, c, j);
+#pragma unroll
+    for (int r = 0; r < G::R; r++)
+      v[q * G::R + r] = ok ? ld(c, j + r * G::NBF) : make_float2(0.f, 0.f);
+  }
+}
+
+// Twiddle and butterfly of pass P in registers.  tw: table of
+// exp(-2 pi i m / (N*TWS)) read with stride TWS.
+template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS>
+__device__ __forceinline__ void pass_compute(float2 *v,
+                                             const float2 *__restrict__ tw) {
+  using G = PassGeom<N, C, T, P, COLMAP>;
+#pragma unroll
+  for (int q = 0; q < G::NB; q++) {
+    int c, j;
+    G::map(q, c, j);
+    if (P > 0) {
+      const int k = j & (G::NS - 1);
+      constexpr int step = N / (G::NS * G::R);
+#pragma unroll
+      for (int r = 1; r < G::R; r++)
+        v[q * G::R + r] = twmul<INV>(v[q * G::R + r], tw[(r * k * step) * TWS]);
+    }
+    dft<G::R, INV>(v + q * G::R);
+  }
+}
+
+// Store the outputs of pass P through `st(c, n, value)`.
+template <int N, int C, int T, int P, bool COLMAP, class ST>
+__device__ __forceinline__ void pass_store(const float2 *v, ST st) {
+  using G = PassGeom<N, C, T, P, COLMAP>;
+#pragma unroll
+  for (int q = 0; q < G::NB; q++) {
+    int c, j;
+    if (!G::map(q, c, j)) continue;
+    const int k = j & (G::NS - 1);
+    const int base = (j / G::NS) * G::NS * G::R + k;
+#pragma unroll
+    for (int r = 0; r < G::R; r++) st(c, base + r * G::NS, v[q * G::R + r]);
+  }
+}
+
+// Exchange the outputs of pass P into the inputs of pass P+1 through LDS,
+// one component at a time.
+template <int N, int C, int T, int P, bool COLMAP>
+__device__ __forceinline__ void exchange(float2 *v, float *lds) {
+  using G0 = PassGeom<N, C, T, P, COLMAP>;
+  using G1 = PassGeom<N, C, T, P + 1, COLMAP>;
+  constexpr int V = G1::NB * G1::R;
+  float tmp[V];
+  pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
+    lds[Lds<N>::idx(c, n)] = x.x;
+  });
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < G1::NB; q++) {
+    int c, j;
+    bool ok = G1::map(q, c, j);
+#pragma unroll
+    for (int r = 0; r < G1::R; r++)
+      tmp[q * G1::R + r] = ok ? lds[Lds<N>::idx(c, j + r * G1::NBF)] : 0.f;
+  }
+  __syncthreads();
+  pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
+    lds[Lds<N>::idx(c, n)] = x.y;
+  });
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < G1::NB; q++) {
+    int c, j;
+    bool ok = G1::map(q, c, j);
+#pragma unroll
+    for (int r = 0; r < G1::R; r++)
+      v[q * G1::R + r] = make_float2(
+          tmp[q * G1::R + r], ok ? lds[Lds<N>::idx(c, j + r * G1::NBF)] : 0.f);
+  }
+  __syncthreads();
+}
+
+// Run passes P..last on values already loaded for pass P.
+template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS>
+struct Passes {
+  __device__ static __forceinline__ void run(float2 *v, float *lds,
+                                             const float2 *__restrict__ tw) {
+    pass_compute<N, C, T, P, COLMAP, INV, TWS>(v, tw);
+    if constexpr (P + 1 < npasses(N)) {
+      exchange<N, C, T, P, COLMAP>(v, lds);
+      Passes<N, C, T, P + 1, COLMAP, INV, TWS>::run(v, lds, tw);
+    }
+  }
+};
+
+// Full transform: load pass-0 inputs with ld, run all passes, leave the
+// outputs of the last pass in v (use pass_store<..., npasses-1> to place them).
+template <int N, int C, int T, bool COLMAP, bool INV, int TWS, class LD>
+__device__ __forceinline__ void transform(float2 *v, float *lds,
+                                          const float2 *__restrict__ tw, LD ld) {
+  pass_load<N, C, T, 0, COLMAP>(v, ld);
+  Passes<N, C, T, 0, COLMAP, INV, TWS>::run(v, lds, tw);
+}
+
+template <int N, int C, int T, bool COLMAP, class ST>
+__device__ __forceinline__ void store_last(const float2 *v, ST st) {
+  pass_store<N, C, T, npasses(N) - 1, COLMAP>(v, st);
+}
+
+}  // namespace pshfft

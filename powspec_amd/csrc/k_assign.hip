@@ -1,0 +1,509 @@
+// k_assign.hip -- NGP/CIC/TSC/PCS mass assignment with grid interlacing on
+// gfx950, replacing the reference's per-particle `omp atomic` scatter
+// (src/genr_mesh.c:50-412) and its zero/subtract/shift passes
+// (src/genr_mesh.c:590-602, 793-858).
+//
+// Design:
+//  1. count:   per particle, the FP64 cell coordinate decides the base cell of
+//              the window on the plain grid and on the half-cell-shifted grid
+//              (same operation order as the reference, contraction off); the
+//              union window is mapped to the 1..8 mesh tiles it touches and a
+//              per-tile counter is incremented.
+//  2. scan:    exclusive prefix sum of the tile counters.
+//  3. fill:    each (particle, tile) pair is written as a 24-byte record.
+//  4. deposit: one workgroup per tile accumulates its records into an LDS
+//              copy of the tile (FP32 LDS atomics, both grids) and writes the
+//              tile to HBM once with coalesced row stores.  Every mesh cell is
+//              written exactly once: no zeroing pass, no global atomics.
+// Surveys assign data and randoms into the same tiles with weights w and
+// -alpha*w, which fuses the data-minus-random subtraction
+// (src/genr_mesh.c:802-806, 841-845).
+#include "psh_internal.h"
+
+namespace {
+
+struct __align__(8) Rec {
+  float f[3];      // window fraction on the plain grid, per axis
+  float w;         // particle weight times the catalogue scale
+  uint32_t bxy;    // base cell x | y << 16 (plain grid)
+  uint32_t bz;     // base cell z | delta bits (shifted grid) << 16
+};
+static_assert(sizeof(Rec) == 24, "record layout");
+
+using DevPart = PshDevPart;
+
+__device__ __forceinline__ int window_width(int scheme) { return scheme + 1; }
+
+__device__ __forceinline__ int wrap(int c, int Ng) {
+  c %= Ng;
+  return c < 0 ? c + Ng : c;
+}
+
+// Base cell (leftmost window cell, unwrapped) and window fraction of a grid
+// coordinate x >= 0, following src/genr_mesh.c:57-66 (NGP), 94-104 (CIC),
+// 152-172 (TSC), 256-263 (PCS).  The fraction is d = x - (int)x for CIC/PCS
+// and the signed offset from the middle node for TSC.
+__device__ __forceinline__ void axis_base(double x, int scheme, int &b,
+                                          double &f) {
+  int x0 = (int)x;
+  double d = x - x0;
+  switch (scheme) {
+    case 0:
+      b = (d >= 0.5) ? x0 + 1 : x0;
+      f = 0;
+      break;
+    case 1:
+      b = x0;
+      f = d;
+      break;
+    case 2:
+      if (d < 0.5) {
+        b = x0 - 1;
+        f = d;
+      } else {
+        b = x0;
+        f = d - 1;
+      }
+      break;
+    default:
+      b = x0 - 1;
+      f = d;
+      break;
+  }
+}
+
+// Per-particle plain/shifted bases.  The shifted grid uses the reference's
+// periodic shift (src/genr_mesh.c:595-600: x >= smin + L -> x - L) before
+// the coordinate (x - smin) * Ng / L.
+struct PartCells {
+  int b[3];        // plain-grid base, wrapped into [0, Ng)
+  int delta[3];    // shifted base - plain base (mod Ng), 0 or 1
+  float f[3];
+  bool ok;
+};
+
+__device__ PartCells part_cells(const double *r, const PshAssignGeom &g) {
+#pragma clang fp contract(off)
+  PartCells pc;
+  pc.ok = true;
+  const double Ngd = (double)g.Ng;
+  for (int a = 0; a < 3; a++) {
+    double x = (r[a] - g.bmin[a]) * Ngd / g.L[a];
+    int b;
+    double f;
+    axis_base(x, g.scheme, b, f);
+    pc.b[a] = wrap(b, g.Ng);
+    pc.f[a] = (float)f;
+    pc.delta[a] = 0;
+    if (g.intlace) {
+      double p = r[a];
+      if (p >= g.smin[a] + g.L[a]) p -= g.L[a];
+      double xs = (p - g.smin[a]) * Ngd / g.L[a];
+      int bs;
+      double fs;
+      axis_base(xs, g.scheme, bs, fs);
+      int dlt = wrap(bs - b, g.Ng);
+      if (dlt > 1) pc.ok = false;
+      pc.delta[a] = dlt;
+    }
+  }
+  return pc;
+}
+
+// Tiles touched along one axis by the union window [b, b + W - 1 + delta].
+__device__ __forceinline__ int axis_tiles(int b, int W, int delta, int Ng,
+                                          int T, int t[2]) {
+  int e = b + W - 1 + delta;
+  t[0] = b / T;
+  int te = wrap(e, Ng) / T;
+  if (te != t[0]) {
+    t[1] = te;
+    return 2;
+  }
+  return 1;
+}
+
+__device__ __forceinline__ int find_part(const DevPart *parts, int nparts,
+                                         size_t gi) {
+  int lo = 0, hi = nparts - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].start <= gi) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256)
+assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
+              size_t ntotal, uint32_t *__restrict__ counts,
+              uint32_t *__restrict__ cursors, Rec *__restrict__ recs,
+              size_t rec_cap, int *__restrict__ flags) {
+  const int W = window_width(g.scheme);
+  const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  for (size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x; gi < ntotal;
+       gi += (size_t)gridDim.x * blockDim.x) {
+    int pi = find_part(parts, nparts, gi);
+    const DevPart &P = parts[pi];
+    const double *r = P.xyzw + 4 * (gi - P.start);
+    double rr[4];
+    {
+      const double2 *r2 = reinterpret_cast<const double2 *>(r);
+      double2 a = r2[0], b = r2[1];
+      rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
+    }
+    PartCells pc = part_cells(rr, g);
+    if (!pc.ok) {
+      atomicOr(flags, 2);
+      continue;
+    }
+    int tx[2], ty[2], tz[2];
+    int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
+    int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
+    int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
+    Rec rec;
+    if (FILL) {
+      rec.f[0] = pc.f[0];
+      rec.f[1] = pc.f[1];
+      rec.f[2] = pc.f[2];
+      rec.w = (float)rr[3] * P.wscale;
+      rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
+      rec.bz = (uint32_t)pc.b[2] |
+               ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2))
+                << 16);
+    }
+    const size_t mbase = (size_t)P.mock * tiles_per_mock;
+    for (int a = 0; a < nx; a++)
+      for (int b = 0; b < ny; b++)
+        for (int c = 0; c < nz; c++) {
+          size_t t = mbase + ((size_t)tx[a] * g.ntile[1] + ty[b]) * g.ntile[2] + tz[c];
+          if (FILL) {
+            uint32_t slot = atomicAdd(&cursors[t], 1u);
+            if (slot < rec_cap) recs[slot] = rec;
+            else atomicOr(flags, 1);
+          } else {
+            atomicAdd(&counts[t], 1u);
+          }
+        }
+  }
+}
+
+// ---- exclusive scan of uint32 counters (3 kernels) -------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
+                                                         uint32_t *sh,
+                                                         uint32_t &total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+      uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[16] = run;
+  }
+  __syncthreads();
+  total = sh[16];
+  uint32_t res = x - v + sh[wid];
+  __syncthreads();
+  return res;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_local(const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+           uint32_t *__restrict__ bsum, size_t n) {
+  __shared__ uint32_t sh[17];
+  size_t base = (size_t)blockIdx.x * kScanBlock + (size_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t tot;
+  uint32_t ex = block_exclusive_scan(s, sh, tot);
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    if (base + i < n) out[base + i] = ex;
+    ex += v[i];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_sums(uint32_t *__restrict__ bsum, int nb) {
+  __shared__ uint32_t sh[17];
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += kScanThreads) {
+    int i = base + threadIdx.x;
+    uint32_t v = (i < nb) ? bsum[i] : 0u;
+    uint32_t tot;
+    uint32_t ex = block_exclusive_scan(v, sh, tot);
+    if (i < nb) bsum[i] = ex + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_add(uint32_t *__restrict__ out, uint32_t *__restrict__ cursors,
+         const uint32_t *__restrict__ bsum, size_t n, int nb) {
+  size_t base = (size_t)blockIdx.x * kScanBlock;
+  uint32_t off = bsum[blockIdx.x];
+  for (int i = threadIdx.x; i < kScanBlock; i += kScanThreads) {
+    size_t k = base + i;
+    if (k < n) {
+      uint32_t v = out[k] + off;
+      out[k] = v;
+      cursors[k] = v;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+__global__ void fill_zero_u32(uint32_t *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// ---- deposit: one workgroup per tile ------------------------------------------
+
+__device__ __forceinline__ void window_weights(int scheme, float f, float *w) {
+  switch (scheme) {
+    case 0:
+      w[0] = 1.f;
+      break;
+    case 1:
+      w[0] = 1.f - f;
+      w[1] = f;
+      break;
+    case 2: {
+      float a = 0.5f - f;
+      w[0] = 0.5f * a * a;
+      w[1] = 0.75f - f * f;
+      w[2] = 1.f - w[0] - w[1];
+      break;
+    }
+    default: {
+      float d2 = f * f;
+      float w3 = d2 * f;
+      float w2 = 1.f + 3.f * (f + d2 - w3);
+      float w1 = 4.f - 6.f * d2 + 3.f * w3;
+      w[3] = w3;
+      w[2] = w2;
+      w[1] = w1;
+      w[0] = 6.f - w1 - w2 - w3;
+      break;
+    }
+  }
+}
+
+template <int SCHEME>
+__device__ __forceinline__ void deposit_one(float *lds, const int *T,
+                                            const int *T0, int Ng, const int *b,
+                                            const float *f, float w) {
+  constexpr int W = SCHEME + 1;
+  float wt[3][4];
+  int l[3][4];
+  unsigned m[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    window_weights(SCHEME, f[a], wt[a]);
+    m[a] = 0;
+#pragma unroll
+    for (int n = 0; n < W; n++) {
+      int c = b[a] + n;
+      if (c >= Ng) c -= Ng;
+      int lc = c - T0[a];
+      l[a][n] = lc;
+      if (lc >= 0 && lc < T[a]) m[a] |= 1u << n;
+    }
+  }
+  if (SCHEME == 3) w *= (1.0f / 216.0f);
+#pragma unroll
+  for (int i = 0; i < W; i++) {
+    if (!(m[0] >> i & 1)) continue;
+    const float wx = wt[0][i] * w;
+    const int rx = l[0][i] * T[1];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      if (!(m[1] >> j & 1)) continue;
+      const float wxy = wx * wt[1][j];
+      const int rxy = (rx + l[1][j]) * T[2];
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        if (!(m[2] >> k & 1)) continue;
+        atomicAdd(&lds[rxy + l[2][k]], wxy * wt[2][k]);
+      }
+    }
+  }
+}
+
+template <int SCHEME, bool INTLACE>
+__global__ void __launch_bounds__(256)
+assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
+               const Rec *__restrict__ recs, size_t rec_cap,
+               float *__restrict__ meshA, float *__restrict__ meshB,
+               size_t mstride) {
+  extern __shared__ float lds[];
+  const int T[3] = {g.tile[0], g.tile[1], g.tile[2]};
+  const int tcells = T[0] * T[1] * T[2];
+  float *ldsA = lds;
+  float *ldsB = lds + tcells;
+  const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  const size_t tile = blockIdx.x;
+  const int mock = (int)(tile / tiles_per_mock);
+  size_t tl = tile - (size_t)mock * tiles_per_mock;
+  const int tzi = (int)(tl % g.ntile[2]);
+  tl /= g.ntile[2];
+  const int tyi = (int)(tl % g.ntile[1]);
+  const int txi = (int)(tl / g.ntile[1]);
+  const int T0[3] = {txi * T[0], tyi * T[1], tzi * T[2]};
+
+  for (int i = threadIdx.x; i < tcells; i += blockDim.x) {
+    ldsA[i] = 0.f;
+    if (INTLACE) ldsB[i] = 0.f;
+  }
+  __syncthreads();
+
+  // records past the buffer were never written (overflow flag set, the run
+  // is repeated with a larger buffer): never read beyond rec_cap
+  const uint32_t cap = rec_cap < 0xffffffffu ? (uint32_t)rec_cap : 0xffffffffu;
+  const uint32_t s0 = min(starts[tile], cap), s1 = min(starts[tile + 1], cap);
+  for (uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    const Rec r = recs[s];
+    int b[3] = {(int)(r.bxy & 0xffffu), (int)(r.bxy >> 16), (int)(r.bz & 0xffffu)};
+    deposit_one<SCHEME>(ldsA, T, T0, g.Ng, b, r.f, r.w);
+    if (INTLACE) {
+      const uint32_t dl = r.bz >> 16;
+      int bs[3];
+      float fs[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        int d = (dl >> a) & 1;
+        bs[a] = b[a] + d;
+        if (bs[a] >= g.Ng) bs[a] -= g.Ng;
+        fs[a] = r.f[a] + 0.5f - (float)d;
+      }
+      deposit_one<SCHEME>(ldsB, T, T0, g.Ng, bs, fs, r.w);
+    }
+  }
+  __syncthreads();
+
+  // write the tile: rows of T[2] floats along z
+  float *outA = meshA + (size_t)mock * mstride;
+  float *outB = INTLACE ? meshB + (size_t)mock * mstride : nullptr;
+  const int nrow = T[0] * T[1];
+  if ((T[2] & 3) == 0) {
+    const int q4 = T[2] >> 2;
+    for (int i = threadIdx.x; i < nrow * q4; i += blockDim.x) {
+      int row = i / q4, q = i - row * q4;
+      int lx = row / T[1], ly = row - lx * T[1];
+      size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
+                    T0[2] + 4 * q;
+      const float4 va = *reinterpret_cast<const float4 *>(ldsA + row * T[2] + 4 * q);
+      *reinterpret_cast<float4 *>(outA + gidx) = va;
+      if (INTLACE) {
+        const float4 vb = *reinterpret_cast<const float4 *>(ldsB + row * T[2] + 4 * q);
+        *reinterpret_cast<float4 *>(outB + gidx) = vb;
+      }
+    }
+  } else {
+    for (int i = threadIdx.x; i < tcells; i += blockDim.x) {
+      int row = i / T[2], lz = i - row * T[2];
+      int lx = row / T[1], ly = row - lx * T[1];
+      size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
+                    T0[2] + lz;
+      outA[gidx] = ldsA[i];
+      if (INTLACE) outB[gidx] = ldsB[i];
+    }
+  }
+}
+
+template <int SCHEME>
+hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
+                          const uint32_t *starts, const Rec *recs,
+                          size_t rec_cap, float *meshA, float *meshB,
+                          size_t mstride, hipStream_t s) {
+  const size_t lds = sizeof(float) * (size_t)g.tile[0] * g.tile[1] * g.tile[2] *
+                     (g.intlace ? 2 : 1);
+  if (g.intlace)
+    hipLaunchKernelGGL((assign_deposit<SCHEME, true>), dim3((unsigned)ntiles),
+                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride);
+  else
+    hipLaunchKernelGGL((assign_deposit<SCHEME, false>), dim3((unsigned)ntiles),
+                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t psh_assign_record_bytes() { return sizeof(Rec); }
+
+hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(fill_zero_u32, dim3(blocks), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+size_t psh_assign_scan_bytes(size_t ntiles) {
+  size_t nsb = (ntiles + kScanBlock - 1) / kScanBlock;
+  return (nsb + 1) * sizeof(uint32_t);
+}
+
+hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
+                          int nparts, size_t ntotal, int nmock,
+                          uint32_t *counts, uint32_t *starts, uint32_t *cursors,
+                          void *records, size_t rec_cap, float *meshA,
+                          float *meshB, size_t mstride, uint32_t *d_scan_tmp,
+                          int *d_flags, hipStream_t s) {
+  if (nparts < 1) return hipErrorInvalidValue;
+  const size_t ntiles = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * nmock;
+  const int nsb = (int)((ntiles + kScanBlock - 1) / kScanBlock);
+  hipError_t e;
+  if ((e = psh_fill_zero_u32(counts, ntiles, s)) != hipSuccess) return e;
+  unsigned pblocks = (unsigned)((ntotal + 255) / 256);
+  if (pblocks > 16384) pblocks = 16384;
+  if (pblocks == 0) pblocks = 1;
+  Rec *recs = reinterpret_cast<Rec *>(records);
+  if (ntotal > 0) {
+    hipLaunchKernelGGL((assign_bucket<false>), dim3(pblocks), dim3(256), 0, s, g,
+                       d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
+                       d_flags);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(scan_local, dim3(nsb), dim3(kScanThreads), 0, s, counts,
+                     starts, d_scan_tmp, ntiles);
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_scan_tmp, nsb);
+  hipLaunchKernelGGL(scan_add, dim3(nsb), dim3(kScanThreads), 0, s, starts,
+                     cursors, d_scan_tmp, ntiles, nsb);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (ntotal > 0) {
+    hipLaunchKernelGGL((assign_bucket<true>), dim3(pblocks), dim3(256), 0, s, g,
+                       d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
+                       d_flags);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  switch (g.scheme) {
+    case 0: return launch_deposit<0>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
+    case 1: return launch_deposit<1>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
+    case 2: return launch_deposit<2>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
+    default: return launch_deposit<3>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
+  }
+}

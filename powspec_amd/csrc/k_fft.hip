@@ -1,0 +1,583 @@
+// k_fft.hip -- 3-D real-to-complex FFT of the density meshes on gfx950,
+// replacing the reference's FFTW r2c/c2r plans (src/fftw_define.h:47-64,
+// executed at src/multipole.c:444,459,493 and src/mp_template.c:98), with the
+// per-mode work of the reference fused into the last (x) pass:
+//   * interlace combine (src/multipole.c:461-487),
+//   * |k| / mu binning with Legendre weights for boxes
+//     (src/multipole.c:778-898, count_mode_sim_lin),
+//   * survey binning and the Y_lm(k) accumulation of the multipole fields
+//     (src/multipole.c:579-660, src/mp_template.c:104-137).
+//
+// Passes over a mesh (layout in psh_internal.h):
+//   z: in-place r2c of each row (N/2-point complex FFT + split post-process)
+//   y: strided column FFTs over 16-column kz tiles
+//   x: strided column FFTs fused with the epilogue; the transformed field is
+//      never written back for boxes.
+// Unnormalised transforms with FFTW's sign convention (-1 forward).
+#include "fft_core.h"
+#include "psh_internal.h"
+
+#include <math.h>
+#include <vector>
+
+using namespace pshfft;
+
+namespace {
+
+template <int N>
+struct ColCfg {
+  static constexpr int C = (N <= 512) ? 16 : 8;
+  static constexpr int T = (N >= 2048) ? 512 : 256;
+};
+
+template <int N>
+struct ZCfg {
+  static constexpr int M = N / 2;
+  static constexpr int C = (4096 / M) > 0 ? 4096 / M : 1;
+  static constexpr int T = 256;
+};
+
+// Largest per-thread value count over the passes of a plan.
+template <int N, int C, int T, bool COLMAP, int P = 0>
+constexpr int max_values() {
+  if constexpr (P >= npasses(N)) {
+    return 1;
+  } else {
+    constexpr int here = PassGeom<N, C, T, P, COLMAP>::NB * PassGeom<N, C, T, P, COLMAP>::R;
+    constexpr int rest = max_values<N, C, T, COLMAP, P + 1>();
+    return here > rest ? here : rest;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real-space Y_lm weights of the survey estimator (src/mp_template.c:69-95),
+// evaluated in FP32 from per-axis position tables.
+// ---------------------------------------------------------------------------
+struct YlmDev {
+  int ell, m;
+  float norm;            // sqrt(2 (2l+1)/(4pi) (l-|m|)!/(l+|m|)!), no sqrt2 at m=0
+  const float *rpos[3];  // (i + smin_g) * L per axis (r-space)
+};
+
+// Real-form spherical harmonic from the associated Legendre recursion (no
+// Condon-Shortley phase; m>0 cos(m phi), m<0 sin(|m| phi)).
+__device__ __forceinline__ float ylm_eval(int ell, int m, float norm, float ct,
+                                          float st, float cp, float sp) {
+  int am = m < 0 ? -m : m;
+  float pmm = 1.f;
+  for (int i = 1; i <= am; i++) pmm *= (float)(2 * i - 1) * st;
+  float plm = pmm;
+  if (ell > am) {
+    float pm1 = ct * (float)(2 * am + 1) * pmm;
+    for (int l = am + 2; l <= ell; l++) {
+      float pl = ((float)(2 * l - 1) * ct * pm1 - (float)(l + am - 1) * pmm) /
+                 (float)(l - am);
+      pmm = pm1;
+      pm1 = pl;
+    }
+    plm = pm1;
+  }
+  if (m == 0) return norm * plm;
+  float c = 1.f, s = 0.f;
+  for (int i = 0; i < am; i++) {
+    float c2 = c * cp - s * sp;
+    s = s * cp + c * sp;
+    c = c2;
+  }
+  return norm * plm * (m > 0 ? c : s);
+}
+
+__device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
+                                              int k) {
+  if (i == 0 && j == 0) return 1.f;  // first row unweighted (mp_template.c:75)
+  float rx = y.rpos[0][i], ry = y.rpos[1][j], rz = y.rpos[2][k];
+  float r2 = rx * rx + ry * ry;
+  float ir3 = rsqrtf(r2 + rz * rz);
+  float ir22 = rsqrtf(r2);
+  float r22 = r2 * ir22;
+  return ylm_eval(y.ell, y.m, y.norm, rz * ir3, r22 * ir3, rx * ir22, ry * ir22);
+}
+
+// ---------------------------------------------------------------------------
+// z pass: real rows -> half-complex rows
+// ---------------------------------------------------------------------------
+template <int N, bool YLM>
+__global__ void __launch_bounds__(256)
+z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
+             size_t nlines, size_t mstride_f, int H,
+             const float2 *__restrict__ tw, YlmDev ylm) {
+  constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
+  constexpr int V = max_values<M, C, T, false>();
+  __shared__ float lds[C * Lds<M>::LN];
+  const size_t line0 = (size_t)blockIdx.x * C;
+  const size_t lpm = (size_t)N * N;
+  float2 v[V];
+
+  transform<M, C, T, false, false, 2>(v, lds, tw, [&](int c, int n) -> float2 {
+    size_t line = line0 + c;
+    if (line >= nlines) return make_float2(0.f, 0.f);
+    size_t mesh = line / lpm, rowi = line - mesh * lpm;
+    const float *row = in + mesh * mstride_f + rowi * (size_t)(2 * H);
+    float2 z = *reinterpret_cast<const float2 *>(row + 2 * n);
+    if (YLM) {
+      int i = (int)(rowi / N), j = (int)(rowi - (size_t)i * N);
+      z.x *= ylm_r_weight(ylm, i, j, 2 * n);
+      z.y *= ylm_r_weight(ylm, i, j, 2 * n + 1);
+    }
+    return z;
+  });
+
+  // post-process: X[k] = E + W^k O, X[M-k] = conj(E - W^k O)
+  constexpr int NP = M / 2 + 1;
+  constexpr int NI = (C * NP + T - 1) / T;
+  float rk[NI], rmk[NI];
+  store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
+    lds[Lds<M>::idx(c, n)] = x.x;
+  });
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NI; it++) {
+    int item = threadIdx.x + it * T;
+    int c = item / NP, k = item - c * NP;
+    if (item < C * NP) {
+      rk[it] = lds[Lds<M>::idx(c, k)];
+      rmk[it] = lds[Lds<M>::idx(c, (M - k) & (M - 1))];
+    }
+  }
+  __syncthreads();
+  store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
+    lds[Lds<M>::idx(c, n)] = x.y;
+  });
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NI; it++) {
+    int item = threadIdx.x + it * T;
+    int c = item / NP, k = item - c * NP;
+    if (item >= C * NP) continue;
+    size_t line = line0 + c;
+    if (line >= nlines) continue;
+    float ik = lds[Lds<M>::idx(c, k)];
+    float imk = lds[Lds<M>::idx(c, (M - k) & (M - 1))];
+    float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
+    float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
+    float2 WO = cmul(tw[k], O);
+    size_t mesh = line / lpm, rowi = line - mesh * lpm;
+    float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
+    orow[k] = cadd(E, WO);
+    if (M - k != k) {
+      float2 d = csub(E, WO);
+      orow[M - k] = make_float2(d.x, -d.y);
+    }
+  }
+}
+
+// inverse z pass: half-complex rows -> real rows (times scale), in place;
+// the imaginary parts of the kz = 0 and kz = N/2 entries are ignored.
+template <int N>
+__global__ void __launch_bounds__(256)
+z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
+             const float2 *__restrict__ tw, float scale) {
+  constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
+  constexpr int V = max_values<M, C, T, false>();
+  __shared__ float lds[C * Lds<M>::LN];
+  const size_t line0 = (size_t)blockIdx.x * C;
+  const size_t lpm = (size_t)N * N;
+  constexpr int NP = M / 2 + 1;
+  constexpr int NI = (C * NP + T - 1) / T;
+  float2 zk[NI], zmk[NI];
+#pragma unroll
+  for (int it = 0; it < NI; it++) {
+    int item = threadIdx.x + it * T;
+    int c = item / NP, k = item - c * NP;
+    zk[it] = zmk[it] = make_float2(0.f, 0.f);
+    size_t line = line0 + c;
+    if (item >= C * NP || line >= nlines) continue;
+    size_t mesh = line / lpm, rowi = line - mesh * lpm;
+    const float2 *row = data + mesh * (mstride_f / 2) + rowi * (size_t)H;
+    float2 xk = row[k], xmk = row[M - k];
+    if (k == 0) {
+      xk.y = 0.f;
+      xmk.y = 0.f;
+    }
+    // Z'[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) conj(W^k)
+    float2 w = tw[k];
+    float2 s = make_float2(xk.x + xmk.x, xk.y - xmk.y);
+    float2 d = cmulc(make_float2(xk.x - xmk.x, xk.y + xmk.y), w);
+    zk[it] = make_float2(s.x - d.y, s.y + d.x);
+    // Z'[M-k] = (X[M-k] + conj X[k]) - i (X[M-k] - conj X[k]) W^k
+    float2 s2 = make_float2(xmk.x + xk.x, xmk.y - xk.y);
+    float2 d2 = cmul(make_float2(xmk.x - xk.x, xmk.y + xk.y), w);
+    zmk[it] = make_float2(s2.x + d2.y, s2.y - d2.x);
+  }
+  __syncthreads();   // all rows read before any LDS use
+  float2 v[V];
+  // two-phase LDS fill (re, then im), reading pass-0 inputs in between
+  float re[V];
+#pragma unroll
+  for (int phase = 0; phase < 2; phase++) {
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+      int item = threadIdx.x + it * T;
+      int c = item / NP, k = item - c * NP;
+      if (item >= C * NP) continue;
+      lds[Lds<M>::idx(c, k)] = phase ? zk[it].y : zk[it].x;
+      if (k != 0 && M - k != k) lds[Lds<M>::idx(c, M - k)] = phase ? zmk[it].y : zmk[it].x;
+    }
+    __syncthreads();
+    pass_load<M, C, T, 0, false>(v, [&](int c, int n) -> float2 {
+      return make_float2(lds[Lds<M>::idx(c, n)], 0.f);
+    });
+    if (phase == 0) {
+#pragma unroll
+      for (int q = 0; q < V; q++) re[q] = v[q].x;
+    } else {
+#pragma unroll
+      for (int q = 0; q < V; q++) v[q] = make_float2(re[q], v[q].x);
+    }
+    __syncthreads();
+  }
+  Passes<M, C, T, 0, false, true, 2>::run(v, lds, tw);
+  store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
+    size_t line = line0 + c;
+    if (line >= nlines) return;
+    size_t mesh = line / lpm, rowi = line - mesh * lpm;
+    float2 *row = data + mesh * (mstride_f / 2) + rowi * (size_t)H;
+    row[n] = make_float2(x.x * scale, x.y * scale);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// strided column passes (y: OS = N*H, ES = H; x: OS = H, ES = N*H)
+// ---------------------------------------------------------------------------
+template <int N, bool INV>
+__global__ void __launch_bounds__(ColCfg<N>::T)
+col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
+           size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
+  constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
+  constexpr int V = max_values<N, C, T, true>();
+  __shared__ float lds[C * Lds<N>::LN];
+  const size_t tile = blockIdx.x;
+  const size_t per_mesh = (size_t)N * ntz;
+  const size_t mesh = tile / per_mesh;
+  const size_t rem = tile - mesh * per_mesh;
+  const int o = (int)(rem / ntz), q = (int)(rem - (size_t)o * ntz);
+  const size_t base = mesh * mstride_c + o * OS + (size_t)q * C;
+  const int ncv = min(C, M + 1 - q * C);
+  float2 v[V];
+  transform<N, C, T, true, INV, 1>(v, lds, tw, [&](int c, int n) -> float2 {
+    return c < ncv ? in[base + n * ES + c] : make_float2(0.f, 0.f);
+  });
+  store_last<N, C, T, true>(v, [&](int c, int n, float2 x) {
+    if (c < ncv) out[base + n * ES + c] = x;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// x pass fused with interlacing and binning
+// ---------------------------------------------------------------------------
+struct XBinDev {
+  const float2 *A, *B, *F0;
+  float2 *store;
+  size_t mstride_c;
+  PshAxisTables ax;
+  PshBins bins;
+  const float2 *phase;
+  double *partials;
+  int nacc;
+  int ell, m;
+  float ynorm;
+};
+
+__device__ __forceinline__ double legendre(int ell, double x) {
+  if (ell == 0) return 1.0;
+  double p0 = 1.0, p1 = x;
+  for (int n = 1; n < ell; n++) {
+    double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
+    p0 = p1;
+    p1 = p2;
+  }
+  return p1;
+}
+
+// Bin of a mode from |k|^2, in the reference's operation order
+// (src/multipole.c:147-158); -1 when outside the bins.
+__device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
+                                        double &kmod) {
+#pragma clang fp contract(off)
+  kmod = b.logscale ? 0.5 * log10(kmod2) : sqrt(kmod2);
+  if (kmod < b.kedge0 || kmod >= b.kedgeN) return -1;
+  int kb = (int)((kmod - b.kedge0) / b.dk);
+  if (kb < 0 || kb >= b.nbin) return -1;
+  return kb;
+}
+
+template <int EPI>
+__device__ __forceinline__ void bin_mode(const XBinDev &a, double *bins, int i,
+                                         int jy, int kz, float2 F, int M,
+                                         size_t pos) {
+#pragma clang fp contract(off)
+  const PshAxisTables &ax = a.ax;
+  double kmod2 = ax.ksq[0][i] + ax.ksq[1][jy];
+  if (kz > 0) kmod2 = kmod2 + ax.ksq[2][kz];
+  if (EPI == PSH_EPI_SIM && kz == 0 && kmod2 == 0.0) return;  // DC skipped
+  double kmod;
+  int kb = mode_bin(kmod2, a.bins, kmod);
+  if (kb < 0) return;
+  double alias = ax.al[0][i] * ax.al[1][jy];
+  if (kz > 0) alias = alias * ax.al[2][kz];
+  const bool single = (kz == 0) || (kz == M);
+  if (EPI == PSH_EPI_SIM) {
+    double p = ((double)F.x * F.x + (double)F.y * F.y) * alias;
+    if (!single) p *= 2;
+    const double mu = (ax.kv[0][i] * a.bins.los[0] + ax.kv[1][jy] * a.bins.los[1] +
+                       ax.kv[2][kz] * a.bins.los[2]) / kmod;
+    for (int l = 0; l < a.bins.nl; l++) {
+      const int ell = a.bins.poles[l];
+      if (!single && (ell & 1)) continue;
+      atomicAdd(&bins[l * a.bins.nbin + kb], p * legendre(ell, mu));
+    }
+  } else if (EPI == PSH_EPI_SURVEY0) {
+    double p = ((double)F.x * F.x + (double)F.y * F.y) * alias;
+    if (!single) p *= 2;
+    atomicAdd(&bins[kb], p);
+  } else {  // PSH_EPI_SURVEYL: Re(F0 conj(Y S)), Y real
+    const float2 f0 = a.F0[pos];
+    float y = 1.f;
+    const double kx = ax.kd[0][i], ky = ax.kd[1][jy], kzz = ax.kd[2][kz];
+    const double k2 = kx * kx + ky * ky;
+    if (k2 != 0.0) {   // kx = ky = 0 is added unweighted (mp_template.c:121)
+      float k2f = (float)k2, kzf = (float)kzz;
+      float ik3 = rsqrtf(k2f + kzf * kzf);
+      float ik22 = rsqrtf(k2f);
+      float k22 = k2f * ik22;
+      y = ylm_eval(a.ell, a.m, a.ynorm, kzf * ik3, k22 * ik3, (float)kx * ik22,
+                   (float)ky * ik22);
+    }
+    double p = ((double)f0.x * F.x + (double)f0.y * F.y) * (double)y * alias;
+    if (!single) p *= 2;
+    atomicAdd(&bins[kb], p);
+  }
+}
+
+template <int N, int EPI, bool INTLACE>
+__global__ void __launch_bounds__(ColCfg<N>::T)
+x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
+  constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
+  constexpr int V = max_values<N, C, T, true>();
+  extern __shared__ double dyn[];
+  double *bins = dyn;
+  const int nacc_pad = (a.nacc + 1) & ~1;
+  float *lds = reinterpret_cast<float *>(bins + nacc_pad);
+  for (int i = threadIdx.x; i < a.nacc; i += T) bins[i] = 0.0;
+  __syncthreads();
+
+  const int mock = blockIdx.y;
+  const size_t ES = (size_t)N * H;
+  const size_t ntiles = (size_t)N * ntz;
+  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int y = (int)(tile / ntz), q = (int)(tile - (size_t)y * ntz);
+    const size_t base = mock * a.mstride_c + (size_t)y * H + (size_t)q * C;
+    const int ncv = min(C, M + 1 - q * C);
+    float2 va[V];
+    transform<N, C, T, true, false, 1>(va, lds, tw, [&](int c, int n) -> float2 {
+      return c < ncv ? a.A[base + n * ES + c] : make_float2(0.f, 0.f);
+    });
+    float2 vb[INTLACE ? V : 1];
+    if (INTLACE) {
+      __syncthreads();
+      transform<N, C, T, true, false, 1>(vb, lds, tw, [&](int c, int n) -> float2 {
+        return c < ncv ? a.B[base + n * ES + c] : make_float2(0.f, 0.f);
+      });
+    }
+    // epilogue over the natural-order outputs of the last pass
+    using G = PassGeom<N, C, T, npasses(N) - 1, true>;
+#pragma unroll
+    for (int qq = 0; qq < G::NB; qq++) {
+      int c, j;
+      if (!G::map(qq, c, j)) continue;
+      if (c >= ncv) continue;
+      const int kz = q * C + c;
+#pragma unroll
+      for (int r = 0; r < G::R; r++) {
+        const int i = j + r * G::NS;
+        float2 F = va[qq * G::R + r];
+        if (INTLACE) {
+          // F0 <- (F0 + e^{i pi (i'+j'+k)/N} F1) / 2   (src/multipole.c:476-484)
+          float2 ph = cmul(cmul(a.phase[i], a.phase[y]), a.phase[kz]);
+          float2 t = cmul(ph, vb[qq * G::R + r]);
+          F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
+        }
+        const size_t pos = base + (size_t)i * ES + c;
+        if (EPI == PSH_EPI_SURVEY0) a.store[pos] = F;
+        bin_mode<EPI>(a, bins, i, y, kz, F, M, pos);
+      }
+    }
+    __syncthreads();
+  }
+  double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc;
+  for (int i = threadIdx.x; i < a.nacc; i += T) out[i] = bins[i];
+}
+
+// ---- dispatch ------------------------------------------------------------------
+
+template <template <int> class F, class... Args>
+hipError_t dispatch_n(int N, Args &&...args) {
+  switch (N) {
+    case 4: return F<4>::run(args...);
+    case 8: return F<8>::run(args...);
+    case 16: return F<16>::run(args...);
+    case 32: return F<32>::run(args...);
+    case 64: return F<64>::run(args...);
+    case 128: return F<128>::run(args...);
+    case 256: return F<256>::run(args...);
+    case 512: return F<512>::run(args...);
+    case 1024: return F<1024>::run(args...);
+    case 2048: return F<2048>::run(args...);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int N>
+struct ZR2C {
+  static hipError_t run(const float *in, float2 *out, int nmesh, size_t mstride,
+                        const float2 *tw, const YlmDev &y, hipStream_t s) {
+    const size_t nlines = (size_t)N * N * nmesh;
+    const unsigned blocks = (unsigned)((nlines + ZCfg<N>::C - 1) / ZCfg<N>::C);
+    const int H = N / 2 + PSH_PAD;
+    if (y.ell > 0)
+      hipLaunchKernelGGL((z_r2c_kernel<N, true>), dim3(blocks), dim3(ZCfg<N>::T), 0,
+                         s, in, out, nlines, mstride, H, tw, y);
+    else
+      hipLaunchKernelGGL((z_r2c_kernel<N, false>), dim3(blocks), dim3(ZCfg<N>::T), 0,
+                         s, in, out, nlines, mstride, H, tw, y);
+    return hipGetLastError();
+  }
+};
+
+template <int N>
+struct ZC2R {
+  static hipError_t run(float2 *data, int nmesh, size_t mstride, const float2 *tw,
+                        float scale, hipStream_t s) {
+    const size_t nlines = (size_t)N * N * nmesh;
+    const unsigned blocks = (unsigned)((nlines + ZCfg<N>::C - 1) / ZCfg<N>::C);
+    hipLaunchKernelGGL((z_c2r_kernel<N>), dim3(blocks), dim3(ZCfg<N>::T), 0, s, data,
+                       nlines, mstride, N / 2 + PSH_PAD, tw, scale);
+    return hipGetLastError();
+  }
+};
+
+template <int N>
+struct Col {
+  static hipError_t run(const float2 *in, float2 *out, int nmesh, size_t mstride,
+                        int axis, const float2 *tw, int inverse, hipStream_t s) {
+    constexpr int C = ColCfg<N>::C;
+    const int H = N / 2 + PSH_PAD;
+    const int ntz = (N / 2 + 1 + C - 1) / C;
+    const size_t OS = axis == 1 ? (size_t)N * H : (size_t)H;
+    const size_t ES = axis == 1 ? (size_t)H : (size_t)N * H;
+    const unsigned blocks = (unsigned)((size_t)N * ntz * nmesh);
+    if (inverse)
+      hipLaunchKernelGGL((col_kernel<N, true>), dim3(blocks), dim3(ColCfg<N>::T), 0, s,
+                         in, out, ntz, OS, ES, mstride / 2, tw);
+    else
+      hipLaunchKernelGGL((col_kernel<N, false>), dim3(blocks), dim3(ColCfg<N>::T), 0, s,
+                         in, out, ntz, OS, ES, mstride / 2, tw);
+    return hipGetLastError();
+  }
+};
+
+template <int N>
+struct XBin {
+  static hipError_t run(const XBinDev &d, int epi, bool il, int grid, int nmock,
+                        const float2 *tw, hipStream_t s) {
+    constexpr int C = ColCfg<N>::C;
+    const int H = N / 2 + PSH_PAD;
+    const int ntz = (N / 2 + 1 + C - 1) / C;
+    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) +
+                         sizeof(float) * C * Lds<N>::LN;
+    const dim3 g(grid, nmock), b(ColCfg<N>::T);
+    if (shmem > 160 * 1024) return hipErrorInvalidValue;
+#define PSH_XB(E, I)                                                          \
+  {                                                                              \
+    auto k = x_bin_kernel<N, E, I>;                                              \
+    if (shmem > 64 * 1024)                                                       \
+      (void)hipFuncSetAttribute((const void *)k,                                 \
+                                hipFuncAttributeMaxDynamicSharedMemorySize,      \
+                                (int)shmem);                                     \
+    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                       \
+    return hipGetLastError();                                                    \
+  }
+    if (epi == PSH_EPI_SIM) {
+      if (il) PSH_XB(PSH_EPI_SIM, true) else PSH_XB(PSH_EPI_SIM, false)
+    } else if (epi == PSH_EPI_SURVEY0) {
+      if (il) PSH_XB(PSH_EPI_SURVEY0, true) else PSH_XB(PSH_EPI_SURVEY0, false)
+    } else {
+      PSH_XB(PSH_EPI_SURVEYL, false)
+    }
+#undef PSH_XB
+  }
+};
+
+}  // namespace
+
+hipError_t psh_fft_twiddles(int N, float2 **d_tw) {
+  std::vector<float2> h(N);
+  for (int m = 0; m < N; m++) {
+    double ang = -2.0 * M_PI * (double)m / (double)N;
+    h[m] = make_float2((float)cos(ang), (float)sin(ang));
+  }
+  hipError_t e = hipMalloc((void **)d_tw, sizeof(float2) * N);
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*d_tw, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
+}
+
+hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
+                         size_t mstride_floats, const float2 *tw,
+                         const PshYlmR &ylm, hipStream_t s) {
+  YlmDev y{};
+  y.ell = ylm.ell;
+  y.m = ylm.m;
+  y.norm = ylm.norm;
+  for (int a = 0; a < 3; a++) y.rpos[a] = ylm.rpos[a];
+  return dispatch_n<ZR2C>(Ng, in, out, nmesh, mstride_floats, tw, y, s);
+}
+
+hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh, size_t mstride_floats,
+                         const float2 *tw, float scale, hipStream_t s) {
+  return dispatch_n<ZC2R>(Ng, inout, nmesh, mstride_floats, tw, scale, s);
+}
+
+hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
+                     const float2 *tw, int inverse, hipStream_t s) {
+  return dispatch_n<Col>(Ng, (const float2 *)data, data, nmesh, mstride_floats, 1, tw,
+                         inverse, s);
+}
+
+hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
+                     size_t mstride_floats, const float2 *tw, int inverse,
+                     hipStream_t s) {
+  return dispatch_n<Col>(Ng, in, out, nmesh, mstride_floats, 0, tw, inverse, s);
+}
+
+int psh_xbin_nacc(const PshBins &b, int epi) {
+  return epi == PSH_EPI_SIM ? b.nl * b.nbin : b.nbin;
+}
+
+hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
+                         hipStream_t s) {
+  XBinDev d{};
+  d.A = a.A;
+  d.B = a.B;
+  d.F0 = a.F0;
+  d.store = a.store;
+  d.mstride_c = a.mstride_floats / 2;
+  d.ax = a.ax;
+  d.bins = a.bins;
+  d.phase = a.phase;
+  d.partials = a.partials;
+  d.nacc = psh_xbin_nacc(a.bins, a.epi);
+  d.ell = a.ell;
+  d.m = a.m;
+  d.ynorm = a.ynorm;
+  return dispatch_n<XBin>(Ng, d, a.epi, a.B != nullptr, a.grid, a.nmock, tw, s);
+}

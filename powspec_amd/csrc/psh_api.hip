@@ -1,0 +1,1279 @@
+// psh_api.hip -- host orchestration of the mesh -> FFT -> P_l(k) path and
+// the C-ABI declared in include/powspec_hip.h.
+//
+// The host side restates the reference's scalar set-up code (box definition
+// src/genr_mesh.c:509-578, k bins src/multipole.c:306-394, alias factors
+// src/multipole.c:46-100, normalisation src/multipole.c:1044-1162) and drives
+// the gfx950 kernels of k_assign.hip, k_fft.hip and k_bin.hip on one HIP
+// stream per context.
+#include "../../include/powspec_hip.h"
+#include "psh_internal.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#define PSH_BOX_CEIL 10   // src/define.h:95
+
+namespace {
+
+std::string g_last_error;
+
+struct Timer {
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+}  // namespace
+
+struct psh_ctx {
+  psh_conf conf{};
+  int dev = 0;
+  int Ng = 0, M = 0, H = 0;
+  size_t mstride = 0;            // floats per mesh
+  int nmock_max = 1;
+  hipStream_t stream = nullptr;
+  bool survey = false;
+  int ellmax = 0;
+
+  float *meshA = nullptr, *meshB = nullptr, *meshS = nullptr;
+
+  // assignment scratch
+  PshAssignGeom ag{};
+  size_t ntiles_mock = 0;
+  uint32_t *counts = nullptr, *starts = nullptr, *cursors = nullptr;
+  uint32_t *scan_tmp = nullptr;
+  size_t tiles_cap = 0;
+  PshDevPart *d_parts = nullptr;
+  int parts_cap = 0;
+  void *records = nullptr;
+  size_t rec_cap = 0;
+  int *d_flags = nullptr;
+
+  // FFT tables
+  float2 *tw = nullptr, *phase = nullptr;
+
+  // geometry
+  bool box_set = false;
+  double L[3] = {0, 0, 0}, bmin[3] = {0, 0, 0}, cmax[3] = {0, 0, 0};
+  int nbin = 0;
+  std::vector<double> kedge, kcen;
+  PshBins bins{};
+  double *d_tab = nullptr;
+  float *d_rpos = nullptr;
+  PshAxisTables ax{};
+  bool geom_valid = false;
+  std::vector<unsigned long long> cnt;
+  std::vector<double> km, lcnt;
+  unsigned long long *d_gcnt = nullptr;
+  double *d_gkm = nullptr, *d_glcnt = nullptr;
+  unsigned long long *d_gpart_cnt = nullptr;
+  double *d_gpart_km = nullptr, *d_gpart_lcnt = nullptr;
+  int ggrid = 0;
+
+  // binning
+  double *partials = nullptr;
+  size_t partials_cap = 0;
+  double *d_bins = nullptr;
+  size_t bins_cap = 0;
+  double *d_bounds = nullptr;
+
+  // device particle staging (host-catalogue runs)
+  double *d_cat = nullptr;
+  size_t d_cat_cap = 0;
+
+  // timing
+  bool profiling = false;
+  Timer tm;
+  double ms[4] = {0, 0, 0, 0};
+
+  // graph of a batched run
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_nmock = 0;
+  size_t graph_ntotal = 0;
+  const void *graph_data = nullptr;
+
+  std::string err;
+};
+
+struct psh_mesh {
+  psh_ctx *ctx = nullptr;
+  int Ng = 0;
+  double bsize[3] = {0, 0, 0};
+  double bmin[3] = {0, 0, 0};
+  int assign = 0, intlace = 0;
+  int num = 0;
+  bool assigned[2] = {false, false};
+  // per-catalogue device particles kept between genr_mesh and powspec
+  double *d_data[2] = {nullptr, nullptr};
+  size_t ndata[2] = {0, 0};
+  double *d_rand[2] = {nullptr, nullptr};
+  size_t nrand[2] = {0, 0};
+};
+
+namespace {
+
+int set_err(psh_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                        \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return set_err(ctx, PSH_ERR_DEVICE, "%s: %s (%s:%d)", #call,               \
+                     hipGetErrorString(e_), __FILE__, __LINE__);                 \
+  } while (0)
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// src/multipole.c:46-100: per-axis alias correction, series branch for
+// pik <= 0.01 (which includes every negative frequency).
+double alias_corr(int intlace, int assign, double pik) {
+  double fac;
+  if (intlace) {
+    if (pik > 0.01) {
+      fac = pik / sin(pik);
+    } else {
+      double t = pik * pik;
+      fac = 1 + (1.0 / 6) * t + (7.0 / 360) * t * t + (31.0 / 15120) * t * t * t;
+    }
+    fac *= fac;
+    switch (assign) {
+      case PSH_ASSIGN_NGP: return fac;
+      case PSH_ASSIGN_CIC: return fac * fac;
+      case PSH_ASSIGN_TSC: return fac * fac * fac;
+      default: fac *= fac; return fac * fac;
+    }
+  }
+  if (pik > 0.01) {
+    fac = sin(pik);
+  } else {
+    double t = pik * pik;
+    fac = (1 - (1.0 / 6) * t + (1.0 / 120) * t * t - (1.0 / 5040) * t * t * t) * pik;
+  }
+  fac *= fac;
+  switch (assign) {
+    case PSH_ASSIGN_NGP: return 1;
+    case PSH_ASSIGN_CIC: return 1 / (1 - (2.0 / 3) * fac);
+    case PSH_ASSIGN_TSC: return 1 / (1 - fac + (2.0 / 15) * fac * fac);
+    default:
+      return 1 / (1 - (4.0 / 3) * fac + 0.4 * fac * fac - (4.0 / 315) * fac * fac * fac);
+  }
+}
+
+// Normalisation of the real-form Y_lm with the associated Legendre function
+// without Condon-Shortley phase.
+float ylm_norm(int ell, int m) {
+  int am = m < 0 ? -m : m;
+  double ratio = 1;
+  for (int i = ell - am + 1; i <= ell + am; i++) ratio /= i;
+  double K = sqrt((2 * ell + 1) / (4 * M_PI) * ratio);
+  if (m != 0) K *= sqrt(2.0);
+  return (float)K;
+}
+
+int validate_conf(const psh_conf *c, psh_ctx *ctx) {
+  if (!c) return set_err(ctx, PSH_ERR_ARG, "configuration not given");
+  if (c->ndata < 1 || c->ndata > 2)
+    return set_err(ctx, PSH_ERR_ARG, "invalid number of catalogues: %d", c->ndata);
+  if (c->gsize <= 1 || c->gsize > 65536)
+    return set_err(ctx, PSH_ERR_ARG, "invalid GRID_SIZE: %d", c->gsize);
+  if (!is_pow2(c->gsize) || c->gsize < 4 || c->gsize > 2048)
+    return set_err(ctx, PSH_ERR_UNSUPPORTED,
+                   "GRID_SIZE %d: this build supports powers of two from 4 to 2048",
+                   c->gsize);
+  if (c->assign < 0 || c->assign > 3)
+    return set_err(ctx, PSH_ERR_ARG, "invalid PARTICLE_ASSIGN: %d", c->assign);
+  if (c->npole < 1 || c->npole > PSH_MAX_ELL + 1)
+    return set_err(ctx, PSH_ERR_ARG, "invalid number of multipoles: %d", c->npole);
+  for (int i = 0; i < c->npole; i++) {
+    if (c->poles[i] < 0 || c->poles[i] > PSH_MAX_ELL)
+      return set_err(ctx, PSH_ERR_ARG, "MULTIPOLE must be between 0 and %d", PSH_MAX_ELL);
+    if (i && c->poles[i] <= c->poles[i - 1])
+      return set_err(ctx, PSH_ERR_ARG, "MULTIPOLE must be sorted and unique");
+  }
+  if (!(c->kbin > 0)) return set_err(ctx, PSH_ERR_ARG, "BIN_SIZE must be > 0");
+  if (c->issim) {
+    for (int a = 0; a < 3; a++)
+      if (!(c->bsize[a] > 0)) return set_err(ctx, PSH_ERR_ARG, "BOX_SIZE must be > 0");
+  }
+  if (c->iscross)
+    return set_err(ctx, PSH_ERR_UNSUPPORTED, "cross power spectra are not supported yet");
+  return PSH_OK;
+}
+
+template <class T>
+int dev_alloc(psh_ctx *ctx, T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(ctx, PSH_ERR_MEMORY, "device allocation of %zu bytes failed: %s",
+                   sizeof(T) * count, hipGetErrorString(e));
+  }
+  return PSH_OK;
+}
+
+// k bins: src/multipole.c:334-394.
+int make_bins(psh_ctx *ctx) {
+  const psh_conf &c = ctx->conf;
+  double bmax = ctx->L[0] > ctx->L[1] ? ctx->L[0] : ctx->L[1];
+  if (bmax < ctx->L[2]) bmax = ctx->L[2];
+  double kny = M_PI * ctx->Ng / bmax;
+  if (c.logscale) kny = log10(kny);
+  double kmax = kny;
+  if (c.kmax > 0 && kmax > c.kmax) kmax = c.kmax;
+  double nb = round((kmax - c.kmin) / c.kbin);
+  if (nb >= 2147483647.0)
+    return set_err(ctx, PSH_ERR_PK, "too many wave number bins due to the small bin size");
+  int nbin = (int)nb;
+  if (c.kmin + c.kbin * nbin > kny) nbin -= 1;
+  if (nbin < 1)
+    return set_err(ctx, PSH_ERR_PK,
+                   "not enough k bins given the Nyquist frequency and the bin size");
+  ctx->nbin = nbin;
+  ctx->kedge.assign(nbin + 1, 0.0);
+  ctx->kcen.assign(nbin, 0.0);
+  for (int i = 0; i <= nbin; i++) ctx->kedge[i] = c.kmin + c.kbin * i;
+  for (int i = 0; i < nbin; i++) ctx->kcen[i] = (ctx->kedge[i] + ctx->kedge[i + 1]) * 0.5;
+  PshBins &b = ctx->bins;
+  b.nbin = nbin;
+  b.logscale = c.logscale;
+  b.kedge0 = ctx->kedge[0];
+  b.kedgeN = ctx->kedge[nbin];
+  b.dk = c.kbin;
+  b.nl = c.npole;
+  for (int l = 0; l < PSH_MAX_NL; l++) b.poles[l] = l < c.npole ? c.poles[l] : 0;
+  for (int a = 0; a < 3; a++) b.los[a] = c.issim ? c.los[a] : 0.0;
+  return PSH_OK;
+}
+
+// Per-axis FP64 tables (signed frequency index for all three axes).
+int make_tables(psh_ctx *ctx) {
+  const int N = ctx->Ng;
+  std::vector<double> h((size_t)12 * N);
+  const double fac = M_PI / N;
+  for (int a = 0; a < 3; a++) {
+    const double vec = 2 * M_PI / ctx->L[a];
+    const double dfac = 1 / ctx->L[a];
+    double *kv = h.data() + (size_t)(4 * a + 0) * N;
+    double *ksq = h.data() + (size_t)(4 * a + 1) * N;
+    double *al = h.data() + (size_t)(4 * a + 2) * N;
+    double *kd = h.data() + (size_t)(4 * a + 3) * N;
+    for (int i = 0; i < N; i++) {
+      const int s = (i <= (N >> 1)) ? i : i - N;
+      double ki = s;
+      al[i] = alias_corr(ctx->conf.intlace, ctx->conf.assign, ki * fac);
+      ki *= vec;
+      kv[i] = ki;
+      ksq[i] = ki * ki;
+      kd[i] = dfac * s;
+    }
+  }
+  int rc = dev_alloc(ctx, &ctx->d_tab, h.size());
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpy(ctx->d_tab, h.data(), sizeof(double) * h.size(),
+                        hipMemcpyHostToDevice));
+  for (int a = 0; a < 3; a++) {
+    ctx->ax.kv[a] = ctx->d_tab + (size_t)(4 * a + 0) * N;
+    ctx->ax.ksq[a] = ctx->d_tab + (size_t)(4 * a + 1) * N;
+    ctx->ax.al[a] = ctx->d_tab + (size_t)(4 * a + 2) * N;
+    ctx->ax.kd[a] = ctx->d_tab + (size_t)(4 * a + 3) * N;
+  }
+  if (ctx->survey) {
+    // r-space positions (i + smin_g) * L with smin_g = bmin * Ng / L
+    // (src/genr_mesh.c:913-914, src/mp_template.c:71-83)
+    std::vector<float> r((size_t)3 * N);
+    for (int a = 0; a < 3; a++) {
+      const double sg = ctx->bmin[a] * N / ctx->L[a];
+      for (int i = 0; i < N; i++) r[(size_t)a * N + i] = (float)((i + sg) * ctx->L[a]);
+    }
+    if ((rc = dev_alloc(ctx, &ctx->d_rpos, r.size()))) return rc;
+    HIPCHK(ctx, hipMemcpy(ctx->d_rpos, r.data(), sizeof(float) * r.size(),
+                          hipMemcpyHostToDevice));
+  }
+  return PSH_OK;
+}
+
+int ensure_geometry(psh_ctx *ctx) {
+  if (ctx->geom_valid) return PSH_OK;
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
+  int grid = ctx->Ng * ctx->Ng;
+  if (grid > 2048) grid = 2048;
+  int rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_gpart_cnt, (size_t)grid * nb))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_gpart_km, (size_t)grid * nb))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_gpart_lcnt, (size_t)grid * nb * nl))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_gcnt, (size_t)nb))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_gkm, (size_t)nb))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_glcnt, (size_t)nb * nl))) return rc;
+  HIPCHK(ctx, psh_geometry(ctx->Ng, ctx->ax, ctx->bins, ctx->d_gpart_cnt,
+                           ctx->d_gpart_km, ctx->d_gpart_lcnt, grid, ctx->d_gcnt,
+                           ctx->d_gkm, ctx->d_glcnt, ctx->stream));
+  ctx->cnt.assign(nb, 0);
+  ctx->km.assign(nb, 0.0);
+  ctx->lcnt.assign((size_t)nb * nl, 0.0);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->cnt.data(), ctx->d_gcnt, sizeof(unsigned long long) * nb,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->km.data(), ctx->d_gkm, sizeof(double) * nb,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->lcnt.data(), ctx->d_glcnt, sizeof(double) * nb * nl,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < nb; i++)
+    if (ctx->cnt[i]) ctx->km[i] /= (double)ctx->cnt[i];
+  ctx->geom_valid = true;
+  return PSH_OK;
+}
+
+int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
+  bool same = ctx->box_set;
+  for (int a = 0; a < 3; a++)
+    if (ctx->L[a] != L[a] || ctx->bmin[a] != bmin[a]) same = false;
+  if (same) return PSH_OK;
+  for (int a = 0; a < 3; a++) {
+    ctx->L[a] = L[a];
+    ctx->bmin[a] = bmin[a];
+  }
+  int rc;
+  if ((rc = make_bins(ctx))) return rc;
+  if ((rc = make_tables(ctx))) return rc;
+  // assignment geometry
+  PshAssignGeom &g = ctx->ag;
+  g.Ng = ctx->Ng;
+  g.scheme = ctx->conf.assign;
+  g.intlace = ctx->conf.intlace;
+  const int tz = ctx->Ng < 32 ? ctx->Ng : 32;
+  const int txy = ctx->Ng < 16 ? ctx->Ng : 16;
+  g.tile[0] = txy;
+  g.tile[1] = txy;
+  g.tile[2] = tz;
+  for (int a = 0; a < 3; a++) g.ntile[a] = ctx->Ng / g.tile[a];
+  g.row_pitch = 2 * ctx->H;
+  for (int a = 0; a < 3; a++) {
+    g.bmin[a] = bmin[a];
+    g.smin[a] = bmin[a] - 0.5 * L[a] / ctx->Ng;   // src/genr_mesh.c:817-818
+    g.L[a] = L[a];
+  }
+  ctx->ntiles_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  ctx->box_set = true;
+  ctx->geom_valid = false;
+  return PSH_OK;
+}
+
+// def_box from coordinate bounds (src/genr_mesh.c:509-578).
+int def_box(psh_ctx *ctx, const double mn[3], const double mx[3]) {
+  const psh_conf &c = ctx->conf;
+  const char ax[3] = {'x', 'y', 'z'};
+  double L[3], bmin[3];
+  for (int i = 0; i < 3; i++) {
+    if (mn[i] > mx[i])
+      return set_err(ctx, PSH_ERR_CATA, "invalid %c coordinate value in the catalogs", ax[i]);
+    if (c.issim) {
+      if (mn[i] < 0) return set_err(ctx, PSH_ERR_CATA, "%c coordinate below 0: %lf", ax[i], mn[i]);
+      if (mx[i] >= c.bsize[i])
+        return set_err(ctx, PSH_ERR_CATA, "%c coordinate not smaller than BOX_SIZE: %lf",
+                       ax[i], mx[i]);
+    }
+  }
+  if (c.issim) {
+    for (int i = 0; i < 3; i++) {
+      L[i] = c.bsize[i];
+      bmin[i] = 0;
+    }
+  } else if (c.has_bsize) {
+    for (int i = 0; i < 3; i++) {
+      if (mx[i] - mn[i] > c.bsize[i])
+        return set_err(ctx, PSH_ERR_CATA,
+                       "BOX_SIZE is too small for the %c coordinates, should be at least %.10lg",
+                       ax[i], mx[i] - mn[i]);
+      bmin[i] = (mx[i] + mn[i] - c.bsize[i]) * 0.5;
+      L[i] = c.bsize[i];
+    }
+  } else {
+    for (int i = 0; i < 3; i++) {
+      L[i] = ceil((mx[i] - mn[i]) * (1 + c.bpad[i]) / PSH_BOX_CEIL) * PSH_BOX_CEIL;
+      bmin[i] = (mx[i] + mn[i] - L[i]) * 0.5;
+    }
+  }
+  for (int i = 0; i < 3; i++) ctx->cmax[i] = mx[i];
+  return set_box(ctx, L, bmin);
+}
+
+int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
+  int rc;
+  const size_t ntiles = ctx->ntiles_mock * nmock;
+  if (ntiles + 1 > ctx->tiles_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->counts, ntiles + 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->starts, ntiles + 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->cursors, ntiles + 1))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->scan_tmp, psh_assign_scan_bytes(ntiles) / 4 + 64)))
+      return rc;
+    ctx->tiles_cap = ntiles + 1;
+  }
+  if (nparts > ctx->parts_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->d_parts, (size_t)nparts))) return rc;
+    ctx->parts_cap = nparts;
+  }
+  if (!ctx->d_flags) {
+    if ((rc = dev_alloc(ctx, &ctx->d_flags, 4))) return rc;
+  }
+  // expected (particle, tile) pairs: prod_a (1 + (W' - 1) / T_a), W' the
+  // union window; 1.5x margin, grown on overflow
+  const int W = ctx->conf.assign + 1 + (ctx->conf.intlace ? 1 : 0);
+  double dup = 1.0;
+  for (int a = 0; a < 3; a++) dup *= 1.0 + (double)(W - 1) / ctx->ag.tile[a];
+  size_t need = (size_t)(ntotal * dup * 1.5) + 4096;
+  if (need > ctx->rec_cap) {
+    if (ctx->records) {
+      (void)hipFree(ctx->records);
+      ctx->records = nullptr;
+    }
+    HIPCHK(ctx, hipMalloc(&ctx->records, need * psh_assign_record_bytes()));
+    ctx->rec_cap = need;
+  }
+  return PSH_OK;
+}
+
+int grow_records(psh_ctx *ctx) {
+  size_t need = ctx->rec_cap * 2;
+  if (ctx->records) (void)hipFree(ctx->records);
+  ctx->records = nullptr;
+  HIPCHK(ctx, hipMalloc(&ctx->records, need * psh_assign_record_bytes()));
+  ctx->rec_cap = need;
+  return PSH_OK;
+}
+
+int ensure_partials(psh_ctx *ctx, int nmock, int grid, int nacc, int slots) {
+  int rc;
+  size_t need = (size_t)nmock * grid * nacc;
+  if (need > ctx->partials_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->partials, need))) return rc;
+    ctx->partials_cap = need;
+  }
+  size_t needb = (size_t)nmock * nacc * slots;
+  if (needb > ctx->bins_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->d_bins, needb))) return rc;
+    ctx->bins_cap = needb;
+  }
+  return PSH_OK;
+}
+
+int xgrid_for(psh_ctx *ctx, int nmock) {
+  const int C = ctx->Ng <= 512 ? 16 : 8;
+  const int ntz = (ctx->M + 1 + C - 1) / C;
+  const long ntiles = (long)ctx->Ng * ntz;
+  long g = 2048 / nmock;
+  if (g < 8) g = 8;
+  if (g > ntiles) g = ntiles;
+  return (int)g;
+}
+
+void ev_record(psh_ctx *ctx, int i) {
+  if (ctx->profiling && ctx->tm.ev[i]) (void)hipEventRecord(ctx->tm.ev[i], ctx->stream);
+}
+
+void ev_collect(psh_ctx *ctx) {
+  if (!ctx->profiling) return;
+  float t;
+  ctx->ms[0] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[1]) == hipSuccess ? t : -1;
+  ctx->ms[1] = hipEventElapsedTime(&t, ctx->tm.ev[1], ctx->tm.ev[2]) == hipSuccess ? t : -1;
+  ctx->ms[2] = hipEventElapsedTime(&t, ctx->tm.ev[2], ctx->tm.ev[3]) == hipSuccess ? t : -1;
+  ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[3]) == hipSuccess ? t : -1;
+}
+
+// Assign parts (device table already built on host) into meshes.
+int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
+               float *meshA, float *meshB) {
+  size_t ntotal = 0;
+  for (auto &p : hp) ntotal += p.n;
+  int rc;
+  if ((rc = ensure_assign_scratch(ctx, nmock, (int)hp.size(), ntotal))) return rc;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_parts, hp.data(), sizeof(PshDevPart) * hp.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, psh_fill_zero_u32((uint32_t *)ctx->d_flags, 4, ctx->stream));
+    HIPCHK(ctx, psh_assign_run(ctx->ag, ctx->d_parts, (int)hp.size(), ntotal, nmock,
+                               ctx->counts, ctx->starts, ctx->cursors, ctx->records,
+                               ctx->rec_cap, meshA, meshB, ctx->mstride, ctx->scan_tmp,
+                               ctx->d_flags, ctx->stream));
+    int flags = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (flags & 2)
+      return set_err(ctx, PSH_ERR_MESH, "inconsistent interlaced cell of a particle");
+    if (!(flags & 1)) return PSH_OK;
+    if ((rc = grow_records(ctx))) return rc;
+  }
+  return set_err(ctx, PSH_ERR_MESH, "assignment record buffer overflow");
+}
+
+// FFTs of the plain (and shifted) meshes of nmock mocks, fused binning.
+int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/) {
+  const int N = ctx->Ng;
+  PshYlmR noy{};
+  float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
+  float2 *B = ctx->conf.intlace ? reinterpret_cast<float2 *>(ctx->meshB) : nullptr;
+  HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
+  if (B)
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
+  HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
+  if (B) HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
+  ev_record(ctx, 2);
+  const int grid = xgrid_for(ctx, nmock);
+  const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
+  int rc;
+  if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
+  PshXBinArgs a{};
+  a.epi = PSH_EPI_SIM;
+  a.A = A;
+  a.B = B;
+  a.nmock = nmock;
+  a.mstride_floats = ctx->mstride;
+  a.ax = ctx->ax;
+  a.bins = ctx->bins;
+  a.phase = ctx->phase;
+  a.partials = ctx->partials;
+  a.grid = grid;
+  HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  HIPCHK(ctx, psh_reduce_partials(ctx->partials, nmock, grid, nacc, out_bins, ctx->stream));
+  return PSH_OK;
+}
+
+// Normalise the raw sums of a simulation box (src/multipole.c:1073-1083).
+void normalise_sim(psh_ctx *ctx, const double *raw, double shot, double norm, double *pl) {
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
+  for (int l = 0; l < nl; l++)
+    for (int i = 0; i < nb; i++) {
+      double v = raw[l * nb + i];
+      if (ctx->cnt[i]) {
+        v /= norm * ctx->cnt[i];
+        v -= shot * ctx->lcnt[i + l * nb] / ctx->cnt[i];
+      }
+      v *= 2 * ctx->conf.poles[l] + 1;
+      pl[l * nb + i] = v;
+    }
+}
+
+void fill_pk_common(psh_ctx *ctx, psh_pk *out) {
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
+  out->nbin = nb;
+  out->nl = nl;
+  for (int l = 0; l <= PSH_MAX_ELL; l++) out->poles[l] = l < nl ? ctx->conf.poles[l] : 0;
+  if (out->k) memcpy(out->k, ctx->kcen.data(), sizeof(double) * nb);
+  if (out->kedge) memcpy(out->kedge, ctx->kedge.data(), sizeof(double) * (nb + 1));
+  if (out->km) memcpy(out->km, ctx->km.data(), sizeof(double) * nb);
+  if (out->cnt)
+    for (int i = 0; i < nb; i++) out->cnt[i] = ctx->cnt[i];
+  if (out->lcnt) memcpy(out->lcnt, ctx->lcnt.data(), sizeof(double) * nb * nl);
+  // log-bin rescale, as the reference writes it (src/multipole.c:1267-1273):
+  // km and the lower edges are exponentiated from the already-rescaled k
+  if (ctx->conf.logscale) {
+    for (int i = 0; i < nb; i++) {
+      if (out->k) out->k[i] = pow(10, out->k[i]);
+      double k = out->k ? out->k[i] : pow(10, ctx->kcen[i]);
+      if (out->km) out->km[i] = pow(10, k);
+      if (out->kedge) out->kedge[i] = pow(10, k);
+    }
+    if (out->kedge) out->kedge[nb] = pow(10, out->kedge[nb]);
+  }
+}
+
+// Survey path for catalogue `c`: data and randoms on the device.
+int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_rand,
+               size_t nr, double alpha, double shot, double norm, double *pl) {
+  const int N = ctx->Ng;
+  const bool il = ctx->conf.intlace;
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
+  std::vector<PshDevPart> hp(2);
+  hp[0] = PshDevPart{d_data, nd, 0, 1.0f, 0};
+  hp[1] = PshDevPart{d_rand, nr, nd, (float)(-alpha), 0};
+  ev_record(ctx, 0);
+  int rc;
+  // plain grid -> A when interlaced (combined into A), else -> B (real field
+  // kept for the multipole weights) and transformed out of place into A
+  float *plain = il ? ctx->meshA : ctx->meshB;
+  if ((rc = run_assign(ctx, hp, 1, plain, il ? ctx->meshB : nullptr))) return rc;
+  ev_record(ctx, 1);
+  float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
+  float2 *B = reinterpret_cast<float2 *>(ctx->meshB);
+  PshYlmR noy{};
+  if (il) {
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, B, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+  } else {
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, A, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+  }
+  ev_record(ctx, 2);
+  const int grid = xgrid_for(ctx, 1);
+  int nslots = 1;
+  for (int n = 1; n < nl; n++) nslots += 2 * ctx->conf.poles[n] + 1;
+  if ((rc = ensure_partials(ctx, 1, grid, nb, nslots))) return rc;
+  PshXBinArgs a{};
+  a.epi = PSH_EPI_SURVEY0;
+  a.A = A;
+  a.B = il ? B : nullptr;
+  a.store = A;
+  a.nmock = 1;
+  a.mstride_floats = ctx->mstride;
+  a.ax = ctx->ax;
+  a.bins = ctx->bins;
+  a.phase = ctx->phase;
+  a.partials = ctx->partials;
+  a.grid = grid;
+  HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb, ctx->d_bins, ctx->stream));
+  if (il) {
+    // back to configuration space: B = c2r(F0) / Ntot (src/multipole.c:489-500)
+    HIPCHK(ctx, psh_fft_x(N, A, B, 1, ctx->mstride, ctx->tw, 1, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, B, 1, ctx->mstride, ctx->tw, 1, ctx->stream));
+    const double ntot = (double)N * N * N;
+    HIPCHK(ctx, psh_fft_z_c2r(N, B, 1, ctx->mstride, ctx->tw, (float)(1.0 / ntot),
+                              ctx->stream));
+  }
+  // multipoles l > 0 (src/multipole.c:1237-1245, src/mp_template.c:48-149);
+  // the first pole is skipped when it is not 0, as in the reference
+  int slot = 1;
+  float2 *S = reinterpret_cast<float2 *>(ctx->meshS);
+  for (int n = 1; n < nl; n++) {
+    const int ell = ctx->conf.poles[n];
+    for (int m = -ell; m <= ell; m++, slot++) {
+      PshYlmR y{};
+      y.ell = ell;
+      y.m = m;
+      y.norm = ylm_norm(ell, m);
+      for (int ax = 0; ax < 3; ax++) y.rpos[ax] = ctx->d_rpos + (size_t)ax * N;
+      HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream));
+      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+      PshXBinArgs b = a;
+      b.epi = PSH_EPI_SURVEYL;
+      b.A = S;
+      b.B = nullptr;
+      b.store = nullptr;
+      b.F0 = A;
+      b.ell = ell;
+      b.m = m;
+      b.ynorm = y.norm;
+      HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
+      HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb,
+                                      ctx->d_bins + (size_t)slot * nb, ctx->stream));
+    }
+  }
+  ev_record(ctx, 3);
+  std::vector<double> raw((size_t)nslots * nb);
+  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ev_collect(ctx);
+  for (int q = 0; q < nl * nb; q++) pl[q] = 0;
+  if (ctx->conf.poles[0] == 0) {   // src/multipole.c:1110-1114
+    for (int i = 0; i < nb; i++) {
+      double v = raw[i];
+      if (ctx->cnt[i]) {
+        v = v / ctx->cnt[i] - shot;
+        v /= norm;
+      }
+      pl[i] = v;
+    }
+  }
+  slot = 1;
+  for (int n = 1; n < nl; n++) {   // src/multipole.c:1146-1153
+    const int ell = ctx->conf.poles[n];
+    std::vector<double> s(nb, 0.0);
+    for (int m = -ell; m <= ell; m++, slot++)
+      for (int i = 0; i < nb; i++) s[i] += raw[(size_t)slot * nb + i];
+    for (int i = 0; i < nb; i++) {
+      double v = s[i];
+      if (ctx->cnt[i]) {
+        double nrm = norm * ctx->cnt[i];
+        v *= 4 * M_PI / nrm;
+      }
+      pl[n * nb + i] = v;
+    }
+  }
+  return PSH_OK;
+}
+
+int bounds_device(psh_ctx *ctx, const double *d, size_t n, double mn[3], double mx[3]) {
+  if (n == 0) return PSH_OK;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  int rc;
+  if (!ctx->d_bounds && (rc = dev_alloc(ctx, &ctx->d_bounds, (size_t)1024 * 6))) return rc;
+  HIPCHK(ctx, psh_coord_bounds(d, n, ctx->d_bounds, blocks, ctx->stream));
+  std::vector<double> h((size_t)blocks * 6);
+  HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->d_bounds, sizeof(double) * h.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int b = 0; b < blocks; b++)
+    for (int a = 0; a < 3; a++) {
+      if (mn[a] > h[b * 6 + a]) mn[a] = h[b * 6 + a];
+      if (mx[a] < h[b * 6 + 3 + a]) mx[a] = h[b * 6 + 3 + a];
+    }
+  return PSH_OK;
+}
+
+int stage_host(psh_ctx *ctx, const psh_data *h, size_t n, size_t offset_records) {
+  if (n == 0) return PSH_OK;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_cat + 4 * offset_records, h, sizeof(psh_data) * n,
+                             hipMemcpyHostToDevice, ctx->stream));
+  return PSH_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// context layer
+// ===========================================================================
+
+extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out) {
+  if (!out) return set_err(nullptr, PSH_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  psh_ctx *ctx = new (std::nothrow) psh_ctx();
+  if (!ctx) return set_err(nullptr, PSH_ERR_MEMORY, "out of host memory");
+  int rc = validate_conf(conf, ctx);
+  if (rc) {
+    delete ctx;
+    return rc;
+  }
+  ctx->conf = *conf;
+  ctx->nmock_max = nmock_max < 1 ? 1 : nmock_max;
+  ctx->Ng = conf->gsize;
+  ctx->M = ctx->Ng / 2;
+  ctx->H = ctx->M + PSH_PAD;
+  ctx->mstride = (size_t)ctx->Ng * ctx->Ng * 2 * ctx->H;
+  ctx->survey = !conf->issim;
+  ctx->ellmax = conf->poles[conf->npole - 1];
+  if (ctx->survey && ctx->nmock_max > 1) {
+    set_err(ctx, PSH_ERR_UNSUPPORTED, "batched mocks are supported for simulation boxes");
+    delete ctx;
+    return PSH_ERR_UNSUPPORTED;
+  }
+  if (hipGetDevice(&ctx->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_err(ctx, PSH_ERR_DEVICE, "no usable HIP device");
+    delete ctx;
+    return PSH_ERR_DEVICE;
+  }
+  const size_t nm = (size_t)ctx->nmock_max;
+  if ((rc = dev_alloc(ctx, &ctx->meshA, ctx->mstride * nm)) ||
+      ((conf->intlace || ctx->survey) && (rc = dev_alloc(ctx, &ctx->meshB, ctx->mstride * nm))) ||
+      (ctx->survey && ctx->ellmax > 0 && (rc = dev_alloc(ctx, &ctx->meshS, ctx->mstride)))) {
+    std::string e = ctx->err;
+    psh_ctx_destroy(ctx);
+    set_err(nullptr, rc, "%s", e.c_str());
+    return rc;
+  }
+  if (psh_fft_twiddles(ctx->Ng, &ctx->tw) != hipSuccess) {
+    psh_ctx_destroy(ctx);
+    return set_err(nullptr, PSH_ERR_DEVICE, "twiddle table upload failed");
+  }
+  {
+    std::vector<float2> ph(ctx->Ng);
+    for (int i = 0; i < ctx->Ng; i++) {
+      int s = (i <= ctx->M) ? i : i - ctx->Ng;
+      double a = M_PI * s / ctx->Ng;
+      ph[i] = make_float2((float)cos(a), (float)sin(a));
+    }
+    if (hipMalloc((void **)&ctx->phase, sizeof(float2) * ctx->Ng) != hipSuccess ||
+        hipMemcpy(ctx->phase, ph.data(), sizeof(float2) * ctx->Ng, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      psh_ctx_destroy(ctx);
+      return set_err(nullptr, PSH_ERR_DEVICE, "phase table upload failed");
+    }
+  }
+  for (int i = 0; i < 5; i++) (void)hipEventCreate(&ctx->tm.ev[i]);
+  if (conf->issim) {
+    double bmin[3] = {0, 0, 0};
+    if ((rc = set_box(ctx, conf->bsize, bmin))) {
+      std::string e = ctx->err;
+      psh_ctx_destroy(ctx);
+      set_err(nullptr, rc, "%s", e.c_str());
+      return rc;
+    }
+  }
+  *out = ctx;
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
+  if (!ctx) return PSH_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  void *bufs[] = {ctx->meshA, ctx->meshB, ctx->meshS, ctx->counts, ctx->starts, ctx->cursors,
+                  ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
+                  ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_gcnt, ctx->d_gkm,
+                  ctx->d_glcnt, ctx->d_gpart_cnt, ctx->d_gpart_km, ctx->d_gpart_lcnt,
+                  ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < 5; i++)
+    if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_set_box(psh_ctx *ctx, const double bsize[3], const double bmin[3]) {
+  if (!ctx || !bsize || !bmin) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  for (int a = 0; a < 3; a++) ctx->cmax[a] = bmin[a] + bsize[a];
+  return set_box(ctx, bsize, bmin);
+}
+
+extern "C" int psh_ctx_nbin(const psh_ctx *ctx) { return ctx && ctx->box_set ? ctx->nbin : 0; }
+
+extern "C" int psh_ctx_set_profiling(psh_ctx *ctx, int on) {
+  if (!ctx) return PSH_ERR_ARG;
+  ctx->profiling = on != 0;
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_timings(const psh_ctx *ctx, double ms[4]) {
+  if (!ctx || !ms) return PSH_ERR_ARG;
+  for (int i = 0; i < 4; i++) ms[i] = ctx->ms[i];
+  return PSH_OK;
+}
+
+extern "C" const char *psh_last_error(const psh_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_last_error.c_str();
+}
+
+extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n,
+                                  double weight_sum, psh_pk *out) {
+  if (!ctx || !out) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (ctx->survey)
+    return set_err(ctx, PSH_ERR_ARG, "psh_ctx_run_device is for simulation boxes");
+  int rc;
+  // box check on all particles (src/genr_mesh.c:517-533)
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  ev_record(ctx, 0);
+  if ((rc = bounds_device(ctx, (const double *)d_data, n, mn, mx))) return rc;
+  if ((rc = def_box(ctx, mn, mx))) return rc;
+  if ((rc = ensure_geometry(ctx))) return rc;
+  std::vector<PshDevPart> hp(1);
+  hp[0] = PshDevPart{(const double *)d_data, n, 0, 1.0f, 0};
+  if ((rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr)))
+    return rc;
+  ev_record(ctx, 1);
+  const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
+  if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
+  if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
+  ev_record(ctx, 3);
+  std::vector<double> raw(nacc);
+  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * nacc,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ev_collect(ctx);
+  // shot noise and normalisation of a box (src/genr_mesh.c:904-910)
+  const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+  const double shot = vol / weight_sum;
+  const double norm = weight_sum * weight_sum / vol;
+  fill_pk_common(ctx, out);
+  if (out->pl[0]) normalise_sim(ctx, raw.data(), shot, norm, out->pl[0]);
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
+                                 const size_t *offsets, const double *weight_sum,
+                                 int nmock, int graph, psh_pk *out) {
+  if (!ctx || !d_data || !offsets || !weight_sum || !out)
+    return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (ctx->survey) return set_err(ctx, PSH_ERR_ARG, "batched runs are for simulation boxes");
+  if (nmock < 1 || nmock > ctx->nmock_max)
+    return set_err(ctx, PSH_ERR_ARG, "nmock %d outside [1, %d]", nmock, ctx->nmock_max);
+  int rc;
+  if ((rc = ensure_geometry(ctx))) return rc;
+  std::vector<PshDevPart> hp(nmock);
+  for (int m = 0; m < nmock; m++)
+    hp[m] = PshDevPart{(const double *)d_data + 4 * offsets[m], offsets[m + 1] - offsets[m],
+                       offsets[m], 1.0f, m};
+  const size_t ntotal = offsets[nmock] - offsets[0];
+  for (int m = 0; m < nmock; m++) hp[m].start = offsets[m] - offsets[0];
+  const int grid = xgrid_for(ctx, nmock);
+  const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
+  if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
+  if ((rc = ensure_assign_scratch(ctx, nmock, nmock, ntotal))) return rc;
+  float *mB = ctx->conf.intlace ? ctx->meshB : nullptr;
+  ev_record(ctx, 0);
+  if (!graph) {
+    if ((rc = run_assign(ctx, hp, nmock, ctx->meshA, mB))) return rc;
+    ev_record(ctx, 1);
+    if ((rc = run_sim_fft_bin(ctx, nmock, ctx->d_bins))) return rc;
+  } else {
+    // capture the launch sequence once per (data pointer, nmock, size);
+    // the part table is uploaded before capture
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_parts, hp.data(), sizeof(PshDevPart) * nmock,
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->gexec || ctx->graph_nmock != nmock || ctx->graph_ntotal != ntotal ||
+        ctx->graph_data != d_data) {
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+      ctx->gexec = nullptr;
+      ctx->graph = nullptr;
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      bool prof = ctx->profiling;
+      ctx->profiling = false;
+      hipError_t e1 = psh_fill_zero_u32((uint32_t *)ctx->d_flags, 4, ctx->stream);
+      hipError_t e2 = psh_assign_run(ctx->ag, ctx->d_parts, nmock, ntotal, nmock, ctx->counts,
+                                     ctx->starts, ctx->cursors, ctx->records, ctx->rec_cap,
+                                     ctx->meshA, mB, ctx->mstride, ctx->scan_tmp,
+                                     ctx->d_flags, ctx->stream);
+      int rc2 = (e1 == hipSuccess && e2 == hipSuccess) ? run_sim_fft_bin(ctx, nmock, ctx->d_bins)
+                                                     : PSH_ERR_DEVICE;
+      ctx->profiling = prof;
+      hipError_t e3 = hipStreamEndCapture(ctx->stream, &ctx->graph);
+      if (rc2 || e3 != hipSuccess)
+        return set_err(ctx, PSH_ERR_DEVICE, "graph capture failed");
+      HIPCHK(ctx, hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+      ctx->graph_nmock = nmock;
+      ctx->graph_ntotal = ntotal;
+      ctx->graph_data = d_data;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+  }
+  ev_record(ctx, 3);
+  std::vector<double> raw((size_t)nmock * nacc);
+  int flags = 0;
+  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (graph) {
+    if (ctx->profiling) {
+      float t;
+      ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[3]) == hipSuccess ? t : -1;
+      ctx->ms[0] = ctx->ms[1] = ctx->ms[2] = -1;
+    }
+  } else {
+    ev_collect(ctx);
+  }
+  if (flags & 1) {
+    // record overflow inside the graph: grow and recapture next call
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+    if ((rc = grow_records(ctx))) return rc;
+    return set_err(ctx, PSH_ERR_MESH, "assignment record buffer overflow (grown; rerun)");
+  }
+  if (flags & 2) return set_err(ctx, PSH_ERR_MESH, "inconsistent interlaced cell of a particle");
+  const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+  for (int m = 0; m < nmock; m++) {
+    fill_pk_common(ctx, &out[m]);
+    const double shot = vol / weight_sum[m];
+    const double norm = weight_sum[m] * weight_sum[m] / vol;
+    if (out[m].pl[0]) normalise_sim(ctx, raw.data() + (size_t)m * nacc, shot, norm, out[m].pl[0]);
+  }
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_define_box(psh_ctx *ctx, const psh_cata *cat) {
+  if (!ctx || !cat) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  // bounds on the host (the catalogues are host arrays here)
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int c = 0; c < cat->num; c++) {
+    for (size_t p = 0; p < cat->ndata[c]; p++)
+      for (int a = 0; a < 3; a++) {
+        double v = cat->data[c][p].x[a];
+        if (mn[a] > v) mn[a] = v;
+        if (mx[a] < v) mx[a] = v;
+      }
+    if (!ctx->conf.issim)
+      for (size_t p = 0; p < cat->nrand[c]; p++)
+        for (int a = 0; a < 3; a++) {
+          double v = cat->rand[c][p].x[a];
+          if (mn[a] > v) mn[a] = v;
+          if (mx[a] < v) mx[a] = v;
+        }
+  }
+  return def_box(ctx, mn, mx);
+}
+
+extern "C" int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out) {
+  if (!ctx || !cat || !out) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (cat->num != ctx->conf.ndata)
+    return set_err(ctx, PSH_ERR_ARG, "catalogue count %d != ndata %d", cat->num, ctx->conf.ndata);
+  int rc;
+  if ((rc = psh_ctx_define_box(ctx, cat))) return rc;
+  if ((rc = ensure_geometry(ctx))) return rc;
+  size_t need = 0;
+  for (int c = 0; c < cat->num; c++) {
+    size_t n = cat->ndata[c] + (ctx->survey ? cat->nrand[c] : 0);
+    if (n > need) need = n;
+  }
+  if (need > ctx->d_cat_cap) {
+    if ((rc = dev_alloc(ctx, &ctx->d_cat, need * 4))) return rc;
+    ctx->d_cat_cap = need;
+  }
+  for (int c = 0; c < cat->num; c++) {
+    if (!ctx->conf.isauto[c]) continue;
+    if ((rc = stage_host(ctx, cat->data[c], cat->ndata[c], 0))) return rc;
+    if (ctx->survey) {
+      if ((rc = stage_host(ctx, cat->rand[c], cat->nrand[c], cat->ndata[c]))) return rc;
+      fill_pk_common(ctx, out);
+      if ((rc = run_survey(ctx, ctx->d_cat, cat->ndata[c], ctx->d_cat + 4 * cat->ndata[c],
+                           cat->nrand[c], cat->alpha[c], cat->shot[c], cat->norm[c],
+                           out->pl[c])))
+        return rc;
+    } else {
+      psh_pk tmp = *out;
+      tmp.pl[0] = out->pl[c];
+      double w = cat->wdata[c];
+      if ((rc = psh_ctx_run_device(ctx, (const psh_data *)ctx->d_cat, cat->ndata[c], w, &tmp)))
+        return rc;
+      // shot/norm of simulation boxes (src/genr_mesh.c:904-910)
+      const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+      cat->shot[c] = vol / w;
+      cat->norm[c] = w * w / vol;
+    }
+  }
+  fill_pk_common(ctx, out);
+  return PSH_OK;
+}
+
+// ===========================================================================
+// drop-in layer
+// ===========================================================================
+
+extern "C" psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat) {
+  printf("Generating meshes for FFT ...");
+  if (!conf) {
+    fprintf(stderr, "\nError: configuration parameters not loaded\n");
+    return nullptr;
+  }
+  if (conf->verbose) printf("\n");
+  fflush(stdout);
+  if (!cat) {
+    fprintf(stderr, "\nError: catalogs not read\n");
+    return nullptr;
+  }
+  psh_mesh *mesh = new (std::nothrow) psh_mesh();
+  if (!mesh) return nullptr;
+  int rc = psh_ctx_create(conf, 1, &mesh->ctx);
+  if (rc) {
+    fprintf(stderr, "\nError: failed to initalise the meshes: %s\n", psh_last_error(nullptr));
+    delete mesh;
+    return nullptr;
+  }
+  psh_ctx *ctx = mesh->ctx;
+  if ((rc = psh_ctx_define_box(ctx, cat))) {
+    fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
+    psh_mesh_destroy(mesh);
+    return nullptr;
+  }
+  mesh->Ng = ctx->Ng;
+  for (int a = 0; a < 3; a++) {
+    mesh->bsize[a] = ctx->L[a];
+    mesh->bmin[a] = ctx->bmin[a];
+  }
+  mesh->assign = conf->assign;
+  mesh->intlace = conf->intlace;
+  mesh->num = cat->num;
+  // copy the particles to the device; the assignment runs with powspec so
+  // that the device holds one catalogue's meshes at a time
+  for (int c = 0; c < cat->num; c++) {
+    if (dev_alloc(ctx, &mesh->d_data[c], cat->ndata[c] * 4) ||
+        hipMemcpy(mesh->d_data[c], cat->data[c], sizeof(psh_data) * cat->ndata[c],
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      fprintf(stderr, "\nError: failed to copy the catalogue to the device\n");
+      psh_mesh_destroy(mesh);
+      return nullptr;
+    }
+    mesh->ndata[c] = cat->ndata[c];
+    if (!conf->issim) {
+      if (dev_alloc(ctx, &mesh->d_rand[c], cat->nrand[c] * 4) ||
+          hipMemcpy(mesh->d_rand[c], cat->rand[c], sizeof(psh_data) * cat->nrand[c],
+                    hipMemcpyHostToDevice) != hipSuccess) {
+        fprintf(stderr, "\nError: failed to copy the catalogue to the device\n");
+        psh_mesh_destroy(mesh);
+        return nullptr;
+      }
+      mesh->nrand[c] = cat->nrand[c];
+    }
+  }
+  // shot noise and normalisation of simulation boxes (src/genr_mesh.c:904-910)
+  if (conf->issim) {
+    for (int c = 0; c < cat->num; c++) {
+      double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+      cat->shot[c] = vol / cat->wdata[c];
+      cat->norm[c] = cat->wdata[c] * cat->wdata[c] / vol;
+    }
+  }
+  // release the catalogues like the reference (src/genr_mesh.c:916-922)
+  for (int c = 0; c < cat->num; c++) {
+    if (cat->data && cat->data[c]) free(cat->data[c]);
+    if (cat->rand && cat->rand[c]) free(cat->rand[c]);
+  }
+  free(cat->data);
+  cat->data = nullptr;
+  free(cat->rand);
+  cat->rand = nullptr;
+  printf("\r\x1B[70C[\x1B[32;1mDONE\x1B[0m]\n");
+  return mesh;
+}
+
+extern "C" psh_pk *psh_powspec(const psh_conf *conf, const psh_cata *cat, psh_mesh *mesh) {
+  printf("Evaluating power spectra ...");
+  if (!conf) {
+    fprintf(stderr, "\nError: configuration parameters not loaded\n");
+    return nullptr;
+  }
+  if (conf->verbose) printf("\n");
+  fflush(stdout);
+  if (!cat || !mesh || !mesh->ctx) {
+    fprintf(stderr, "\nError: %s\n", !cat ? "catalogs not read" : "meshes not generated");
+    return nullptr;
+  }
+  psh_ctx *ctx = mesh->ctx;
+  int rc = ensure_geometry(ctx);
+  if (rc) {
+    fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
+    return nullptr;
+  }
+  const int nb = ctx->nbin, nl = conf->npole;
+  psh_pk *pk = (psh_pk *)calloc(1, sizeof(psh_pk));
+  if (!pk) return nullptr;
+  pk->k = (double *)calloc(nb, sizeof(double));
+  pk->kedge = (double *)calloc(nb + 1, sizeof(double));
+  pk->km = (double *)calloc(nb, sizeof(double));
+  pk->cnt = (uint64_t *)calloc(nb, sizeof(uint64_t));
+  pk->lcnt = (double *)calloc((size_t)nb * nl, sizeof(double));
+  for (int c = 0; c < 2; c++)
+    if (conf->isauto[c]) pk->pl[c] = (double *)calloc((size_t)nb * nl, sizeof(double));
+  if (!pk->k || !pk->kedge || !pk->km || !pk->cnt || !pk->lcnt) {
+    psh_powspec_destroy(pk);
+    return nullptr;
+  }
+  for (int c = 0; c < mesh->num; c++) {
+    if (!conf->isauto[c]) continue;
+    if (conf->issim) {
+      psh_pk tmp = *pk;
+      tmp.pl[0] = pk->pl[c];
+      rc = psh_ctx_run_device(ctx, (const psh_data *)mesh->d_data[c], mesh->ndata[c],
+                              cat->wdata[c], &tmp);
+    } else {
+      fill_pk_common(ctx, pk);
+      rc = run_survey(ctx, mesh->d_data[c], mesh->ndata[c], mesh->d_rand[c], mesh->nrand[c],
+                      cat->alpha[c], cat->shot[c], cat->norm[c], pk->pl[c]);
+    }
+    if (rc) {
+      fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
+      psh_powspec_destroy(pk);
+      return nullptr;
+    }
+  }
+  fill_pk_common(ctx, pk);
+  // release the device meshes and particles; keep the metadata
+  // (src/multipole.c:1248-1264)
+  for (int c = 0; c < 2; c++) {
+    if (mesh->d_data[c]) (void)hipFree(mesh->d_data[c]);
+    if (mesh->d_rand[c]) (void)hipFree(mesh->d_rand[c]);
+    mesh->d_data[c] = mesh->d_rand[c] = nullptr;
+  }
+  psh_ctx_destroy(ctx);
+  mesh->ctx = nullptr;
+  printf("\r\x1B[70C[\x1B[32;1mDONE\x1B[0m]\n");
+  return pk;
+}
+
+extern "C" void psh_mesh_destroy(psh_mesh *mesh) {
+  if (!mesh) return;
+  for (int c = 0; c < 2; c++) {
+    if (mesh->d_data[c]) (void)hipFree(mesh->d_data[c]);
+    if (mesh->d_rand[c]) (void)hipFree(mesh->d_rand[c]);
+  }
+  if (mesh->ctx) psh_ctx_destroy(mesh->ctx);
+  delete mesh;
+}
+
+extern "C" void psh_powspec_destroy(psh_pk *pk) {
+  if (!pk) return;
+  free(pk->k);
+  free(pk->kedge);
+  free(pk->km);
+  free(pk->cnt);
+  free(pk->lcnt);
+  free(pk->pl[0]);
+  free(pk->pl[1]);
+  free(pk->xpl);
+  free(pk);
+}
+
+extern "C" int psh_mesh_info(const psh_mesh *mesh, int *Ng, double bsize[3], double bmin[3],
+                             int *assign, int *intlace) {
+  if (!mesh) return PSH_ERR_ARG;
+  if (Ng) *Ng = mesh->Ng;
+  for (int a = 0; a < 3; a++) {
+    if (bsize) bsize[a] = mesh->bsize[a];
+    if (bmin) bmin[a] = mesh->bmin[a];
+  }
+  if (assign) *assign = mesh->assign;
+  if (intlace) *intlace = mesh->intlace;
+  return PSH_OK;
+}
+
+// ===========================================================================
+// device helpers
+// ===========================================================================
+
+extern "C" int psh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int psh_device_malloc(void **ptr, size_t bytes) {
+  return hipMalloc(ptr, bytes) == hipSuccess ? PSH_OK : PSH_ERR_MEMORY;
+}
+
+extern "C" int psh_device_free(void *ptr) {
+  return hipFree(ptr) == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? PSH_OK
+                                                                         : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PSH_OK
+                                                                         : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_device_synchronize(void) {
+  return hipDeviceSynchronize() == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_device_arch(char *buf, int len) {
+  if (!buf || len < 1) return PSH_ERR_ARG;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
+    return PSH_ERR_DEVICE;
+  snprintf(buf, len, "%s", p.gcnArchName);
+  return PSH_OK;
+}

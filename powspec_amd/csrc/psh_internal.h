@@ -1,0 +1,169 @@
+// psh_internal.h -- shared declarations of the gfx950 kernels and the host
+// orchestration of the mesh -> FFT -> P_l(k) path.
+//
+// Memory layout of one mesh in HBM (Ng = grid size, M = Ng/2):
+//   real field     [Ng][Ng][2H] float,  rows padded to 2H floats
+//   complex field  [Ng][Ng][H]  float2, same bytes (in-place r2c),
+//                  kz = 0..M stored (M+1 entries), H = M + PSH_PAD
+// The padding keeps every 16-column kz tile 128-byte aligned and lets the
+// z-direction r2c run in place.  x is the slowest index and z the fastest,
+// as in the reference (src/define.h:134).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define PSH_PAD 16          // complex padding per row (one 16-column tile)
+#define PSH_MAX_NL 7
+
+// Per-axis FP64 tables used by the binning epilogues, device resident.
+// For axis a and index i (signed frequency i' for x,y; i for z):
+//   kv[a][i] = vec_a * i'          (vec_a = 2 pi / L_a)
+//   ksq[a][i] = kv * kv
+//   al[a][i] = alias correction factor (src/multipole.c:46-100)
+//   kd[a][i] = i' / L_a            (direction of k for Y_lm(k))
+struct PshAxisTables {
+  const double *kv[3];
+  const double *ksq[3];
+  const double *al[3];
+  const double *kd[3];
+};
+
+// Binning geometry (src/multipole.c:306-421).
+struct PshBins {
+  int nbin;
+  int logscale;
+  double kedge0;   // kedge[0]
+  double kedgeN;   // kedge[nbin]
+  double dk;
+  int nl;
+  int poles[PSH_MAX_NL];
+  double los[3];
+};
+
+// What the fused last FFT pass does with each mode.
+enum PshEpilogue {
+  PSH_EPI_SIM = 0,        // sim: |F|^2 alias mult L_l(mu), DC skipped
+  PSH_EPI_SURVEY0 = 1,    // survey l=0: |F|^2 alias mult, DC kept; F stored
+  PSH_EPI_SURVEYL = 2,    // survey l>0: Re(F0 conj(Y_lm(k) S)) alias mult
+  PSH_EPI_STORE = 3,      // plain transform, result stored
+};
+
+// ---------------------------------------------------------------------------
+// Assignment (k_assign.hip)
+// ---------------------------------------------------------------------------
+struct PshAssignGeom {
+  int Ng;
+  int scheme;           // 0..3
+  int intlace;          // second (half-cell shifted) grid
+  int tile[3];          // tile extent per axis
+  int ntile[3];         // Ng / tile
+  int row_pitch;        // floats per mesh row (2H)
+  double bmin[3];       // box corner of the plain grid
+  double smin[3];       // box corner of the shifted grid
+  double L[3];          // box size
+};
+
+// One catalogue part of a (possibly batched) assignment; the table lives in
+// device memory (uploaded before any graph capture).
+struct PshDevPart {
+  const double *xyzw;   // device psh_data records (x,y,z,w)
+  size_t n;             // number of records
+  size_t start;         // first global particle index of this part
+  float wscale;         // 1 for data, -alpha for randoms
+  int mock;             // destination mesh pair index
+};
+
+size_t psh_assign_record_bytes();
+// Bytes of scan scratch for ntiles counters.
+size_t psh_assign_scan_bytes(size_t ntiles);
+
+// Count the (particle, tile) pairs, fill tile buckets and deposit the plain
+// (meshA) and shifted (meshB, interlacing only) meshes of nmock mocks; mock
+// m's meshes start at meshA + m*mstride, meshB + m*mstride.
+// counts, starts (ntiles+1), cursors: uint32 scratch; records: rec_cap
+// records; flags: bit 0 record overflow, bit 1 inconsistent shifted base.
+hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
+    int nparts, size_t ntotal, int nmock, uint32_t *counts, uint32_t *starts,
+    uint32_t *cursors, void *records, size_t rec_cap, float *meshA,
+    float *meshB, size_t mstride, uint32_t *d_scan_tmp, int *d_flags,
+    hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// FFT (k_fft.hip)
+// ---------------------------------------------------------------------------
+
+// Twiddle table exp(-2 pi i m / N), m = 0..N-1, float2, from FP64.
+hipError_t psh_fft_twiddles(int N, float2 **d_tw);
+
+// Forward r2c along z, in place or out of place, for nmesh meshes laid out
+// contiguously (mesh stride mstride floats).  ylm: optional Y_lm(r) weight
+// (ell > 0) of the survey estimator with the reference's first-row rule.
+struct PshYlmR {
+  int ell, m;            // ell = 0: no weighting
+  float norm;            // normalisation of the real-form Y_lm
+  const float *rpos[3];  // per-axis (i + smin_g) * L, device arrays of Ng
+};
+// tw: twiddle table of N = Ng entries.
+hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
+    size_t mstride_floats, const float2 *tw, const PshYlmR &ylm,
+    hipStream_t s);
+
+// Inverse c2r along z in place, output scaled by `scale`.
+hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh,
+    size_t mstride_floats, const float2 *tw, float scale, hipStream_t s);
+
+// Complex FFT along y (axis 1) in place; dir +1 inverse, -1 forward.
+hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
+    const float2 *tw, int inverse, hipStream_t s);
+
+// Complex FFT along x (axis 0) from `in` to `out` (may alias); no epilogue.
+hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
+    size_t mstride_floats, const float2 *tw, int inverse, hipStream_t s);
+
+// Forward FFT along x fused with the binning epilogue.
+struct PshXBinArgs {
+  int epi;                 // PshEpilogue
+  const float2 *A;         // plain-grid field (per mock stride mstride)
+  const float2 *B;         // shifted-grid field or nullptr
+  float2 *store;           // F store target for SURVEY0 (usually == A)
+  const float2 *F0;        // SURVEYL: the l=0 field
+  int ell, m;              // SURVEYL: Y_lm(k) weight
+  float ynorm;             // SURVEYL: normalisation of the real-form Y_lm
+  int nmock;
+  size_t mstride_floats;
+  PshAxisTables ax;
+  PshBins bins;
+  const float2 *phase;     // interlace phase tables [3][Ng] exp(i pi i'/Ng)
+  double *partials;        // [grid][nacc] per workgroup partial sums
+  int grid;                // number of persistent workgroups per mock
+};
+int psh_xbin_nacc(const PshBins &b, int epi);
+hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
+    hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Geometry and reductions (k_bin.hip)
+// ---------------------------------------------------------------------------
+
+// cnt (uint64), km (sum of kmod*mult) and lcnt (sum of mult*L_l(mu), DC
+// skipped) per bin.  Writes partials then reduces into out_cnt, out_km,
+// out_lcnt[nl*nbin].
+hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
+    unsigned long long *part_cnt, double *part_km, double *part_lcnt,
+    int grid, unsigned long long *out_cnt, double *out_km, double *out_lcnt,
+    hipStream_t s);
+
+// Sum partials[nmock][grid][nacc] over grid in a fixed order into
+// out[nmock][nacc].
+hipError_t psh_reduce_partials(const double *partials, int nmock, int grid,
+    int nacc, double *out, hipStream_t s);
+
+// Fill a float buffer with zeros (graph-safe replacement of memset).
+hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s);
+
+// Min/max of the coordinates of n psh_data records (FP64, exact).
+// out: [blocks][6] partial (min x,y,z, max x,y,z).
+hipError_t psh_coord_bounds(const double *xyzw, size_t n, double *partial,
+    int blocks, hipStream_t s);

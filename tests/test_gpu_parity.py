@@ -1,0 +1,159 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C-ABI.
+
+Tolerances: mode counts bit-exact; for P_l per bin,
+    |P_l(gpu) - P_l(oracle)| <= TOL * (2l+1) * S_b
+with S_b = |P_0(oracle)| + shot (boxes) or |P_0| + shot/norm (surveys), the
+un-subtracted monopole amplitude of the bin, and TOL = 1e-4 (FP32 meshes and
+FFTs against FP64).
+"""
+import math
+
+import numpy as np
+import pytest
+
+TOL = 1e-4
+
+pytestmark = pytest.mark.gpu
+
+
+def _uniform(n, L, seed, weights=False):
+    rng = np.random.default_rng(seed)
+    d = np.empty((n, 4))
+    d[:, :3] = rng.uniform(0.0, 1.0, (n, 3)) * np.asarray(L)
+    d[:, 3] = rng.uniform(0.5, 1.5, n) if weights else 1.0
+    return d
+
+
+def _check(got, ref, poles, shot, norm=None, tol=TOL):
+    assert got["nbin"] == ref["nbin"]
+    assert np.array_equal(got["cnt"], ref["cnt"]), "mode counts differ"
+    assert np.allclose(got["kedge"], ref["kedge"], rtol=0, atol=0)
+    assert np.allclose(got["km"], ref["km"], rtol=1e-12)
+    scale = np.abs(ref["pl"][0][0]) + (shot if norm is None else shot / norm)
+    has = ref["cnt"] > 0
+    worst = 0.0
+    for li, ell in enumerate(poles):
+        d = np.abs(got["pl"][0][li] - ref["pl"][0][li])[has] / scale[has]
+        worst = max(worst, float(d.max()) if d.size else 0.0)
+        assert np.all(d <= tol * (2 * ell + 1)), (ell, float(d.max()))
+        assert np.all(got["pl"][0][li][~has] == ref["pl"][0][li][~has])
+    return worst
+
+
+def test_device_is_gfx950(psh):
+    assert psh.device_arch().startswith("gfx950")
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+@pytest.mark.parametrize("intlace", [False, True])
+def test_sim_small_all_schemes(psh, oracle, scheme, intlace):
+    Ng, L, n = 32, 1000.0, 30000
+    d = _uniform(n, [L] * 3, 100 + scheme, weights=True)
+    kw = dict(gsize=Ng, assign=scheme, intlace=intlace, bsize=L, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, d[:, 3].sum())
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    w = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"scheme={scheme} intlace={intlace} worst={w:.2e}")
+
+
+@pytest.mark.parametrize("Ng", [8, 16, 64, 128])
+def test_sim_grid_sizes(psh, oracle, Ng):
+    L, n = 500.0, 20000
+    d = _uniform(n, [L] * 3, Ng)
+    kw = dict(gsize=Ng, assign=2, intlace=True, bsize=L, kbin=0.02, poles=[0, 1, 2, 3, 4, 6],
+              los=[0.6, 0.0, 0.8])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, float(n))
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    _check(got, ref, [0, 1, 2, 3, 4, 6], ref["shot"][0])
+
+
+def test_sim_config1_exact_setup(psh, oracle):
+    """L=1000, 1e6 uniform particles, CIC, Ng=256, no interlacing, l=0,2,4,
+    dk=0.01 (the reference's CPU-runnable case)."""
+    Ng, L, n = 256, 1000.0, 1_000_000
+    d = _uniform(n, [L] * 3, 1)
+    kw = dict(gsize=Ng, assign=1, intlace=False, bsize=L, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, float(n))
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    assert ref["nbin"] == 80
+    w = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"config-1 worst={w:.2e}")
+
+
+def test_sim_clustered_pcs_interlaced(psh, oracle):
+    """Clustered particles (many per cell, strong large-scale power)."""
+    Ng, L = 128, 1000.0
+    rng = np.random.default_rng(3)
+    centres = rng.uniform(0, L, (400, 3))
+    pts = (centres[rng.integers(0, 400, 300000)] + rng.normal(0, 15.0, (300000, 3))) % L
+    d = np.empty((len(pts), 4))
+    d[:, :3] = pts
+    d[:, 3] = 1.0
+    kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.005, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    w = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"clustered worst={w:.2e}")
+
+
+def test_sim_boundary_particles(psh, oracle):
+    """Particles on cell edges, on the last half cell (periodic shift of the
+    interlaced grid) and at the largest coordinate below L."""
+    Ng, L = 16, 160.0
+    h = L / Ng
+    pts = [[0, 0, 0], [h, 2 * h, 3 * h], [L - 1e-9, L - h / 2, L - h / 4],
+           [h / 2, 1.5 * h, 15.5 * h], [np.nextafter(L, 0)] * 3, [7.5 * h, 8 * h, 0.25 * h]]
+    d = np.array([p + [1.0] for p in pts] * 50)
+    for scheme in range(4):
+        kw = dict(gsize=Ng, assign=scheme, intlace=True, bsize=L, kbin=0.05, poles=[0, 2])
+        ctx = psh.Context(psh.make_conf(**kw))
+        got = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
+        ref = oracle.run(oracle.make_conf(**kw), [d])
+        _check(got, ref, [0, 2], ref["shot"][0])
+
+
+def test_run_host_dropin_sim(psh, oracle):
+    Ng, L, n = 32, 300.0, 5000
+    d = _uniform(n, [L] * 3, 9, weights=True)
+    kw = dict(gsize=Ng, assign=1, intlace=True, bsize=L, kbin=0.03, poles=[0, 2])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_host([d])
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    _check(got, ref, [0, 2], ref["shot"][0])
+    assert math.isclose(got["shot"][0], ref["shot"][0], rel_tol=1e-15)
+    assert math.isclose(got["norm"][0], ref["norm"][0], rel_tol=1e-15)
+
+
+def test_box_errors(psh):
+    d = _uniform(100, [100.0] * 3, 2)
+    d[3, 1] = 100.0   # not smaller than BOX_SIZE
+    ctx = psh.Context(psh.make_conf(gsize=16, bsize=100.0, kbin=0.05))
+    with pytest.raises(psh.PshError) as e:
+        ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), 100.0)
+    assert e.value.code == psh.ERR_CATA
+
+
+def test_batch_matches_plain(psh):
+    Ng, L = 32, 1000.0
+    rng = np.random.default_rng(11)
+    sizes = rng.integers(2000, 9000, 6)
+    cats = [_uniform(int(s), [L] * 3, 200 + i) for i, s in enumerate(sizes)]
+    allp = np.concatenate(cats)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    kw = dict(gsize=Ng, assign=2, intlace=True, bsize=L, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw), nmock_max=len(cats))
+    dev = psh.DeviceArray.from_numpy(allp)
+    batch = ctx.run_batch(dev, off, [float(s) for s in sizes], graph=False)
+    graph = ctx.run_batch(dev, off, [float(s) for s in sizes], graph=True)
+    graph2 = ctx.run_batch(dev, off, [float(s) for s in sizes], graph=True)
+    single = psh.Context(psh.make_conf(**kw))
+    for m, c in enumerate(cats):
+        p = single.run_device(psh.DeviceArray.from_numpy(c), len(c), float(len(c)))
+        for r in (batch[m], graph[m], graph2[m]):
+            assert np.array_equal(r["cnt"], p["cnt"])
+            scale = np.abs(p["pl"][0][0]) + 1e9 / len(c)
+            assert np.all(np.abs(r["pl"][0] - p["pl"][0]) <= 1e-5 * scale)

@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Benchmark of the mesh -> FFT -> P_l(k) path on one MI355X.
+
+Workload (default): the clustered periodic box at production size --
+Gaussian field on a 512^3 grid with P(k) = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2)
+in L = 1000 Mpc/h, lognormal, Poisson-sampled at 0.02 (h/Mpc)^3 (~2e7
+particles), Zel'dovich RSD along z with f = 0.8; PCS, N_g = 1024,
+interlaced, FP32 meshes and FFTs, FP64 bin sums, l = 0,2,4, dk = 0.005.
+
+A step is one full pass of the path over the catalogue already resident in
+HBM: coordinate bounds, mass assignment of both interlaced meshes, 3-D r2c
+FFTs, interlace combine, binning and normalisation (psh_ctx_run_device).
+
+Printed: one JSON line with meshes/s (value), particles/s, the roofline of
+the dominant kernel (HIP events on the library stream), batched-vs-plain
+throughput of many small mocks, and the CPU oracle baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particles/s + P_l(k) meshes/s at N_g=1024, batched vs plain; % of HBM peak"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def c2_conf(P):
+    return dict(gsize=1024, assign=P.PCS, intlace=True, bsize=1000.0, kbin=0.005,
+                poles=[0, 2, 4])
+
+
+def stage_bytes(Ng, nmesh, npart):
+    """Algorithmic (compulsory) HBM bytes per launch of each stage."""
+    ntot = float(Ng) ** 3
+    nc = float(Ng) * Ng * (Ng // 2 + 1)
+    return {
+        "deposit": nmesh * 4 * ntot,                    # each mesh written once
+        "z_r2c": nmesh * (4 * ntot + 8 * nc),            # read real, write half-complex
+        "y_fft": nmesh * 16 * nc,                        # read + write complex
+        "x_fft_bin": nmesh * 8 * nc,                     # read complex; bins negligible
+        "assign_total": npart * 32 + nmesh * 4 * ntot,   # particles once + meshes once
+    }
+
+
+def run_c2(P, steps, warmup):
+    from powspec_amd import synth
+    t = time.time()
+    data = synth.clustered_box(ng_gen=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
+    n = len(data)
+    log(f"C2 catalogue: {n} particles generated in {time.time() - t:.1f} s")
+    dev = P.DeviceArray.from_numpy(data)
+    conf = c2_conf(P)
+    ctx = P.Context(P.make_conf(**conf))
+    L = P.load()
+    for _ in range(warmup):
+        res = ctx.run_device(dev, n, float(n))
+    L.psh_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = ctx.run_device(dev, n, float(n))
+    L.psh_device_synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) / steps * 1e3
+    log(f"C2: {ms_step:.3f} ms/step over {steps} steps")
+    # stage breakdown: HIP events on the library's stream, separate runs
+    ctx.set_profiling(True)
+    reps = []
+    for _ in range(3):
+        ctx.run_device(dev, n, float(n))
+        reps.append(ctx.timings())
+    ctx.set_profiling(False)
+    ms = np.median(np.array(reps), axis=0)
+    stages = {"assign_total": ms[0], "zy_fft": ms[1], "x_fft_bin": ms[2], "total": ms[3],
+              "deposit": ms[4], "z_r2c": ms[5], "y_fft": ms[6], "bucket": ms[7]}
+    log("C2 stages (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+    return data, res, ms_step, stages, n
+
+
+def roofline(stages, Ng, nmesh, npart):
+    b = stage_bytes(Ng, nmesh, npart)
+    cands = {k: stages[k] for k in ("deposit", "z_r2c", "y_fft", "x_fft_bin")}
+    dom = max(cands, key=lambda k: cands[k])
+    per_launch = b[dom]
+    # z/y passes launch once per mesh: per-launch bytes and time are halves
+    launches = nmesh if dom in ("z_r2c", "y_fft") else 1
+    t = stages[dom] / launches
+    ach = per_launch / launches / (t * 1e-3) / 1e9
+    table = {}
+    for k in cands:
+        nl = nmesh if k in ("z_r2c", "y_fft") else 1
+        if stages[k] > 0:
+            table[k] = {"ms": round(stages[k], 4),
+                        "GBps": round(b[k] / (stages[k] * 1e-3) / 1e9, 1),
+                        "launches": nl}
+    return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": per_launch / launches, "ms_per_launch": t,
+            "stages": table}
+
+
+def run_batched(P, data, nmock, seed=1000):
+    """Many small clustered mocks (TSC, N_g = 256, interlaced): one batched
+    launch sequence replayed as a hipGraph vs one plain run per mock."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(200_000, 3_000_001, nmock)
+    parts = []
+    for m in range(nmock):
+        r = np.random.default_rng(seed + m)
+        idx = r.integers(0, len(data), sizes[m])
+        shift = r.uniform(0, 1000.0, 3)
+        d = data[idx].copy()
+        d[:, :3] = np.minimum(np.mod(d[:, :3] + shift, 1000.0), 1000.0 * (1 - 1e-15))
+        parts.append(d)
+    allp = np.concatenate(parts)
+    del parts
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    log(f"batched mocks: {nmock} mocks, {len(allp)} particles")
+    dev = P.DeviceArray.from_numpy(allp)
+    kw = dict(gsize=256, assign=P.TSC, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
+    ctx = P.Context(P.make_conf(**kw), nmock_max=nmock)
+    L = P.load()
+    ws = [float(s) for s in sizes]
+    ctx.run_batch(dev, off, ws, graph=True)           # capture + warm-up
+    L.psh_device_synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.run_batch(dev, off, ws, graph=True)
+    L.psh_device_synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    # plain: one mock per launch sequence
+    single = P.Context(P.make_conf(**kw))
+    ptr0 = dev.ptr.value
+
+    class View:  # device pointer into the concatenated buffer
+        def __init__(self, p):
+            import ctypes
+            self.ptr = ctypes.c_void_p(p)
+
+    for m in range(min(nmock, 4)):
+        single.run_device(View(ptr0 + 32 * int(off[m])), int(sizes[m]), ws[m])
+    L.psh_device_synchronize()
+    t0 = time.perf_counter()
+    for m in range(nmock):
+        single.run_device(View(ptr0 + 32 * int(off[m])), int(sizes[m]), ws[m])
+    L.psh_device_synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"batched: {nmock / t_batch:.1f} meshes/s, plain: {nmock / t_plain:.1f} meshes/s")
+    return {"workload": f"{nmock} clustered mocks, N_p in [2e5,3e6] (total {len(allp)}), "
+                        "TSC N_g=256 interlaced, l=0,2,4, dk=0.01",
+            "batched_meshes_per_s": round(nmock / t_batch, 1),
+            "plain_meshes_per_s": round(nmock / t_plain, 1),
+            "batched_over_plain": round(t_plain / t_batch, 2),
+            "batched_particles_per_s": round(len(allp) / t_batch, 1)}
+
+
+def cpu_baseline(c2_data):
+    """FP64 oracle (OpenMP, own FFT) on a bounded sample of the C2 workload:
+    N_g = 512 (1/8 of the cells) and every 8th particle (same particles per
+    cell), PCS interlaced, l = 0,2,4; the per-stage times are scaled to
+    N_g = 1024 (assignment and binning x8, FFT x8*30/27 for N log N)."""
+    from oracle import oracle as O
+    O.lib()
+    sample = np.ascontiguousarray(c2_data[::8])
+    conf = O.make_conf(gsize=512, assign=3, intlace=True, bsize=1000.0, kbin=0.005,
+                       poles=[0, 2, 4])
+    t = time.perf_counter()
+    r = O.run(conf, [sample])
+    wall = time.perf_counter() - t
+    ta, tf, tb, tt = r["timings"]
+    est = ta * 8 + tf * 8 * 30 / 27 + tb * 8
+    threads = O.lib().pso_num_threads()
+    log(f"cpu oracle sample: {wall:.1f} s wall (assign {ta:.1f}, fft {tf:.1f}, bin {tb:.1f});"
+        f" C2 estimate {est:.1f} s")
+    return {"value": round(1.0 / est, 5), "unit": "meshes/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle/psoracle.c (FP64, OpenMP {threads} threads, own radix-2 FFT) "
+                      f"on every 8th C2 particle ({len(sample)}) at N_g=512, PCS interlaced; "
+                      f"measured {wall:.2f} s, stage times scaled x8 (assign, bin) and "
+                      f"x8.9 (FFT) to N_g=1024 -> {est:.1f} s per C2 mesh",
+            "particles_per_s": round(len(c2_data) / est, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nmock", type=int, default=256)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build "
+              "measures the single-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import powspec_amd as P
+    P.load()
+    if P.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    arch = P.device_arch()
+    data, res, ms_step, stages, n = run_c2(P, args.steps, args.warmup)
+    rl = roofline(stages, 1024, 2, n)
+    extra = {}
+    if not args.no_batched and args.nmock > 0:
+        extra = run_batched(P, data, args.nmock)
+    cpu = None if args.no_cpu else cpu_baseline(data)
+    line = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_step, 3),
+        "unit": "meshes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "clustered box, lognormal Gaussian field 512^3, Poisson "
+                               "nbar=0.02 (h/Mpc)^3, Zel'dovich RSD f=0.8; PCS, N_g=1024, "
+                               "interlaced, l=0,2,4, dk=0.005, L=1000 Mpc/h",
+                   "N_g": 1024, "N_p": n, "assign": "PCS", "interlaced": True,
+                   "precision": "FP32 meshes/FFT, FP64 bin sums", "arch": arch},
+        "particles_per_s": round(n / (ms_step * 1e-3), 1),
+        "stage_ms": {k: round(float(v), 4) for k, v in stages.items()},
+        "roofline": rl,
+        "batched": extra or None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -174,9 +174,11 @@ int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     psh_pk *out);
 
 /* Per-stage device time of the last run, milliseconds, measured with HIP
- * events on the context's stream: [0] assign, [1] fft (z,y passes),
- * [2] last FFT pass fused with binning, [3] total. */
-int psh_ctx_timings(const psh_ctx *ctx, double ms[4]);
+ * events on the context's stream (profiling on): [0] assignment incl. the
+ * coordinate-bounds pass, [1] z and y FFT passes, [2] x FFT pass fused with
+ * interlacing and binning, [3] total, [4] deposit kernel, [5] z passes,
+ * [6] y passes, [7] bucketing kernels (count, scan, fill). */
+int psh_ctx_timings(const psh_ctx *ctx, double ms[8]);
 
 /* Enable/disable per-stage event timing (adds event records between
  * stages; off by default). */

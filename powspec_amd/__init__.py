@@ -241,7 +241,7 @@ class Context:
         self._check(self.L.psh_ctx_set_profiling(self.h, int(on)))
 
     def timings(self):
-        t = (C.c_double * 4)()
+        t = (C.c_double * 8)()
         self._check(self.L.psh_ctx_timings(self.h, t))
         return list(t)
 

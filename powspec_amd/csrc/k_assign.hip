@@ -134,6 +134,29 @@ __device__ __forceinline__ int find_part(const DevPart *parts, int nparts,
   return lo;
 }
 
+// Wave-aggregated `old = atomicAdd(&ctr[t], 1)` over the lanes with `active`:
+// lanes of a wave that target the same counter are served by one atomic of
+// the group size (catalogues are usually ordered in space, so a wave mostly
+// hits one tile); returns each lane's old value plus its rank in the group.
+__device__ __forceinline__ uint32_t agg_increment(uint32_t *ctr, uint32_t t,
+                                                  bool active) {
+  const int lane = __lane_id();
+  uint64_t pend = __ballot(active);
+  uint32_t slot = 0;
+  while (pend) {
+    const int leader = __ffsll((unsigned long long)pend) - 1;
+    const uint32_t tl = __shfl(t, leader);
+    const uint64_t same = __ballot(((pend >> lane) & 1ull) && t == tl);
+    uint32_t old = 0;
+    if (lane == leader) old = atomicAdd(&ctr[tl], (uint32_t)__popcll(same));
+    old = __shfl(old, leader);
+    if ((same >> lane) & 1ull)
+      slot = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    pend &= ~same;
+  }
+  return slot;
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(256)
 assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
@@ -141,51 +164,75 @@ assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
               uint32_t *__restrict__ cursors, Rec *__restrict__ recs,
               size_t rec_cap, int *__restrict__ flags) {
   const int W = window_width(g.scheme);
-  const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
-  for (size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x; gi < ntotal;
-       gi += (size_t)gridDim.x * blockDim.x) {
-    int pi = find_part(parts, nparts, gi);
-    const DevPart &P = parts[pi];
-    const double *r = P.xyzw + 4 * (gi - P.start);
-    double rr[4];
-    {
-      const double2 *r2 = reinterpret_cast<const double2 *>(r);
-      double2 a = r2[0], b = r2[1];
-      rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
-    }
-    PartCells pc = part_cells(rr, g);
-    if (!pc.ok) {
-      atomicOr(flags, 2);
-      continue;
-    }
-    int tx[2], ty[2], tz[2];
-    int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
-    int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
-    int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
+  const uint32_t tiles_per_mock = (uint32_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of iterations so that the
+  // wave-level aggregation below sees converged waves
+  const size_t iters = (ntotal + stride - 1) / stride;
+  size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  float wmax = 0.f;
+  for (size_t it = 0; it < iters; it++, gi += stride) {
+    const bool live = gi < ntotal;
+    uint32_t tiles[8];
+    int nt = 0;
     Rec rec;
-    if (FILL) {
-      rec.f[0] = pc.f[0];
-      rec.f[1] = pc.f[1];
-      rec.f[2] = pc.f[2];
-      rec.w = (float)rr[3] * P.wscale;
-      rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
-      rec.bz = (uint32_t)pc.b[2] |
-               ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2))
-                << 16);
-    }
-    const size_t mbase = (size_t)P.mock * tiles_per_mock;
-    for (int a = 0; a < nx; a++)
-      for (int b = 0; b < ny; b++)
-        for (int c = 0; c < nz; c++) {
-          size_t t = mbase + ((size_t)tx[a] * g.ntile[1] + ty[b]) * g.ntile[2] + tz[c];
-          if (FILL) {
-            uint32_t slot = atomicAdd(&cursors[t], 1u);
-            if (slot < rec_cap) recs[slot] = rec;
-            else atomicOr(flags, 1);
-          } else {
-            atomicAdd(&counts[t], 1u);
-          }
+    if (live) {
+      int pi = find_part(parts, nparts, gi);
+      const DevPart &P = parts[pi];
+      const double *r = P.xyzw + 4 * (gi - P.start);
+      double rr[4];
+      {
+        const double2 *r2 = reinterpret_cast<const double2 *>(r);
+        double2 a = r2[0], b = r2[1];
+        rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
+      }
+      PartCells pc = part_cells(rr, g);
+      if (!pc.ok) {
+        atomicOr(flags, 2);
+      } else {
+        int tx[2], ty[2], tz[2];
+        int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
+        int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
+        int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
+        const uint32_t mbase = (uint32_t)P.mock * tiles_per_mock;
+        for (int a = 0; a < nx; a++)
+          for (int b = 0; b < ny; b++)
+            for (int c = 0; c < nz; c++)
+              tiles[nt++] = mbase + ((uint32_t)tx[a] * g.ntile[1] + ty[b]) * g.ntile[2] + tz[c];
+        if (FILL) {
+          rec.f[0] = pc.f[0];
+          rec.f[1] = pc.f[1];
+          rec.f[2] = pc.f[2];
+          rec.w = (float)rr[3] * P.wscale;
+          wmax = fmaxf(wmax, fabsf(rec.w));
+          rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
+          rec.bz = (uint32_t)pc.b[2] |
+                   ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2))
+                    << 16);
         }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool act = k < nt;
+      if (!__ballot(act)) break;
+      const uint32_t t = act ? tiles[k] : 0u;
+      if (FILL) {
+        const uint32_t slot = agg_increment(cursors, t, act);
+        if (act) {
+          if (slot < rec_cap) recs[slot] = rec;
+          else atomicOr(flags, 1);
+        }
+      } else {
+        agg_increment(counts, t, act);
+      }
+    }
+  }
+  if (FILL) {
+    // largest |weight| of the launch, for the fixed-point scale of the
+    // deposit (flags[1] holds the bits of a non-negative float)
+    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
+    if (__lane_id() == 0) atomicMax(reinterpret_cast<unsigned *>(flags) + 1, __float_as_uint(wmax));
   }
 }
 
@@ -312,8 +359,12 @@ __device__ __forceinline__ void window_weights(int scheme, float f, float *w) {
   }
 }
 
+// Accumulate one particle's window into an LDS tile of 32-bit fixed-point
+// cells (value * 2^s as a signed integer, ds_add_u32; FP32 LDS atomics are
+// an order of magnitude slower on gfx950).  Integer addition is associative,
+// so the tile does not depend on the order of the records.
 template <int SCHEME>
-__device__ __forceinline__ void deposit_one(float *lds, const int *T,
+__device__ __forceinline__ void deposit_one(int *lds, const int *T,
                                             const int *T0, int Ng, const int *b,
                                             const float *f, float w) {
   constexpr int W = SCHEME + 1;
@@ -347,10 +398,22 @@ __device__ __forceinline__ void deposit_one(float *lds, const int *T,
 #pragma unroll
       for (int k = 0; k < W; k++) {
         if (!(m[2] >> k & 1)) continue;
-        atomicAdd(&lds[rxy + l[2][k]], wxy * wt[2][k]);
+        atomicAdd(&lds[rxy + l[2][k]], __float2int_rn(wxy * wt[2][k]));
       }
     }
   }
+}
+
+// Fixed-point exponent of a tile: a cell value is a sum of at most cnt
+// contributions of magnitude <= max|w| (window weights are <= 1 and sum to 1
+// per axis), so |value| * 2^s < 2^30 < 2^31.  The quantum is therefore
+// cnt * max|w| * 2^-30 of the largest possible cell value.
+__device__ __forceinline__ int tile_exponent(const int *flags, uint32_t cnt) {
+  const float maxw = __uint_as_float((unsigned)flags[1]);
+  const double bound = (double)maxw * (double)cnt;
+  if (!(bound > 0.0)) return 30;
+  int s = 30 - (int)ceil(log2(bound * (1.0 + 1e-7)));
+  return s < -60 ? -60 : (s > 60 ? 60 : s);
 }
 
 template <int SCHEME, bool INTLACE>
@@ -358,12 +421,12 @@ __global__ void __launch_bounds__(256)
 assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                const Rec *__restrict__ recs, size_t rec_cap,
                float *__restrict__ meshA, float *__restrict__ meshB,
-               size_t mstride) {
-  extern __shared__ float lds[];
+               size_t mstride, const int *__restrict__ flags, size_t ntotal) {
+  extern __shared__ int ldsq[];
   const int T[3] = {g.tile[0], g.tile[1], g.tile[2]};
   const int tcells = T[0] * T[1] * T[2];
-  float *ldsA = lds;
-  float *ldsB = lds + tcells;
+  int *ldsA = ldsq;
+  int *ldsB = ldsq + tcells;
   const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
   const size_t tile = blockIdx.x;
   const int mock = (int)(tile / tiles_per_mock);
@@ -373,10 +436,10 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   const int tyi = (int)(tl % g.ntile[1]);
   const int txi = (int)(tl / g.ntile[1]);
   const int T0[3] = {txi * T[0], tyi * T[1], tzi * T[2]};
-
+  (void)ntotal;
   for (int i = threadIdx.x; i < tcells; i += blockDim.x) {
-    ldsA[i] = 0.f;
-    if (INTLACE) ldsB[i] = 0.f;
+    ldsA[i] = 0;
+    if (INTLACE) ldsB[i] = 0;
   }
   __syncthreads();
 
@@ -384,10 +447,42 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   // is repeated with a larger buffer): never read beyond rec_cap
   const uint32_t cap = rec_cap < 0xffffffffu ? (uint32_t)rec_cap : 0xffffffffu;
   const uint32_t s0 = min(starts[tile], cap), s1 = min(starts[tile + 1], cap);
-  for (uint32_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
-    const Rec r = recs[s];
+  const uint32_t cnt = s1 > s0 ? s1 - s0 : 0u;
+  const int sexp = tile_exponent(flags, cnt);
+  const float scale = ldexpf(1.f, sexp);
+  const float inv = ldexpf(1.f, -sexp);
+  // Visit the bucket in the order i -> (i * S) mod cnt with S ~ 0.618 cnt and
+  // gcd(S, cnt) = 1 (a permutation): neighbouring lanes take records far
+  // apart in fill order, i.e. far apart in the (usually space-ordered)
+  // catalogue, so their windows rarely share LDS banks.
+  uint32_t S = 1u;
+  if (cnt > 2) {
+    S = ((uint32_t)((float)cnt * 0.618034f)) | 1u;
+    if (S >= cnt) S = 1u;
+    for (;;) {
+      uint32_t a = S, b = cnt;
+      while (b) {
+        uint32_t t = a % b;
+        a = b;
+        b = t;
+      }
+      if (a == 1u) break;
+      S += 2u;
+      if (S >= cnt) {
+        S = 1u;
+        break;
+      }
+    }
+  }
+  uint32_t x = cnt ? (uint32_t)(((uint64_t)threadIdx.x * S) % cnt) : 0u;
+  const uint32_t D = cnt ? (uint32_t)(((uint64_t)blockDim.x * S) % cnt) : 0u;
+  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const Rec r = recs[s0 + x];
+    x += D;
+    if (x >= cnt) x -= cnt;
     int b[3] = {(int)(r.bxy & 0xffffu), (int)(r.bxy >> 16), (int)(r.bz & 0xffffu)};
-    deposit_one<SCHEME>(ldsA, T, T0, g.Ng, b, r.f, r.w);
+    const float ws = r.w * scale;
+    deposit_one<SCHEME>(ldsA, T, T0, g.Ng, b, r.f, ws);
     if (INTLACE) {
       const uint32_t dl = r.bz >> 16;
       int bs[3];
@@ -399,37 +494,35 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
         if (bs[a] >= g.Ng) bs[a] -= g.Ng;
         fs[a] = r.f[a] + 0.5f - (float)d;
       }
-      deposit_one<SCHEME>(ldsB, T, T0, g.Ng, bs, fs, r.w);
+      deposit_one<SCHEME>(ldsB, T, T0, g.Ng, bs, fs, ws);
     }
   }
   __syncthreads();
 
-  // write the tile: rows of T[2] floats along z
+  // write the tile: rows of T[2] floats along z, 4 cells per thread
   float *outA = meshA + (size_t)mock * mstride;
   float *outB = INTLACE ? meshB + (size_t)mock * mstride : nullptr;
   const int nrow = T[0] * T[1];
-  if ((T[2] & 3) == 0) {
-    const int q4 = T[2] >> 2;
-    for (int i = threadIdx.x; i < nrow * q4; i += blockDim.x) {
-      int row = i / q4, q = i - row * q4;
-      int lx = row / T[1], ly = row - lx * T[1];
-      size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
-                    T0[2] + 4 * q;
-      const float4 va = *reinterpret_cast<const float4 *>(ldsA + row * T[2] + 4 * q);
-      *reinterpret_cast<float4 *>(outA + gidx) = va;
-      if (INTLACE) {
-        const float4 vb = *reinterpret_cast<const float4 *>(ldsB + row * T[2] + 4 * q);
-        *reinterpret_cast<float4 *>(outB + gidx) = vb;
-      }
-    }
-  } else {
-    for (int i = threadIdx.x; i < tcells; i += blockDim.x) {
-      int row = i / T[2], lz = i - row * T[2];
-      int lx = row / T[1], ly = row - lx * T[1];
-      size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
-                    T0[2] + lz;
-      outA[gidx] = ldsA[i];
-      if (INTLACE) outB[gidx] = ldsB[i];
+  const int q4 = T[2] >> 2;   // T[2] is a multiple of 4 (>= 4)
+  for (int i = threadIdx.x; i < nrow * q4; i += blockDim.x) {
+    const int row = i / q4, q = i - row * q4;
+    const int lx = row / T[1], ly = row - lx * T[1];
+    const size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
+                        T0[2] + 4 * q;
+    const int li = row * T[2] + 4 * q;
+    float4 va;
+    va.x = (float)ldsA[li] * inv;
+    va.y = (float)ldsA[li + 1] * inv;
+    va.z = (float)ldsA[li + 2] * inv;
+    va.w = (float)ldsA[li + 3] * inv;
+    *reinterpret_cast<float4 *>(outA + gidx) = va;
+    if (INTLACE) {
+      float4 vb;
+      vb.x = (float)ldsB[li] * inv;
+      vb.y = (float)ldsB[li + 1] * inv;
+      vb.z = (float)ldsB[li + 2] * inv;
+      vb.w = (float)ldsB[li + 3] * inv;
+      *reinterpret_cast<float4 *>(outB + gidx) = vb;
     }
   }
 }
@@ -438,15 +531,19 @@ template <int SCHEME>
 hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
                           const uint32_t *starts, const Rec *recs,
                           size_t rec_cap, float *meshA, float *meshB,
-                          size_t mstride, hipStream_t s) {
-  const size_t lds = sizeof(float) * (size_t)g.tile[0] * g.tile[1] * g.tile[2] *
+                          size_t mstride, const int *flags, size_t ntotal,
+                          hipStream_t s) {
+  const size_t lds = sizeof(int) * (size_t)g.tile[0] * g.tile[1] * g.tile[2] *
                      (g.intlace ? 2 : 1);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
   if (g.intlace)
     hipLaunchKernelGGL((assign_deposit<SCHEME, true>), dim3((unsigned)ntiles),
-                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride);
+                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride,
+                       flags, ntotal);
   else
     hipLaunchKernelGGL((assign_deposit<SCHEME, false>), dim3((unsigned)ntiles),
-                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride);
+                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride,
+                       flags, ntotal);
   return hipGetLastError();
 }
 
@@ -472,7 +569,7 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
                           uint32_t *counts, uint32_t *starts, uint32_t *cursors,
                           void *records, size_t rec_cap, float *meshA,
                           float *meshB, size_t mstride, uint32_t *d_scan_tmp,
-                          int *d_flags, hipStream_t s) {
+                          int *d_flags, hipStream_t s, hipEvent_t ev_deposit) {
   if (nparts < 1) return hipErrorInvalidValue;
   const size_t ntiles = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * nmock;
   const int nsb = (int)((ntiles + kScanBlock - 1) / kScanBlock);
@@ -500,10 +597,11 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
                        d_flags);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  if (ev_deposit) (void)hipEventRecord(ev_deposit, s);
   switch (g.scheme) {
-    case 0: return launch_deposit<0>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
-    case 1: return launch_deposit<1>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
-    case 2: return launch_deposit<2>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
-    default: return launch_deposit<3>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, s);
+    case 0: return launch_deposit<0>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, d_flags, ntotal, s);
+    case 1: return launch_deposit<1>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, d_flags, ntotal, s);
+    case 2: return launch_deposit<2>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, d_flags, ntotal, s);
+    default: return launch_deposit<3>(g, ntiles, starts, recs, rec_cap, meshA, meshB, mstride, d_flags, ntotal, s);
   }
 }

@@ -80,28 +80,40 @@ geometry_kernel(int N, PshAxisTables ax, PshBins b,
     part_lcnt[o * b.nl * nb + i] = lc[i];
 }
 
+// out[m][a] = sum over g of part[m][g][a].  A block takes 16 accumulators;
+// its 16 slices of threads each sum a contiguous range of g in order, and the
+// slice sums are added in slice order: the result does not depend on timing.
+constexpr int kRedAcc = 16, kRedSlices = 16;
+
 template <class T>
-__global__ void reduce_kernel(const T *__restrict__ part, int nmock, int grid,
-                              int nacc, T *__restrict__ out) {
-  const size_t total = (size_t)nmock * nacc;
-  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const int m = (int)(idx / nacc), a = (int)(idx - (size_t)m * nacc);
+__global__ void __launch_bounds__(kRedAcc * kRedSlices)
+reduce_kernel(const T *__restrict__ part, int grid, int nacc, T *__restrict__ out) {
+  __shared__ T sh[kRedSlices][kRedAcc];
+  const int l = threadIdx.x % kRedAcc, slice = threadIdx.x / kRedAcc;
+  const int a = blockIdx.x * kRedAcc + l;
+  const int m = blockIdx.y;
+  T s = 0;
+  if (a < nacc) {
     const T *p = part + (size_t)m * grid * nacc + a;
-    T s = 0;
-    for (int g = 0; g < grid; g++) s += p[(size_t)g * nacc];
-    out[idx] = s;
+    const int g0 = (int)((long)slice * grid / kRedSlices);
+    const int g1 = (int)((long)(slice + 1) * grid / kRedSlices);
+#pragma unroll 8
+    for (int g = g0; g < g1; g++) s += p[(size_t)g * nacc];
+  }
+  sh[slice][l] = s;
+  __syncthreads();
+  if (slice == 0 && a < nacc) {
+    T t = sh[0][l];
+    for (int q = 1; q < kRedSlices; q++) t += sh[q][l];
+    out[(size_t)m * nacc + a] = t;
   }
 }
 
 template <class T>
 hipError_t launch_reduce(const T *part, int nmock, int grid, int nacc, T *out,
                          hipStream_t s) {
-  const size_t total = (size_t)nmock * nacc;
-  unsigned blocks = (unsigned)((total + 255) / 256);
-  if (blocks > 4096) blocks = 4096;
-  if (blocks == 0) blocks = 1;
-  hipLaunchKernelGGL(reduce_kernel<T>, dim3(blocks), dim3(256), 0, s, part, nmock,
+  const dim3 blocks((unsigned)((nacc + kRedAcc - 1) / kRedAcc), (unsigned)nmock);
+  hipLaunchKernelGGL(reduce_kernel<T>, blocks, dim3(kRedAcc * kRedSlices), 0, s, part,
                      grid, nacc, out);
   return hipGetLastError();
 }

@@ -288,15 +288,19 @@ struct XBinDev {
   float ynorm;
 };
 
-__device__ __forceinline__ double legendre(int ell, double x) {
-  if (ell == 0) return 1.0;
-  double p0 = 1.0, p1 = x;
-  for (int n = 1; n < ell; n++) {
-    double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
-    p0 = p1;
-    p1 = p2;
+// Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference)
+// written as polynomials in x^2, FP32.
+__device__ __forceinline__ float legendre_f(int ell, float x) {
+  const float x2 = x * x;
+  switch (ell) {
+    case 0: return 1.f;
+    case 1: return x;
+    case 2: return 1.5f * x2 - 0.5f;
+    case 3: return x * (2.5f * x2 - 1.5f);
+    case 4: return (4.375f * x2 - 3.75f) * x2 + 0.375f;
+    case 5: return x * ((7.875f * x2 - 8.75f) * x2 + 1.875f);
+    default: return ((14.4375f * x2 - 19.6875f) * x2 + 6.5625f) * x2 - 0.3125f;
   }
-  return p1;
 }
 
 // Bin of a mode from |k|^2, in the reference's operation order
@@ -311,58 +315,81 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
+// Bin of mode (i, jy, kz).  t = (|k| - kedge0) / dk is first evaluated in
+// FP32; its error is below 1e-6 (|t| + kedge0/dk + 8), so when t lies farther
+// than that from an integer, floor(t) is the bin the FP64 expression gives.
+// Otherwise (and for log bins) the FP64 expression decides.  kmodf: FP32 |k|.
+__device__ __forceinline__ int fast_bin(const PshAxisTables &ax, const PshBins &b,
+                                        int i, int jy, int kz, float &kmodf) {
+  const float k2f = (ax.ksqf[0][i] + ax.ksqf[1][jy]) + ax.ksqf[2][kz];
+  kmodf = sqrtf(k2f);
+  if (!b.logscale) {
+    const float tf = (kmodf - b.kedge0f) * b.invdkf;
+    const float fl = floorf(tf);
+    const float fr = tf - fl;
+    const float eps = 1e-6f * (fabsf(tf) + b.k0dk + 8.f);
+    if (fr > eps && fr < 1.f - eps) {
+      const int k = (int)fl;
+      return (fl >= 0.f && k < b.nbin) ? k : -1;
+    }
+  }
+  double kmod2 = ax.ksq[0][i] + ax.ksq[1][jy];
+  if (kz > 0) kmod2 = kmod2 + ax.ksq[2][kz];
+  double kmod;
+  return mode_bin(kmod2, b, kmod);
+}
+
 template <int EPI>
 __device__ __forceinline__ void bin_mode(const XBinDev &a, double *bins, int i,
                                          int jy, int kz, float2 F, int M,
                                          size_t pos) {
-#pragma clang fp contract(off)
   const PshAxisTables &ax = a.ax;
-  double kmod2 = ax.ksq[0][i] + ax.ksq[1][jy];
-  if (kz > 0) kmod2 = kmod2 + ax.ksq[2][kz];
-  if (EPI == PSH_EPI_SIM && kz == 0 && kmod2 == 0.0) return;  // DC skipped
-  double kmod;
-  int kb = mode_bin(kmod2, a.bins, kmod);
+  if (EPI == PSH_EPI_SIM && i == 0 && jy == 0 && kz == 0) return;  // DC skipped
+  float kmodf;
+  const int kb = fast_bin(ax, a.bins, i, jy, kz, kmodf);
   if (kb < 0) return;
-  double alias = ax.al[0][i] * ax.al[1][jy];
-  if (kz > 0) alias = alias * ax.al[2][kz];
+  const float alias = (ax.alf[0][i] * ax.alf[1][jy]) * ax.alf[2][kz];
   const bool single = (kz == 0) || (kz == M);
+  const float mult = single ? 1.f : 2.f;
   if (EPI == PSH_EPI_SIM) {
-    double p = ((double)F.x * F.x + (double)F.y * F.y) * alias;
-    if (!single) p *= 2;
-    const double mu = (ax.kv[0][i] * a.bins.los[0] + ax.kv[1][jy] * a.bins.los[1] +
-                       ax.kv[2][kz] * a.bins.los[2]) / kmod;
+    const float p = (F.x * F.x + F.y * F.y) * alias * mult;
+    const float mu = (ax.kvf[0][i] * a.bins.losf[0] + ax.kvf[1][jy] * a.bins.losf[1] +
+                      ax.kvf[2][kz] * a.bins.losf[2]) / kmodf;
     for (int l = 0; l < a.bins.nl; l++) {
       const int ell = a.bins.poles[l];
-      if (!single && (ell & 1)) continue;
-      atomicAdd(&bins[l * a.bins.nbin + kb], p * legendre(ell, mu));
+      if (!single && (ell & 1)) continue;   // the two half-spaces cancel
+      atomicAdd(&bins[l * a.bins.nbin + kb], (double)(p * legendre_f(ell, mu)));
     }
   } else if (EPI == PSH_EPI_SURVEY0) {
-    double p = ((double)F.x * F.x + (double)F.y * F.y) * alias;
-    if (!single) p *= 2;
-    atomicAdd(&bins[kb], p);
+    atomicAdd(&bins[kb], (double)((F.x * F.x + F.y * F.y) * alias * mult));
   } else {  // PSH_EPI_SURVEYL: Re(F0 conj(Y S)), Y real
     const float2 f0 = a.F0[pos];
     float y = 1.f;
-    const double kx = ax.kd[0][i], ky = ax.kd[1][jy], kzz = ax.kd[2][kz];
-    const double k2 = kx * kx + ky * ky;
-    if (k2 != 0.0) {   // kx = ky = 0 is added unweighted (mp_template.c:121)
-      float k2f = (float)k2, kzf = (float)kzz;
-      float ik3 = rsqrtf(k2f + kzf * kzf);
-      float ik22 = rsqrtf(k2f);
-      float k22 = k2f * ik22;
-      y = ylm_eval(a.ell, a.m, a.ynorm, kzf * ik3, k22 * ik3, (float)kx * ik22,
-                   (float)ky * ik22);
+    const float kx = (float)ax.kd[0][i], ky = (float)ax.kd[1][jy];
+    const float kzz = (float)ax.kd[2][kz];
+    const float k2 = kx * kx + ky * ky;
+    if (k2 != 0.f) {   // kx = ky = 0 is added unweighted (mp_template.c:121)
+      const float ik3 = rsqrtf(k2 + kzz * kzz);
+      const float ik22 = rsqrtf(k2);
+      const float k22 = k2 * ik22;
+      y = ylm_eval(a.ell, a.m, a.ynorm, kzz * ik3, k22 * ik3, kx * ik22, ky * ik22);
     }
-    double p = ((double)f0.x * F.x + (double)f0.y * F.y) * (double)y * alias;
-    if (!single) p *= 2;
-    atomicAdd(&bins[kb], p);
+    atomicAdd(&bins[kb], (double)((f0.x * F.x + f0.y * F.y) * y * alias * mult));
   }
 }
 
+// x pass: T chosen so that each thread holds 16 values per field.
+template <int N>
+struct XCfg {
+  static constexpr int C = ColCfg<N>::C;
+  static constexpr int T0 = C * N / 16;
+  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+};
+
 template <int N, int EPI, bool INTLACE>
-__global__ void __launch_bounds__(ColCfg<N>::T)
+__global__ void __launch_bounds__(XCfg<N>::T)
 x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
-  constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
+  constexpr int C = XCfg<N>::C, T = XCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
   extern __shared__ double dyn[];
   double *bins = dyn;
@@ -397,14 +424,15 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       if (!G::map(qq, c, j)) continue;
       if (c >= ncv) continue;
       const int kz = q * C + c;
+      float2 pyz = make_float2(1.f, 0.f);
+      if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
 #pragma unroll
       for (int r = 0; r < G::R; r++) {
         const int i = j + r * G::NS;
         float2 F = va[qq * G::R + r];
         if (INTLACE) {
           // F0 <- (F0 + e^{i pi (i'+j'+k)/N} F1) / 2   (src/multipole.c:476-484)
-          float2 ph = cmul(cmul(a.phase[i], a.phase[y]), a.phase[kz]);
-          float2 t = cmul(ph, vb[qq * G::R + r]);
+          const float2 t = cmul(cmul(a.phase[i], pyz), vb[qq * G::R + r]);
           F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
         }
         const size_t pos = base + (size_t)i * ES + c;
@@ -495,7 +523,7 @@ struct XBin {
     const int ntz = (N / 2 + 1 + C - 1) / C;
     const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) +
                          sizeof(float) * C * Lds<N>::LN;
-    const dim3 g(grid, nmock), b(ColCfg<N>::T);
+    const dim3 g(grid, nmock), b(XCfg<N>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
 #define PSH_XB(E, I)                                                          \
   {                                                                              \

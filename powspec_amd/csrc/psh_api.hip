@@ -25,8 +25,21 @@ namespace {
 
 std::string g_last_error;
 
+// Stage events of one run (recorded only when profiling is on).
+enum {
+  E_START = 0,    // run entry
+  E_ASSIGN,       // bucket kernels start
+  E_DEPOSIT,      // deposit kernel start
+  E_ASSIGNED,     // meshes complete
+  E_ZDONE,        // z passes done
+  E_YDONE,        // y passes done
+  E_XDONE,        // fused x pass + binning done
+  E_END,          // results reduced
+  E_COUNT
+};
+
 struct Timer {
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[E_COUNT] = {};
 };
 
 }  // namespace
@@ -90,7 +103,7 @@ struct psh_ctx {
   // timing
   bool profiling = false;
   Timer tm;
-  double ms[4] = {0, 0, 0, 0};
+  double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   // graph of a batched run
   hipGraphExec_t gexec = nullptr;
@@ -262,7 +275,13 @@ int make_bins(psh_ctx *ctx) {
   b.dk = c.kbin;
   b.nl = c.npole;
   for (int l = 0; l < PSH_MAX_NL; l++) b.poles[l] = l < c.npole ? c.poles[l] : 0;
-  for (int a = 0; a < 3; a++) b.los[a] = c.issim ? c.los[a] : 0.0;
+  for (int a = 0; a < 3; a++) {
+    b.los[a] = c.issim ? c.los[a] : 0.0;
+    b.losf[a] = (float)b.los[a];
+  }
+  b.kedge0f = (float)b.kedge0;
+  b.invdkf = (float)(1.0 / c.kbin);
+  b.k0dk = (float)fabs(b.kedge0 / c.kbin);
   return PSH_OK;
 }
 
@@ -288,15 +307,27 @@ int make_tables(psh_ctx *ctx) {
       kd[i] = dfac * s;
     }
   }
-  int rc = dev_alloc(ctx, &ctx->d_tab, h.size());
+  // FP32 roundings of kv, ksq, al appended after the FP64 tables
+  std::vector<float> hf((size_t)9 * N);
+  for (int a = 0; a < 3; a++)
+    for (int t = 0; t < 3; t++)
+      for (int i = 0; i < N; i++)
+        hf[(size_t)(3 * a + t) * N + i] = (float)h[(size_t)(4 * a + t) * N + i];
+  const size_t nd = h.size() + (hf.size() + 1) / 2;
+  int rc = dev_alloc(ctx, &ctx->d_tab, nd);
   if (rc) return rc;
   HIPCHK(ctx, hipMemcpy(ctx->d_tab, h.data(), sizeof(double) * h.size(),
                         hipMemcpyHostToDevice));
+  float *dtf = reinterpret_cast<float *>(ctx->d_tab + h.size());
+  HIPCHK(ctx, hipMemcpy(dtf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
   for (int a = 0; a < 3; a++) {
     ctx->ax.kv[a] = ctx->d_tab + (size_t)(4 * a + 0) * N;
     ctx->ax.ksq[a] = ctx->d_tab + (size_t)(4 * a + 1) * N;
     ctx->ax.al[a] = ctx->d_tab + (size_t)(4 * a + 2) * N;
     ctx->ax.kd[a] = ctx->d_tab + (size_t)(4 * a + 3) * N;
+    ctx->ax.kvf[a] = dtf + (size_t)(3 * a + 0) * N;
+    ctx->ax.ksqf[a] = dtf + (size_t)(3 * a + 1) * N;
+    ctx->ax.alf[a] = dtf + (size_t)(3 * a + 2) * N;
   }
   if (ctx->survey) {
     // r-space positions (i + smin_g) * L with smin_g = bmin * Ng / L
@@ -361,11 +392,21 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   g.Ng = ctx->Ng;
   g.scheme = ctx->conf.assign;
   g.intlace = ctx->conf.intlace;
-  const int tz = ctx->Ng < 32 ? ctx->Ng : 32;
-  const int txy = ctx->Ng < 16 ? ctx->Ng : 16;
-  g.tile[0] = txy;
-  g.tile[1] = txy;
-  g.tile[2] = tz;
+  // 32-bit fixed-point LDS tiles (<= 64 KB): 8x16x32 cells per grid by
+  // default (measured best of 7 shapes for PCS interlaced at Ng = 1024);
+  // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4)
+  int tl[3] = {8, 16, 32};
+  if (const char *env = getenv("PSH_TILE")) {
+    int a = 0, b = 0, cz = 0;
+    if (sscanf(env, "%d,%d,%d", &a, &b, &cz) == 3 && is_pow2(a) && is_pow2(b) &&
+        is_pow2(cz) && cz >= 4 &&
+        (size_t)a * b * cz * 4 * (ctx->conf.intlace ? 2 : 1) <= 64 * 1024) {
+      tl[0] = a;
+      tl[1] = b;
+      tl[2] = cz;
+    }
+  }
+  for (int a = 0; a < 3; a++) g.tile[a] = ctx->Ng < tl[a] ? ctx->Ng : tl[a];
   for (int a = 0; a < 3; a++) g.ntile[a] = ctx->Ng / g.tile[a];
   g.row_pitch = 2 * ctx->H;
   for (int a = 0; a < 3; a++) {
@@ -491,13 +532,23 @@ void ev_record(psh_ctx *ctx, int i) {
   if (ctx->profiling && ctx->tm.ev[i]) (void)hipEventRecord(ctx->tm.ev[i], ctx->stream);
 }
 
+double ev_ms(psh_ctx *ctx, int a, int b) {
+  float t;
+  return hipEventElapsedTime(&t, ctx->tm.ev[a], ctx->tm.ev[b]) == hipSuccess ? t : -1.0;
+}
+
+// ms: [0] assignment (incl. bounds), [1] z+y passes, [2] fused x pass,
+// [3] total, [4] deposit kernel, [5] z passes, [6] y passes, [7] bucketing
 void ev_collect(psh_ctx *ctx) {
   if (!ctx->profiling) return;
-  float t;
-  ctx->ms[0] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[1]) == hipSuccess ? t : -1;
-  ctx->ms[1] = hipEventElapsedTime(&t, ctx->tm.ev[1], ctx->tm.ev[2]) == hipSuccess ? t : -1;
-  ctx->ms[2] = hipEventElapsedTime(&t, ctx->tm.ev[2], ctx->tm.ev[3]) == hipSuccess ? t : -1;
-  ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[3]) == hipSuccess ? t : -1;
+  ctx->ms[0] = ev_ms(ctx, E_START, E_ASSIGNED);
+  ctx->ms[1] = ev_ms(ctx, E_ASSIGNED, E_YDONE);
+  ctx->ms[2] = ev_ms(ctx, E_YDONE, E_XDONE);
+  ctx->ms[3] = ev_ms(ctx, E_START, E_END);
+  ctx->ms[4] = ev_ms(ctx, E_DEPOSIT, E_ASSIGNED);
+  ctx->ms[5] = ev_ms(ctx, E_ASSIGNED, E_ZDONE);
+  ctx->ms[6] = ev_ms(ctx, E_ZDONE, E_YDONE);
+  ctx->ms[7] = ev_ms(ctx, E_ASSIGN, E_DEPOSIT);
 }
 
 // Assign parts (device table already built on host) into meshes.
@@ -511,10 +562,13 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_parts, hp.data(), sizeof(PshDevPart) * hp.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, psh_fill_zero_u32((uint32_t *)ctx->d_flags, 4, ctx->stream));
+    ev_record(ctx, E_ASSIGN);
     HIPCHK(ctx, psh_assign_run(ctx->ag, ctx->d_parts, (int)hp.size(), ntotal, nmock,
                                ctx->counts, ctx->starts, ctx->cursors, ctx->records,
                                ctx->rec_cap, meshA, meshB, ctx->mstride, ctx->scan_tmp,
-                               ctx->d_flags, ctx->stream));
+                               ctx->d_flags, ctx->stream,
+                               ctx->profiling ? ctx->tm.ev[E_DEPOSIT] : nullptr));
+    ev_record(ctx, E_ASSIGNED);
     int flags = 0;
     HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
                                ctx->stream));
@@ -536,9 +590,10 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
   if (B)
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
+  ev_record(ctx, E_ZDONE);
   HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
   if (B) HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
-  ev_record(ctx, 2);
+  ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, nmock);
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   int rc;
@@ -555,6 +610,7 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   a.partials = ctx->partials;
   a.grid = grid;
   HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  ev_record(ctx, E_XDONE);
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, nmock, grid, nacc, out_bins, ctx->stream));
   return PSH_OK;
 }
@@ -607,13 +663,13 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
   std::vector<PshDevPart> hp(2);
   hp[0] = PshDevPart{d_data, nd, 0, 1.0f, 0};
   hp[1] = PshDevPart{d_rand, nr, nd, (float)(-alpha), 0};
-  ev_record(ctx, 0);
+  ev_record(ctx, E_START);
   int rc;
   // plain grid -> A when interlaced (combined into A), else -> B (real field
   // kept for the multipole weights) and transformed out of place into A
   float *plain = il ? ctx->meshA : ctx->meshB;
   if ((rc = run_assign(ctx, hp, 1, plain, il ? ctx->meshB : nullptr))) return rc;
-  ev_record(ctx, 1);
+
   float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
   float2 *B = reinterpret_cast<float2 *>(ctx->meshB);
   PshYlmR noy{};
@@ -626,7 +682,7 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, A, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
     HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
   }
-  ev_record(ctx, 2);
+  ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, 1);
   int nslots = 1;
   for (int n = 1; n < nl; n++) nslots += 2 * ctx->conf.poles[n] + 1;
@@ -681,7 +737,7 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
                                       ctx->d_bins + (size_t)slot * nb, ctx->stream));
     }
   }
-  ev_record(ctx, 3);
+  ev_record(ctx, E_END);
   std::vector<double> raw((size_t)nslots * nb);
   HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
                              hipMemcpyDeviceToHost, ctx->stream));
@@ -804,7 +860,7 @@ extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out
       return set_err(nullptr, PSH_ERR_DEVICE, "phase table upload failed");
     }
   }
-  for (int i = 0; i < 5; i++) (void)hipEventCreate(&ctx->tm.ev[i]);
+  for (int i = 0; i < E_COUNT; i++) (void)hipEventCreate(&ctx->tm.ev[i]);
   if (conf->issim) {
     double bmin[3] = {0, 0, 0};
     if ((rc = set_box(ctx, conf->bsize, bmin))) {
@@ -830,7 +886,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
                   ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
-  for (int i = 0; i < 5; i++)
+  for (int i = 0; i < E_COUNT; i++)
     if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -851,9 +907,9 @@ extern "C" int psh_ctx_set_profiling(psh_ctx *ctx, int on) {
   return PSH_OK;
 }
 
-extern "C" int psh_ctx_timings(const psh_ctx *ctx, double ms[4]) {
+extern "C" int psh_ctx_timings(const psh_ctx *ctx, double ms[8]) {
   if (!ctx || !ms) return PSH_ERR_ARG;
-  for (int i = 0; i < 4; i++) ms[i] = ctx->ms[i];
+  for (int i = 0; i < 8; i++) ms[i] = ctx->ms[i];
   return PSH_OK;
 }
 
@@ -870,7 +926,7 @@ extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n
   int rc;
   // box check on all particles (src/genr_mesh.c:517-533)
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  ev_record(ctx, 0);
+  ev_record(ctx, E_START);
   if ((rc = bounds_device(ctx, (const double *)d_data, n, mn, mx))) return rc;
   if ((rc = def_box(ctx, mn, mx))) return rc;
   if ((rc = ensure_geometry(ctx))) return rc;
@@ -878,11 +934,11 @@ extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n
   hp[0] = PshDevPart{(const double *)d_data, n, 0, 1.0f, 0};
   if ((rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr)))
     return rc;
-  ev_record(ctx, 1);
+
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
   if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
-  ev_record(ctx, 3);
+  ev_record(ctx, E_END);
   std::vector<double> raw(nacc);
   HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * nacc,
                              hipMemcpyDeviceToHost, ctx->stream));
@@ -918,10 +974,10 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, nmock, ntotal))) return rc;
   float *mB = ctx->conf.intlace ? ctx->meshB : nullptr;
-  ev_record(ctx, 0);
+  ev_record(ctx, E_START);
   if (!graph) {
     if ((rc = run_assign(ctx, hp, nmock, ctx->meshA, mB))) return rc;
-    ev_record(ctx, 1);
+
     if ((rc = run_sim_fft_bin(ctx, nmock, ctx->d_bins))) return rc;
   } else {
     // capture the launch sequence once per (data pointer, nmock, size);
@@ -956,7 +1012,7 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     }
     HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
   }
-  ev_record(ctx, 3);
+  ev_record(ctx, E_END);
   std::vector<double> raw((size_t)nmock * nacc);
   int flags = 0;
   HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
@@ -967,7 +1023,7 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   if (graph) {
     if (ctx->profiling) {
       float t;
-      ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[0], ctx->tm.ev[3]) == hipSuccess ? t : -1;
+      ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[E_START], ctx->tm.ev[E_END]) == hipSuccess ? t : -1;
       ctx->ms[0] = ctx->ms[1] = ctx->ms[2] = -1;
     }
   } else {

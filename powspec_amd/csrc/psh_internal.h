@@ -28,6 +28,10 @@ struct PshAxisTables {
   const double *ksq[3];
   const double *al[3];
   const double *kd[3];
+  // FP32 roundings of kv, ksq, al for the per-mode fast path
+  const float *kvf[3];
+  const float *ksqf[3];
+  const float *alf[3];
 };
 
 // Binning geometry (src/multipole.c:306-421).
@@ -40,6 +44,10 @@ struct PshBins {
   int nl;
   int poles[PSH_MAX_NL];
   double los[3];
+  // FP32 fast path: t = (|k| - kedge0) / dk decided in FP32 when it lies
+  // farther than an error bound from an integer, else recomputed in FP64
+  float kedge0f, invdkf, k0dk;
+  float losf[3];
 };
 
 // What the fused last FFT pass does with each mode.
@@ -88,7 +96,7 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
     int nparts, size_t ntotal, int nmock, uint32_t *counts, uint32_t *starts,
     uint32_t *cursors, void *records, size_t rec_cap, float *meshA,
     float *meshB, size_t mstride, uint32_t *d_scan_tmp, int *d_flags,
-    hipStream_t s);
+    hipStream_t s, hipEvent_t ev_deposit = nullptr);
 
 // ---------------------------------------------------------------------------
 // FFT (k_fft.hip)

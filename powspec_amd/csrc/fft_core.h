@@ -163,12 +163,18 @@ __device__ __forceinline__ void dft<16, false>(float2 *v) { dft16<false>(v); }
 template <>
 __device__ __forceinline__ void dft<16, true>(float2 *v) { dft16<true>(v); }
 
-// LDS index of element n of line c (one float per element, padded)
-template <int N>
+// LDS index of element n of line c (one float per element, padded).
+// COLMAP (lanes vary the line fastest): element-major [n][c] with 8 floats of
+// padding every 8 elements; ROWMAP (lanes vary the element fastest):
+// line-major [c][n] with one float of padding every 16 elements and an odd
+// line stride.  Both were chosen with a bank-conflict simulation of every
+// Stockham exchange of the plans used here (32 and 64 banks).
+template <int N, int C, bool COLMAP>
 struct Lds {
   static constexpr int LN = N + (N >> 4) + 1;
+  static constexpr int SIZE = COLMAP ? N * C + ((N + 7) >> 3) * 8 : C * LN;
   __device__ static __forceinline__ int idx(int c, int n) {
-    return c * LN + n + (n >> 4);
+    return COLMAP ? n * C + c + ((n >> 3) << 3) : c * LN + n + (n >> 4);
   }
 };
 
@@ -259,7 +265,7 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
   constexpr int V = G1::NB * G1::R;
   float tmp[V];
   pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
-    lds[Lds<N>::idx(c, n)] = x.x;
+    lds[Lds<N, C, COLMAP>::idx(c, n)] = x.x;
   });
   __syncthreads();
 #pragma unroll
@@ -268,11 +274,11 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
     bool ok = G1::map(q, c, j);
 #pragma unroll
     for (int r = 0; r < G1::R; r++)
-      tmp[q * G1::R + r] = ok ? lds[Lds<N>::idx(c, j + r * G1::NBF)] : 0.f;
+      tmp[q * G1::R + r] = ok ? lds[Lds<N, C, COLMAP>::idx(c, j + r * G1::NBF)] : 0.f;
   }
   __syncthreads();
   pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
-    lds[Lds<N>::idx(c, n)] = x.y;
+    lds[Lds<N, C, COLMAP>::idx(c, n)] = x.y;
   });
   __syncthreads();
 #pragma unroll
@@ -282,7 +288,7 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
 #pragma unroll
     for (int r = 0; r < G1::R; r++)
       v[q * G1::R + r] = make_float2(
-          tmp[q * G1::R + r], ok ? lds[Lds<N>::idx(c, j + r * G1::NBF)] : 0.f);
+          tmp[q * G1::R + r], ok ? lds[Lds<N, C, COLMAP>::idx(c, j + r * G1::NBF)] : 0.f);
   }
   __syncthreads();
 }

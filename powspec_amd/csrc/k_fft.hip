@@ -108,7 +108,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
              const float2 *__restrict__ tw, YlmDev ylm) {
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
-  __shared__ float lds[C * Lds<M>::LN];
+  __shared__ float lds[Lds<M, C, false>::SIZE];
   const size_t line0 = (size_t)blockIdx.x * C;
   const size_t lpm = (size_t)N * N;
   float2 v[V];
@@ -132,7 +132,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
   constexpr int NI = (C * NP + T - 1) / T;
   float rk[NI], rmk[NI];
   store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
-    lds[Lds<M>::idx(c, n)] = x.x;
+    lds[Lds<M, C, false>::idx(c, n)] = x.x;
   });
   __syncthreads();
 #pragma unroll
@@ -140,13 +140,13 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     int item = threadIdx.x + it * T;
     int c = item / NP, k = item - c * NP;
     if (item < C * NP) {
-      rk[it] = lds[Lds<M>::idx(c, k)];
-      rmk[it] = lds[Lds<M>::idx(c, (M - k) & (M - 1))];
+      rk[it] = lds[Lds<M, C, false>::idx(c, k)];
+      rmk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
     }
   }
   __syncthreads();
   store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
-    lds[Lds<M>::idx(c, n)] = x.y;
+    lds[Lds<M, C, false>::idx(c, n)] = x.y;
   });
   __syncthreads();
 #pragma unroll
@@ -156,8 +156,8 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     if (item >= C * NP) continue;
     size_t line = line0 + c;
     if (line >= nlines) continue;
-    float ik = lds[Lds<M>::idx(c, k)];
-    float imk = lds[Lds<M>::idx(c, (M - k) & (M - 1))];
+    float ik = lds[Lds<M, C, false>::idx(c, k)];
+    float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
     float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
     float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
     float2 WO = cmul(tw[k], O);
@@ -179,7 +179,7 @@ z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
              const float2 *__restrict__ tw, float scale) {
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
-  __shared__ float lds[C * Lds<M>::LN];
+  __shared__ float lds[Lds<M, C, false>::SIZE];
   const size_t line0 = (size_t)blockIdx.x * C;
   const size_t lpm = (size_t)N * N;
   constexpr int NP = M / 2 + 1;
@@ -220,12 +220,12 @@ z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
       int item = threadIdx.x + it * T;
       int c = item / NP, k = item - c * NP;
       if (item >= C * NP) continue;
-      lds[Lds<M>::idx(c, k)] = phase ? zk[it].y : zk[it].x;
-      if (k != 0 && M - k != k) lds[Lds<M>::idx(c, M - k)] = phase ? zmk[it].y : zmk[it].x;
+      lds[Lds<M, C, false>::idx(c, k)] = phase ? zk[it].y : zk[it].x;
+      if (k != 0 && M - k != k) lds[Lds<M, C, false>::idx(c, M - k)] = phase ? zmk[it].y : zmk[it].x;
     }
     __syncthreads();
     pass_load<M, C, T, 0, false>(v, [&](int c, int n) -> float2 {
-      return make_float2(lds[Lds<M>::idx(c, n)], 0.f);
+      return make_float2(lds[Lds<M, C, false>::idx(c, n)], 0.f);
     });
     if (phase == 0) {
 #pragma unroll
@@ -255,7 +255,7 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
            size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
   constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
-  __shared__ float lds[C * Lds<N>::LN];
+  __shared__ float lds[Lds<N, C, true>::SIZE];
   const size_t tile = blockIdx.x;
   const size_t per_mesh = (size_t)N * ntz;
   const size_t mesh = tile / per_mesh;
@@ -286,6 +286,7 @@ struct XBinDev {
   int nacc;
   int ell, m;
   float ynorm;
+  float vec0f;   // 2 pi / L_x
 };
 
 // Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference)
@@ -339,34 +340,62 @@ __device__ __forceinline__ int fast_bin(const PshAxisTables &ax, const PshBins &
   return mode_bin(kmod2, b, kmod);
 }
 
+// Per-column constants of the epilogue (y index fixed per tile, kz per column).
+struct ColConst {
+  float ksqyz;   // ksq_y + ksq_z (FP32)
+  float ayz;     // alias_y * alias_z
+  float dyz;     // k_y los_y + k_z los_z
+  float2 pyz;    // interlace phase e^{i pi (j' + k) / N}
+};
+
+// Epilogue of one mode.  xt = {alias_x, ksq_x, phase_x.re, phase_x.im} of
+// index i (LDS copy of the per-axis tables).
 template <int EPI>
 __device__ __forceinline__ void bin_mode(const XBinDev &a, double *bins, int i,
                                          int jy, int kz, float2 F, int M,
-                                         size_t pos) {
-  const PshAxisTables &ax = a.ax;
+                                         size_t pos, const float4 xt,
+                                         const ColConst &cc, float kvx) {
   if (EPI == PSH_EPI_SIM && i == 0 && jy == 0 && kz == 0) return;  // DC skipped
-  float kmodf;
-  const int kb = fast_bin(ax, a.bins, i, jy, kz, kmodf);
+  const PshBins &b = a.bins;
+  // bin: FP32 with an exactness margin, FP64 in the reference's order near
+  // a bin edge (see fast_bin)
+  const float k2f = xt.y + cc.ksqyz;
+  const float rs = rsqrtf(k2f);
+  const float kmodf = k2f * rs;
+  int kb;
+  {
+    const float tf = (kmodf - b.kedge0f) * b.invdkf;
+    const float fl = floorf(tf);
+    const float fr = tf - fl;
+    const float eps = 2e-6f * (fabsf(tf) + b.k0dk + 8.f);
+    if (!b.logscale && fr > eps && fr < 1.f - eps) {
+      kb = (fl >= 0.f && (int)fl < b.nbin) ? (int)fl : -1;
+    } else {
+      double kmod2 = a.ax.ksq[0][i] + a.ax.ksq[1][jy];
+      if (kz > 0) kmod2 = kmod2 + a.ax.ksq[2][kz];
+      double kmod;
+      kb = mode_bin(kmod2, b, kmod);
+    }
+  }
   if (kb < 0) return;
-  const float alias = (ax.alf[0][i] * ax.alf[1][jy]) * ax.alf[2][kz];
+  const float alias = xt.x * cc.ayz;
   const bool single = (kz == 0) || (kz == M);
   const float mult = single ? 1.f : 2.f;
   if (EPI == PSH_EPI_SIM) {
     const float p = (F.x * F.x + F.y * F.y) * alias * mult;
-    const float mu = (ax.kvf[0][i] * a.bins.losf[0] + ax.kvf[1][jy] * a.bins.losf[1] +
-                      ax.kvf[2][kz] * a.bins.losf[2]) / kmodf;
-    for (int l = 0; l < a.bins.nl; l++) {
-      const int ell = a.bins.poles[l];
+    const float mu = (kvx * b.losf[0] + cc.dyz) * rs;
+    for (int l = 0; l < b.nl; l++) {
+      const int ell = b.poles[l];
       if (!single && (ell & 1)) continue;   // the two half-spaces cancel
-      atomicAdd(&bins[l * a.bins.nbin + kb], (double)(p * legendre_f(ell, mu)));
+      atomicAdd(&bins[l * b.nbin + kb], (double)(p * legendre_f(ell, mu)));
     }
   } else if (EPI == PSH_EPI_SURVEY0) {
     atomicAdd(&bins[kb], (double)((F.x * F.x + F.y * F.y) * alias * mult));
   } else {  // PSH_EPI_SURVEYL: Re(F0 conj(Y S)), Y real
     const float2 f0 = a.F0[pos];
     float y = 1.f;
-    const float kx = (float)ax.kd[0][i], ky = (float)ax.kd[1][jy];
-    const float kzz = (float)ax.kd[2][kz];
+    const float kx = (float)a.ax.kd[0][i], ky = (float)a.ax.kd[1][jy];
+    const float kzz = (float)a.ax.kd[2][kz];
     const float k2 = kx * kx + ky * ky;
     if (k2 != 0.f) {   // kx = ky = 0 is added unweighted (mp_template.c:121)
       const float ik3 = rsqrtf(k2 + kzz * kzz);
@@ -378,24 +407,35 @@ __device__ __forceinline__ void bin_mode(const XBinDev &a, double *bins, int i,
   }
 }
 
-// x pass: T chosen so that each thread holds 16 values per field.
+// x pass: T chosen so that each thread holds 32 values per field (256
+// threads at N = 1024: the full VGPR budget, no spills).
 template <int N>
 struct XCfg {
   static constexpr int C = ColCfg<N>::C;
-  static constexpr int T0 = C * N / 16;
+  static constexpr int T0 = C * N / 32;
   static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
 };
 
+template <int N>
+constexpr size_t xbin_static_lds() {
+  return sizeof(float) * Lds<N, XCfg<N>::C, true>::SIZE + sizeof(float4) * N;
+}
+
 template <int N, int EPI, bool INTLACE>
 __global__ void __launch_bounds__(XCfg<N>::T)
-x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
+x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw, float vec0f) {
   constexpr int C = XCfg<N>::C, T = XCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
   extern __shared__ double dyn[];
   double *bins = dyn;
   const int nacc_pad = (a.nacc + 1) & ~1;
-  float *lds = reinterpret_cast<float *>(bins + nacc_pad);
+  float4 *xtab = reinterpret_cast<float4 *>(bins + nacc_pad);
+  float *lds = reinterpret_cast<float *>(xtab + N);
   for (int i = threadIdx.x; i < a.nacc; i += T) bins[i] = 0.0;
+  for (int i = threadIdx.x; i < N; i += T) {
+    const float2 ph = a.phase[i];
+    xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
+  }
   __syncthreads();
 
   const int mock = blockIdx.y;
@@ -416,6 +456,9 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         return c < ncv ? a.B[base + n * ES + c] : make_float2(0.f, 0.f);
       });
     }
+    const float ksqy = a.ax.ksqf[1][y], aly = a.ax.alf[1][y];
+    const float dy = a.ax.kvf[1][y] * a.bins.losf[1];
+    const float2 phy = a.phase[y];
     // epilogue over the natural-order outputs of the last pass
     using G = PassGeom<N, C, T, npasses(N) - 1, true>;
 #pragma unroll
@@ -424,20 +467,25 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       if (!G::map(qq, c, j)) continue;
       if (c >= ncv) continue;
       const int kz = q * C + c;
-      float2 pyz = make_float2(1.f, 0.f);
-      if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
+      ColConst cc;
+      cc.ksqyz = ksqy + a.ax.ksqf[2][kz];
+      cc.ayz = aly * a.ax.alf[2][kz];
+      cc.dyz = dy + a.ax.kvf[2][kz] * a.bins.losf[2];
+      cc.pyz = cmul(phy, a.phase[kz]);
 #pragma unroll
       for (int r = 0; r < G::R; r++) {
         const int i = j + r * G::NS;
+        const float4 xt = xtab[i];
         float2 F = va[qq * G::R + r];
         if (INTLACE) {
           // F0 <- (F0 + e^{i pi (i'+j'+k)/N} F1) / 2   (src/multipole.c:476-484)
-          const float2 t = cmul(cmul(a.phase[i], pyz), vb[qq * G::R + r]);
+          const float2 t = cmul(cmul(make_float2(xt.z, xt.w), cc.pyz), vb[qq * G::R + r]);
           F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
         }
         const size_t pos = base + (size_t)i * ES + c;
         if (EPI == PSH_EPI_SURVEY0) a.store[pos] = F;
-        bin_mode<EPI>(a, bins, i, y, kz, F, M, pos);
+        const float kvx = (float)(i <= M ? i : i - N) * vec0f;
+        bin_mode<EPI>(a, bins, i, y, kz, F, M, pos, xt, cc, kvx);
       }
     }
     __syncthreads();
@@ -521,8 +569,7 @@ struct XBin {
     constexpr int C = ColCfg<N>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
-    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) +
-                         sizeof(float) * C * Lds<N>::LN;
+    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) + xbin_static_lds<N>();
     const dim3 g(grid, nmock), b(XCfg<N>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
 #define PSH_XB(E, I)                                                          \
@@ -532,7 +579,7 @@ struct XBin {
       (void)hipFuncSetAttribute((const void *)k,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                 (int)shmem);                                     \
-    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                       \
+    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw, d.vec0f);                       \
     return hipGetLastError();                                                    \
   }
     if (epi == PSH_EPI_SIM) {
@@ -607,5 +654,6 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.ell = a.ell;
   d.m = a.m;
   d.ynorm = a.ynorm;
+  d.vec0f = a.vec0f;
   return dispatch_n<XBin>(Ng, d, a.epi, a.B != nullptr, a.grid, a.nmock, tw, s);
 }

@@ -609,6 +609,7 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   a.phase = ctx->phase;
   a.partials = ctx->partials;
   a.grid = grid;
+  a.vec0f = (float)(2 * M_PI / ctx->L[0]);
   HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
   ev_record(ctx, E_XDONE);
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, nmock, grid, nacc, out_bins, ctx->stream));
@@ -699,6 +700,7 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
   a.phase = ctx->phase;
   a.partials = ctx->partials;
   a.grid = grid;
+  a.vec0f = (float)(2 * M_PI / ctx->L[0]);
   HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb, ctx->d_bins, ctx->stream));
   if (il) {

@@ -139,6 +139,7 @@ struct PshXBinArgs {
   const float2 *F0;        // SURVEYL: the l=0 field
   int ell, m;              // SURVEYL: Y_lm(k) weight
   float ynorm;             // SURVEYL: normalisation of the real-form Y_lm
+  float vec0f;             // 2 pi / L_x (FP32)
   int nmock;
   size_t mstride_floats;
   PshAxisTables ax;

@@ -178,6 +178,16 @@ struct Lds {
   }
 };
 
+// threadIdx.x hidden from loop-invariant code motion.  In persistent kernels
+// every per-lane address derived from the thread index is invariant over the
+// tile loop; hoisted, those addresses stay live across the whole loop and
+// spill.  Recomputing them at each use costs a few integer operations.
+__device__ __forceinline__ int opaque_tid() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // Per-pass thread geometry.
 template <int N, int C, int T, int P, bool COLMAP>
 struct PassGeom {
@@ -187,7 +197,7 @@ struct PassGeom {
   static constexpr int NBT = C * NBF;         // butterflies in the tile
   static constexpr int NB = (NBT + T - 1) / T;// butterflies per thread
   __device__ static __forceinline__ bool map(int q, int &c, int &j) {
-    int b = (int)threadIdx.x + q * T;
+    int b = opaque_tid() + q * T;
     if (COLMAP) {
       c = b % C;
       j = b / C;
@@ -319,5 +329,54 @@ template <int N, int C, int T, bool COLMAP, class ST>
 __device__ __forceinline__ void store_last(const float2 *v, ST st) {
   pass_store<N, C, T, npasses(N) - 1, COLMAP>(v, st);
 }
+
+// Row-split access for strided columns.  Element j + r*S of line c (S = NBF
+// of the first pass for loads, NS of the last pass for stores) is passed as
+// (r, c, j): with r a compile-time constant after unrolling, a caller forms
+// the address as a per-tile uniform row base (r) plus one per-lane offset
+// (c, j) shared by all values of the thread, which keeps 64-bit addresses
+// out of the registers.
+template <int N, int C, int T, bool COLMAP, class LD>
+__device__ __forceinline__ void pass_load_rows(float2 *v, LD ld) {
+  using G = PassGeom<N, C, T, 0, COLMAP>;
+#pragma unroll
+  for (int q = 0; q < G::NB; q++) {
+    int c, j;
+    const bool ok = G::map(q, c, j);
+#pragma unroll
+    for (int r = 0; r < G::R; r++)
+      v[q * G::R + r] = ok ? ld(r, c, j) : make_float2(0.f, 0.f);
+  }
+}
+
+template <int N, int C, int T, bool COLMAP, bool INV, int TWS, class LD>
+__device__ __forceinline__ void transform_rows(float2 *v, float *lds,
+                                               const float2 *__restrict__ tw, LD ld) {
+  pass_load_rows<N, C, T, COLMAP>(v, ld);
+  Passes<N, C, T, 0, COLMAP, INV, TWS>::run(v, lds, tw);
+}
+
+// Natural-order outputs of the last pass: element j + r*NS of line c.
+template <int N, int C, int T, bool COLMAP, class ST>
+__device__ __forceinline__ void store_last_rows(const float2 *v, ST st) {
+  using G = PassGeom<N, C, T, npasses(N) - 1, COLMAP>;
+  static_assert(G::NS * G::R == N, "last pass");
+#pragma unroll
+  for (int q = 0; q < G::NB; q++) {
+    int c, j;
+    if (!G::map(q, c, j)) continue;
+#pragma unroll
+    for (int r = 0; r < G::R; r++) st(r, c, j, v[q * G::R + r]);
+  }
+}
+
+// Row strides of the split access.
+template <int N, int C, int T, bool COLMAP>
+struct RowSplit {
+  static constexpr int LOAD = PassGeom<N, C, T, 0, COLMAP>::NBF;
+  static constexpr int STORE = PassGeom<N, C, T, npasses(N) - 1, COLMAP>::NS;
+  // largest per-lane row index (must keep j * ES + c within 32 bits)
+  static constexpr int JMAX = (LOAD > STORE ? LOAD : STORE);
+};
 
 }  // namespace pshfft

@@ -26,8 +26,12 @@ namespace {
 
 template <int N>
 struct ColCfg {
-  static constexpr int C = (N <= 512) ? 16 : 8;
-  static constexpr int T = (N >= 2048) ? 512 : 256;
+  // 16 columns: each row segment is one 128-byte line (8-column tiles reach
+  // about 60% of the read bandwidth of 16-column ones); 32 values per thread
+  static constexpr int C = 16;
+  static constexpr int T0 = C * N / 32;
+  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+  static constexpr size_t LDS = sizeof(float) * Lds<N, C, true>::SIZE;
 };
 
 template <int N>
@@ -255,7 +259,7 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
            size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
   constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
-  __shared__ float lds[Lds<N, C, true>::SIZE];
+  extern __shared__ float lds[];
   const size_t tile = blockIdx.x;
   const size_t per_mesh = (size_t)N * ntz;
   const size_t mesh = tile / per_mesh;
@@ -263,12 +267,18 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
   const int o = (int)(rem / ntz), q = (int)(rem - (size_t)o * ntz);
   const size_t base = mesh * mstride_c + o * OS + (size_t)q * C;
   const int ncv = min(C, M + 1 - q * C);
+  using RS = RowSplit<N, C, T, true>;
+  static_assert((size_t)RS::JMAX * N * (N / 2 + PSH_PAD) + C < (1ull << 32),
+                "lane offsets are 32-bit");
+  // columns q*C + c < H always lie inside the padded row; those past kz = M
+  // are read and discarded
   float2 v[V];
-  transform<N, C, T, true, INV, 1>(v, lds, tw, [&](int c, int n) -> float2 {
-    return c < ncv ? in[base + n * ES + c] : make_float2(0.f, 0.f);
+  transform_rows<N, C, T, true, INV, 1>(v, lds, tw, [&](int r, int c, int j) -> float2 {
+    const float2 x = (in + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * (uint32_t)ES + c];
+    return c < ncv ? x : make_float2(0.f, 0.f);
   });
-  store_last<N, C, T, true>(v, [&](int c, int n, float2 x) {
-    if (c < ncv) out[base + n * ES + c] = x;
+  store_last_rows<N, C, T, true>(v, [&](int r, int c, int j, float2 x) {
+    if (c < ncv) (out + base + (size_t)(r * RS::STORE) * ES)[(uint32_t)j * (uint32_t)ES + c] = x;
   });
 }
 
@@ -287,22 +297,11 @@ struct XBinDev {
   int ell, m;
   float ynorm;
   float vec0f;   // 2 pi / L_x
+  // SIM: L_ell(mu) = mu^(ell & 1) * (((c3 x + c2) x + c1) x + c0), x = mu^2,
+  // for the requested pole l; bit l of oddmask set when ell is odd
+  float legc[PSH_MAX_NL][4];
+  int oddmask;
 };
-
-// Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference)
-// written as polynomials in x^2, FP32.
-__device__ __forceinline__ float legendre_f(int ell, float x) {
-  const float x2 = x * x;
-  switch (ell) {
-    case 0: return 1.f;
-    case 1: return x;
-    case 2: return 1.5f * x2 - 0.5f;
-    case 3: return x * (2.5f * x2 - 1.5f);
-    case 4: return (4.375f * x2 - 3.75f) * x2 + 0.375f;
-    case 5: return x * ((7.875f * x2 - 8.75f) * x2 + 1.875f);
-    default: return ((14.4375f * x2 - 19.6875f) * x2 + 6.5625f) * x2 - 0.3125f;
-  }
-}
 
 // Bin of a mode from |k|^2, in the reference's operation order
 // (src/multipole.c:147-158); -1 when outside the bins.
@@ -316,176 +315,205 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
-// Bin of mode (i, jy, kz).  t = (|k| - kedge0) / dk is first evaluated in
-// FP32; its error is below 1e-6 (|t| + kedge0/dk + 8), so when t lies farther
-// than that from an integer, floor(t) is the bin the FP64 expression gives.
-// Otherwise (and for log bins) the FP64 expression decides.  kmodf: FP32 |k|.
-__device__ __forceinline__ int fast_bin(const PshAxisTables &ax, const PshBins &b,
-                                        int i, int jy, int kz, float &kmodf) {
-  const float k2f = (ax.ksqf[0][i] + ax.ksqf[1][jy]) + ax.ksqf[2][kz];
-  kmodf = sqrtf(k2f);
-  if (!b.logscale) {
-    const float tf = (kmodf - b.kedge0f) * b.invdkf;
-    const float fl = floorf(tf);
-    const float fr = tf - fl;
-    const float eps = 1e-6f * (fabsf(tf) + b.k0dk + 8.f);
-    if (fr > eps && fr < 1.f - eps) {
-      const int k = (int)fl;
-      return (fl >= 0.f && k < b.nbin) ? k : -1;
-    }
-  }
-  double kmod2 = ax.ksq[0][i] + ax.ksq[1][jy];
-  if (kz > 0) kmod2 = kmod2 + ax.ksq[2][kz];
+// x pass: 16 values per thread and field.  (The interlaced kernel, which
+// holds both fields, spills a little at 1024 threads; 512 threads with 32
+// values per field spill as well and run slower.)  With T a multiple of C, a
+// thread works on the same column (kz) in every pass and in the binning loop.
+template <int N, bool IL>
+struct XCfg {
+  static constexpr int C = N >= 2048 ? 8 : 16;
+  static constexpr int T0 = C * N / 16;
+  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+  static_assert(T % C == 0, "a thread keeps one column");
+};
+
+// LDS of the x pass besides the bins: ksq_x (FP64), xtab, twiddles, exchange
+// tile.  The twiddles are read from LDS: read from global memory they are
+// invariant over the persistent tile loop and stay live in registers.
+template <int N>
+constexpr size_t xbin_static_lds() {
+  return sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
+         sizeof(float) * Lds<N, XCfg<N, false>::C, true>::SIZE;
+}
+
+// Bin of a mode from its FP32 |k|, kmodf: -1 outside the bins, -2 when
+// undecided.  t = (|k| - kedge0) / dk evaluated in FP32 errs by less than
+// 2e-6 (|t| + kedge0/dk + 8), so when t lies farther than that from an
+// integer, floor(t) is the bin the FP64 expression gives.  Log bins are
+// always undecided.
+__device__ __forceinline__ int bin_fast(const PshBins &b, float kmodf) {
+  const float tf = (kmodf - b.kedge0f) * b.invdkf;
+  const float fl = floorf(tf);
+  const float fr = tf - fl;
+  const float eps = 2e-6f * (fabsf(tf) + b.k0dk + 8.f);
+  if (b.logscale || !(fr > eps && fr < 1.f - eps)) return -2;
+  return (fl >= 0.f && fl < (float)b.nbin) ? (int)fl : -1;
+}
+
+// The FP64 bin of the reference: |k|^2 = (ksq_x + ksq_y) + ksq_z with ksq_z
+// added only off the kz = 0 plane (src/multipole.c:147-158).
+__device__ __forceinline__ int bin_exact(const PshBins &b, double ksqx, double ksqy,
+                                      double ksqz, int kz) {
+#pragma clang fp contract(off)
+  double kmod2 = ksqx + ksqy;
+  if (kz > 0) kmod2 = kmod2 + ksqz;
   double kmod;
   return mode_bin(kmod2, b, kmod);
 }
 
-// Per-column constants of the epilogue (y index fixed per tile, kz per column).
-struct ColConst {
-  float ksqyz;   // ksq_y + ksq_z (FP32)
-  float ayz;     // alias_y * alias_z
-  float dyz;     // k_y los_y + k_z los_z
-  float2 pyz;    // interlace phase e^{i pi (j' + k) / N}
-};
-
-// Epilogue of one mode.  xt = {alias_x, ksq_x, phase_x.re, phase_x.im} of
-// index i (LDS copy of the per-axis tables).
-template <int EPI>
-__device__ __forceinline__ void bin_mode(const XBinDev &a, double *bins, int i,
-                                         int jy, int kz, float2 F, int M,
-                                         size_t pos, const float4 xt,
-                                         const ColConst &cc, float kvx) {
-  if (EPI == PSH_EPI_SIM && i == 0 && jy == 0 && kz == 0) return;  // DC skipped
-  const PshBins &b = a.bins;
-  // bin: FP32 with an exactness margin, FP64 in the reference's order near
-  // a bin edge (see fast_bin)
-  const float k2f = xt.y + cc.ksqyz;
-  const float rs = rsqrtf(k2f);
-  const float kmodf = k2f * rs;
-  int kb;
-  {
-    const float tf = (kmodf - b.kedge0f) * b.invdkf;
-    const float fl = floorf(tf);
-    const float fr = tf - fl;
-    const float eps = 2e-6f * (fabsf(tf) + b.k0dk + 8.f);
-    if (!b.logscale && fr > eps && fr < 1.f - eps) {
-      kb = (fl >= 0.f && (int)fl < b.nbin) ? (int)fl : -1;
-    } else {
-      double kmod2 = a.ax.ksq[0][i] + a.ax.ksq[1][jy];
-      if (kz > 0) kmod2 = kmod2 + a.ax.ksq[2][kz];
-      double kmod;
-      kb = mode_bin(kmod2, b, kmod);
-    }
-  }
-  if (kb < 0) return;
-  const float alias = xt.x * cc.ayz;
-  const bool single = (kz == 0) || (kz == M);
-  const float mult = single ? 1.f : 2.f;
-  if (EPI == PSH_EPI_SIM) {
-    const float p = (F.x * F.x + F.y * F.y) * alias * mult;
-    const float mu = (kvx * b.losf[0] + cc.dyz) * rs;
-    for (int l = 0; l < b.nl; l++) {
-      const int ell = b.poles[l];
-      if (!single && (ell & 1)) continue;   // the two half-spaces cancel
-      atomicAdd(&bins[l * b.nbin + kb], (double)(p * legendre_f(ell, mu)));
-    }
-  } else if (EPI == PSH_EPI_SURVEY0) {
-    atomicAdd(&bins[kb], (double)((F.x * F.x + F.y * F.y) * alias * mult));
-  } else {  // PSH_EPI_SURVEYL: Re(F0 conj(Y S)), Y real
-    const float2 f0 = a.F0[pos];
-    float y = 1.f;
-    const float kx = (float)a.ax.kd[0][i], ky = (float)a.ax.kd[1][jy];
-    const float kzz = (float)a.ax.kd[2][kz];
-    const float k2 = kx * kx + ky * ky;
-    if (k2 != 0.f) {   // kx = ky = 0 is added unweighted (mp_template.c:121)
-      const float ik3 = rsqrtf(k2 + kzz * kzz);
-      const float ik22 = rsqrtf(k2);
-      const float k22 = k2 * ik22;
-      y = ylm_eval(a.ell, a.m, a.ynorm, kzz * ik3, k22 * ik3, kx * ik22, ky * ik22);
-    }
-    atomicAdd(&bins[kb], (double)((f0.x * F.x + f0.y * F.y) * y * alias * mult));
-  }
-}
-
-// x pass: T chosen so that each thread holds 32 values per field (256
-// threads at N = 1024: the full VGPR budget, no spills).
-template <int N>
-struct XCfg {
-  static constexpr int C = ColCfg<N>::C;
-  static constexpr int T0 = C * N / 32;
-  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
-};
-
-template <int N>
-constexpr size_t xbin_static_lds() {
-  return sizeof(float) * Lds<N, XCfg<N>::C, true>::SIZE + sizeof(float4) * N;
-}
-
+// Forward x transform of one (y, kz-tile) column tile fused with the
+// interlace combine and the binning of its modes.  Two phases per tile:
+//  1. (registers, unrolled) F = (F_A + e^{i pi (i'+j'+k)/N} F_B) / 2
+//     (src/multipole.c:476-484); each mode reduced to one float -- |F|^2, or
+//     Re(F0 conj F) for l > 0 survey fields -- written to LDS in natural
+//     order, F itself stored for the l = 0 survey field;
+//  2. (LDS, one loop) bin, alias correction, half-space multiplicity and
+//     Legendre weights, accumulated in per-workgroup FP64 LDS bins
+//     (src/multipole.c:147-177, src/mp_template.c:107-139).
+// xtab[i] = {alias_x, ksq_x, phase_x.re, phase_x.im} in LDS.
 template <int N, int EPI, bool INTLACE>
-__global__ void __launch_bounds__(XCfg<N>::T)
-x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw, float vec0f) {
-  constexpr int C = XCfg<N>::C, T = XCfg<N>::T, M = N / 2;
+__global__ void __launch_bounds__((XCfg<N, INTLACE>::T))
+x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
+  constexpr int C = XCfg<N, INTLACE>::C, T = XCfg<N, INTLACE>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
+  using L = Lds<N, C, true>;
+  using G = PassGeom<N, C, T, npasses(N) - 1, true>;
   extern __shared__ double dyn[];
   double *bins = dyn;
   const int nacc_pad = (a.nacc + 1) & ~1;
-  float4 *xtab = reinterpret_cast<float4 *>(bins + nacc_pad);
-  float *lds = reinterpret_cast<float *>(xtab + N);
+  double *ksqx = bins + nacc_pad;
+  float4 *xtab = reinterpret_cast<float4 *>(ksqx + N);
+  float2 *twl = reinterpret_cast<float2 *>(xtab + N);
+  float *lds = reinterpret_cast<float *>(twl + N);
   for (int i = threadIdx.x; i < a.nacc; i += T) bins[i] = 0.0;
   for (int i = threadIdx.x; i < N; i += T) {
     const float2 ph = a.phase[i];
     xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
+    ksqx[i] = a.ax.ksq[0][i];
+    twl[i] = tw[i];
   }
   __syncthreads();
 
+  const PshBins &b = a.bins;
   const int mock = blockIdx.y;
   const size_t ES = (size_t)N * H;
+  const uint32_t ES32 = (uint32_t)ES;
+  using RS = RowSplit<N, C, T, true>;
+  static_assert(G::NS == RS::STORE, "phase 1 walks the last pass outputs");
+  static_assert((size_t)RS::JMAX * N * (N / 2 + PSH_PAD) + C < (1ull << 32),
+                "lane offsets are 32-bit");
   const size_t ntiles = (size_t)N * ntz;
   for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int y = (int)(tile / ntz), q = (int)(tile - (size_t)y * ntz);
     const size_t base = mock * a.mstride_c + (size_t)y * H + (size_t)q * C;
     const int ncv = min(C, M + 1 - q * C);
+    // columns q*C + c < H always lie inside the padded row; those past
+    // kz = M are read and discarded
     float2 va[V];
-    transform<N, C, T, true, false, 1>(va, lds, tw, [&](int c, int n) -> float2 {
-      return c < ncv ? a.A[base + n * ES + c] : make_float2(0.f, 0.f);
+    transform_rows<N, C, T, true, false, 1>(va, lds, twl, [&](int r, int cc, int j) -> float2 {
+      const float2 x = (a.A + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * ES32 + cc];
+      return cc < ncv ? x : make_float2(0.f, 0.f);
     });
     float2 vb[INTLACE ? V : 1];
     if (INTLACE) {
       __syncthreads();
-      transform<N, C, T, true, false, 1>(vb, lds, tw, [&](int c, int n) -> float2 {
-        return c < ncv ? a.B[base + n * ES + c] : make_float2(0.f, 0.f);
+      transform_rows<N, C, T, true, false, 1>(vb, lds, twl, [&](int r, int cc, int j) -> float2 {
+        const float2 x = (a.B + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * ES32 + cc];
+        return cc < ncv ? x : make_float2(0.f, 0.f);
       });
     }
-    const float ksqy = a.ax.ksqf[1][y], aly = a.ax.alf[1][y];
-    const float dy = a.ax.kvf[1][y] * a.bins.losf[1];
-    const float2 phy = a.phase[y];
-    // epilogue over the natural-order outputs of the last pass
-    using G = PassGeom<N, C, T, npasses(N) - 1, true>;
+    const int c = opaque_tid() % C;   // the thread's column in every pass
+    const bool colok = c < ncv;
+    const int kz = min(q * C + c, M);
+
+    // phase 1
+    float2 pyz = make_float2(1.f, 0.f);
+    if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
-      int c, j;
-      if (!G::map(qq, c, j)) continue;
-      if (c >= ncv) continue;
-      const int kz = q * C + c;
-      ColConst cc;
-      cc.ksqyz = ksqy + a.ax.ksqf[2][kz];
-      cc.ayz = aly * a.ax.alf[2][kz];
-      cc.dyz = dy + a.ax.kvf[2][kz] * a.bins.losf[2];
-      cc.pyz = cmul(phy, a.phase[kz]);
+      const int bt = opaque_tid() + qq * T;
+      const int j = bt / C;
+      const bool ok = colok && bt < G::NBT;
 #pragma unroll
       for (int r = 0; r < G::R; r++) {
+        if (!ok) continue;
         const int i = j + r * G::NS;
-        const float4 xt = xtab[i];
         float2 F = va[qq * G::R + r];
         if (INTLACE) {
-          // F0 <- (F0 + e^{i pi (i'+j'+k)/N} F1) / 2   (src/multipole.c:476-484)
-          const float2 t = cmul(cmul(make_float2(xt.z, xt.w), cc.pyz), vb[qq * G::R + r]);
+          const float4 xt = xtab[i];
+          const float2 t = cmul(cmul(make_float2(xt.z, xt.w), pyz), vb[qq * G::R + r]);
           F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
         }
-        const size_t pos = base + (size_t)i * ES + c;
-        if (EPI == PSH_EPI_SURVEY0) a.store[pos] = F;
-        const float kvx = (float)(i <= M ? i : i - N) * vec0f;
-        bin_mode<EPI>(a, bins, i, y, kz, F, M, pos, xt, cc, kvx);
+        // element (i = j + r NS, c): row base uniform, lane offset j ES + c
+        const size_t rbase = base + (size_t)(r * RS::STORE) * ES;
+        const uint32_t lo = (uint32_t)j * ES32 + c;
+        float v;
+        if (EPI == PSH_EPI_SURVEYL) {
+          const float2 f0 = (a.F0 + rbase)[lo];
+          v = f0.x * F.x + f0.y * F.y;
+        } else {
+          v = F.x * F.x + F.y * F.y;
+        }
+        if (EPI == PSH_EPI_SURVEY0) (a.store + rbase)[lo] = F;
+        lds[L::idx(c, i)] = v;
+      }
+    }
+    __syncthreads();
+
+    // phase 2: column c, rows i = threadIdx.x / C + k T / C
+    if (colok) {
+      const float ksqyz = a.ax.ksqf[1][y] + a.ax.ksqf[2][kz];
+      const float dyz = a.ax.kvf[1][y] * b.losf[1] + a.ax.kvf[2][kz] * b.losf[2];
+      const bool single = (kz == 0) || (kz == M);
+      const float wyz = a.ax.alf[1][y] * a.ax.alf[2][kz] * (single ? 1.f : 2.f);
+      const double ksqy = a.ax.ksq[1][y], ksqz = a.ax.ksq[2][kz];
+      const bool dcrow = (EPI == PSH_EPI_SIM) && y == 0 && kz == 0;
+      // the two half-spaces cancel odd poles off the kz = 0, M planes
+      const int lmask = single ? ~0 : ~a.oddmask;
+      float kdy = 0.f, kdz = 0.f;
+      if (EPI == PSH_EPI_SURVEYL) {
+        kdy = (float)a.ax.kd[1][y];
+        kdz = (float)a.ax.kd[2][kz];
+      }
+#pragma unroll 4
+      for (int k = 0; k < N * C / T; k++) {
+        const int i = opaque_tid() / C + k * (T / C);
+        const float4 xt = xtab[i];
+        const float v = lds[L::idx(c, i)];
+        const float k2f = xt.y + ksqyz;
+        const float rs = rsqrtf(k2f);
+        int kb = bin_fast(b, k2f * rs);
+        if (kb == -2) kb = bin_exact(b, ksqx[i], ksqy, ksqz, kz);
+        if (dcrow && i == 0) kb = -1;   // DC mode skipped
+        if (kb < 0) continue;
+        const float w = v * (xt.x * wyz);
+        if (EPI == PSH_EPI_SIM) {
+          const float kvx = (float)(i <= M ? i : i - N) * a.vec0f;
+          const float mu = (kvx * b.losf[0] + dyz) * rs;
+          const float x2 = mu * mu;
+#pragma unroll
+          for (int l = 0; l < PSH_MAX_NL; l++) {
+            if (l >= b.nl) break;
+            if (!((lmask >> l) & 1)) continue;
+            const float *cf = a.legc[l];
+            float pl = ((cf[3] * x2 + cf[2]) * x2 + cf[1]) * x2 + cf[0];
+            if ((a.oddmask >> l) & 1) pl *= mu;
+            atomicAdd(&bins[l * b.nbin + kb], (double)(w * pl));
+          }
+        } else if (EPI == PSH_EPI_SURVEY0) {
+          atomicAdd(&bins[kb], (double)w);
+        } else {  // Y_lm(k) weight; kx = ky = 0 is added unweighted (mp_template.c:121)
+          float yw = 1.f;
+          const float kdx = (float)a.ax.kd[0][i];
+          const float k2 = kdx * kdx + kdy * kdy;
+          if (k2 != 0.f) {
+            const float ik3 = rsqrtf(k2 + kdz * kdz);
+            const float ik22 = rsqrtf(k2);
+            yw = ylm_eval(a.ell, a.m, a.ynorm, kdz * ik3, k2 * ik22 * ik3, kdx * ik22,
+                          kdy * ik22);
+          }
+          atomicAdd(&bins[kb], (double)(w * yw));
+        }
       }
     }
     __syncthreads();
@@ -552,12 +580,14 @@ struct Col {
     const size_t OS = axis == 1 ? (size_t)N * H : (size_t)H;
     const size_t ES = axis == 1 ? (size_t)H : (size_t)N * H;
     const unsigned blocks = (unsigned)((size_t)N * ntz * nmesh);
-    if (inverse)
-      hipLaunchKernelGGL((col_kernel<N, true>), dim3(blocks), dim3(ColCfg<N>::T), 0, s,
-                         in, out, ntz, OS, ES, mstride / 2, tw);
-    else
-      hipLaunchKernelGGL((col_kernel<N, false>), dim3(blocks), dim3(ColCfg<N>::T), 0, s,
-                         in, out, ntz, OS, ES, mstride / 2, tw);
+    constexpr size_t shmem = ColCfg<N>::LDS;
+    static_assert(shmem <= 160 * 1024, "column tile exceeds the LDS");
+    auto k = inverse ? col_kernel<N, true> : col_kernel<N, false>;
+    if (shmem > 64 * 1024)
+      (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N>::T), shmem, s, in, out, ntz, OS, ES,
+                       mstride / 2, tw);
     return hipGetLastError();
   }
 };
@@ -566,11 +596,11 @@ template <int N>
 struct XBin {
   static hipError_t run(const XBinDev &d, int epi, bool il, int grid, int nmock,
                         const float2 *tw, hipStream_t s) {
-    constexpr int C = ColCfg<N>::C;
+    constexpr int C = XCfg<N, false>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
     const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) + xbin_static_lds<N>();
-    const dim3 g(grid, nmock), b(XCfg<N>::T);
+    const dim3 g(grid, nmock), b(il ? XCfg<N, true>::T : XCfg<N, false>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
 #define PSH_XB(E, I)                                                          \
   {                                                                              \
@@ -579,7 +609,7 @@ struct XBin {
       (void)hipFuncSetAttribute((const void *)k,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                 (int)shmem);                                     \
-    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw, d.vec0f);                       \
+    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                                \
     return hipGetLastError();                                                    \
   }
     if (epi == PSH_EPI_SIM) {
@@ -655,5 +685,19 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.m = a.m;
   d.ynorm = a.ynorm;
   d.vec0f = a.vec0f;
+  // Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference) as
+  // polynomials in mu^2
+  static const float kLeg[7][4] = {
+      {1.f, 0.f, 0.f, 0.f},         {1.f, 0.f, 0.f, 0.f},
+      {-0.5f, 1.5f, 0.f, 0.f},      {-1.5f, 2.5f, 0.f, 0.f},
+      {0.375f, -3.75f, 4.375f, 0.f}, {1.875f, -8.75f, 7.875f, 0.f},
+      {-0.3125f, 6.5625f, -19.6875f, 14.4375f}};
+  d.oddmask = 0;
+  for (int l = 0; l < a.bins.nl && l < PSH_MAX_NL; l++) {
+    const int ell = a.bins.poles[l];
+    if (ell < 0 || ell > 6) return hipErrorInvalidValue;
+    for (int t = 0; t < 4; t++) d.legc[l][t] = kLeg[ell][t];
+    if (ell & 1) d.oddmask |= 1 << l;
+  }
   return dispatch_n<XBin>(Ng, d, a.epi, a.B != nullptr, a.grid, a.nmock, tw, s);
 }

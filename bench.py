@@ -67,25 +67,44 @@ def run_c2(P, steps, warmup):
     for _ in range(warmup):
         res = ctx.run_device(dev, n, float(n))
     L.psh_device_synchronize()
+    # HIP events on the library's stream bracket every stage of every timed
+    # step (each step ends with the copy of its bins to the host)
+    ctx.set_profiling(True)
+    reps = []
     t0 = time.perf_counter()
     for _ in range(steps):
         res = ctx.run_device(dev, n, float(n))
+        reps.append(ctx.timings())
     L.psh_device_synchronize()
     t1 = time.perf_counter()
+    ctx.set_profiling(False)
     ms_step = (t1 - t0) / steps * 1e3
     log(f"C2: {ms_step:.3f} ms/step over {steps} steps")
-    # stage breakdown: HIP events on the library's stream, separate runs
-    ctx.set_profiling(True)
-    reps = []
-    for _ in range(3):
-        ctx.run_device(dev, n, float(n))
-        reps.append(ctx.timings())
-    ctx.set_profiling(False)
-    ms = np.median(np.array(reps), axis=0)
+    ms = np.mean(np.array(reps), axis=0)
     stages = {"assign_total": ms[0], "zy_fft": ms[1], "x_fft_bin": ms[2], "total": ms[3],
               "deposit": ms[4], "z_r2c": ms[5], "y_fft": ms[6], "bucket": ms[7]}
-    log("C2 stages (ms): " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+    log("C2 stages (ms, mean over the timed steps): " +
+        ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
     return data, res, ms_step, stages, n
+
+
+KERNELS = {"deposit": "assign_deposit", "z_r2c": "z_r2c_kernel", "y_fft": "col_kernel",
+           "x_fft_bin": "x_bin_kernel"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
+    None when no summary for it is present."""
+    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch") and kernel in ln:
+                    return float(ln.split("=")[-1])
+    except OSError:
+        pass
+    return None
 
 
 def roofline(stages, Ng, nmesh, npart):
@@ -93,7 +112,8 @@ def roofline(stages, Ng, nmesh, npart):
     cands = {k: stages[k] for k in ("deposit", "z_r2c", "y_fft", "x_fft_bin")}
     dom = max(cands, key=lambda k: cands[k])
     per_launch = b[dom]
-    # z/y passes launch once per mesh: per-launch bytes and time are halves
+    # the z and y passes launch once per mesh (plain and shifted grid); the
+    # deposit and the fused x pass cover both meshes in one launch
     launches = nmesh if dom in ("z_r2c", "y_fft") else 1
     t = stages[dom] / launches
     ach = per_launch / launches / (t * 1e-3) / 1e9
@@ -104,10 +124,24 @@ def roofline(stages, Ng, nmesh, npart):
             table[k] = {"ms": round(stages[k], 4),
                         "GBps": round(b[k] / (stages[k] * 1e-3) / 1e9, 1),
                         "launches": nl}
-    return {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": per_launch / launches, "ms_per_launch": t,
+    return {"kernel": KERNELS[dom], "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(KERNELS[dom]),
+            "bytes_per_launch": per_launch / launches, "ms_per_launch": round(t, 4),
             "stages": table}
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size (torch), for scaling the HBM
+    counters: FETCH_SIZE and WRITE_SIZE of this kernel should read nbytes."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "launches": reps}), flush=True)
 
 
 def run_batched(P, data, nmock, seed=1000):
@@ -201,7 +235,12 @@ def main():
     ap.add_argument("--nmock", type=int, default=256)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build "
               "measures the single-GPU path", file=sys.stderr)

@@ -182,10 +182,24 @@ struct Lds {
 // every per-lane address derived from the thread index is invariant over the
 // tile loop; hoisted, those addresses stay live across the whole loop and
 // spill.  Recomputing them at each use costs a few integer operations.
+// The mask restores the range 0 <= t < T (T a power of two, the block size)
+// that the bound checks of the geometry below fold against.
+template <int T>
 __device__ __forceinline__ int opaque_tid() {
+  static_assert((T & (T - 1)) == 0, "power-of-two block size");
   int t = (int)threadIdx.x;
   asm volatile("" : "+v"(t));
-  return t;
+  return t & (T - 1);
+}
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (linear workgroup id mod 8), each with its own L2; mapping workgroup b
+// of nblk to tile (b mod 8) * nblk/8 + b/8 gives every XCD a contiguous range
+// of tiles, so neighbouring tiles -- which share 128-byte lines -- meet in one
+// L2.  Identity when nblk is not a multiple of 8.
+__device__ __forceinline__ size_t xcd_tile(size_t b, size_t nblk) {
+  if (nblk & 7) return b;
+  return (b & 7) * (nblk >> 3) + (b >> 3);
 }
 
 // Per-pass thread geometry.
@@ -197,7 +211,7 @@ struct PassGeom {
   static constexpr int NBT = C * NBF;         // butterflies in the tile
   static constexpr int NB = (NBT + T - 1) / T;// butterflies per thread
   __device__ static __forceinline__ bool map(int q, int &c, int &j) {
-    int b = opaque_tid() + q * T;
+    int b = opaque_tid<T>() + q * T;
     if (COLMAP) {
       c = b % C;
       j = b / C;
@@ -356,6 +370,32 @@ __device__ __forceinline__ void transform_rows(float2 *v, float *lds,
   Passes<N, C, T, 0, COLMAP, INV, TWS>::run(v, lds, tw);
 }
 
+// Two fields of the same tile advanced pass by pass together (each exchanged
+// through the same LDS buffer in turn), so that neither sits finished in
+// registers while the other is transformed.
+template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS>
+struct Passes2 {
+  __device__ static __forceinline__ void run(float2 *va, float2 *vb, float *lds,
+                                             const float2 *__restrict__ tw) {
+    pass_compute<N, C, T, P, COLMAP, INV, TWS>(va, tw);
+    pass_compute<N, C, T, P, COLMAP, INV, TWS>(vb, tw);
+    if constexpr (P + 1 < npasses(N)) {
+      exchange<N, C, T, P, COLMAP>(va, lds);
+      exchange<N, C, T, P, COLMAP>(vb, lds);
+      Passes2<N, C, T, P + 1, COLMAP, INV, TWS>::run(va, vb, lds, tw);
+    }
+  }
+};
+
+template <int N, int C, int T, bool COLMAP, bool INV, int TWS, class LDA, class LDB>
+__device__ __forceinline__ void transform_rows2(float2 *va, float2 *vb, float *lds,
+                                                const float2 *__restrict__ tw, LDA lda,
+                                                LDB ldb) {
+  pass_load_rows<N, C, T, COLMAP>(va, lda);
+  pass_load_rows<N, C, T, COLMAP>(vb, ldb);
+  Passes2<N, C, T, 0, COLMAP, INV, TWS>::run(va, vb, lds, tw);
+}
+
 // Natural-order outputs of the last pass: element j + r*NS of line c.
 template <int N, int C, int T, bool COLMAP, class ST>
 __device__ __forceinline__ void store_last_rows(const float2 *v, ST st) {
@@ -370,6 +410,24 @@ __device__ __forceinline__ void store_last_rows(const float2 *v, ST st) {
   }
 }
 
+// Access to element `off` of a wave-uniform row base.  When every lane
+// offset fits 32 bits in bytes (BYTE32), the address is formed as a 64-bit
+// scalar base plus a zero-extended 32-bit byte offset, which the global
+// load/store instructions take directly (SGPR base + VGPR offset): one VGPR
+// per lane instead of a 64-bit address per access.
+template <bool BYTE32>
+__device__ __forceinline__ const float2 *row_elem(const float2 *rowbase, uint32_t off) {
+  if constexpr (BYTE32)
+    return reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(rowbase) +
+                                            (size_t)(off * 8u));
+  else
+    return rowbase + off;
+}
+template <bool BYTE32>
+__device__ __forceinline__ float2 *row_elem(float2 *rowbase, uint32_t off) {
+  return const_cast<float2 *>(row_elem<BYTE32>(static_cast<const float2 *>(rowbase), off));
+}
+
 // Row strides of the split access.
 template <int N, int C, int T, bool COLMAP>
 struct RowSplit {
@@ -377,6 +435,10 @@ struct RowSplit {
   static constexpr int STORE = PassGeom<N, C, T, npasses(N) - 1, COLMAP>::NS;
   // largest per-lane row index (must keep j * ES + c within 32 bits)
   static constexpr int JMAX = (LOAD > STORE ? LOAD : STORE);
+  // lane offsets of a column pass over an x stride of N (N/2 + pad) complex
+  // values fit 32 bits in elements always, in bytes when BYTE32
+  static constexpr size_t XSPAN = (size_t)JMAX * N * (N / 2 + 16) + C;
+  static constexpr bool BYTE32 = XSPAN * 8 < (1ull << 32);
 };
 
 }  // namespace pshfft

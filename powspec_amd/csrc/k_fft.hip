@@ -268,17 +268,19 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
   const size_t base = mesh * mstride_c + o * OS + (size_t)q * C;
   const int ncv = min(C, M + 1 - q * C);
   using RS = RowSplit<N, C, T, true>;
-  static_assert((size_t)RS::JMAX * N * (N / 2 + PSH_PAD) + C < (1ull << 32),
-                "lane offsets are 32-bit");
+  static_assert(PSH_PAD == 16 && RS::XSPAN < (1ull << 32), "lane offsets are 32-bit");
   // columns q*C + c < H always lie inside the padded row; those past kz = M
   // are read and discarded
   float2 v[V];
   transform_rows<N, C, T, true, INV, 1>(v, lds, tw, [&](int r, int c, int j) -> float2 {
-    const float2 x = (in + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * (uint32_t)ES + c];
+    const float2 x = *row_elem<RS::BYTE32>(in + base + (size_t)(r * RS::LOAD) * ES,
+                                           (uint32_t)j * (uint32_t)ES + c);
     return c < ncv ? x : make_float2(0.f, 0.f);
   });
   store_last_rows<N, C, T, true>(v, [&](int r, int c, int j, float2 x) {
-    if (c < ncv) (out + base + (size_t)(r * RS::STORE) * ES)[(uint32_t)j * (uint32_t)ES + c] = x;
+    if (c < ncv)
+      *row_elem<RS::BYTE32>(out + base + (size_t)(r * RS::STORE) * ES,
+                            (uint32_t)j * (uint32_t)ES + c) = x;
   });
 }
 
@@ -315,14 +317,17 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
-// x pass: 16 values per thread and field.  (The interlaced kernel, which
-// holds both fields, spills a little at 1024 threads; 512 threads with 32
-// values per field spill as well and run slower.)  With T a multiple of C, a
-// thread works on the same column (kz) in every pass and in the binning loop.
+// x pass: 1024 threads (16 waves, 128 VGPRs).  The plain kernel takes
+// 16-column tiles (128-byte row segments) and 16 values per thread.  The
+// interlaced kernel transforms both fields pass by pass together; to stay
+// within the registers it takes 8-column tiles (8 values per thread and
+// field), whose 64-byte half lines are completed by the neighbouring tile on
+// the same XCD (xcd_tile).  With T a multiple of C, a thread works on the
+// same column (kz) in every pass and in the binning loop.
 template <int N, bool IL>
 struct XCfg {
-  static constexpr int C = N >= 2048 ? 8 : 16;
-  static constexpr int T0 = C * N / 16;
+  static constexpr int C = (IL || N >= 2048) ? 8 : 16;
+  static constexpr int T0 = C * N / (IL ? 8 : 16);
   static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
   static_assert(T % C == 0, "a thread keeps one column");
 };
@@ -333,7 +338,7 @@ struct XCfg {
 template <int N>
 constexpr size_t xbin_static_lds() {
   return sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
-         sizeof(float) * Lds<N, XCfg<N, false>::C, true>::SIZE;
+         sizeof(float) * Lds<N, 16, true>::SIZE;
 }
 
 // Bin of a mode from its FP32 |k|, kmodf: -1 outside the bins, -2 when
@@ -400,29 +405,29 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   const uint32_t ES32 = (uint32_t)ES;
   using RS = RowSplit<N, C, T, true>;
   static_assert(G::NS == RS::STORE, "phase 1 walks the last pass outputs");
-  static_assert((size_t)RS::JMAX * N * (N / 2 + PSH_PAD) + C < (1ull << 32),
-                "lane offsets are 32-bit");
+  static_assert(PSH_PAD == 16 && RS::XSPAN < (1ull << 32), "lane offsets are 32-bit");
   const size_t ntiles = (size_t)N * ntz;
-  for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const size_t t0 = xcd_tile(blockIdx.x, gridDim.x);
+  for (size_t tile = t0; tile < ntiles; tile += gridDim.x) {
     const int y = (int)(tile / ntz), q = (int)(tile - (size_t)y * ntz);
     const size_t base = mock * a.mstride_c + (size_t)y * H + (size_t)q * C;
     const int ncv = min(C, M + 1 - q * C);
     // columns q*C + c < H always lie inside the padded row; those past
     // kz = M are read and discarded
-    float2 va[V];
-    transform_rows<N, C, T, true, false, 1>(va, lds, twl, [&](int r, int cc, int j) -> float2 {
-      const float2 x = (a.A + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * ES32 + cc];
-      return cc < ncv ? x : make_float2(0.f, 0.f);
-    });
-    float2 vb[INTLACE ? V : 1];
-    if (INTLACE) {
-      __syncthreads();
-      transform_rows<N, C, T, true, false, 1>(vb, lds, twl, [&](int r, int cc, int j) -> float2 {
-        const float2 x = (a.B + base + (size_t)(r * RS::LOAD) * ES)[(uint32_t)j * ES32 + cc];
+    auto ld_field = [&](const float2 *f) {
+      return [=](int r, int cc, int j) -> float2 {
+        const float2 x = *row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES,
+                                               (uint32_t)j * ES32 + cc);
         return cc < ncv ? x : make_float2(0.f, 0.f);
-      });
-    }
-    const int c = opaque_tid() % C;   // the thread's column in every pass
+      };
+    };
+    float2 va[V];
+    float2 vb[INTLACE ? V : 1];
+    if (INTLACE)
+      transform_rows2<N, C, T, true, false, 1>(va, vb, lds, twl, ld_field(a.A), ld_field(a.B));
+    else
+      transform_rows<N, C, T, true, false, 1>(va, lds, twl, ld_field(a.A));
+    const int c = opaque_tid<T>() % C;   // the thread's column in every pass
     const bool colok = c < ncv;
     const int kz = min(q * C + c, M);
 
@@ -431,7 +436,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
-      const int bt = opaque_tid() + qq * T;
+      const int bt = opaque_tid<T>() + qq * T;
       const int j = bt / C;
       const bool ok = colok && bt < G::NBT;
 #pragma unroll
@@ -449,12 +454,12 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         const uint32_t lo = (uint32_t)j * ES32 + c;
         float v;
         if (EPI == PSH_EPI_SURVEYL) {
-          const float2 f0 = (a.F0 + rbase)[lo];
+          const float2 f0 = *row_elem<RS::BYTE32>(a.F0 + rbase, lo);
           v = f0.x * F.x + f0.y * F.y;
         } else {
           v = F.x * F.x + F.y * F.y;
         }
-        if (EPI == PSH_EPI_SURVEY0) (a.store + rbase)[lo] = F;
+        if (EPI == PSH_EPI_SURVEY0) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
         lds[L::idx(c, i)] = v;
       }
     }
@@ -477,7 +482,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       }
 #pragma unroll 4
       for (int k = 0; k < N * C / T; k++) {
-        const int i = opaque_tid() / C + k * (T / C);
+        const int i = opaque_tid<T>() / C + k * (T / C);
         const float4 xt = xtab[i];
         const float v = lds[L::idx(c, i)];
         const float k2f = xt.y + ksqyz;
@@ -596,7 +601,7 @@ template <int N>
 struct XBin {
   static hipError_t run(const XBinDev &d, int epi, bool il, int grid, int nmock,
                         const float2 *tw, hipStream_t s) {
-    constexpr int C = XCfg<N, false>::C;
+    const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
     const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) + xbin_static_lds<N>();

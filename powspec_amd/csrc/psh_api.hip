@@ -518,11 +518,13 @@ int ensure_partials(psh_ctx *ctx, int nmock, int grid, int nacc, int slots) {
   return PSH_OK;
 }
 
+// Persistent workgroups per mock of the fused x pass: about 2048 in total, a
+// multiple of 8 (one share per XCD), at most one per tile.
 int xgrid_for(psh_ctx *ctx, int nmock) {
-  const int C = ctx->Ng <= 512 ? 16 : 8;
+  const int C = 16;
   const int ntz = (ctx->M + 1 + C - 1) / C;
   const long ntiles = (long)ctx->Ng * ntz;
-  long g = 2048 / nmock;
+  long g = (2048 / nmock) & ~7L;
   if (g < 8) g = 8;
   if (g > ntiles) g = ntiles;
   return (int)g;

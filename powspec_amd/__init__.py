@@ -162,8 +162,9 @@ class DeviceArray:
     def __init__(self, nbytes: int):
         self.ptr = C.c_void_p()
         self.nbytes = int(nbytes)
-        if load().psh_device_malloc(C.byref(self.ptr), max(1, self.nbytes)) != 0:
-            raise MemoryError(f"device allocation of {nbytes} bytes failed")
+        rc = load().psh_device_malloc(C.byref(self.ptr), max(1, self.nbytes))
+        if rc != 0:
+            raise PshError(rc, f"device allocation of {nbytes} bytes failed")
 
     @classmethod
     def from_numpy(cls, a: np.ndarray) -> "DeviceArray":

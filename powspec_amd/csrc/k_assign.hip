@@ -359,12 +359,38 @@ __device__ __forceinline__ void window_weights(int scheme, float f, float *w) {
   }
 }
 
-// Accumulate one particle's window into an LDS tile of 32-bit fixed-point
-// cells (value * 2^s as a signed integer, ds_add_u32; FP32 LDS atomics are
-// an order of magnitude slower on gfx950).  Integer addition is associative,
-// so the tile does not depend on the order of the records.
-template <int SCHEME>
-__device__ __forceinline__ void deposit_one(int *lds, const int *T,
+// Fixed-point LDS accumulators (value * 2^s as a signed integer; FP32 LDS
+// atomics are an order of magnitude slower on gfx950).  Integer addition is
+// associative, so a tile does not depend on the order of its records.
+//  - 32-bit (ds_add_u32): the exponent leaves about 2^-30 * cells-per-tile
+//    relative precision per contribution -- ample for boxes, whose fields
+//    are sums of positive weights;
+//  - 64-bit (ds_add_u64): each contribution keeps its FP32 precision; used
+//    when weights of both signs meet (survey data minus alpha * randoms),
+//    where large-scale modes are small differences of large sums.
+template <bool WIDE>
+struct Acc;
+template <>
+struct Acc<false> {
+  using T = int;
+  static constexpr int TOP = 30;
+  __device__ static __forceinline__ void add(int *p, float v) {
+    atomicAdd(p, __float2int_rn(v));
+  }
+};
+template <>
+struct Acc<true> {
+  using T = long long;
+  static constexpr int TOP = 62;
+  __device__ static __forceinline__ void add(long long *p, float v) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(p),
+              (unsigned long long)__float2ll_rn(v));
+  }
+};
+
+// Accumulate one particle's window into an LDS tile of fixed-point cells.
+template <int SCHEME, bool WIDE>
+__device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const int *T,
                                             const int *T0, int Ng, const int *b,
                                             const float *f, float w) {
   constexpr int W = SCHEME + 1;
@@ -398,7 +424,7 @@ __device__ __forceinline__ void deposit_one(int *lds, const int *T,
 #pragma unroll
       for (int k = 0; k < W; k++) {
         if (!(m[2] >> k & 1)) continue;
-        atomicAdd(&lds[rxy + l[2][k]], __float2int_rn(wxy * wt[2][k]));
+        Acc<WIDE>::add(&lds[rxy + l[2][k]], wxy * wt[2][k]);
       }
     }
   }
@@ -406,27 +432,29 @@ __device__ __forceinline__ void deposit_one(int *lds, const int *T,
 
 // Fixed-point exponent of a tile: a cell value is a sum of at most cnt
 // contributions of magnitude <= max|w| (window weights are <= 1 and sum to 1
-// per axis), so |value| * 2^s < 2^30 < 2^31.  The quantum is therefore
-// cnt * max|w| * 2^-30 of the largest possible cell value.
+// per axis), so |value| * 2^s < 2^TOP, one bit below the accumulator's sign.
+template <int TOP>
 __device__ __forceinline__ int tile_exponent(const int *flags, uint32_t cnt) {
   const float maxw = __uint_as_float((unsigned)flags[1]);
   const double bound = (double)maxw * (double)cnt;
-  if (!(bound > 0.0)) return 30;
-  int s = 30 - (int)ceil(log2(bound * (1.0 + 1e-7)));
-  return s < -60 ? -60 : (s > 60 ? 60 : s);
+  if (!(bound > 0.0)) return TOP;
+  int s = TOP - (int)ceil(log2(bound * (1.0 + 1e-7)));
+  return s < -100 ? -100 : (s > 100 ? 100 : s);
 }
 
-template <int SCHEME, bool INTLACE>
+template <int SCHEME, bool INTLACE, bool WIDE>
 __global__ void __launch_bounds__(256)
 assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                const Rec *__restrict__ recs, size_t rec_cap,
                float *__restrict__ meshA, float *__restrict__ meshB,
                size_t mstride, const int *__restrict__ flags, size_t ntotal) {
-  extern __shared__ int ldsq[];
+  using AT = typename Acc<WIDE>::T;
+  extern __shared__ long long ldsraw[];
+  AT *ldsq = reinterpret_cast<AT *>(ldsraw);
   const int T[3] = {g.tile[0], g.tile[1], g.tile[2]};
   const int tcells = T[0] * T[1] * T[2];
-  int *ldsA = ldsq;
-  int *ldsB = ldsq + tcells;
+  AT *ldsA = ldsq;
+  AT *ldsB = ldsq + tcells;
   const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
   const size_t tile = blockIdx.x;
   const int mock = (int)(tile / tiles_per_mock);
@@ -448,7 +476,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   const uint32_t cap = rec_cap < 0xffffffffu ? (uint32_t)rec_cap : 0xffffffffu;
   const uint32_t s0 = min(starts[tile], cap), s1 = min(starts[tile + 1], cap);
   const uint32_t cnt = s1 > s0 ? s1 - s0 : 0u;
-  const int sexp = tile_exponent(flags, cnt);
+  const int sexp = tile_exponent<Acc<WIDE>::TOP>(flags, cnt);
   const float scale = ldexpf(1.f, sexp);
   const float inv = ldexpf(1.f, -sexp);
   // Visit the bucket in the order i -> (i * S) mod cnt with S ~ 0.618 cnt and
@@ -482,7 +510,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
     if (x >= cnt) x -= cnt;
     int b[3] = {(int)(r.bxy & 0xffffu), (int)(r.bxy >> 16), (int)(r.bz & 0xffffu)};
     const float ws = r.w * scale;
-    deposit_one<SCHEME>(ldsA, T, T0, g.Ng, b, r.f, ws);
+    deposit_one<SCHEME, WIDE>(ldsA, T, T0, g.Ng, b, r.f, ws);
     if (INTLACE) {
       const uint32_t dl = r.bz >> 16;
       int bs[3];
@@ -494,7 +522,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
         if (bs[a] >= g.Ng) bs[a] -= g.Ng;
         fs[a] = r.f[a] + 0.5f - (float)d;
       }
-      deposit_one<SCHEME>(ldsB, T, T0, g.Ng, bs, fs, ws);
+      deposit_one<SCHEME, WIDE>(ldsB, T, T0, g.Ng, bs, fs, ws);
     }
   }
   __syncthreads();
@@ -533,17 +561,18 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
                           size_t rec_cap, float *meshA, float *meshB,
                           size_t mstride, const int *flags, size_t ntotal,
                           hipStream_t s) {
-  const size_t lds = sizeof(int) * (size_t)g.tile[0] * g.tile[1] * g.tile[2] *
-                     (g.intlace ? 2 : 1);
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
-  if (g.intlace)
-    hipLaunchKernelGGL((assign_deposit<SCHEME, true>), dim3((unsigned)ntiles),
-                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride,
-                       flags, ntotal);
-  else
-    hipLaunchKernelGGL((assign_deposit<SCHEME, false>), dim3((unsigned)ntiles),
-                       dim3(256), lds, s, g, starts, recs, rec_cap, meshA, meshB, mstride,
-                       flags, ntotal);
+  const size_t lds = (g.wide ? sizeof(long long) : sizeof(int)) * (size_t)g.tile[0] *
+                     g.tile[1] * g.tile[2] * (g.intlace ? 2 : 1);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  auto k = g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true>
+                               : assign_deposit<SCHEME, false, true>)
+                  : (g.intlace ? assign_deposit<SCHEME, true, false>
+                               : assign_deposit<SCHEME, false, false>);
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  hipLaunchKernelGGL(k, dim3((unsigned)ntiles), dim3(256), lds, s, g, starts, recs, rec_cap,
+                     meshA, meshB, mstride, flags, ntotal);
   return hipGetLastError();
 }
 

@@ -557,7 +557,11 @@ void ev_collect(psh_ctx *ctx) {
 int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
                float *meshA, float *meshB) {
   size_t ntotal = 0;
-  for (auto &p : hp) ntotal += p.n;
+  ctx->ag.wide = 0;
+  for (auto &p : hp) {
+    ntotal += p.n;
+    if (p.wscale < 0.f) ctx->ag.wide = 1;   // data - alpha * randoms
+  }
   int rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, (int)hp.size(), ntotal))) return rc;
   for (int attempt = 0; attempt < 4; attempt++) {
@@ -1307,6 +1311,10 @@ extern "C" int psh_device_count(void) {
 }
 
 extern "C" int psh_device_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return PSH_ERR_ARG;
+  *ptr = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return PSH_ERR_DEVICE;
   return hipMalloc(ptr, bytes) == hipSuccess ? PSH_OK : PSH_ERR_MEMORY;
 }
 

@@ -65,6 +65,7 @@ struct PshAssignGeom {
   int Ng;
   int scheme;           // 0..3
   int intlace;          // second (half-cell shifted) grid
+  int wide;             // 64-bit fixed-point tiles (weights of both signs)
   int tile[3];          // tile extent per axis
   int ntile[3];         // Ng / tile
   int row_pitch;        // floats per mesh row (2H)

@@ -157,3 +157,45 @@ def test_batch_matches_plain(psh):
             assert np.array_equal(r["cnt"], p["cnt"])
             scale = np.abs(p["pl"][0][0]) + 1e9 / len(c)
             assert np.all(np.abs(r["pl"][0] - p["pl"][0]) <= 1e-5 * scale)
+
+
+def _survey_cats(nrand, seed, nz=3e-4, p0=1e4):
+    """Randoms uniform in a shell sector (r in [500, 900], RA in [0, 60) deg,
+    Dec in [-20, 20) deg); data drawn from them with a modulated selection.
+    Completeness weights in [0.8, 1.2], FKP weights 1/(1 + n p0)."""
+    rng = np.random.default_rng(seed)
+    r = np.cbrt(rng.uniform(500.0 ** 3, 900.0 ** 3, nrand))
+    ra = np.radians(rng.uniform(0.0, 60.0, nrand))
+    dec = np.arcsin(rng.uniform(np.sin(np.radians(-20.0)), np.sin(np.radians(20.0)), nrand))
+    xyz = np.stack([r * np.cos(dec) * np.cos(ra), r * np.cos(dec) * np.sin(ra),
+                    r * np.sin(dec)], axis=1)
+    sel = 0.25 * (1 + 0.8 * np.sin(2 * np.pi * xyz[:, 0] / 210.0)
+                  * np.cos(2 * np.pi * xyz[:, 2] / 170.0))
+    dsel = rng.uniform(0, 1, nrand) < sel
+    dat = xyz[dsel] + rng.normal(0, 3.0, (int(dsel.sum()), 3))
+    out = []
+    for pts in (dat, xyz):
+        wc = rng.uniform(0.8, 1.2, len(pts))
+        nzv = np.full(len(pts), nz)
+        out.append((pts, wc, 1.0 / (1.0 + nzv * p0), nzv))
+    return out
+
+
+@pytest.mark.parametrize("scheme,intlace,Ng,poles", [
+    (1, False, 64, [0, 2, 4]),
+    (1, True, 64, [0, 2, 4]),
+    (2, True, 32, [0, 1, 2, 3, 4]),
+    (3, False, 32, [0, 2]),
+    (0, True, 32, [0, 2, 4, 6]),
+])
+def test_survey_parity(psh, oracle, scheme, intlace, Ng, poles):
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(120000, 40 + scheme)
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    d = np.column_stack([dp, wd])
+    r = np.column_stack([rp, wr])
+    kw = dict(gsize=Ng, assign=scheme, intlace=intlace, issim=False, kbin=0.01, poles=poles)
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_host([d], [r], [sums])
+    ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
+    w = _check(got, ref, poles, sums["shot"], sums["norm"])
+    print(f"survey scheme={scheme} intlace={intlace} Ng={Ng} worst={w:.2e}")

@@ -194,6 +194,7 @@ class _PkArrays:
         self.cnt = np.zeros(nbin, np.uint64)
         self.lcnt = np.zeros(nl * nbin)
         self.pl = [np.zeros(nl * nbin), np.zeros(nl * nbin)]
+        self.xpl = np.zeros(nl * nbin)
         self.pk = Pk()
         self.pk.k = self.k.ctypes.data
         self.pk.kedge = self.kedge.ctypes.data
@@ -202,6 +203,7 @@ class _PkArrays:
         self.pk.lcnt = self.lcnt.ctypes.data
         self.pk.pl[0] = self.pl[0].ctypes.data
         self.pk.pl[1] = self.pl[1].ctypes.data
+        self.pk.xpl = self.xpl.ctypes.data
         self.nbin, self.nl = nbin, nl
 
     def result(self):
@@ -209,7 +211,8 @@ class _PkArrays:
         return {"nbin": nb, "k": self.k.copy(), "kedge": self.kedge.copy(),
                 "km": self.km.copy(), "cnt": self.cnt.copy(),
                 "lcnt": self.lcnt.reshape(nl, nb).copy(),
-                "pl": [p.reshape(nl, nb).copy() for p in self.pl]}
+                "pl": [p.reshape(nl, nb).copy() for p in self.pl],
+                "xpl": self.xpl.reshape(nl, nb).copy()}
 
 
 class Context:

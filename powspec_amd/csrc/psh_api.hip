@@ -224,8 +224,8 @@ int validate_conf(const psh_conf *c, psh_ctx *ctx) {
     for (int a = 0; a < 3; a++)
       if (!(c->bsize[a] > 0)) return set_err(ctx, PSH_ERR_ARG, "BOX_SIZE must be > 0");
   }
-  if (c->iscross)
-    return set_err(ctx, PSH_ERR_UNSUPPORTED, "cross power spectra are not supported yet");
+  if (c->iscross && c->ndata != 2)
+    return set_err(ctx, PSH_ERR_ARG, "cross power spectra need two catalogues");
   return PSH_OK;
 }
 
@@ -662,14 +662,22 @@ void fill_pk_common(psh_ctx *ctx, psh_pk *out) {
 }
 
 // Survey path for catalogue `c`: data and randoms on the device.
-int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_rand,
-               size_t nr, double alpha, double shot, double norm, double *pl) {
+// Number of raw bin slots of a survey run: l = 0, then 2l+1 per further pole.
+int survey_slots(const psh_ctx *ctx) {
+  int nslots = 1;
+  for (int n = 1; n < ctx->conf.npole; n++) nslots += 2 * ctx->conf.poles[n] + 1;
+  return nslots;
+}
+
+// Raw survey sums of the field of `hp` (data parts with weight 1, random
+// parts with -alpha; several catalogues' parts give the field of their sum):
+// raw[0..nb) = sum |F0|^2 alias mult (src/multipole.c:579-660), then per pole
+// n >= 1 and m = -l..l: sum Re(F0 conj(Y_lm(k) S_lm)) with S_lm the transform
+// of Y_lm(r) times the field (src/mp_template.c:48-149).
+int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
   const int N = ctx->Ng;
   const bool il = ctx->conf.intlace;
   const int nb = ctx->nbin, nl = ctx->conf.npole;
-  std::vector<PshDevPart> hp(2);
-  hp[0] = PshDevPart{d_data, nd, 0, 1.0f, 0};
-  hp[1] = PshDevPart{d_rand, nr, nd, (float)(-alpha), 0};
   ev_record(ctx, E_START);
   int rc;
   // plain grid -> A when interlaced (combined into A), else -> B (real field
@@ -691,8 +699,7 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
   }
   ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, 1);
-  int nslots = 1;
-  for (int n = 1; n < nl; n++) nslots += 2 * ctx->conf.poles[n] + 1;
+  const int nslots = survey_slots(ctx);
   if ((rc = ensure_partials(ctx, 1, grid, nb, nslots))) return rc;
   PshXBinArgs a{};
   a.epi = PSH_EPI_SURVEY0;
@@ -746,23 +753,37 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
     }
   }
   ev_record(ctx, E_END);
-  std::vector<double> raw((size_t)nslots * nb);
+  raw.assign((size_t)nslots * nb, 0.0);
   HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ev_collect(ctx);
+  return PSH_OK;
+}
+
+// Survey normalisation (src/multipole.c:1096-1153): l = 0 as (S/cnt - shot)
+// / norm when poles[0] == 0, l > 0 as S * 4pi / (norm cnt) summed over m.
+// Cross spectra (shot = 0, norm = sqrt(norm0 norm1), fac = 2pi for l > 0 over
+// the two orderings) reuse it with cross = true.
+void normalise_survey(const psh_ctx *ctx, const double *raw, double shot, double norm,
+                      bool cross, double *pl) {
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
   for (int q = 0; q < nl * nb; q++) pl[q] = 0;
   if (ctx->conf.poles[0] == 0) {   // src/multipole.c:1110-1114
     for (int i = 0; i < nb; i++) {
       double v = raw[i];
       if (ctx->cnt[i]) {
-        v = v / ctx->cnt[i] - shot;
-        v /= norm;
+        if (cross) {
+          v /= norm * ctx->cnt[i];
+        } else {
+          v = v / ctx->cnt[i] - shot;
+          v /= norm;
+        }
       }
       pl[i] = v;
     }
   }
-  slot = 1;
+  int slot = 1;
   for (int n = 1; n < nl; n++) {   // src/multipole.c:1146-1153
     const int ell = ctx->conf.poles[n];
     std::vector<double> s(nb, 0.0);
@@ -772,12 +793,11 @@ int run_survey(psh_ctx *ctx, const double *d_data, size_t nd, const double *d_ra
       double v = s[i];
       if (ctx->cnt[i]) {
         double nrm = norm * ctx->cnt[i];
-        v *= 4 * M_PI / nrm;
+        v *= (cross ? 2 : 4) * M_PI / nrm;
       }
       pl[n * nb + i] = v;
     }
   }
-  return PSH_OK;
 }
 
 int bounds_device(psh_ctx *ctx, const double *d, size_t n, double mn[3], double mx[3]) {
@@ -926,38 +946,123 @@ extern "C" const char *psh_last_error(const psh_ctx *ctx) {
   return g_last_error.c_str();
 }
 
+// Raw box sums of the field of `hp` (several parts: the field of their sum):
+// sum |F|^2 alias mult L_l(mu) per pole and bin (src/multipole.c:778-898),
+// after the box check of every particle (src/genr_mesh.c:517-533).
+int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
+  int rc;
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  ev_record(ctx, E_START);
+  for (auto &p : hp)
+    if ((rc = bounds_device(ctx, p.xyzw, p.n, mn, mx))) return rc;
+  if ((rc = def_box(ctx, mn, mx))) return rc;
+  if ((rc = ensure_geometry(ctx))) return rc;
+  if ((rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr)))
+    return rc;
+  const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
+  if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
+  if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
+  ev_record(ctx, E_END);
+  raw.assign(nacc, 0.0);
+  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * nacc,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ev_collect(ctx);
+  return PSH_OK;
+}
+
+// Cross normalisation of boxes (src/multipole.c:1048-1061): no shot noise,
+// divided by sqrt(norm0 norm1) cnt, times 2l+1.
+void normalise_sim_cross(const psh_ctx *ctx, const double *raw, double norm, double *pl) {
+  const int nb = ctx->nbin, nl = ctx->conf.npole;
+  for (int l = 0; l < nl; l++)
+    for (int i = 0; i < nb; i++) {
+      double v = raw[l * nb + i];
+      if (ctx->cnt[i]) v /= norm * ctx->cnt[i];
+      pl[l * nb + i] = v * (2 * ctx->conf.poles[l] + 1);
+    }
+}
+
 extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n,
                                   double weight_sum, psh_pk *out) {
   if (!ctx || !out) return set_err(ctx, PSH_ERR_ARG, "null argument");
   if (ctx->survey)
     return set_err(ctx, PSH_ERR_ARG, "psh_ctx_run_device is for simulation boxes");
   int rc;
-  // box check on all particles (src/genr_mesh.c:517-533)
-  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  ev_record(ctx, E_START);
-  if ((rc = bounds_device(ctx, (const double *)d_data, n, mn, mx))) return rc;
-  if ((rc = def_box(ctx, mn, mx))) return rc;
-  if ((rc = ensure_geometry(ctx))) return rc;
   std::vector<PshDevPart> hp(1);
   hp[0] = PshDevPart{(const double *)d_data, n, 0, 1.0f, 0};
-  if ((rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr)))
-    return rc;
-
-  const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
-  if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
-  if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
-  ev_record(ctx, E_END);
-  std::vector<double> raw(nacc);
-  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * nacc,
-                             hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  ev_collect(ctx);
+  std::vector<double> raw;
+  if ((rc = sim_raw(ctx, hp, raw))) return rc;
   // shot noise and normalisation of a box (src/genr_mesh.c:904-910)
   const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
   const double shot = vol / weight_sum;
   const double norm = weight_sum * weight_sum / vol;
   fill_pk_common(ctx, out);
   if (out->pl[0]) normalise_sim(ctx, raw.data(), shot, norm, out->pl[0]);
+  return PSH_OK;
+}
+
+// One or two catalogues already on the device: auto spectra of those with
+// isauto set, and the cross spectrum when iscross is set (src/multipole.c:
+// 1209-1245).  The cross sums come from the field of the two catalogues'
+// sum: with S[.] the raw sums of a field, the l = 0 and box terms
+// Re(F_0 F_1*) = (S[0+1] - S[0] - S[1]) / 2, and the survey l > 0 terms,
+// which the reference adds over both orderings, S[0+1] - S[0] - S[1].
+struct CatDev {
+  const double *data, *rand;   // device psh_data records
+  size_t nd, nr;
+  double wdata, alpha, shot, norm;
+};
+
+int run_catalogues(psh_ctx *ctx, const CatDev *cats, int num, psh_pk *out) {
+  int rc;
+  const bool cross = ctx->conf.iscross && num == 2;
+  std::vector<double> raw[2], rawx;
+  for (int c = 0; c < num; c++) {
+    if (!ctx->conf.isauto[c] && !cross) continue;
+    std::vector<PshDevPart> hp;
+    hp.push_back(PshDevPart{cats[c].data, cats[c].nd, 0, 1.0f, 0});
+    if (ctx->survey) {
+      hp.push_back(PshDevPart{cats[c].rand, cats[c].nr, cats[c].nd, (float)(-cats[c].alpha), 0});
+      if ((rc = survey_raw(ctx, hp, raw[c]))) return rc;
+    } else {
+      if ((rc = sim_raw(ctx, hp, raw[c]))) return rc;
+    }
+  }
+  if (cross) {
+    std::vector<PshDevPart> hp;
+    size_t start = 0;
+    for (int c = 0; c < 2; c++) {
+      hp.push_back(PshDevPart{cats[c].data, cats[c].nd, start, 1.0f, 0});
+      start += cats[c].nd;
+      if (ctx->survey) {
+        hp.push_back(PshDevPart{cats[c].rand, cats[c].nr, start, (float)(-cats[c].alpha), 0});
+        start += cats[c].nr;
+      }
+    }
+    if ((rc = ctx->survey ? survey_raw(ctx, hp, rawx) : sim_raw(ctx, hp, rawx))) return rc;
+    const size_t nb = ctx->nbin;
+    for (size_t q = 0; q < rawx.size(); q++) {
+      const double d = rawx[q] - raw[0][q] - raw[1][q];
+      // l = 0 slot of a survey and every box term carry Re(F_0 F_1*) once
+      rawx[q] = (!ctx->survey || q < nb) ? 0.5 * d : d;
+    }
+  }
+  fill_pk_common(ctx, out);
+  for (int c = 0; c < num; c++) {
+    if (!ctx->conf.isauto[c] || !out->pl[c]) continue;
+    if (ctx->survey)
+      normalise_survey(ctx, raw[c].data(), cats[c].shot, cats[c].norm, false, out->pl[c]);
+    else
+      normalise_sim(ctx, raw[c].data(), cats[c].shot, cats[c].norm, out->pl[c]);
+  }
+  if (cross && out->xpl) {
+    const double nrm = sqrt(cats[0].norm * cats[1].norm);
+    if (ctx->survey)
+      normalise_survey(ctx, rawx.data(), 0.0, nrm, true, out->xpl);
+    else
+      normalise_sim_cross(ctx, rawx.data(), nrm, out->xpl);
+  }
   return PSH_OK;
 }
 
@@ -1085,38 +1190,38 @@ extern "C" int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out) {
   if ((rc = psh_ctx_define_box(ctx, cat))) return rc;
   if ((rc = ensure_geometry(ctx))) return rc;
   size_t need = 0;
-  for (int c = 0; c < cat->num; c++) {
-    size_t n = cat->ndata[c] + (ctx->survey ? cat->nrand[c] : 0);
-    if (n > need) need = n;
-  }
+  for (int c = 0; c < cat->num; c++) need += cat->ndata[c] + (ctx->survey ? cat->nrand[c] : 0);
   if (need > ctx->d_cat_cap) {
     if ((rc = dev_alloc(ctx, &ctx->d_cat, need * 4))) return rc;
     ctx->d_cat_cap = need;
   }
+  CatDev cd[2] = {};
+  size_t off = 0;
+  const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
   for (int c = 0; c < cat->num; c++) {
-    if (!ctx->conf.isauto[c]) continue;
-    if ((rc = stage_host(ctx, cat->data[c], cat->ndata[c], 0))) return rc;
+    cd[c].data = ctx->d_cat + 4 * off;
+    cd[c].nd = cat->ndata[c];
+    if ((rc = stage_host(ctx, cat->data[c], cat->ndata[c], off))) return rc;
+    off += cat->ndata[c];
     if (ctx->survey) {
-      if ((rc = stage_host(ctx, cat->rand[c], cat->nrand[c], cat->ndata[c]))) return rc;
-      fill_pk_common(ctx, out);
-      if ((rc = run_survey(ctx, ctx->d_cat, cat->ndata[c], ctx->d_cat + 4 * cat->ndata[c],
-                           cat->nrand[c], cat->alpha[c], cat->shot[c], cat->norm[c],
-                           out->pl[c])))
-        return rc;
+      cd[c].rand = ctx->d_cat + 4 * off;
+      cd[c].nr = cat->nrand[c];
+      if ((rc = stage_host(ctx, cat->rand[c], cat->nrand[c], off))) return rc;
+      off += cat->nrand[c];
+      cd[c].alpha = cat->alpha[c];
+      cd[c].shot = cat->shot[c];
+      cd[c].norm = cat->norm[c];
     } else {
-      psh_pk tmp = *out;
-      tmp.pl[0] = out->pl[c];
-      double w = cat->wdata[c];
-      if ((rc = psh_ctx_run_device(ctx, (const psh_data *)ctx->d_cat, cat->ndata[c], w, &tmp)))
-        return rc;
       // shot/norm of simulation boxes (src/genr_mesh.c:904-910)
-      const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+      const double w = cat->wdata[c];
       cat->shot[c] = vol / w;
       cat->norm[c] = w * w / vol;
+      cd[c].shot = cat->shot[c];
+      cd[c].norm = cat->norm[c];
     }
+    cd[c].wdata = cat->wdata[c];
   }
-  fill_pk_common(ctx, out);
-  return PSH_OK;
+  return run_catalogues(ctx, cd, cat->num, out);
 }
 
 // ===========================================================================
@@ -1232,23 +1337,28 @@ extern "C" psh_pk *psh_powspec(const psh_conf *conf, const psh_cata *cat, psh_me
     psh_powspec_destroy(pk);
     return nullptr;
   }
-  for (int c = 0; c < mesh->num; c++) {
-    if (!conf->isauto[c]) continue;
-    if (conf->issim) {
-      psh_pk tmp = *pk;
-      tmp.pl[0] = pk->pl[c];
-      rc = psh_ctx_run_device(ctx, (const psh_data *)mesh->d_data[c], mesh->ndata[c],
-                              cat->wdata[c], &tmp);
-    } else {
-      fill_pk_common(ctx, pk);
-      rc = run_survey(ctx, mesh->d_data[c], mesh->ndata[c], mesh->d_rand[c], mesh->nrand[c],
-                      cat->alpha[c], cat->shot[c], cat->norm[c], pk->pl[c]);
-    }
-    if (rc) {
-      fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
+  if (conf->iscross) {
+    pk->xpl = (double *)calloc((size_t)nb * nl, sizeof(double));
+    if (!pk->xpl) {
       psh_powspec_destroy(pk);
       return nullptr;
     }
+  }
+  CatDev cd[2] = {};
+  for (int c = 0; c < mesh->num; c++) {
+    cd[c].data = mesh->d_data[c];
+    cd[c].nd = mesh->ndata[c];
+    cd[c].rand = mesh->d_rand[c];
+    cd[c].nr = mesh->nrand[c];
+    cd[c].wdata = cat->wdata[c];
+    cd[c].alpha = cat->alpha[c];
+    cd[c].shot = cat->shot[c];
+    cd[c].norm = cat->norm[c];
+  }
+  if ((rc = run_catalogues(ctx, cd, mesh->num, pk))) {
+    fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
+    psh_powspec_destroy(pk);
+    return nullptr;
   }
   fill_pk_common(ctx, pk);
   // release the device meshes and particles; keep the metadata

@@ -199,3 +199,133 @@ def test_survey_parity(psh, oracle, scheme, intlace, Ng, poles):
     ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
     w = _check(got, ref, poles, sums["shot"], sums["norm"])
     print(f"survey scheme={scheme} intlace={intlace} Ng={Ng} worst={w:.2e}")
+
+
+def _check_cross(got, ref, poles, scale, tol=TOL):
+    has = ref["cnt"] > 0
+    worst = 0.0
+    for li, ell in enumerate(poles):
+        d = np.abs(got["xpl"][li] - ref["xpl"][li])[has] / scale[has]
+        worst = max(worst, float(d.max()) if d.size else 0.0)
+        assert np.all(d <= tol * (2 * ell + 1)), (ell, float(d.max()))
+    return worst
+
+
+@pytest.mark.parametrize("scheme,intlace,isauto", [
+    (1, False, [True, True]), (3, True, [True, False]), (2, True, [False, False])])
+def test_cross_sim(psh, oracle, scheme, intlace, isauto):
+    """Two correlated boxes: the second shares half of the first's particles."""
+    Ng, L = 32, 800.0
+    rng = np.random.default_rng(70 + scheme)
+    c = rng.uniform(0, L, (60, 3))
+    p0 = (c[rng.integers(0, 60, 40000)] + rng.normal(0, 25.0, (40000, 3))) % L
+    p1 = np.concatenate([p0[:20000], rng.uniform(0, L, (15000, 3))])
+    d0 = np.column_stack([p0, np.ones(len(p0))])
+    d1 = np.column_stack([p1, rng.uniform(0.5, 1.5, len(p1))])
+    kw = dict(gsize=Ng, assign=scheme, intlace=intlace, bsize=L, kbin=0.02, poles=[0, 2, 4],
+              ndata=2, isauto=isauto, iscross=True)
+    got = psh.Context(psh.make_conf(**kw)).run_host([d0, d1])
+    ref = oracle.run(oracle.make_conf(**kw), [d0, d1])
+    assert np.array_equal(got["cnt"], ref["cnt"])
+    s = [np.abs(ref["pl"][c][0]) + ref["shot"][c] for c in range(2)]
+    for c in range(2):
+        if isauto[c]:
+            _check({**got, "pl": [got["pl"][c]]}, {**ref, "pl": [ref["pl"][c]]}, [0, 2, 4],
+                   ref["shot"][c])
+    if not all(isauto):   # autos not returned: scale from a full oracle run
+        full = oracle.run(oracle.make_conf(**{**kw, "isauto": [True, True]}), [d0, d1])
+        s = [np.abs(full["pl"][c][0]) + full["shot"][c] for c in range(2)]
+    w = _check_cross(got, ref, [0, 2, 4], np.sqrt(s[0] * s[1]))
+    print(f"cross sim scheme={scheme} il={intlace} worst={w:.2e}")
+
+
+@pytest.mark.parametrize("scheme,intlace", [(1, True), (2, False)])
+def test_cross_survey(psh, oracle, scheme, intlace):
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(100000, 80 + scheme)
+    wd, wr, s0 = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    d0 = np.column_stack([dp, wd])
+    r0 = np.column_stack([rp, wr])
+    # second tracer: half of the first's objects plus new ones, its own randoms
+    (ep, ewc, efkp, enz), (qp, qwc, qfkp, qnz) = _survey_cats(80000, 90 + scheme)
+    pts = np.concatenate([dp[: len(dp) // 2], ep])
+    wc = np.concatenate([dwc[: len(dp) // 2], ewc])
+    fk = np.concatenate([dfkp[: len(dp) // 2], efkp])
+    nz = np.concatenate([dnz[: len(dp) // 2], enz])
+    wd1, wr1, s1 = oracle.survey_sums(wc, fk, nz, qwc, qfkp, qnz)
+    d1 = np.column_stack([pts, wd1])
+    r1 = np.column_stack([qp, wr1])
+    kw = dict(gsize=32, assign=scheme, intlace=intlace, issim=False, kbin=0.01,
+              poles=[0, 2, 4], ndata=2, isauto=[True, True], iscross=True)
+    got = psh.Context(psh.make_conf(**kw)).run_host([d0, d1], [r0, r1], [s0, s1])
+    ref = oracle.run(oracle.make_conf(**kw), [d0, d1], [r0, r1], [s0, s1])
+    for c, sm in enumerate((s0, s1)):
+        _check({**got, "pl": [got["pl"][c]]}, {**ref, "pl": [ref["pl"][c]]}, [0, 2, 4],
+               sm["shot"], sm["norm"])
+    sc = [np.abs(ref["pl"][c][0]) + sm["shot"] / sm["norm"] for c, sm in enumerate((s0, s1))]
+    w = _check_cross(got, ref, [0, 2, 4], np.sqrt(sc[0] * sc[1]))
+    print(f"cross survey scheme={scheme} il={intlace} worst={w:.2e}")
+
+
+def test_dropin_entry_points(psh, oracle):
+    """psh_genr_mesh / psh_powspec / destructors with a CATA-layout struct whose
+    particle arrays come from malloc (the library frees them, as the reference
+    does), two boxes with the cross spectrum."""
+    import ctypes as C
+    libc = C.CDLL("libc.so.6")
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    L = psh.load()
+    L.psh_genr_mesh.restype = C.c_void_p
+    L.psh_powspec.restype = C.POINTER(psh.Pk)
+    L.psh_genr_mesh.argtypes = [C.c_void_p, C.c_void_p]
+    L.psh_powspec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.psh_mesh_destroy.argtypes = [C.c_void_p]
+    L.psh_powspec_destroy.argtypes = [C.c_void_p]
+    Ng, Lb = 32, 500.0
+    cats = [_uniform(20000, [Lb] * 3, 300), _uniform(15000, [Lb] * 3, 301, weights=True)]
+    cats[1][:5000, :3] = cats[0][:5000, :3]
+    kw = dict(gsize=Ng, assign=2, intlace=True, bsize=Lb, kbin=0.03, poles=[0, 2],
+              ndata=2, isauto=[True, True], iscross=True)
+    conf = psh.make_conf(**kw)
+    # the particle arrays and the two pointer arrays come from malloc: the
+    # library frees all of them (src/genr_mesh.c:916-922)
+    psz = C.sizeof(C.c_void_p)
+    dptr_p = libc.malloc(2 * psz)
+    rptr_p = libc.malloc(2 * psz)
+    C.memset(rptr_p, 0, 2 * psz)
+    dptr = (C.c_void_p * 2).from_address(dptr_p)
+    nd = (C.c_size_t * 2)(*[len(c) for c in cats])
+    vals = {k: (C.c_double * 2)() for k in ("wdata", "wrand", "alpha", "shot", "norm")}
+    for i, c in enumerate(cats):
+        p = libc.malloc(c.nbytes)
+        C.memmove(p, c.ctypes.data, c.nbytes)
+        dptr[i] = p
+        vals["wdata"][i] = float(c[:, 3].sum())
+    cat = psh.Cata()
+    cat.num = 2
+    cat.data = dptr_p
+    cat.rand = rptr_p
+    cat.ndata = C.cast(nd, C.c_void_p)
+    cat.nrand = C.cast((C.c_size_t * 2)(), C.c_void_p)
+    for k, v in vals.items():
+        setattr(cat, k, C.cast(v, C.c_void_p))
+    mesh = L.psh_genr_mesh(C.byref(conf), C.byref(cat))
+    assert mesh
+    assert not cat.data and not cat.rand              # freed like the reference
+    pk = L.psh_powspec(C.byref(conf), C.byref(cat), mesh)
+    assert pk
+    nb, nl = pk.contents.nbin, pk.contents.nl
+    arr = lambda p, n, t=C.c_double: np.ctypeslib.as_array(C.cast(p, C.POINTER(t)), (n,)).copy()
+    got = {"nbin": nb, "cnt": arr(pk.contents.cnt, nb, C.c_uint64),
+           "kedge": arr(pk.contents.kedge, nb + 1), "km": arr(pk.contents.km, nb),
+           "pl": [arr(pk.contents.pl[c], nl * nb).reshape(nl, nb) for c in range(2)],
+           "xpl": arr(pk.contents.xpl, nl * nb).reshape(nl, nb)}
+    L.psh_powspec_destroy(pk)
+    L.psh_mesh_destroy(C.c_void_p(mesh))
+    ref = oracle.run(oracle.make_conf(**kw), cats)
+    for c in range(2):
+        assert math.isclose(vals["shot"][c], ref["shot"][c], rel_tol=1e-15)
+        _check({**got, "pl": [got["pl"][c]]}, {**ref, "pl": [ref["pl"][c]]}, [0, 2],
+               ref["shot"][c])
+    s = [np.abs(ref["pl"][c][0]) + ref["shot"][c] for c in range(2)]
+    _check_cross(got, ref, [0, 2], np.sqrt(s[0] * s[1]))

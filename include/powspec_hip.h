@@ -33,6 +33,7 @@ extern "C" {
 #define PSH_OK 0
 #define PSH_ERR_MEMORY (-1)      /* POWSPEC_ERR_MEMORY (src/define.h:112)  */
 #define PSH_ERR_ARG (-2)         /* POWSPEC_ERR_ARG    (src/define.h:113)  */
+#define PSH_ERR_FILE (-3)        /* POWSPEC_ERR_FILE   (src/define.h:114)  */
 #define PSH_ERR_CATA (-11)       /* POWSPEC_ERR_CATA   (src/define.h:119)  */
 #define PSH_ERR_MESH (-13)       /* POWSPEC_ERR_MESH   (src/define.h:121)  */
 #define PSH_ERR_PK (-14)         /* POWSPEC_ERR_PK     (src/define.h:122)  */
@@ -132,6 +133,32 @@ void psh_powspec_destroy(psh_pk *pk);
 /* Mesh metadata accessors (for the output writer, src/save_res.c:70-84). */
 int psh_mesh_info(const psh_mesh *mesh, int *Ng, double bsize[3],
     double bmin[3], int *assign, int *intlace);
+
+/* ---- output files ---------------------------------------------------------- */
+
+/* Mesh metadata the output header reports (src/save_res.c:70-84). */
+typedef struct {
+  int Ng;
+  double bsize[3];
+  double bmin[3];
+  int assign;
+  int intlace;
+} psh_mesh_meta;
+
+/* Writes the reference's output files (src/save_res.c:35-230): one auto
+ * power spectrum file per catalogue with isauto set (fauto[c]) and, when
+ * iscross is set, the cross spectrum file fcross.  header != 0 adds the
+ * catalogue, mesh and normalisation lines (OUTPUT_HEADER).  Numbers are
+ * printed with the reference's formats (%.10lg, %lg for the box, %zu for
+ * mode counts), so the files are byte-identical for identical values.
+ * Returns PSH_OK, PSH_ERR_ARG or PSH_ERR_FILE. */
+int psh_write_pk(const psh_conf *conf, const psh_cata *cat, const psh_mesh_meta *meta,
+    const psh_pk *pk, const char *const fauto[2], const char *fcross, int header);
+
+/* Replaces save_res (src/save_res.c:35): psh_write_pk with the metadata of a
+ * psh_mesh kept by psh_powspec. */
+int psh_save_res(const psh_conf *conf, const psh_cata *cat, const psh_mesh *mesh,
+    const psh_pk *pk, const char *const fauto[2], const char *fcross, int header);
 
 /* ---- context layer -------------------------------------------------------- */
 

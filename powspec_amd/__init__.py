@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libpowspec_hip.so")
 NGP, CIC, TSC, PCS = 0, 1, 2, 3
 MAX_ELL = 6
 OK = 0
-ERR_MEMORY, ERR_ARG, ERR_CATA, ERR_MESH, ERR_PK = -1, -2, -11, -13, -14
+ERR_MEMORY, ERR_ARG, ERR_FILE, ERR_CATA, ERR_MESH, ERR_PK = -1, -2, -3, -11, -13, -14
 ERR_DEVICE, ERR_UNSUPPORTED = -20, -21
 
 
@@ -64,8 +64,16 @@ EXPORTS = [
     "psh_ctx_nbin", "psh_ctx_run_host", "psh_ctx_define_box", "psh_ctx_run_device",
     "psh_ctx_run_batch", "psh_ctx_timings", "psh_ctx_set_profiling", "psh_last_error",
     "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
-    "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch",
+    "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
+    "psh_save_res",
 ]
+
+
+class MeshMeta(C.Structure):
+    """psh_mesh_meta: mesh metadata of the output header."""
+
+    _fields_ = [("Ng", C.c_int), ("bsize", C.c_double * 3), ("bmin", C.c_double * 3),
+                ("assign", C.c_int), ("intlace", C.c_int)]
 
 _lib = None
 

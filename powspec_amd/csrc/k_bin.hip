@@ -61,8 +61,11 @@ geometry_kernel(int N, PshAxisTables ax, PshBins b,
       atomicAdd(&cnt[kb], single ? 1ull : 2ull);
       atomicAdd(&km[kb], single ? kmod : 2 * kmod);
       if (k == 0 && kmod2 == 0.0) continue;  // DC: no multipole weight
+      // mu divides by |k| in both binnings (src/multipole.c:940, 966): with
+      // log bins kmod is 0.5 log10 |k|^2, not |k|
+      const double kabs = b.logscale ? sqrt(kmod2) : kmod;
       const double mu = (ax.kv[0][i] * b.los[0] + ax.kv[1][j] * b.los[1] +
-                         ax.kv[2][k] * b.los[2]) / kmod;
+                         ax.kv[2][k] * b.los[2]) / kabs;
       const double num = single ? 1.0 : 2.0;
       for (int l = 0; l < b.nl; l++) {
         if (!single && (b.poles[l] & 1)) continue;

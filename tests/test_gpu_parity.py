@@ -329,3 +329,66 @@ def test_dropin_entry_points(psh, oracle):
                ref["shot"][c])
     s = [np.abs(ref["pl"][c][0]) + ref["shot"][c] for c in range(2)]
     _check_cross(got, ref, [0, 2], np.sqrt(s[0] * s[1]))
+
+
+@pytest.mark.parametrize("issim", [True, False])
+def test_log_bins(psh, oracle, issim):
+    """LOG_SCALE binning (0.5 log10 |k|^2, src/multipole.c:147-158) with the
+    reference's rescale of k, km and the edges (Q7, src/multipole.c:1267-1273)."""
+    kw = dict(gsize=64, assign=2, intlace=True, logscale=True, kmin=math.log10(0.004),
+              kmax=math.log10(0.5), kbin=0.05, poles=[0, 2, 4])
+    if issim:
+        d = _uniform(60000, [600.0] * 3, 17, weights=True)
+        kw["bsize"] = 600.0
+        got = psh.Context(psh.make_conf(**kw)).run_host([d])
+        ref = oracle.run(oracle.make_conf(**kw), [d])
+        shot, norm = ref["shot"][0], None
+    else:
+        (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(100000, 21)
+        wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+        d = np.column_stack([dp, wd])
+        r = np.column_stack([rp, wr])
+        kw["issim"] = False
+        got = psh.Context(psh.make_conf(**kw)).run_host([d], [r], [sums])
+        ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
+        shot, norm = sums["shot"], sums["norm"]
+    assert ref["nbin"] > 10
+    assert np.array_equal(got["k"], ref["k"])
+    w = _check(got, ref, [0, 2, 4], shot, norm)
+    print(f"log bins issim={issim} nbin={ref['nbin']} worst={w:.2e}")
+
+
+def test_save_res_dropin(psh, tmp_path):
+    """psh_save_res writes what psh_powspec returned, in the reference's format."""
+    import ctypes as C
+    d = _uniform(20000, [400.0] * 3, 23)
+    kw = dict(gsize=32, assign=1, intlace=False, bsize=400.0, kbin=0.04, poles=[0, 2])
+    ctx = psh.Context(psh.make_conf(**kw))
+    res = ctx.run_host([d])
+    L = psh.load()
+    pk = psh.Pk()
+    keep = [res["k"], res["kedge"], res["km"], res["cnt"], res["pl"][0].ravel().copy()]
+    pk.nbin, pk.nl = res["nbin"], 2
+    pk.poles[0], pk.poles[1] = 0, 2
+    pk.k, pk.kedge, pk.km = (a.ctypes.data for a in keep[:3])
+    pk.cnt = keep[3].ctypes.data
+    pk.pl[0] = keep[4].ctypes.data
+    meta = psh.MeshMeta()
+    meta.Ng, meta.assign, meta.intlace = 32, 1, 0
+    meta.bsize[:] = [400.0] * 3
+    cat = psh.Cata()
+    vals = {k: (C.c_double * 2)(res[k][0] if k in ("shot", "norm") else float(len(d)), 0.0)
+            for k in ("wdata", "shot", "norm")}
+    nd = (C.c_size_t * 2)(len(d), 0)
+    cat.num = 1
+    cat.ndata = C.cast(nd, C.c_void_p)
+    for k, v in vals.items():
+        setattr(cat, k, C.cast(v, C.c_void_p))
+    conf = psh.make_conf(**kw)
+    f = str(tmp_path / "pk.txt")
+    fa = (C.c_char_p * 2)(f.encode(), None)
+    assert L.psh_write_pk(C.byref(conf), C.byref(cat), C.byref(meta), C.byref(pk), fa, None, 1) == 0
+    tab = np.loadtxt(f)
+    assert np.allclose(tab[:, 0], res["k"], rtol=1e-9)
+    assert np.array_equal(tab[:, 4].astype(np.uint64), res["cnt"])
+    assert np.allclose(tab[:, 5], res["pl"][0][0], rtol=1e-9, atol=1e-300)

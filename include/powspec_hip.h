@@ -35,6 +35,7 @@ extern "C" {
 #define PSH_ERR_ARG (-2)         /* POWSPEC_ERR_ARG    (src/define.h:113)  */
 #define PSH_ERR_FILE (-3)        /* POWSPEC_ERR_FILE   (src/define.h:114)  */
 #define PSH_ERR_CATA (-11)       /* POWSPEC_ERR_CATA   (src/define.h:119)  */
+#define PSH_ERR_CNVT (-12)       /* POWSPEC_ERR_CNVT   (src/define.h:120)  */
 #define PSH_ERR_MESH (-13)       /* POWSPEC_ERR_MESH   (src/define.h:121)  */
 #define PSH_ERR_PK (-14)         /* POWSPEC_ERR_PK     (src/define.h:122)  */
 #define PSH_ERR_DEVICE (-20)     /* HIP runtime failure (no reference code) */
@@ -159,6 +160,39 @@ int psh_write_pk(const psh_conf *conf, const psh_cata *cat, const psh_mesh_meta 
  * psh_mesh kept by psh_powspec. */
 int psh_save_res(const psh_conf *conf, const psh_cata *cat, const psh_mesh *mesh,
     const psh_pk *pk, const char *const fauto[2], const char *fcross, int header);
+
+/* ---- survey coordinate conversion ------------------------------------------ */
+
+/* The CONF fields of the conversion (src/load_conf.h:67-72): cosmology with
+ * omega_k = 1 - omega_m - omega_l as load_conf sets it, DE_EOS_W,
+ * CMVDST_ERR, and the Z_CMVDST_CONV samples (already read, host arrays;
+ * NULL/0 selects the Gauss-Legendre integration), plus DATA_CONVERT /
+ * RAND_CONVERT per catalogue. */
+typedef struct {
+  double omega_m, omega_l, omega_k;
+  double eos_w;
+  double ecdst;
+  const double *zsp, *dsp;
+  size_t nsp;
+  int dcnvt[2], rcnvt[2];
+} psh_cnvt;
+
+/* Replaces cnvt_coord (src/cnvt_coord.c:551-606): (RA [deg], Dec [deg],
+ * redshift) of the flagged catalogues -> comoving (x, y, z) in place, on the
+ * GPU, in FP64.  Integration: the Gauss-Legendre order is chosen by the
+ * reference's convergence test over the catalogues' redshift range
+ * (src/cnvt_coord.c:138-239, 306-341, 494-512); negative redshifts are an
+ * error.  Interpolation: natural cubic spline of the samples with the
+ * reference's binary search; redshifts outside [z_0, z_last) give HUGE_VAL
+ * distances as in the reference.  Returns PSH_OK or PSH_ERR_CNVT /
+ * PSH_ERR_ARG / PSH_ERR_DEVICE. */
+int psh_cnvt_coord(const psh_cnvt *cv, psh_cata *cat);
+
+/* The same on device-resident records: narr arrays d_data[i] of n[i]
+ * records, all converted with one order.  *order_out (may be NULL) receives
+ * the integration order (0 for interpolation). */
+int psh_cnvt_device(const psh_cnvt *cv, psh_data *const *d_data, const size_t *n,
+    int narr, int *order_out);
 
 /* ---- context layer -------------------------------------------------------- */
 

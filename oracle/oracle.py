@@ -109,6 +109,18 @@ def lib():
         L.pso_run.restype = C.c_int
         L.pso_last_timings.argtypes = [_dp]
         L.pso_num_threads.restype = C.c_int
+        L.pso_legauss.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.pso_cmvdist.argtypes = [C.c_double] * 4 + [C.c_int, C.c_double]
+        L.pso_cmvdist.restype = C.c_double
+        L.pso_cmvdist_order.argtypes = [C.c_double] * 7
+        L.pso_cmvdist_order.restype = C.c_int
+        L.pso_spline_ypp.argtypes = [_dp, _dp, C.c_size_t, _dp]
+        L.pso_spline_ypp.restype = C.c_int
+        L.pso_spline_dist.argtypes = [_dp, _dp, _dp, C.c_size_t, C.c_double]
+        L.pso_spline_dist.restype = C.c_double
+        L.pso_cnvt.argtypes = [_dp, C.c_size_t] + [C.c_double] * 4 + [C.c_int, C.c_void_p,
+                                                                     C.c_void_p, C.c_void_p,
+                                                                     C.c_size_t]
         _lib = L
     return _lib
 
@@ -372,3 +384,59 @@ def survey_sums(wc_d, wfkp_d, nz_d, wc_r, wfkp_r, nz_r):
         norm = alpha * s2n[1]
     return wd, wr, {"wdata": sw[0], "wrand": sw[1], "alpha": alpha, "shot": shot,
                     "norm": norm}
+
+
+# ---- survey coordinate conversion (src/cnvt_coord.c) -----------------------
+
+def legauss(n):
+    """(positive abscissas, their weights, weight of x = 0 or 0)."""
+    x = np.zeros(17)
+    w = np.zeros(17)
+    w0 = np.zeros(1)
+    lib().pso_legauss(n, x, w, w0)
+    return x[: n // 2].copy(), w[: n // 2].copy(), float(w0[0])
+
+
+def eos_widx(eos_w):
+    """Power index of the dark-energy term (src/cnvt_coord.c:322)."""
+    return 0.0 if eos_w == -1 else 3 * (1 + eos_w)
+
+
+def cmvdist(om, ol, ok, widx, n, z):
+    return lib().pso_cmvdist(om, ol, ok, widx, n, z)
+
+
+def cmvdist_order(om, ol, ok, widx, err, zmin, zmax):
+    return lib().pso_cmvdist_order(om, ol, ok, widx, err, zmin, zmax)
+
+
+def spline_ypp(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    ypp = np.zeros_like(x)
+    if lib().pso_spline_ypp(x, y, len(x), ypp):
+        raise ValueError("spline needs two or more samples")
+    return ypp
+
+
+def spline_dist(zs, ds, ypp, z):
+    zs, ds, ypp = (np.ascontiguousarray(a, np.float64) for a in (zs, ds, ypp))
+    return lib().pso_spline_dist(zs, ds, ypp, len(zs), z)
+
+
+def cnvt(data, om=0.31, ol=0.69, ok=0.0, eos_w=-1.0, order=None, err=1e-8,
+         zs=None, ds=None):
+    """(RA, Dec, z, w) records -> (x, y, z, w), a converted copy."""
+    d = np.ascontiguousarray(data, np.float64).copy()
+    widx = eos_widx(eos_w)
+    if zs is None:
+        if order is None:
+            order = cmvdist_order(om, ol, ok, widx, err, d[:, 2].min(), d[:, 2].max())
+        lib().pso_cnvt(d, len(d), om, ol, ok, widx, order, None, None, None, 0)
+    else:
+        zs = np.ascontiguousarray(zs, np.float64)
+        ds = np.ascontiguousarray(ds, np.float64)
+        ypp = spline_ypp(zs, ds)
+        lib().pso_cnvt(d, len(d), om, ol, ok, widx, 0, zs.ctypes.data, ds.ctypes.data,
+                       ypp.ctypes.data, len(zs))
+    return d

@@ -26,6 +26,11 @@
  *   pso_legpoly        reference header compiled as-is, scipy eval_legendre
  *   pso_run            composition of the above; physics sanity (uniform
  *                      box -> P_0 ~ 0 after shot subtraction)
+ *   pso_legauss        numpy.polynomial.legendre.leggauss
+ *   pso_cmvdist(_order) closed forms: Einstein-de Sitter and de Sitter
+ *                      distances; convergence of the order
+ *   pso_spline_*       scipy CubicSpline(bc_type='natural'); exact on
+ *                      linear samples; out-of-range rule
  * FFTW3 (the reference's external FFT) is absent from this image, so the
  * reference itself is unbuildable here; the FFT is pinned against the DFT
  * definition instead.
@@ -186,6 +191,36 @@ void pso_last_timings(double t[4]);
 
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int pso_num_threads(void);
+
+/* ---- survey coordinate conversion (src/cnvt_coord.c) -------------------- */
+
+/* Gauss-Legendre rule of order n (4..32): the n/2 positive abscissas, their
+ * weights and, for odd n, the weight of x = 0 (w0); written from the
+ * Legendre polynomials (Newton iteration), not from a table. */
+void pso_legauss(int n, double *x, double *w, double *w0);
+
+/* Radial comoving distance [Mpc/h] by n-point Gauss-Legendre integration of
+ * c/(100 E(z)) from 0 to z, E^2 = Om (1+z)^3 + Ok (1+z)^2 + OL (1+z)^widx
+ * (src/cnvt_coord.c:241-292). */
+double pso_cmvdist(double om, double ol, double ok, double widx, int n, double z);
+
+/* Order chosen by the reference's convergence test on 128 redshifts in
+ * [zmin, zmax] (src/cnvt_coord.c:138-239, 306-341); -1 on failure. */
+int pso_cmvdist_order(double om, double ol, double ok, double widx, double err,
+    double zmin, double zmax);
+
+/* Natural cubic spline: second derivatives of the samples
+ * (math/cspline.c cspline_ypp), and the interpolation with the reference's
+ * binary search (src/cnvt_coord.c:97-111), HUGE_VAL outside [z0, z_last). */
+int pso_spline_ypp(const double *x, const double *y, size_t n, double *ypp);
+double pso_spline_dist(const double *zs, const double *ds, const double *ypp,
+    size_t n, double z);
+
+/* In-place (RA, Dec, z) -> (x, y, z) of n records (src/cnvt_coord.c:122-134,
+ * 372-386): integration when zs == NULL, else spline interpolation. */
+void pso_cnvt(double *d, size_t n, double om, double ol, double ok,
+    double widx, int order, const double *zs, const double *ds,
+    const double *ypp, size_t nsp);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ NGP, CIC, TSC, PCS = 0, 1, 2, 3
 MAX_ELL = 6
 OK = 0
 ERR_MEMORY, ERR_ARG, ERR_FILE, ERR_CATA, ERR_MESH, ERR_PK = -1, -2, -3, -11, -13, -14
+ERR_CNVT = -12
 ERR_DEVICE, ERR_UNSUPPORTED = -20, -21
 
 
@@ -65,8 +66,52 @@ EXPORTS = [
     "psh_ctx_run_batch", "psh_ctx_timings", "psh_ctx_set_profiling", "psh_last_error",
     "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
-    "psh_save_res",
+    "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device",
 ]
+
+
+class Cnvt(C.Structure):
+    """psh_cnvt: the CONF fields of the coordinate conversion."""
+
+    _fields_ = [("omega_m", C.c_double), ("omega_l", C.c_double), ("omega_k", C.c_double),
+                ("eos_w", C.c_double), ("ecdst", C.c_double), ("zsp", C.c_void_p),
+                ("dsp", C.c_void_p), ("nsp", C.c_size_t), ("dcnvt", C.c_int * 2),
+                ("rcnvt", C.c_int * 2)]
+
+
+def make_cnvt(omega_m=0.31, omega_l=None, eos_w=-1.0, ecdst=1e-8, samples=None,
+              dcnvt=(True, True), rcnvt=(True, True)):
+    """psh_cnvt as load_conf sets it (omega_k = 1 - Om - OL, or 0 when
+    OMEGA_LAMBDA is not given, src/load_conf.c:758-767).  samples: optional
+    (z, d) arrays of Z_CMVDST_CONV; kept alive on the returned object."""
+    c = Cnvt()
+    c.omega_m = omega_m
+    if omega_l is None:
+        c.omega_l, c.omega_k = 1 - omega_m, 0.0
+    else:
+        c.omega_l, c.omega_k = omega_l, 1 - omega_m - omega_l
+    c.eos_w, c.ecdst = eos_w, ecdst
+    if samples is not None:
+        z = np.ascontiguousarray(samples[0], np.float64)
+        d = np.ascontiguousarray(samples[1], np.float64)
+        c._keep = (z, d)
+        c.zsp, c.dsp, c.nsp = z.ctypes.data, d.ctypes.data, len(z)
+    c.dcnvt[:] = [int(v) for v in dcnvt]
+    c.rcnvt[:] = [int(v) for v in rcnvt]
+    return c
+
+
+def cnvt_device(cv: Cnvt, arrays):
+    """Convert device arrays (DeviceArray, record count) in place; returns the
+    integration order (0 for interpolation)."""
+    n = len(arrays)
+    ptrs = (C.c_void_p * max(n, 1))(*[a.ptr for a, _ in arrays])
+    cnt = (C.c_size_t * max(n, 1))(*[int(k) for _, k in arrays])
+    order = C.c_int()
+    rc = load().psh_cnvt_device(C.byref(cv), ptrs, cnt, n, C.byref(order))
+    if rc:
+        raise PshError(rc, "coordinate conversion failed")
+    return order.value
 
 
 class MeshMeta(C.Structure):
@@ -181,6 +226,15 @@ class DeviceArray:
         if a.nbytes and load().psh_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes) != 0:
             raise RuntimeError("host to device copy failed")
         return d
+
+    def to_numpy(self, shape, dtype=np.float64) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the allocation holds")
+        rc = load().psh_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes)
+        if rc:
+            raise PshError(rc, "device to host copy failed")
+        return out
 
     def free(self):
         if self.ptr:

@@ -45,8 +45,11 @@ int main(void) {
   F(psh_conf, gsize); F(psh_conf, poles); F(psh_conf, bsize); F(psh_conf, los);
   F(psh_conf, kmin); F(psh_conf, kbin); F(psh_conf, isauto); F(psh_conf, verbose);
   F(psh_pk, k); F(psh_pk, cnt); F(psh_pk, pl); F(psh_pk, xpl);
-  printf("codes %d %d %d %d %d %d %d %d\n", PSH_ERR_MEMORY, PSH_ERR_ARG, PSH_ERR_FILE,
-         PSH_ERR_CATA, PSH_ERR_MESH, PSH_ERR_PK, PSH_ERR_DEVICE, PSH_ERR_UNSUPPORTED);
+  printf("codes %d %d %d %d %d %d %d %d %d\n", PSH_ERR_MEMORY, PSH_ERR_ARG, PSH_ERR_FILE,
+         PSH_ERR_CATA, PSH_ERR_CNVT, PSH_ERR_MESH, PSH_ERR_PK, PSH_ERR_DEVICE,
+         PSH_ERR_UNSUPPORTED);
+  printf("psh_cnvt %zu\n", sizeof(psh_cnvt));
+  F(psh_cnvt, zsp); F(psh_cnvt, nsp); F(psh_cnvt, rcnvt);
   printf("psh_mesh_meta %zu\n", sizeof(psh_mesh_meta));
   F(psh_mesh_meta, bmin); F(psh_mesh_meta, intlace);
   return 0;
@@ -74,12 +77,13 @@ def test_struct_layout_matches_header():
     assert v["psh_conf"][0] == C.sizeof(P.Conf)
     assert v["psh_pk"][0] == C.sizeof(P.Pk)
     for tname, cls in (("psh_cata", P.Cata), ("psh_conf", P.Conf), ("psh_pk", P.Pk),
-                       ("psh_mesh_meta", P.MeshMeta)):
+                       ("psh_mesh_meta", P.MeshMeta), ("psh_cnvt", P.Cnvt)):
         for key, (off,) in ((k, o) for k, o in v.items() if k.startswith(tname + ".")):
             field = key.split(".", 1)[1]
             assert getattr(cls, field).offset == off, key
-    assert v["codes"] == [P.ERR_MEMORY, P.ERR_ARG, P.ERR_FILE, P.ERR_CATA, P.ERR_MESH,
-                          P.ERR_PK, P.ERR_DEVICE, P.ERR_UNSUPPORTED]
+    assert v["codes"] == [P.ERR_MEMORY, P.ERR_ARG, P.ERR_FILE, P.ERR_CATA, P.ERR_CNVT,
+                          P.ERR_MESH, P.ERR_PK, P.ERR_DEVICE, P.ERR_UNSUPPORTED]
+    assert v["psh_cnvt"][0] == C.sizeof(P.Cnvt)
     assert v["psh_mesh_meta"][0] == C.sizeof(P.MeshMeta)
 
 

@@ -4,8 +4,9 @@
 Workload (default): the clustered periodic box at production size --
 Gaussian field on a 512^3 grid with P(k) = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2)
 in L = 1000 Mpc/h, lognormal, Poisson-sampled at 0.02 (h/Mpc)^3 (~2e7
-particles), Zel'dovich RSD along z with f = 0.8; PCS, N_g = 1024,
-interlaced, FP32 meshes and FFTs, FP64 bin sums, l = 0,2,4, dk = 0.005.
+particles), Zel'dovich RSD along z with f = 0.8, generated on the GPU from
+Philox streams (seed 2); PCS, N_g = 1024, interlaced, FP32 meshes and FFTs,
+FP64 bin sums, l = 0,2,4, dk = 0.005.
 
 A step is one full pass of the path over the catalogue already resident in
 HBM: coordinate bounds, mass assignment of both interlaced meshes, 3-D r2c
@@ -55,12 +56,12 @@ def stage_bytes(Ng, nmesh, npart):
 
 
 def run_c2(P, steps, warmup):
-    from powspec_amd import synth
     t = time.time()
-    data = synth.clustered_box(ng_gen=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
-    n = len(data)
+    # C2 recipe on the GPU (same Philox streams as powspec_amd.synth.clustered_box)
+    dev, n = P.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
+    P.load().psh_device_synchronize()
+    data = dev.to_numpy((n, 4))   # host copy for the CPU baseline and the mocks
     log(f"C2 catalogue: {n} particles generated in {time.time() - t:.1f} s")
-    dev = P.DeviceArray.from_numpy(data)
     conf = c2_conf(P)
     ctx = P.Context(P.make_conf(**conf))
     L = P.load()

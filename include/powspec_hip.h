@@ -194,6 +194,45 @@ int psh_cnvt_coord(const psh_cnvt *cv, psh_cata *cat);
 int psh_cnvt_device(const psh_cnvt *cv, psh_data *const *d_data, const size_t *n,
     int narr, int *order_out);
 
+/* ---- synthetic catalogues ------------------------------------------------- */
+/* No counterpart in the reference: seeded generators for the benchmark and
+ * test workloads.  All draw from Philox4x32-10 streams (counter = (index,
+ * stream, draw), key = seed) that powspec_amd/synth.py reproduces on the
+ * host, so CPU and GPU generators see the same random numbers. */
+
+/* The four Philox words of counters (i, stream, draw), i < n -> d_out[4i..]. */
+int psh_gen_philox(uint32_t *d_out, size_t n, uint64_t seed, uint32_t stream, uint32_t draw);
+
+/* n records uniform in [0, L)^3 (kept below the box edge), unit weights. */
+int psh_gen_uniform_box(psh_data *d_out, size_t n, const double L[3], uint64_t seed);
+
+/* Clustered periodic box: Gaussian field with P(k) = A (k/0.1)^nidx
+ * exp(-(k/kc)^2) on an ng^3 grid, lognormal, Poisson sampled at nbar (or at
+ * npart expected particles when npart > 0), uniform in the cell, Zel'dovich
+ * shift f Psi_z along z, periodic.  *d_out is allocated by the library (free
+ * with psh_device_free); *n_out receives the particle count. */
+typedef struct {
+  int ng;
+  double L;
+  double nbar, npart;
+  double f;
+  double A, nidx, kc;
+  uint64_t seed;
+} psh_box_spec;
+int psh_gen_clustered_box(const psh_box_spec *g, psh_data **d_out, size_t *n_out);
+
+/* Survey sky sample: RA uniform in ra[], sin(Dec) uniform in dec[] (degrees),
+ * redshift from a Gaussian (zmu, zsigma) truncated to [zlo, zhi); records
+ * (RA, Dec, z, w_fkp) with n(z) = nbar0 exp(-(z-zmu)^2 / (2 zsigma^2)) and
+ * w_fkp = 1 / (1 + n(z) p0); d_nz (may be NULL) receives n(z). */
+typedef struct {
+  double ra[2], dec[2];
+  double zmu, zsigma, zlo, zhi;
+  double nbar0, p0;
+  uint64_t seed;
+} psh_sky_spec;
+int psh_gen_sky(const psh_sky_spec *g, psh_data *d_out, double *d_nz, size_t n);
+
 /* ---- context layer -------------------------------------------------------- */
 
 typedef struct psh_ctx psh_ctx;

@@ -66,8 +66,10 @@ EXPORTS = [
     "psh_ctx_run_batch", "psh_ctx_timings", "psh_ctx_set_profiling", "psh_last_error",
     "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
-    "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device",
+    "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
+    "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky",
 ]
+
 
 
 class Cnvt(C.Structure):
@@ -380,3 +382,73 @@ class Context:
         res["shot"] = [vals["shot"][0], vals["shot"][1]]
         res["norm"] = [vals["norm"][0], vals["norm"][1]]
         return res
+
+
+# ---- synthetic catalogues on the GPU (psh_gen_*) ---------------------------
+
+class BoxSpec(C.Structure):
+    """psh_box_spec: clustered box generator parameters."""
+
+    _fields_ = [("ng", C.c_int), ("L", C.c_double), ("nbar", C.c_double),
+                ("npart", C.c_double), ("f", C.c_double), ("A", C.c_double),
+                ("nidx", C.c_double), ("kc", C.c_double), ("seed", C.c_uint64)]
+
+
+class SkySpec(C.Structure):
+    """psh_sky_spec: survey sky generator parameters."""
+
+    _fields_ = [("ra", C.c_double * 2), ("dec", C.c_double * 2), ("zmu", C.c_double),
+                ("zsigma", C.c_double), ("zlo", C.c_double), ("zhi", C.c_double),
+                ("nbar0", C.c_double), ("p0", C.c_double), ("seed", C.c_uint64)]
+
+
+def gen_philox(n, seed, stream, draw=0) -> np.ndarray:
+    """(n, 4) uint32 Philox words of counters (i, stream, draw) on the GPU."""
+    d = DeviceArray(16 * n)
+    rc = load().psh_gen_philox(d.ptr, C.c_size_t(n), C.c_uint64(seed), C.c_uint32(stream),
+                               C.c_uint32(draw))
+    if rc:
+        raise PshError(rc, "psh_gen_philox failed")
+    return d.to_numpy((n, 4), np.uint32)
+
+
+def gen_uniform_box(n, L=1000.0, seed=1) -> DeviceArray:
+    d = DeviceArray(32 * n)
+    Ls = (C.c_double * 3)(*np.broadcast_to(np.asarray(L, float), (3,)))
+    rc = load().psh_gen_uniform_box(d.ptr, C.c_size_t(n), Ls, C.c_uint64(seed))
+    if rc:
+        raise PshError(rc, "psh_gen_uniform_box failed")
+    return d
+
+
+class _Owned(DeviceArray):
+    """A device pointer allocated inside the library."""
+
+    def __init__(self, ptr, nbytes):   # noqa: D401 - no allocation here
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+
+def gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2, A=1e4, n=-1.5, kc=0.5,
+                      npart=0.0):
+    """(DeviceArray, count) of the clustered box (same recipe and streams as
+    synth.clustered_box)."""
+    spec = BoxSpec(ng, L, nbar, npart or 0.0, f, A, n, kc, seed)
+    ptr = C.c_void_p()
+    cnt = C.c_size_t()
+    rc = load().psh_gen_clustered_box(C.byref(spec), C.byref(ptr), C.byref(cnt))
+    if rc:
+        raise PshError(rc, "psh_gen_clustered_box failed")
+    return _Owned(ptr, 32 * max(cnt.value, 1)), cnt.value
+
+
+def gen_sky(n, seed, ra=(0.0, 60.0), dec=(-30.0, 30.0), zmu=0.55, zsigma=0.08, zlo=0.4,
+            zhi=0.7, nbar0=3e-4, p0=1e4):
+    """(DeviceArray of (RA, Dec, z, w_fkp), DeviceArray of n(z))."""
+    spec = SkySpec((C.c_double * 2)(*ra), (C.c_double * 2)(*dec), zmu, zsigma, zlo, zhi,
+                   nbar0, p0, seed)
+    d, nz = DeviceArray(32 * n), DeviceArray(8 * n)
+    rc = load().psh_gen_sky(C.byref(spec), d.ptr, nz.ptr, C.c_size_t(n))
+    if rc:
+        raise PshError(rc, "psh_gen_sky failed")
+    return d, nz

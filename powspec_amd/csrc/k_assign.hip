@@ -305,6 +305,7 @@ scan_sums(uint32_t *__restrict__ bsum, int nb) {
   if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
+template <bool CURSORS>
 __global__ void __launch_bounds__(kScanThreads)
 scan_add(uint32_t *__restrict__ out, uint32_t *__restrict__ cursors,
          const uint32_t *__restrict__ bsum, size_t n, int nb) {
@@ -315,7 +316,7 @@ scan_add(uint32_t *__restrict__ out, uint32_t *__restrict__ cursors,
     if (k < n) {
       uint32_t v = out[k] + off;
       out[k] = v;
-      cursors[k] = v;
+      if (CURSORS) cursors[k] = v;
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
@@ -588,6 +589,17 @@ hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t psh_exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t *tmp, size_t n,
+                                  hipStream_t s) {
+  if (n == 0) return hipErrorInvalidValue;
+  const int nsb = (int)((n + kScanBlock - 1) / kScanBlock);
+  hipLaunchKernelGGL(scan_local, dim3(nsb), dim3(kScanThreads), 0, s, in, out, tmp, n);
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, tmp, nsb);
+  hipLaunchKernelGGL(scan_add<false>, dim3(nsb), dim3(kScanThreads), 0, s, out,
+                     (uint32_t *)nullptr, tmp, n, nsb);
+  return hipGetLastError();
+}
+
 size_t psh_assign_scan_bytes(size_t ntiles) {
   size_t nsb = (ntiles + kScanBlock - 1) / kScanBlock;
   return (nsb + 1) * sizeof(uint32_t);
@@ -617,7 +629,7 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
   hipLaunchKernelGGL(scan_local, dim3(nsb), dim3(kScanThreads), 0, s, counts,
                      starts, d_scan_tmp, ntiles);
   hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, d_scan_tmp, nsb);
-  hipLaunchKernelGGL(scan_add, dim3(nsb), dim3(kScanThreads), 0, s, starts,
+  hipLaunchKernelGGL(scan_add<true>, dim3(nsb), dim3(kScanThreads), 0, s, starts,
                      cursors, d_scan_tmp, ntiles, nsb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (ntotal > 0) {

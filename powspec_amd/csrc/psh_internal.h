@@ -170,6 +170,11 @@ hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
 hipError_t psh_reduce_partials(const double *partials, int nmock, int grid,
     int nacc, double *out, hipStream_t s);
 
+// Exclusive scan of n uint32 counts into out[0..n] (out[n] = total); tmp
+// holds psh_assign_scan_bytes(n) bytes.  in and out must not overlap.
+hipError_t psh_exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t *tmp,
+                                  size_t n, hipStream_t s);
+
 // Fill a float buffer with zeros (graph-safe replacement of memset).
 hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 

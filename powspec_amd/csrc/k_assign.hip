@@ -157,6 +157,53 @@ __device__ __forceinline__ uint32_t agg_increment(uint32_t *ctr, uint32_t t,
   return slot;
 }
 
+// Tiles (mock-local indices) touched by particle `gi` of its part and, with
+// FILL, its record.  Returns the number of tiles (0 and flag bit 2 when the
+// shifted grid's base is inconsistent).
+template <bool FILL>
+__device__ __forceinline__ int particle_tiles(const PshAssignGeom &g, const DevPart &P,
+                                              size_t gi, int W, uint32_t *tiles, Rec &rec,
+                                              int *flags) {
+  const double *r = P.xyzw + 4 * (gi - P.start);
+  double rr[4];
+  {
+    const double2 *r2 = reinterpret_cast<const double2 *>(r);
+    double2 a = r2[0], b = r2[1];
+    rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
+  }
+  PartCells pc = part_cells(rr, g);
+  if (!pc.ok) {
+    atomicOr(flags, 2);
+    return 0;
+  }
+  int tx[2], ty[2], tz[2];
+  int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
+  int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
+  int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
+  int nt = 0;
+  for (int a = 0; a < nx; a++)
+    for (int b = 0; b < ny; b++)
+      for (int c = 0; c < nz; c++)
+        tiles[nt++] = ((uint32_t)tx[a] * g.ntile[1] + ty[b]) * g.ntile[2] + tz[c];
+  if (FILL) {
+    rec.f[0] = pc.f[0];
+    rec.f[1] = pc.f[1];
+    rec.f[2] = pc.f[2];
+    rec.w = (float)rr[3] * P.wscale;
+    rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
+    rec.bz = (uint32_t)pc.b[2] |
+             ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2)) << 16);
+  }
+  return nt;
+}
+
+// Largest |weight| of a launch, for the fixed-point scale of the deposit
+// (flags[1] holds the bits of a non-negative float).
+__device__ __forceinline__ void publish_wmax(float wmax, int *flags) {
+  for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
+  if (__lane_id() == 0) atomicMax(reinterpret_cast<unsigned *>(flags) + 1, __float_as_uint(wmax));
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(256)
 assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
@@ -177,40 +224,11 @@ assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
     int nt = 0;
     Rec rec;
     if (live) {
-      int pi = find_part(parts, nparts, gi);
-      const DevPart &P = parts[pi];
-      const double *r = P.xyzw + 4 * (gi - P.start);
-      double rr[4];
-      {
-        const double2 *r2 = reinterpret_cast<const double2 *>(r);
-        double2 a = r2[0], b = r2[1];
-        rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
-      }
-      PartCells pc = part_cells(rr, g);
-      if (!pc.ok) {
-        atomicOr(flags, 2);
-      } else {
-        int tx[2], ty[2], tz[2];
-        int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
-        int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
-        int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
-        const uint32_t mbase = (uint32_t)P.mock * tiles_per_mock;
-        for (int a = 0; a < nx; a++)
-          for (int b = 0; b < ny; b++)
-            for (int c = 0; c < nz; c++)
-              tiles[nt++] = mbase + ((uint32_t)tx[a] * g.ntile[1] + ty[b]) * g.ntile[2] + tz[c];
-        if (FILL) {
-          rec.f[0] = pc.f[0];
-          rec.f[1] = pc.f[1];
-          rec.f[2] = pc.f[2];
-          rec.w = (float)rr[3] * P.wscale;
-          wmax = fmaxf(wmax, fabsf(rec.w));
-          rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
-          rec.bz = (uint32_t)pc.b[2] |
-                   ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2))
-                    << 16);
-        }
-      }
+      const DevPart &P = parts[find_part(parts, nparts, gi)];
+      nt = particle_tiles<FILL>(g, P, gi, W, tiles, rec, flags);
+      const uint32_t mbase = (uint32_t)P.mock * tiles_per_mock;
+      for (int k = 0; k < nt; k++) tiles[k] += mbase;
+      if (FILL && nt) wmax = fmaxf(wmax, fabsf(rec.w));
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
@@ -228,12 +246,76 @@ assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
       }
     }
   }
-  if (FILL) {
-    // largest |weight| of the launch, for the fixed-point scale of the
-    // deposit (flags[1] holds the bits of a non-negative float)
-    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-    if (__lane_id() == 0) atomicMax(reinterpret_cast<unsigned *>(flags) + 1, __float_as_uint(wmax));
+  if (FILL) publish_wmax(wmax, flags);
+}
+
+// Bucketing with an LDS histogram of one mock's tiles (meshes of at most
+// kLdsTiles tiles per mock).  Workgroup b takes particles [b kChunk,
+// (b+1) kChunk) -- a split that depends on the total only, so captured
+// graphs stay valid for any mock sizes -- and histograms the (particle,
+// tile) pairs of the mock of its first particle in LDS; particles of another
+// mock (at a part boundary) use global atomics directly.
+//  count: the histogram is added to the global counters, one atomic per
+//         nonzero tile;
+//  fill:  one atomic per nonzero tile reserves the workgroup's range of each
+//         bucket, then a second sweep over the chunk (L2-resident) hands out
+//         slots with LDS atomics.
+// The deposit accumulates in integers, so the order of records within a
+// bucket does not matter.
+constexpr int kChunk = 16384;
+constexpr int kLdsTiles = 32768;
+constexpr int kLdsThreads = 1024;
+
+template <bool FILL>
+__global__ void __launch_bounds__(kLdsThreads)
+assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
+                  size_t ntotal, uint32_t *__restrict__ counts,
+                  uint32_t *__restrict__ cursors, Rec *__restrict__ recs, size_t rec_cap,
+                  int *__restrict__ flags) {
+  extern __shared__ uint32_t hist[];
+  const int W = window_width(g.scheme);
+  const uint32_t tpm = (uint32_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  const size_t c0 = (size_t)blockIdx.x * kChunk;
+  const size_t c1 = min(ntotal, c0 + (size_t)kChunk);
+  const int mock0 = parts[find_part(parts, nparts, c0)].mock;
+  const uint32_t mbase0 = (uint32_t)mock0 * tpm;
+  for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  uint32_t tiles[8];
+  Rec rec;
+  // pass A: histogram of this mock's pairs (count: others go to global)
+  for (size_t gi = c0 + threadIdx.x; gi < c1; gi += blockDim.x) {
+    const DevPart &P = parts[find_part(parts, nparts, gi)];
+    const int nt = particle_tiles<false>(g, P, gi, W, tiles, rec, flags);
+    for (int k = 0; k < nt; k++) {
+      if (P.mock == mock0) atomicAdd(&hist[tiles[k]], 1u);
+      else if (!FILL) atomicAdd(&counts[(uint32_t)P.mock * tpm + tiles[k]], 1u);
+    }
   }
+  __syncthreads();
+  if (!FILL) {
+    for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x)
+      if (hist[i]) atomicAdd(&counts[mbase0 + i], hist[i]);
+    return;
+  }
+  for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x)
+    if (hist[i]) hist[i] = atomicAdd(&cursors[mbase0 + i], hist[i]);
+  __syncthreads();
+  // pass B: records
+  float wmax = 0.f;
+  for (size_t gi = c0 + threadIdx.x; gi < c1; gi += blockDim.x) {
+    const DevPart &P = parts[find_part(parts, nparts, gi)];
+    const int nt = particle_tiles<true>(g, P, gi, W, tiles, rec, flags);
+    if (nt) wmax = fmaxf(wmax, fabsf(rec.w));
+    for (int k = 0; k < nt; k++) {
+      const uint32_t slot = P.mock == mock0
+                                ? atomicAdd(&hist[tiles[k]], 1u)
+                                : atomicAdd(&cursors[(uint32_t)P.mock * tpm + tiles[k]], 1u);
+      if (slot < rec_cap) recs[slot] = rec;
+      else atomicOr(flags, 1);
+    }
+  }
+  publish_wmax(wmax, flags);
 }
 
 // ---- exclusive scan of uint32 counters (3 kernels) -------------------------
@@ -620,10 +702,25 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
   if (pblocks > 16384) pblocks = 16384;
   if (pblocks == 0) pblocks = 1;
   Rec *recs = reinterpret_cast<Rec *>(records);
+  const size_t tpm = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  const bool lds = tpm <= (size_t)kLdsTiles;
+  const unsigned cblocks = (unsigned)((ntotal + kChunk - 1) / kChunk);
+  const size_t lds_bytes = sizeof(uint32_t) * tpm;
+  if (lds && lds_bytes > 64 * 1024) {
+    (void)hipFuncSetAttribute((const void *)assign_bucket_lds<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    (void)hipFuncSetAttribute((const void *)assign_bucket_lds<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+  }
   if (ntotal > 0) {
-    hipLaunchKernelGGL((assign_bucket<false>), dim3(pblocks), dim3(256), 0, s, g,
-                       d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
-                       d_flags);
+    if (lds)
+      hipLaunchKernelGGL((assign_bucket_lds<false>), dim3(cblocks), dim3(kLdsThreads),
+                         lds_bytes, s, g, d_parts, nparts, ntotal, counts, cursors, recs,
+                         rec_cap, d_flags);
+    else
+      hipLaunchKernelGGL((assign_bucket<false>), dim3(pblocks), dim3(256), 0, s, g,
+                         d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
+                         d_flags);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(scan_local, dim3(nsb), dim3(kScanThreads), 0, s, counts,
@@ -633,9 +730,14 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
                      cursors, d_scan_tmp, ntiles, nsb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (ntotal > 0) {
-    hipLaunchKernelGGL((assign_bucket<true>), dim3(pblocks), dim3(256), 0, s, g,
-                       d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
-                       d_flags);
+    if (lds)
+      hipLaunchKernelGGL((assign_bucket_lds<true>), dim3(cblocks), dim3(kLdsThreads),
+                         lds_bytes, s, g, d_parts, nparts, ntotal, counts, cursors, recs,
+                         rec_cap, d_flags);
+    else
+      hipLaunchKernelGGL((assign_bucket<true>), dim3(pblocks), dim3(256), 0, s, g,
+                         d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
+                         d_flags);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (ev_deposit) (void)hipEventRecord(ev_deposit, s);

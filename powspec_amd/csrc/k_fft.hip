@@ -317,18 +317,26 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
-// x pass: 1024 threads (16 waves, 128 VGPRs).  The plain kernel takes
+// x pass: up to 1024 threads (16 waves, 128 VGPRs).  The plain kernel takes
 // 16-column tiles (128-byte row segments) and 16 values per thread.  The
-// interlaced kernel transforms both fields pass by pass together; to stay
-// within the registers it takes 8-column tiles (8 values per thread and
-// field), whose 64-byte half lines are completed by the neighbouring tile on
-// the same XCD (xcd_tile).  With T a multiple of C, a thread works on the
-// same column (kz) in every pass and in the binning loop.
+// interlaced kernel transforms both fields pass by pass together with 8
+// values per thread and field, which bounds its tile at 1024 threads x 8 / N
+// columns: 16 up to N = 512, 8 above (64-byte half lines, completed by the
+// neighbouring tile on the same XCD, see xcd_tile).  With T a multiple of C,
+// a thread works on the same column (kz) in every pass and in the binning
+// loop.
 template <int N, bool IL>
 struct XCfg {
-  static constexpr int C = (IL || N >= 2048) ? 8 : 16;
-  static constexpr int T0 = C * N / (IL ? 8 : 16);
+  static constexpr int C = IL ? (N <= 512 ? 16 : 8) : (N >= 2048 ? 8 : 16);
+  // up to N = 256 the first pass's radix-16 butterflies set the values per
+  // thread whatever T is: T = its butterfly count leaves no thread idle
+  static constexpr bool SMALL = IL && N <= 256;
+  static constexpr int T0 = SMALL ? C * N / radix(N, 0) : C * N / (IL ? 8 : 16);
   static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+  // waves per SIMD the register allocation must allow: 4 (128 VGPRs) lets
+  // workgroups share a CU and hide each other's barriers; the small
+  // interlaced kernel holds 2 x 16 values per thread and gets 2 (256 VGPRs)
+  static constexpr int WAVES = SMALL ? 2 : 4;
   static_assert(T % C == 0, "a thread keeps one column");
 };
 
@@ -377,7 +385,7 @@ __device__ __forceinline__ int bin_exact(const PshBins &b, double ksqx, double k
 //     (src/multipole.c:147-177, src/mp_template.c:107-139).
 // xtab[i] = {alias_x, ksq_x, phase_x.re, phase_x.im} in LDS.
 template <int N, int EPI, bool INTLACE>
-__global__ void __launch_bounds__((XCfg<N, INTLACE>::T))
+__global__ void __launch_bounds__((XCfg<N, INTLACE>::T), (XCfg<N, INTLACE>::WAVES))
 x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   constexpr int C = XCfg<N, INTLACE>::C, T = XCfg<N, INTLACE>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();

@@ -63,7 +63,13 @@ def run_c2(P, steps, warmup):
     data = dev.to_numpy((n, 4))   # host copy for the CPU baseline and the mocks
     log(f"C2 catalogue: {n} particles generated in {time.time() - t:.1f} s")
     conf = c2_conf(P)
+    # deposit tiles tuned on this catalogue before the timed region (a planning
+    # step: candidates timed on the same data; results do not depend on it)
+    t = time.time()
+    tile, tms, ncand = P.autotune(P.make_conf(**conf), nmock=1, data=dev, offsets=[0, n])
+    log(f"C2 tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
     ctx = P.Context(P.make_conf(**conf))
+    ctx.set_tiles(tile)
     L = P.load()
     for _ in range(warmup):
         res = ctx.run_device(dev, n, float(n))
@@ -86,7 +92,7 @@ def run_c2(P, steps, warmup):
               "deposit": ms[4], "z_r2c": ms[5], "y_fft": ms[6], "bucket": ms[7]}
     log("C2 stages (ms, mean over the timed steps): " +
         ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
-    return data, res, ms_step, stages, n
+    return data, res, ms_step, stages, n, tile
 
 
 KERNELS = {"deposit": "assign_deposit", "z_r2c": "z_r2c_kernel", "y_fft": "col_kernel",
@@ -164,7 +170,12 @@ def run_batched(P, data, nmock, seed=1000):
     log(f"batched mocks: {nmock} mocks, {len(allp)} particles")
     dev = P.DeviceArray.from_numpy(allp)
     kw = dict(gsize=256, assign=P.TSC, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
+    t = time.time()
+    tile, tms, ncand = P.autotune(P.make_conf(**kw), nmock=nmock, data=dev, offsets=off)
+    log(f"batched tiles: {tile} ({tms:.1f} ms per batch, {ncand} candidates, "
+        f"{time.time() - t:.1f} s)")
     ctx = P.Context(P.make_conf(**kw), nmock_max=nmock)
+    ctx.set_tiles(tile)
     L = P.load()
     ws = [float(s) for s in sizes]
     ctx.run_batch(dev, off, ws, graph=True)           # capture + warm-up
@@ -177,6 +188,7 @@ def run_batched(P, data, nmock, seed=1000):
     t_batch = (time.perf_counter() - t0) / reps
     # plain: one mock per launch sequence
     single = P.Context(P.make_conf(**kw))
+    single.set_tiles(tile)
     ptr0 = dev.ptr.value
 
     class View:  # device pointer into the concatenated buffer
@@ -198,6 +210,7 @@ def run_batched(P, data, nmock, seed=1000):
             "batched_meshes_per_s": round(nmock / t_batch, 1),
             "plain_meshes_per_s": round(nmock / t_plain, 1),
             "batched_over_plain": round(t_plain / t_batch, 2),
+            "tile": list(tile),
             "batched_particles_per_s": round(len(allp) / t_batch, 1)}
 
 
@@ -252,7 +265,7 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     arch = P.device_arch()
-    data, res, ms_step, stages, n = run_c2(P, args.steps, args.warmup)
+    data, res, ms_step, stages, n, tile = run_c2(P, args.steps, args.warmup)
     rl = roofline(stages, 1024, 2, n)
     extra = {}
     if not args.no_batched and args.nmock > 0:
@@ -275,7 +288,8 @@ def main():
                                "nbar=0.02 (h/Mpc)^3, Zel'dovich RSD f=0.8; PCS, N_g=1024, "
                                "interlaced, l=0,2,4, dk=0.005, L=1000 Mpc/h",
                    "N_g": 1024, "N_p": n, "assign": "PCS", "interlaced": True,
-                   "precision": "FP32 meshes/FFT, FP64 bin sums", "arch": arch},
+                   "precision": "FP32 meshes/FFT, FP64 bin sums", "arch": arch,
+                   "tile": list(tile)},
         "particles_per_s": round(n / (ms_step * 1e-3), 1),
         "stage_ms": {k: round(float(v), 4) for k, v in stages.items()},
         "roofline": rl,

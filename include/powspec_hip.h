@@ -298,6 +298,38 @@ int psh_device_synchronize(void);
 /* Name of the device architecture (e.g. "gfx950"). */
 int psh_device_arch(char *buf, int len);
 
+/* ---- tuning ----------------------------------------------------------------- */
+/* No counterpart in the reference (its FFTW plans use FFTW_ESTIMATE): the
+ * deposit tile shape sets the bucketing duplication, the deposit's LDS
+ * footprint and which bucketing kernel runs; results do not depend on it
+ * (integer accumulation). */
+
+/* Tiles of a context (cells per axis, powers of two, z >= 4, both grids in
+ * 64-bit fixed point within 160 KB of LDS); overrides the default 8x16x32
+ * and any cache.  PSH_ERR_ARG when invalid. */
+int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
+
+/* Applies the fastest entry of a tuning cache for the context's device
+ * architecture, grid size, scheme, interlacing and batch size (rounded down
+ * to a power of two).  PSH_ERR_FILE / PSH_ERR_ARG when none applies. */
+int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path);
+
+typedef struct {
+  int tile[3];     /* fastest shape                          */
+  double ms;       /* its median time per run                */
+  int ncand;       /* candidates timed                       */
+} psh_tune;
+
+/* Times candidate tile shapes (2^11..2^14 cells) for the grid size, scheme
+ * and interlacing of conf on nmock mocks (batched when > 1): the caller's
+ * device catalogues d_data with offsets[nmock + 1] (unit weights assumed for
+ * the timing runs), or, when d_data is NULL, a uniform box of npart
+ * particles per mock generated on the device.  Appends "arch Ng scheme
+ * intlace nmock npart tx ty tz ms" to cache_path (when not NULL) and returns
+ * the winner. */
+int psh_autotune(const psh_conf *conf, const psh_data *d_data, const size_t *offsets,
+    size_t npart, int nmock, const char *cache_path, psh_tune *out);
+
 #ifdef __cplusplus
 }
 #endif

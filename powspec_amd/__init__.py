@@ -67,8 +67,32 @@ EXPORTS = [
     "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
-    "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky",
+    "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
+    "psh_ctx_load_tuning", "psh_autotune",
 ]
+
+
+class Tune(C.Structure):
+    """psh_tune: result of psh_autotune."""
+
+    _fields_ = [("tile", C.c_int * 3), ("ms", C.c_double), ("ncand", C.c_int)]
+
+
+def autotune(conf, npart=0, nmock=1, cache_path=None, data=None, offsets=None):
+    """Time candidate deposit tiles for conf's grid/scheme/interlacing on the
+    device catalogues `data` (DeviceArray) split by `offsets`, or on a
+    generated uniform box of npart per mock; returns (tile, ms, candidates)
+    and appends the winner to cache_path."""
+    t = Tune()
+    off = None
+    if data is not None:
+        off = (C.c_size_t * (nmock + 1))(*[int(o) for o in offsets])
+    rc = load().psh_autotune(C.byref(conf), data.ptr if data is not None else None, off,
+                             C.c_size_t(int(npart)), int(nmock),
+                             cache_path.encode() if cache_path else None, C.byref(t))
+    if rc:
+        raise PshError(rc, "psh_autotune failed")
+    return tuple(t.tile), t.ms, t.ncand
 
 
 
@@ -312,6 +336,12 @@ class Context:
         t = (C.c_double * 8)()
         self._check(self.L.psh_ctx_timings(self.h, t))
         return list(t)
+
+    def set_tiles(self, tile):
+        self._check(self.L.psh_ctx_set_tiles(self.h, (C.c_int * 3)(*tile)))
+
+    def load_tuning(self, cache_path):
+        self._check(self.L.psh_ctx_load_tuning(self.h, str(cache_path).encode()))
 
     def set_box(self, bsize, bmin=(0.0, 0.0, 0.0)):
         b = (C.c_double * 3)(*np.broadcast_to(np.asarray(bsize, float), (3,)))

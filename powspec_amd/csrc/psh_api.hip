@@ -16,6 +16,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -55,6 +57,7 @@ struct psh_ctx {
   int ellmax = 0;
 
   float *meshA = nullptr, *meshB = nullptr, *meshS = nullptr;
+  int tile_req[3] = {0, 0, 0};   // deposit tiles set by psh_ctx_set_tiles / tuning
 
   // assignment scratch
   PshAssignGeom ag{};
@@ -152,6 +155,16 @@ int set_err(psh_ctx *ctx, int code, const char *fmt, ...) {
   } while (0)
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Deposit tile shapes the kernels accept: powers of two, rows of at least 4
+// cells along z, LDS of both grids within 160 KB at `cell_bytes` per cell
+// (4: 32-bit fixed point of boxes, 8: 64-bit of surveys).
+bool tiles_ok(const int t[3], int intlace, int cell_bytes) {
+  for (int a = 0; a < 3; a++)
+    if (!is_pow2(t[a]) || t[a] > 2048) return false;
+  return t[2] >= 4 &&
+         (size_t)t[0] * t[1] * t[2] * cell_bytes * (intlace ? 2 : 1) <= 160 * 1024;
+}
 
 // src/multipole.c:46-100: per-axis alias correction, series branch for
 // pik <= 0.01 (which includes every negative frequency).
@@ -397,15 +410,13 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4)
   int tl[3] = {8, 16, 32};
   if (const char *env = getenv("PSH_TILE")) {
-    int a = 0, b = 0, cz = 0;
-    if (sscanf(env, "%d,%d,%d", &a, &b, &cz) == 3 && is_pow2(a) && is_pow2(b) &&
-        is_pow2(cz) && cz >= 4 &&
-        (size_t)a * b * cz * 4 * (ctx->conf.intlace ? 2 : 1) <= 64 * 1024) {
-      tl[0] = a;
-      tl[1] = b;
-      tl[2] = cz;
-    }
+    int t[3] = {0, 0, 0};
+    if (sscanf(env, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 &&
+        tiles_ok(t, ctx->conf.intlace, ctx->survey ? 8 : 4))
+      for (int a = 0; a < 3; a++) tl[a] = t[a];
   }
+  if (ctx->tile_req[0])
+    for (int a = 0; a < 3; a++) tl[a] = ctx->tile_req[a];
   for (int a = 0; a < 3; a++) g.tile[a] = ctx->Ng < tl[a] ? ctx->Ng : tl[a];
   for (int a = 0; a < 3; a++) g.ntile[a] = ctx->Ng / g.tile[a];
   g.row_pitch = 2 * ctx->H;
@@ -562,6 +573,10 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
     ntotal += p.n;
     if (p.wscale < 0.f) ctx->ag.wide = 1;   // data - alpha * randoms
   }
+  if (ctx->ag.wide && !tiles_ok(ctx->ag.tile, ctx->conf.intlace, 8))
+    return set_err(ctx, PSH_ERR_UNSUPPORTED,
+                   "tile %dx%dx%d too large for 64-bit accumulation of weights of both signs",
+                   ctx->ag.tile[0], ctx->ag.tile[1], ctx->ag.tile[2]);
   int rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, (int)hp.size(), ntotal))) return rc;
   for (int attempt = 0; attempt < 4; attempt++) {
@@ -1453,5 +1468,168 @@ extern "C" int psh_device_arch(char *buf, int len) {
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
     return PSH_ERR_DEVICE;
   snprintf(buf, len, "%s", p.gcnArchName);
+  return PSH_OK;
+}
+
+// ===========================================================================
+// deposit tile tuning
+// ===========================================================================
+
+extern "C" int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]) {
+  if (!ctx || !tile) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (!tiles_ok(tile, ctx->conf.intlace, ctx->survey ? 8 : 4))
+    return set_err(ctx, PSH_ERR_ARG, "invalid tile shape %dx%dx%d", tile[0], tile[1], tile[2]);
+  for (int a = 0; a < 3; a++) ctx->tile_req[a] = tile[a];
+  if (ctx->box_set) {   // re-derive the assignment geometry with the new tiles
+    double L[3] = {ctx->L[0], ctx->L[1], ctx->L[2]};
+    double bmin[3] = {ctx->bmin[0], ctx->bmin[1], ctx->bmin[2]};
+    return set_box(ctx, L, bmin);
+  }
+  return PSH_OK;
+}
+
+namespace {
+
+std::string arch_name() {
+  hipDeviceProp_t p;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
+    return "unknown";
+  std::string a = p.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+// nmock class of a cache entry: 1, or the batch size rounded down to a power of 2
+int nmock_class(int nmock) {
+  int c = 1;
+  while (c * 2 <= nmock) c *= 2;
+  return c;
+}
+
+}  // namespace
+
+extern "C" int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path) {
+  if (!ctx || !cache_path) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  FILE *fp = fopen(cache_path, "r");
+  if (!fp) return set_err(ctx, PSH_ERR_FILE, "cannot open tuning cache %s", cache_path);
+  const std::string arch = arch_name();
+  const int cls = nmock_class(ctx->nmock_max);
+  char line[512];
+  int best[3] = {0, 0, 0};
+  double best_ms = 1e300;
+  while (fgets(line, sizeof line, fp)) {
+    char a[64];
+    int Ng, scheme, il, nm, t[3];
+    double npart, ms;
+    if (line[0] == '#' ||
+        sscanf(line, "%63s %d %d %d %d %lf %d %d %d %lf", a, &Ng, &scheme, &il, &nm, &npart,
+               &t[0], &t[1], &t[2], &ms) != 10)
+      continue;
+    if (arch != a || Ng != ctx->Ng || scheme != ctx->conf.assign ||
+        il != ctx->conf.intlace || nm != cls || !tiles_ok(t, il, ctx->survey ? 8 : 4))
+      continue;
+    if (ms < best_ms) {   // the latest measurement of equal speed wins ties
+      best_ms = ms;
+      for (int k = 0; k < 3; k++) best[k] = t[k];
+    }
+  }
+  fclose(fp);
+  if (!best[0]) return set_err(ctx, PSH_ERR_ARG, "no tuning entry for this configuration");
+  return psh_ctx_set_tiles(ctx, best);
+}
+
+extern "C" int psh_autotune(const psh_conf *conf, const psh_data *d_data, const size_t *offsets,
+                            size_t npart, int nmock, const char *cache_path, psh_tune *out) {
+  if (!conf || !out || nmock < 1 || (!d_data && npart == 0) || (d_data && !offsets))
+    return PSH_ERR_ARG;
+  memset(out, 0, sizeof *out);
+  // tiles are geometry only: tune on a uniform periodic box of the same size
+  psh_conf c = *conf;
+  c.issim = 1;
+  c.ndata = 1;
+  c.iscross = 0;
+  c.isauto[0] = 1;
+  c.isauto[1] = 0;
+  if (!c.has_bsize) {
+    c.has_bsize = 1;
+    c.bsize[0] = c.bsize[1] = c.bsize[2] = 1000.0;
+  }
+  psh_ctx *ctx = nullptr;
+  int rc = psh_ctx_create(&c, nmock, &ctx);
+  if (rc) return rc;
+  std::vector<size_t> off(nmock + 1);
+  std::vector<double> ws(nmock);
+  psh_data *d = nullptr;
+  const psh_data *src = d_data;
+  if (d_data) {   // the caller's catalogues (unit weights assumed for timing)
+    for (int m = 0; m <= nmock; m++) off[m] = offsets[m];
+    for (int m = 0; m < nmock; m++) ws[m] = (double)(off[m + 1] - off[m]);
+  } else {
+    const size_t ntot = npart * (size_t)nmock;
+    if (hipMalloc(&d, sizeof(psh_data) * ntot) != hipSuccess) {
+      psh_ctx_destroy(ctx);
+      return PSH_ERR_MEMORY;
+    }
+    rc = psh_gen_uniform_box(d, ntot, c.bsize, 12345);
+    for (int m = 0; m <= nmock; m++) off[m] = (size_t)m * npart;
+    for (int m = 0; m < nmock; m++) ws[m] = (double)npart;
+    src = d;
+  }
+  const int nb = rc ? 0 : psh_ctx_nbin(ctx);
+  std::vector<double> k(nb), ke(nb + 1), km(nb), lc((size_t)nb * c.npole), pl((size_t)nb * c.npole);
+  std::vector<uint64_t> cnt(nb);
+  std::vector<psh_pk> pk(nmock);
+  for (int m = 0; m < nmock; m++) {
+    pk[m] = psh_pk{};
+    pk[m].k = k.data();
+    pk[m].kedge = ke.data();
+    pk[m].km = km.data();
+    pk[m].cnt = cnt.data();
+    pk[m].lcnt = lc.data();
+    pk[m].pl[0] = pl.data();
+  }
+  // candidate shapes: 2^11..2^14 cells, z rows of 16..128 cells
+  const int opts[4] = {4, 8, 16, 32}, zopts[4] = {16, 32, 64, 128};
+  double best_ms = 1e300;
+  for (int a = 0; a < 4 && !rc; a++)
+    for (int b = 0; b < 4 && !rc; b++)
+      for (int z = 0; z < 4 && !rc; z++) {
+        const int t[3] = {opts[a], opts[b], zopts[z]};
+        const size_t cells = (size_t)t[0] * t[1] * t[2];
+        if (cells < 2048 || cells > 16384 || t[0] > t[1] || t[0] > c.gsize ||
+            t[1] > c.gsize || t[2] > c.gsize || !tiles_ok(t, c.intlace, conf->issim ? 4 : 8))
+          continue;
+        if ((rc = psh_ctx_set_tiles(ctx, t))) break;
+        double times[4];
+        for (int r = 0; r < 4 && !rc; r++) {
+          const auto t0 = std::chrono::steady_clock::now();
+          rc = nmock == 1
+                   ? psh_ctx_run_device(ctx, src + off[0], off[1] - off[0], ws[0], &pk[0])
+                   : psh_ctx_run_batch(ctx, src, off.data(), ws.data(), nmock, 0, pk.data());
+          times[r] = std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - t0).count();
+        }
+        if (rc) break;
+        std::sort(times + 1, times + 4);   // the first run warms up
+        const double ms = times[2];
+        out->ncand++;
+        if (ms < best_ms) {
+          best_ms = ms;
+          for (int q = 0; q < 3; q++) out->tile[q] = t[q];
+        }
+      }
+  if (d) (void)hipFree(d);
+  psh_ctx_destroy(ctx);
+  if (rc) return rc;
+  out->ms = best_ms;
+  if (cache_path) {
+    FILE *fp = fopen(cache_path, "a");
+    if (!fp) return PSH_ERR_FILE;
+    fprintf(fp, "%s %d %d %d %d %.0f %d %d %d %.4f\n", arch_name().c_str(), c.gsize, c.assign,
+            c.intlace, nmock_class(nmock), (double)(off[nmock] - off[0]) / nmock, out->tile[0],
+            out->tile[1], out->tile[2], best_ms);
+    if (fclose(fp)) return PSH_ERR_FILE;
+  }
   return PSH_OK;
 }

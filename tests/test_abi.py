@@ -50,6 +50,8 @@ int main(void) {
          PSH_ERR_UNSUPPORTED);
   printf("psh_cnvt %zu\n", sizeof(psh_cnvt));
   printf("psh_box_spec %zu\npsh_sky_spec %zu\n", sizeof(psh_box_spec), sizeof(psh_sky_spec));
+  printf("psh_tune %zu\n", sizeof(psh_tune));
+  F(psh_tune, ms); F(psh_tune, ncand);
   F(psh_box_spec, seed); F(psh_sky_spec, nbar0); F(psh_sky_spec, seed);
   F(psh_cnvt, zsp); F(psh_cnvt, nsp); F(psh_cnvt, rcnvt);
   printf("psh_mesh_meta %zu\n", sizeof(psh_mesh_meta));
@@ -80,7 +82,8 @@ def test_struct_layout_matches_header():
     assert v["psh_pk"][0] == C.sizeof(P.Pk)
     for tname, cls in (("psh_cata", P.Cata), ("psh_conf", P.Conf), ("psh_pk", P.Pk),
                        ("psh_mesh_meta", P.MeshMeta), ("psh_cnvt", P.Cnvt),
-                       ("psh_box_spec", P.BoxSpec), ("psh_sky_spec", P.SkySpec)):
+                       ("psh_box_spec", P.BoxSpec), ("psh_sky_spec", P.SkySpec),
+                       ("psh_tune", P.Tune)):
         for key, (off,) in ((k, o) for k, o in v.items() if k.startswith(tname + ".")):
             field = key.split(".", 1)[1]
             assert getattr(cls, field).offset == off, key
@@ -89,6 +92,7 @@ def test_struct_layout_matches_header():
     assert v["psh_cnvt"][0] == C.sizeof(P.Cnvt)
     assert v["psh_box_spec"][0] == C.sizeof(P.BoxSpec)
     assert v["psh_sky_spec"][0] == C.sizeof(P.SkySpec)
+    assert v["psh_tune"][0] == C.sizeof(P.Tune)
     assert v["psh_mesh_meta"][0] == C.sizeof(P.MeshMeta)
 
 

@@ -1,0 +1,51 @@
+"""Deposit tile tuning: every tile shape gives the oracle's spectra (to the
+same tolerance as the default), the autotuner picks a valid shape and its
+cache entry is found again by psh_ctx_load_tuning."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import _check, _uniform
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("tile", [(4, 4, 128), (8, 8, 64), (16, 16, 16), (32, 32, 16),
+                                  (4, 32, 64)])
+def test_tile_shapes_keep_parity(psh, oracle, tile):
+    Ng, L, n = 64, 500.0, 60000
+    d = _uniform(n, [L] * 3, 51, weights=True)
+    kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.02, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    ctx.set_tiles(tile)
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, d[:, 3].sum())
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    _check(got, ref, [0, 2, 4], ref["shot"][0])
+
+
+def test_invalid_tiles_rejected(psh):
+    ctx = psh.Context(psh.make_conf(gsize=64, bsize=500.0, kbin=0.02, intlace=True))
+    for bad in [(3, 8, 32), (8, 8, 2), (64, 64, 64)]:
+        with pytest.raises(psh.PshError) as e:
+            ctx.set_tiles(bad)
+        assert e.value.code == psh.ERR_ARG
+
+
+def test_autotune_and_cache(psh, tmp_path):
+    cache = tmp_path / "tune.txt"
+    conf = psh.make_conf(gsize=64, assign=2, intlace=True, bsize=500.0, kbin=0.02)
+    tile, ms, ncand = psh.autotune(conf, 100000, 1, str(cache))
+    d0 = psh.DeviceArray.from_numpy(_uniform(50000, [500.0] * 3, 53))
+    t2, ms2, n2 = psh.autotune(conf, nmock=1, data=d0, offsets=[0, 50000])
+    assert n2 == ncand and ms2 > 0
+    assert ncand >= 8 and ms > 0
+    assert 2048 <= tile[0] * tile[1] * tile[2] <= 16384
+    line = cache.read_text().split()
+    assert line[0].startswith("gfx950") and [int(v) for v in line[6:9]] == list(tile)
+    ctx = psh.Context(conf)
+    ctx.load_tuning(str(cache))
+    d = _uniform(20000, [500.0] * 3, 52)
+    res = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
+    assert res["nbin"] > 0
+    other = psh.Context(psh.make_conf(gsize=32, assign=2, intlace=True, bsize=500.0, kbin=0.02))
+    with pytest.raises(psh.PshError):
+        other.load_tuning(str(cache))     # no entry for Ng = 32

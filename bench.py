@@ -214,6 +214,52 @@ def run_batched(P, data, nmock, seed=1000):
             "batched_particles_per_s": round(len(allp) / t_batch, 1)}
 
 
+def run_survey(P, steps=3):
+    """Survey workload (C4 recipe): sky cap RA in [0, 60) deg, Dec in [-30, 30)
+    deg (uniform sin Dec), redshift from a Gaussian n(z) (0.55, 0.08)
+    truncated to [0.4, 0.7], 5e6 data + 1e8 randoms generated on the GPU,
+    w_fkp = 1/(1 + n(z) 1e4), converted to comoving coordinates on the GPU
+    (flat LCDM, Om = 0.31), CIC, N_g = 512, interlaced, l = 0,2,4, dk = 0.01.
+    Timed: the survey estimator with both catalogues in HBM."""
+    t = time.time()
+    nd, nr = 5_000_000, 100_000_000
+    kw = dict(ra=(0.0, 60.0), dec=(-30.0, 30.0), zmu=0.55, zsigma=0.08, zlo=0.4, zhi=0.7,
+              nbar0=5e-3, p0=1e4)
+    dd, dnz = P.gen_sky(nd, 4001, **kw)
+    rr, rnz = P.gen_sky(nr, 4002, **kw)
+    cv = P.make_cnvt(omega_m=0.31)
+    order = P.cnvt_device(cv, [(dd, nd), (rr, nr)])
+    P.load().psh_device_synchronize()
+    t_gen = time.time() - t
+    # catalogue sums (src/read_cata.c:165-183) with completeness weights 1
+    wd = dd.to_numpy((nd, 4))[:, 3]
+    wr = rr.to_numpy((nr, 4))[:, 3]
+    nzd, nzr = dnz.to_numpy((nd,)), rnz.to_numpy((nr,))
+    alpha = nd / nr
+    shot = float(np.sum(wd * wd)) + alpha * alpha * float(np.sum(wr * wr))
+    norm = alpha * float(np.sum(wr * wr * nzr))
+    del wd, wr, nzd, nzr
+    conf = P.make_conf(gsize=512, assign=P.CIC, intlace=True, issim=False, kbin=0.01,
+                       poles=[0, 2, 4])
+    ctx = P.Context(conf)
+    res = ctx.run_survey_device(dd, nd, rr, nr, alpha, shot, norm)   # warm-up
+    L = P.load()
+    L.psh_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = ctx.run_survey_device(dd, nd, rr, nr, alpha, shot, norm)
+    L.psh_device_synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    log(f"survey: {ms:.2f} ms per P_l evaluation (generation + conversion {t_gen:.1f} s, "
+        f"Gauss-Legendre order {order}), nbin {res['nbin']}")
+    return {"workload": "survey sky cap 60x60 deg, z in [0.4,0.7] Gaussian n(z), 5e6 data + "
+                        "1e8 randoms, GPU coordinate conversion, CIC N_g=512 interlaced, "
+                        "l=0,2,4, dk=0.01",
+            "ms_per_evaluation": round(ms, 3), "evaluations_per_s": round(1e3 / ms, 3),
+            "particles_per_s": round((nd + nr) / (ms * 1e-3), 1), "nbin": res["nbin"],
+            "fft_transforms": 2 + (5 + 9)}
+
+
 def cpu_baseline(c2_data):
     """FP64 oracle (OpenMP, own FFT) on a bounded sample of the C2 workload:
     N_g = 512 (1/8 of the cells) and every 8th particle (same particles per
@@ -249,6 +295,7 @@ def main():
     ap.add_argument("--nmock", type=int, default=256)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -270,6 +317,7 @@ def main():
     extra = {}
     if not args.no_batched and args.nmock > 0:
         extra = run_batched(P, data, args.nmock)
+    survey = None if args.no_survey else run_survey(P)
     cpu = None if args.no_cpu else cpu_baseline(data)
     line = {
         "metric": METRIC,
@@ -294,6 +342,7 @@ def main():
         "stage_ms": {k: round(float(v), 4) for k, v in stages.items()},
         "roofline": rl,
         "batched": extra or None,
+        "survey": survey,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -273,6 +273,14 @@ int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     const size_t *offsets, const double *weight_sum, int nmock, int graph,
     psh_pk *out);
 
+/* A survey (one data/random pair) with both catalogues in device memory:
+ * box from their coordinate range (src/genr_mesh.c:427-578), then the
+ * survey estimator of psh_powspec.  alpha, shot and norm are the
+ * catalogue sums of src/read_cata.c:165-183 (see psh_cata). */
+int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, size_t nd,
+    const psh_data *d_rand, size_t nr, double alpha, double shot, double norm,
+    psh_pk *out);
+
 /* Per-stage device time of the last run, milliseconds, measured with HIP
  * events on the context's stream (profiling on): [0] assignment incl. the
  * coordinate-bounds pass, [1] z and y FFT passes, [2] x FFT pass fused with

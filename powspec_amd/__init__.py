@@ -68,7 +68,7 @@ EXPORTS = [
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
-    "psh_ctx_load_tuning", "psh_autotune",
+    "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device",
 ]
 
 
@@ -357,6 +357,26 @@ class Context:
         self._check(self.L.psh_ctx_run_device(self.h, d_data.ptr, n, float(weight_sum),
                                               C.byref(arr.pk)))
         return arr.result()
+
+    def run_survey_device(self, d_data: DeviceArray, nd: int, d_rand: DeviceArray, nr: int,
+                          alpha: float, shot: float, norm: float):
+        nb = None
+        arr = None
+        # the box (and so nbin) is known only after the bounds pass: size the
+        # output generously, then trim
+        cap = 1 << 14
+        arr = _PkArrays(cap, self.conf.npole)
+        self._check(self.L.psh_ctx_run_survey_device(
+            self.h, d_data.ptr, C.c_size_t(nd), d_rand.ptr, C.c_size_t(nr), C.c_double(alpha),
+            C.c_double(shot), C.c_double(norm), C.byref(arr.pk)))
+        nb = self.nbin()
+        res = arr.result()
+        out = {"nbin": nb, "k": res["k"][:nb], "kedge": res["kedge"][:nb + 1],
+               "km": res["km"][:nb], "cnt": res["cnt"][:nb]}
+        flat = lambda a: a.ravel()[: self.conf.npole * nb].reshape(self.conf.npole, nb)
+        out["pl"] = [flat(p) for p in res["pl"]]
+        out["xpl"] = flat(res["xpl"])
+        return out
 
     def run_batch(self, d_data: DeviceArray, offsets, weight_sums, graph=False):
         nm = len(offsets) - 1

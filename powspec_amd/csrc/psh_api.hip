@@ -1471,6 +1471,32 @@ extern "C" int psh_device_arch(char *buf, int len) {
   return PSH_OK;
 }
 
+extern "C" int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, size_t nd,
+                                         const psh_data *d_rand, size_t nr, double alpha,
+                                         double shot, double norm, psh_pk *out) {
+  if (!ctx || !out || (nd && !d_data) || (nr && !d_rand))
+    return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (!ctx->survey) return set_err(ctx, PSH_ERR_ARG, "psh_ctx_run_survey_device is for surveys");
+  if (ctx->conf.ndata != 1 || ctx->conf.iscross)
+    return set_err(ctx, PSH_ERR_ARG, "one catalogue pair per call");
+  int rc;
+  // box from the coordinate range of data and randoms (src/genr_mesh.c:427-578)
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  if ((rc = bounds_device(ctx, (const double *)d_data, nd, mn, mx)) ||
+      (rc = bounds_device(ctx, (const double *)d_rand, nr, mn, mx)) ||
+      (rc = def_box(ctx, mn, mx)) || (rc = ensure_geometry(ctx)))
+    return rc;
+  CatDev cd[2] = {};
+  cd[0].data = (const double *)d_data;
+  cd[0].nd = nd;
+  cd[0].rand = (const double *)d_rand;
+  cd[0].nr = nr;
+  cd[0].alpha = alpha;
+  cd[0].shot = shot;
+  cd[0].norm = norm;
+  return run_catalogues(ctx, cd, 1, out);
+}
+
 // ===========================================================================
 // deposit tile tuning
 // ===========================================================================

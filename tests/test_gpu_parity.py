@@ -392,3 +392,19 @@ def test_save_res_dropin(psh, tmp_path):
     assert np.allclose(tab[:, 0], res["k"], rtol=1e-9)
     assert np.array_equal(tab[:, 4].astype(np.uint64), res["cnt"])
     assert np.allclose(tab[:, 5], res["pl"][0][0], rtol=1e-9, atol=1e-300)
+
+
+def test_survey_device_entry(psh, oracle):
+    """psh_ctx_run_survey_device (catalogues in HBM) gives the oracle's survey
+    spectra, like the host entry point."""
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(100000, 61)
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    d = np.column_stack([dp, wd])
+    r = np.column_stack([rp, wr])
+    kw = dict(gsize=32, assign=2, intlace=True, issim=False, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_survey_device(psh.DeviceArray.from_numpy(d), len(d),
+                                psh.DeviceArray.from_numpy(r), len(r),
+                                sums["alpha"], sums["shot"], sums["norm"])
+    ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
+    _check(got, ref, [0, 2, 4], sums["shot"], sums["norm"])

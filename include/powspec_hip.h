@@ -306,6 +306,25 @@ int psh_device_synchronize(void);
 /* Name of the device architecture (e.g. "gfx950"). */
 int psh_device_arch(char *buf, int len);
 
+/* ---- learned alias correction (no counterpart in the reference) ------------ */
+/* A perceptron with inputs padded to 32, three hidden layers of 64 (ReLU) and
+ * one output, trained and evaluated with bf16 matrix-core products and FP32
+ * accumulation.  Parameters: one FP32 device array of psh_mlp_param_count()
+ * values, W1[32][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][16]
+ * b4[16] (only output column 0 is used; padding entries stay zero when
+ * initialised to zero).  Samples: X[B][32] FP32 rows, targets T[B]. */
+int psh_mlp_param_count(void);
+/* Y[s] = network output of row s. */
+int psh_mlp_forward(const float *d_params, const float *d_X, int B, float *d_Y);
+/* Gradient of loss = mean (y - t)^2 over the B samples into d_grads
+ * (overwritten); *loss (host, may be NULL) receives the loss. */
+int psh_mlp_grad(const float *d_params, const float *d_X, const float *d_T, int B,
+    float *d_grads, float *loss);
+/* Adam (beta 0.9, 0.999) over `epochs` passes of contiguous mini-batches of
+ * `batch` samples; *loss_out receives the final loss over all samples. */
+int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, int batch,
+    int epochs, float lr, unsigned seed, float *loss_out);
+
 /* ---- tuning ----------------------------------------------------------------- */
 /* No counterpart in the reference (its FFTW plans use FFTW_ESTIMATE): the
  * deposit tile shape sets the bucketing duplication, the deposit's LDS

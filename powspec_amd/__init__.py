@@ -68,7 +68,8 @@ EXPORTS = [
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
-    "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device",
+    "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device", "psh_mlp_param_count",
+    "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train",
 ]
 
 

@@ -1,0 +1,257 @@
+// k_mlp.hip -- the learned alias correction: a 4-layer perceptron (inputs
+// padded to 32, three hidden layers of 64 with ReLU, one output) trained and
+// evaluated on gfx950 matrix cores: every product -- forward, weight
+// gradients H^T dZ and input gradients dZ W^T -- runs as bf16 MFMA
+// (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; FP32 master weights,
+// Adam updates.  No counterpart in the reference.
+#include <math.h>
+
+#include <hip/hip_runtime.h>
+
+#include "powspec_hip.h"
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int IN = 32, H = 64, OUT = 16;   // padded widths
+// parameter layout (FP32): W1[IN][H] b1[H] W2[H][H] b2[H] W3[H][H] b3[H] W4[H][OUT] b4[OUT]
+constexpr int oW1 = 0, ob1 = oW1 + IN * H, oW2 = ob1 + H, ob2 = oW2 + H * H, oW3 = ob2 + H,
+              ob3 = oW3 + H * H, oW4 = ob3 + H, ob4 = oW4 + H * OUT, NPAR = ob4 + OUT;
+
+__device__ inline short bf16(float x) {   // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// C(r, n) = sum_k A(r, k) B(k, n) for the wave's 16 rows, K and N multiples
+// of 16.  Operand element (i, j) at base[i * s0 + j * s1] (FP32, converted
+// to bf16 on load), so transposed operands cost nothing.  Lane layout of
+// the 16x16x16 MFMA: A[l%16][4(l/16)+j], B[4(l/16)+j][l%16],
+// C[4(l/16)+i][l%16].
+template <int K, int N>
+__device__ inline void wave_mma(const float *A, int a0, int a1, const float *B, int b0, int b1,
+                                float *C, int c0, int c1) {
+  const int l = (int)(threadIdx.x & 63);
+  const int r = l & 15, q = l >> 4;
+#pragma unroll
+  for (int nt = 0; nt < N / 16; nt++) {
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < K / 16; kt++) {
+      s4 a, b;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int k = kt * 16 + 4 * q + j;
+        a[j] = bf16(A[r * a0 + k * a1]);
+        b[j] = bf16(B[k * b0 + (nt * 16 + r) * b1]);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) C[(4 * q + i) * c0 + (nt * 16 + r) * c1] = acc[i];
+  }
+}
+
+// bias + optional ReLU on a wave's [16][n] LDS block
+__device__ inline void bias_act(float *Z, const float *b, int n, bool relu) {
+  const int l = (int)(threadIdx.x & 63);
+  for (int e = l; e < 16 * n; e += 64) {
+    float v = Z[e] + b[e % n];
+    Z[e] = relu ? fmaxf(v, 0.f) : v;
+  }
+}
+
+struct WaveBufs {
+  float x[16 * IN], h1[16 * H], h2[16 * H], h3[16 * H], y[16 * OUT];
+  float d1[16 * H], d2[16 * H], t[64 * OUT];
+};
+
+// forward of the wave's 16 samples; rows past B are zero
+__device__ inline void forward(const float *P, const float *X, int s0, int B, WaveBufs &w) {
+  const int l = (int)(threadIdx.x & 63);
+  for (int e = l; e < 16 * IN; e += 64) {
+    const int s = s0 + e / IN;
+    w.x[e] = s < B ? X[(size_t)s * IN + e % IN] : 0.f;
+  }
+  __builtin_amdgcn_wave_barrier();
+  wave_mma<IN, H>(w.x, IN, 1, P + oW1, H, 1, w.h1, H, 1);
+  __builtin_amdgcn_wave_barrier();
+  bias_act(w.h1, P + ob1, H, true);
+  __builtin_amdgcn_wave_barrier();
+  wave_mma<H, H>(w.h1, H, 1, P + oW2, H, 1, w.h2, H, 1);
+  __builtin_amdgcn_wave_barrier();
+  bias_act(w.h2, P + ob2, H, true);
+  __builtin_amdgcn_wave_barrier();
+  wave_mma<H, H>(w.h2, H, 1, P + oW3, H, 1, w.h3, H, 1);
+  __builtin_amdgcn_wave_barrier();
+  bias_act(w.h3, P + ob3, H, true);
+  __builtin_amdgcn_wave_barrier();
+  wave_mma<H, OUT>(w.h3, H, 1, P + oW4, OUT, 1, w.y, OUT, 1);
+  __builtin_amdgcn_wave_barrier();
+  bias_act(w.y, P + ob4, OUT, false);
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kWaves = 4;
+
+__global__ void __launch_bounds__(64 * kWaves) mlp_forward_kernel(const float *P, const float *X,
+                                                                  int B, float *Y) {
+  __shared__ WaveBufs wb[kWaves];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int s0 = (blockIdx.x * kWaves + wv) * 16;
+  forward(P, X, s0, B, wb[wv]);
+  if (l < 16 && s0 + l < B) Y[s0 + l] = wb[wv].y[l * OUT];
+}
+
+// dW[m][n] += sum_s A[s][m] D[s][n] over the wave's 16 samples (M = rows of
+// the weight), via MFMA with K = 16 samples; db[n] += sum_s D[s][n]
+template <int M, int N>
+__device__ inline void weight_grad(const float *A, int lda, const float *D, int ldd, float *G,
+                                   float *gb, float *tmp) {
+  const int l = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int mt = 0; mt < M / 16; mt++) {
+    // tmp(r, n) = sum_s A[s][mt*16 + r] D[s][n]
+    wave_mma<16, N>(A + mt * 16, 1, lda, D, ldd, 1, tmp, N, 1);
+    __builtin_amdgcn_wave_barrier();
+    for (int e = l; e < 16 * N; e += 64) atomicAdd(&G[(mt * 16 + e / N) * N + e % N], tmp[e]);
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int n = l; n < N; n += 64) {
+    float s = 0.f;
+    for (int r = 0; r < 16; r++) s += D[r * ldd + n];
+    atomicAdd(&gb[n], s);
+  }
+}
+
+// loss = sum (y - t)^2 / B_total over the batch, gradient accumulated in G
+__global__ void __launch_bounds__(64 * kWaves) mlp_grad_kernel(const float *P, const float *X,
+                                                               const float *T, int B,
+                                                               float inv_total, float *G,
+                                                               float *loss) {
+  __shared__ WaveBufs wb[kWaves];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  WaveBufs &w = wb[wv];
+  const int s0 = (blockIdx.x * kWaves + wv) * 16;
+  forward(P, X, s0, B, w);
+  // dY (only output 0 is trained) into y
+  float lsum = 0.f;
+  for (int e = l; e < 16 * OUT; e += 64) {
+    const int s = s0 + e / OUT;
+    float g = 0.f;
+    if (e % OUT == 0 && s < B) {
+      const float d = w.y[e] - T[s];
+      lsum += d * d * inv_total;
+      g = 2.f * d * inv_total;
+    }
+    w.y[e] = g;
+  }
+  for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o);
+  if (l == 0) atomicAdd(loss, lsum);
+  __builtin_amdgcn_wave_barrier();
+  // layer 4
+  weight_grad<H, OUT>(w.h3, H, w.y, OUT, G + oW4, G + ob4, w.t);
+  wave_mma<OUT, H>(w.y, OUT, 1, P + oW4, 1, OUT, w.d1, H, 1);   // dH3 = dY W4^T
+  __builtin_amdgcn_wave_barrier();
+  for (int e = l; e < 16 * H; e += 64) w.d1[e] = w.h3[e] > 0.f ? w.d1[e] : 0.f;
+  __builtin_amdgcn_wave_barrier();
+  // layer 3
+  weight_grad<H, H>(w.h2, H, w.d1, H, G + oW3, G + ob3, w.t);
+  wave_mma<H, H>(w.d1, H, 1, P + oW3, 1, H, w.d2, H, 1);        // dH2 = dZ3 W3^T
+  __builtin_amdgcn_wave_barrier();
+  for (int e = l; e < 16 * H; e += 64) w.d2[e] = w.h2[e] > 0.f ? w.d2[e] : 0.f;
+  __builtin_amdgcn_wave_barrier();
+  // layer 2
+  weight_grad<H, H>(w.h1, H, w.d2, H, G + oW2, G + ob2, w.t);
+  wave_mma<H, H>(w.d2, H, 1, P + oW2, 1, H, w.d1, H, 1);        // dH1 = dZ2 W2^T
+  __builtin_amdgcn_wave_barrier();
+  for (int e = l; e < 16 * H; e += 64) w.d1[e] = w.h1[e] > 0.f ? w.d1[e] : 0.f;
+  __builtin_amdgcn_wave_barrier();
+  // layer 1
+  weight_grad<IN, H>(w.x, IN, w.d1, H, G + oW1, G + ob1, w.t);
+}
+
+__global__ void adam_kernel(float *P, float *G, float *m, float *v, int n, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float g = G[i];
+    m[i] = b1 * m[i] + (1.f - b1) * g;
+    v[i] = b2 * v[i] + (1.f - b2) * g * g;
+    P[i] -= lr * (m[i] * c1) / (sqrtf(v[i] * c2) + eps);
+    G[i] = 0.f;
+  }
+}
+
+__global__ void zero_kernel(float *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+unsigned blocks_for(int B) { return (unsigned)((B + 16 * kWaves - 1) / (16 * kWaves)); }
+
+}  // namespace
+
+extern "C" int psh_mlp_param_count(void) { return NPAR; }
+
+extern "C" int psh_mlp_forward(const float *d_params, const float *d_X, int B, float *d_Y) {
+  if (!d_params || !d_X || !d_Y || B < 0) return PSH_ERR_ARG;
+  if (B == 0) return PSH_OK;
+  hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
+                     d_params, d_X, B, d_Y);
+  return hipDeviceSynchronize() == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_mlp_grad(const float *d_params, const float *d_X, const float *d_T, int B,
+                            float *d_grads, float *loss) {
+  if (!d_params || !d_X || !d_T || !d_grads || B < 1) return PSH_ERR_ARG;
+  float *d_loss = nullptr;
+  if (hipMalloc(&d_loss, sizeof(float)) != hipSuccess) return PSH_ERR_MEMORY;
+  hipLaunchKernelGGL(zero_kernel, dim3(64), dim3(256), 0, nullptr, d_grads, NPAR);
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
+                     d_params, d_X, d_T, B, 1.f / (float)B, d_grads, d_loss);
+  float h = 0.f;
+  const bool ok = hipMemcpy(&h, d_loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+  (void)hipFree(d_loss);
+  if (loss) *loss = h;
+  return ok ? PSH_OK : PSH_ERR_DEVICE;
+}
+
+extern "C" int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B,
+                             int batch, int epochs, float lr, unsigned seed, float *loss_out) {
+  if (!d_params || !d_X || !d_T || B < 1 || batch < 16 || epochs < 0) return PSH_ERR_ARG;
+  float *buf = nullptr;
+  if (hipMalloc(&buf, sizeof(float) * (3 * NPAR + 1)) != hipSuccess) return PSH_ERR_MEMORY;
+  float *G = buf, *m = buf + NPAR, *v = buf + 2 * NPAR, *d_loss = buf + 3 * NPAR;
+  hipLaunchKernelGGL(zero_kernel, dim3(64), dim3(256), 0, nullptr, buf, 3 * NPAR + 1);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int step = 0;
+  // mini-batches are contiguous slices of a sample order the caller shuffles
+  // (seed reserved for a device-side shuffle)
+  (void)seed;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int s0 = 0; s0 < B; s0 += batch) {
+      const int nb = B - s0 < batch ? B - s0 : batch;
+      hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(nb)), dim3(64 * kWaves), 0, nullptr,
+                         d_params, d_X + (size_t)s0 * IN, d_T + s0, nb, 1.f / (float)nb, G,
+                         d_loss);
+      step++;
+      const float c1 = 1.f / (1.f - powf(b1, (float)step));
+      const float c2 = 1.f / (1.f - powf(b2, (float)step));
+      hipLaunchKernelGGL(adam_kernel, dim3(64), dim3(256), 0, nullptr, d_params, G, m, v, NPAR,
+                         lr, b1, b2, eps, c1, c2);
+    }
+  }
+  // final loss over the whole set
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
+  hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
+                     d_params, d_X, d_T, B, 1.f / (float)B, G, d_loss);
+  float h = 0.f;
+  const bool ok = hipMemcpy(&h, d_loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+  (void)hipFree(buf);
+  if (loss_out) *loss_out = h;
+  return ok ? PSH_OK : PSH_ERR_DEVICE;
+}

@@ -34,10 +34,9 @@ using DevPart = PshDevPart;
 
 __device__ __forceinline__ int window_width(int scheme) { return scheme + 1; }
 
-__device__ __forceinline__ int wrap(int c, int Ng) {
-  c %= Ng;
-  return c < 0 ? c + Ng : c;
-}
+// Periodic wrap into [0, Ng); Ng is a power of two (the context accepts no
+// other grid size), so two's complement masking is the modulo.
+__device__ __forceinline__ int wrap(int c, int Ng) { return c & (Ng - 1); }
 
 // Base cell (leftmost window cell, unwrapped) and window fraction of a grid
 // coordinate x >= 0, following src/genr_mesh.c:57-66 (NGP), 94-104 (CIC),
@@ -110,12 +109,13 @@ __device__ PartCells part_cells(const double *r, const PshAssignGeom &g) {
   return pc;
 }
 
-// Tiles touched along one axis by the union window [b, b + W - 1 + delta].
+// Tiles touched along one axis by the union window [b, b + W - 1 + delta]
+// (b in [0, Ng), tiles of 2^TS cells).
 __device__ __forceinline__ int axis_tiles(int b, int W, int delta, int Ng,
-                                          int T, int t[2]) {
+                                          int TS, int t[2]) {
   int e = b + W - 1 + delta;
-  t[0] = b / T;
-  int te = wrap(e, Ng) / T;
+  t[0] = b >> TS;
+  int te = wrap(e, Ng) >> TS;
   if (te != t[0]) {
     t[1] = te;
     return 2;
@@ -134,42 +134,92 @@ __device__ __forceinline__ int find_part(const DevPart *parts, int nparts,
   return lo;
 }
 
-// Wave-aggregated `old = atomicAdd(&ctr[t], 1)` over the lanes with `active`:
-// lanes of a wave that target the same counter are served by one atomic of
-// the group size (catalogues are usually ordered in space, so a wave mostly
-// hits one tile); returns each lane's old value plus its rank in the group.
-__device__ __forceinline__ uint32_t agg_increment(uint32_t *ctr, uint32_t t,
-                                                  bool active) {
-  const int lane = __lane_id();
-  uint64_t pend = __ballot(active);
-  uint32_t slot = 0;
-  while (pend) {
-    const int leader = __ffsll((unsigned long long)pend) - 1;
-    const uint32_t tl = __shfl(t, leader);
-    const uint64_t same = __ballot(((pend >> lane) & 1ull) && t == tl);
-    uint32_t old = 0;
-    if (lane == leader) old = atomicAdd(&ctr[tl], (uint32_t)__popcll(same));
-    old = __shfl(old, leader);
-    if ((same >> lane) & 1ull)
-      slot = old + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-    pend &= ~same;
-  }
-  return slot;
+// One particle as loaded: position and weight, its part's weight scale and
+// destination mock.
+struct Loaded {
+  double r[4];
+  float wscale;
+  int mock;
+};
+
+// A part descriptor held in scalar registers (the part of a workgroup's
+// first particle).
+struct UPart {
+  const double *xyzw;
+  size_t start, n;
+  float wscale;
+  int mock;
+};
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
 }
 
-// Tiles (mock-local indices) touched by particle `gi` of its part and, with
-// FILL, its record.  Returns the number of tiles (0 and flag bit 2 when the
-// shifted grid's base is inconsistent).
-template <bool FILL>
-__device__ __forceinline__ int particle_tiles(const PshAssignGeom &g, const DevPart &P,
-                                              size_t gi, int W, uint32_t *tiles, Rec &rec,
+// The part containing particle `gi0`; every lane passes the same gi0.
+__device__ __forceinline__ UPart uniform_part(const DevPart *parts, int nparts, size_t gi0) {
+  const DevPart &P = parts[find_part(parts, nparts, gi0)];
+  UPart u;
+  u.xyzw = reinterpret_cast<const double *>(uniform64(reinterpret_cast<uint64_t>(P.xyzw)));
+  u.start = uniform64(P.start);
+  u.n = uniform64(P.n);
+  u.wscale = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(P.wscale)));
+  u.mock = __builtin_amdgcn_readfirstlane(P.mock);
+  return u;
+}
+
+typedef __attribute__((address_space(1))) const double gdouble;
+
+// Load particle gi (zeros when !live).  P0 is the part the caller expects gi
+// to lie in (chunks rarely cross part boundaries), so the table search runs
+// only for particles outside it; callers keep the next particle's loads in
+// flight while they process the current one.
+__device__ __forceinline__ Loaded load_particle(const DevPart *parts, int nparts, UPart P0,
+                                                size_t gi, bool live) {
+  Loaded L;
+  L.r[0] = L.r[1] = L.r[2] = L.r[3] = 0.0;
+  L.wscale = 0.f;
+  L.mock = 0;
+  if (!live) return L;
+  const double *px = P0.xyzw;
+  size_t ps = P0.start;
+  float pw = P0.wscale;
+  int pm = P0.mock;
+  if (gi < P0.start || gi - P0.start >= P0.n) {
+    const int q = find_part(parts, nparts, gi);
+    px = parts[q].xyzw;
+    ps = parts[q].start;
+    pw = parts[q].wscale;
+    pm = parts[q].mock;
+  }
+  L.wscale = pw;
+  L.mock = pm;
+  const gdouble *g = (const gdouble *)(px + 4 * (gi - ps));   // device memory
+  L.r[0] = g[0];
+  L.r[1] = g[1];
+  L.r[2] = g[2];
+  L.r[3] = g[3];
+  return L;
+}
+
+// Tiles (mock-local indices) touched by a particle rr = (x, y, z, w) and,
+// with FILL, its record.  Returns the number of tiles (0 and flag bit 2 when the
+// shifted grid's base is inconsistent).  With CHECK and g.check_box, a
+// coordinate outside [bmin, bmin + L) (bmin = 0 for simulation boxes) sets
+// flag bit 4: the box check of
+// simulation catalogues (src/genr_mesh.c:517-533) folded into the count
+// pass, which reads every particle anyway (NaN passes, as it passes the
+// reference's min/max comparisons).
+template <bool FILL, bool CHECK = false>
+__device__ __forceinline__ int particle_tiles(const PshAssignGeom &g, const double *rr,
+                                              float wscale, int W, uint32_t *tiles, Rec &rec,
                                               int *flags) {
-  const double *r = P.xyzw + 4 * (gi - P.start);
-  double rr[4];
-  {
-    const double2 *r2 = reinterpret_cast<const double2 *>(r);
-    double2 a = r2[0], b = r2[1];
-    rr[0] = a.x; rr[1] = a.y; rr[2] = b.x; rr[3] = b.y;
+  if (CHECK && g.check_box) {
+    bool bad = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) bad |= (rr[a] < g.bmin[a]) || (rr[a] >= g.bmin[a] + g.L[a]);
+    if (bad) atomicOr(flags, 4);
   }
   PartCells pc = part_cells(rr, g);
   if (!pc.ok) {
@@ -177,9 +227,9 @@ __device__ __forceinline__ int particle_tiles(const PshAssignGeom &g, const DevP
     return 0;
   }
   int tx[2], ty[2], tz[2];
-  int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile[0], tx);
-  int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile[1], ty);
-  int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile[2], tz);
+  int nx = axis_tiles(pc.b[0], W, pc.delta[0], g.Ng, g.tile_shift[0], tx);
+  int ny = axis_tiles(pc.b[1], W, pc.delta[1], g.Ng, g.tile_shift[1], ty);
+  int nz = axis_tiles(pc.b[2], W, pc.delta[2], g.Ng, g.tile_shift[2], tz);
   int nt = 0;
   for (int a = 0; a < nx; a++)
     for (int b = 0; b < ny; b++)
@@ -189,7 +239,7 @@ __device__ __forceinline__ int particle_tiles(const PshAssignGeom &g, const DevP
     rec.f[0] = pc.f[0];
     rec.f[1] = pc.f[1];
     rec.f[2] = pc.f[2];
-    rec.w = (float)rr[3] * P.wscale;
+    rec.w = (float)rr[3] * wscale;
     rec.bxy = (uint32_t)pc.b[0] | ((uint32_t)pc.b[1] << 16);
     rec.bz = (uint32_t)pc.b[2] |
              ((uint32_t)(pc.delta[0] | (pc.delta[1] << 1) | (pc.delta[2] << 2)) << 16);
@@ -204,6 +254,37 @@ __device__ __forceinline__ void publish_wmax(float wmax, int *flags) {
   if (__lane_id() == 0) atomicMax(reinterpret_cast<unsigned *>(flags) + 1, __float_as_uint(wmax));
 }
 
+// Run-aggregated increment of LDS or global counters: lanes with `act` whose counter index `t`
+// equals the previous lane's form a run (space-ordered catalogues put
+// neighbouring particles into one tile, so a wave holds a few long runs);
+// the run's first lane adds the run length with one LDS atomic and every
+// lane gets old + its rank in the run (RET), or nothing (count only).
+// Shuffled catalogues make every lane its own run: one atomic per lane, as
+// without aggregation.
+template <bool RET>
+__device__ __forceinline__ uint32_t run_increment(uint32_t *hist, uint32_t t, bool act) {
+  const int lane = __lane_id();
+  const uint32_t tp = __shfl_up(t, 1, 64);
+  const bool actp = __shfl_up(act ? 1 : 0, 1, 64) != 0;
+  const bool head = act && (lane == 0 || !actp || tp != t);
+  const uint64_t heads = __ballot(head);
+  const uint64_t bound = heads | ~__ballot(act);   // a run ends before these lanes
+  uint32_t old = 0;
+  if (head) {
+    const uint64_t after = lane == 63 ? 0ull : bound & (~0ull << (lane + 1));
+    const int next = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    const uint32_t len = (uint32_t)(next - lane);
+    if (RET) old = atomicAdd(&hist[t], len);
+    else atomicAdd(&hist[t], len);
+  }
+  if (!RET) return 0u;
+  // first lane of this lane's run: the highest head at or below it
+  const uint64_t below = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int hl = below ? 63 - __clzll((long long)below) : lane;
+  const uint32_t base = __shfl(old, hl, 64);
+  return base + (uint32_t)(lane - hl);
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(256)
 assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
@@ -216,17 +297,24 @@ assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
   // every lane of a wave runs the same number of iterations so that the
   // wave-level aggregation below sees converged waves
   const size_t iters = (ntotal + stride - 1) / stride;
+  const UPart P0 = uniform_part(parts, nparts, blockIdx.x * (size_t)blockDim.x);
   size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   float wmax = 0.f;
+  // software pipeline: the next particle's loads are in flight while the
+  // current one is processed
+  bool cur_live = gi < ntotal;
+  Loaded cur = load_particle(parts, nparts, P0, gi, cur_live);
+#pragma unroll 1
   for (size_t it = 0; it < iters; it++, gi += stride) {
-    const bool live = gi < ntotal;
+    const size_t gn = gi + stride;
+    const bool nxt_live = it + 1 < iters && gn < ntotal;
+    const Loaded nxt = load_particle(parts, nparts, P0, gn, nxt_live);
     uint32_t tiles[8];
     int nt = 0;
     Rec rec;
-    if (live) {
-      const DevPart &P = parts[find_part(parts, nparts, gi)];
-      nt = particle_tiles<FILL>(g, P, gi, W, tiles, rec, flags);
-      const uint32_t mbase = (uint32_t)P.mock * tiles_per_mock;
+    if (cur_live) {
+      nt = particle_tiles<FILL, !FILL>(g, cur.r, cur.wscale, W, tiles, rec, flags);
+      const uint32_t mbase = (uint32_t)cur.mock * tiles_per_mock;
       for (int k = 0; k < nt; k++) tiles[k] += mbase;
       if (FILL && nt) wmax = fmaxf(wmax, fabsf(rec.w));
     }
@@ -236,15 +324,17 @@ assign_bucket(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
       if (!__ballot(act)) break;
       const uint32_t t = act ? tiles[k] : 0u;
       if (FILL) {
-        const uint32_t slot = agg_increment(cursors, t, act);
+        const uint32_t slot = run_increment<true>(cursors, t, act);
         if (act) {
           if (slot < rec_cap) recs[slot] = rec;
           else atomicOr(flags, 1);
         }
       } else {
-        agg_increment(counts, t, act);
+        run_increment<false>(counts, t, act);
       }
     }
+    cur = nxt;
+    cur_live = nxt_live;
   }
   if (FILL) publish_wmax(wmax, flags);
 }
@@ -267,7 +357,7 @@ constexpr int kLdsTiles = 32768;
 constexpr int kLdsThreads = 1024;
 
 template <bool FILL>
-__global__ void __launch_bounds__(kLdsThreads)
+__global__ void __launch_bounds__(kLdsThreads, 8)   // 8 waves per SIMD: two workgroups per CU
 assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
                   size_t ntotal, uint32_t *__restrict__ counts,
                   uint32_t *__restrict__ cursors, Rec *__restrict__ recs, size_t rec_cap,
@@ -277,19 +367,45 @@ assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts
   const uint32_t tpm = (uint32_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
   const size_t c0 = (size_t)blockIdx.x * kChunk;
   const size_t c1 = min(ntotal, c0 + (size_t)kChunk);
-  const int mock0 = parts[find_part(parts, nparts, c0)].mock;
+  const UPart P0 = uniform_part(parts, nparts, c0);
+  const int mock0 = P0.mock;
   const uint32_t mbase0 = (uint32_t)mock0 * tpm;
   for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   uint32_t tiles[8];
   Rec rec;
+  // every lane of a wave runs the same number of iterations (the chunk is a
+  // multiple of the workgroup size), so the run aggregation sees whole waves
+  static_assert(kChunk % kLdsThreads == 0, "chunk of whole workgroup sweeps");
   // pass A: histogram of this mock's pairs (count: others go to global)
-  for (size_t gi = c0 + threadIdx.x; gi < c1; gi += blockDim.x) {
-    const DevPart &P = parts[find_part(parts, nparts, gi)];
-    const int nt = particle_tiles<false>(g, P, gi, W, tiles, rec, flags);
-    for (int k = 0; k < nt; k++) {
-      if (P.mock == mock0) atomicAdd(&hist[tiles[k]], 1u);
-      else if (!FILL) atomicAdd(&counts[(uint32_t)P.mock * tpm + tiles[k]], 1u);
+  {
+    // software pipeline: the next particle's loads are in flight while the
+    // current one is processed
+    size_t gi = c0 + threadIdx.x;
+    bool cur_live = gi < c1;
+    Loaded cur = load_particle(parts, nparts, P0, gi, cur_live);
+#pragma unroll 1
+    for (int it = 0; it < kChunk / kLdsThreads; it++) {
+      const size_t gn = gi + kLdsThreads;
+      const bool nxt_live = it + 1 < kChunk / kLdsThreads && gn < c1;
+      const Loaded nxt = load_particle(parts, nparts, P0, gn, nxt_live);
+      int nt = 0;
+      int mock = mock0;
+      if (cur_live) {
+        nt = particle_tiles<false, !FILL>(g, cur.r, cur.wscale, W, tiles, rec, flags);
+        mock = cur.mock;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const bool act = k < nt;
+        if (!__ballot(act)) break;
+        const bool here = act && mock == mock0;
+        run_increment<false>(hist, here ? tiles[k] : 0u, here);
+        if (!FILL && act && !here) atomicAdd(&counts[(uint32_t)mock * tpm + tiles[k]], 1u);
+      }
+      cur = nxt;
+      cur_live = nxt_live;
+      gi = gn;
     }
   }
   __syncthreads();
@@ -303,16 +419,39 @@ assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts
   __syncthreads();
   // pass B: records
   float wmax = 0.f;
-  for (size_t gi = c0 + threadIdx.x; gi < c1; gi += blockDim.x) {
-    const DevPart &P = parts[find_part(parts, nparts, gi)];
-    const int nt = particle_tiles<true>(g, P, gi, W, tiles, rec, flags);
-    if (nt) wmax = fmaxf(wmax, fabsf(rec.w));
-    for (int k = 0; k < nt; k++) {
-      const uint32_t slot = P.mock == mock0
-                                ? atomicAdd(&hist[tiles[k]], 1u)
-                                : atomicAdd(&cursors[(uint32_t)P.mock * tpm + tiles[k]], 1u);
-      if (slot < rec_cap) recs[slot] = rec;
-      else atomicOr(flags, 1);
+  {
+    // software pipeline: the next particle's loads are in flight while the
+    // current one is processed
+    size_t gi = c0 + threadIdx.x;
+    bool cur_live = gi < c1;
+    Loaded cur = load_particle(parts, nparts, P0, gi, cur_live);
+#pragma unroll 1
+    for (int it = 0; it < kChunk / kLdsThreads; it++) {
+      const size_t gn = gi + kLdsThreads;
+      const bool nxt_live = it + 1 < kChunk / kLdsThreads && gn < c1;
+      const Loaded nxt = load_particle(parts, nparts, P0, gn, nxt_live);
+      int nt = 0;
+      int mock = mock0;
+      if (cur_live) {
+        nt = particle_tiles<true>(g, cur.r, cur.wscale, W, tiles, rec, flags);
+        mock = cur.mock;
+        if (nt) wmax = fmaxf(wmax, fabsf(rec.w));
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const bool act = k < nt;
+        if (!__ballot(act)) break;
+        const bool here = act && mock == mock0;
+        uint32_t slot = run_increment<true>(hist, here ? tiles[k] : 0u, here);
+        if (act && !here) slot = atomicAdd(&cursors[(uint32_t)mock * tpm + tiles[k]], 1u);
+        if (act) {
+          if (slot < rec_cap) recs[slot] = rec;
+          else atomicOr(flags, 1);
+        }
+      }
+      cur = nxt;
+      cur_live = nxt_live;
+      gi = gn;
     }
   }
   publish_wmax(wmax, flags);

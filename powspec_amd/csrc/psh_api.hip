@@ -405,6 +405,7 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   g.Ng = ctx->Ng;
   g.scheme = ctx->conf.assign;
   g.intlace = ctx->conf.intlace;
+  g.check_box = ctx->conf.issim ? 1 : 0;   // simulation boxes: [0, L) per axis
   // 32-bit fixed-point LDS tiles (<= 64 KB): 8x16x32 cells per grid by
   // default (measured best of 7 shapes for PCS interlaced at Ng = 1024);
   // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4)
@@ -419,6 +420,10 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
     for (int a = 0; a < 3; a++) tl[a] = ctx->tile_req[a];
   for (int a = 0; a < 3; a++) g.tile[a] = ctx->Ng < tl[a] ? ctx->Ng : tl[a];
   for (int a = 0; a < 3; a++) g.ntile[a] = ctx->Ng / g.tile[a];
+  for (int a = 0; a < 3; a++) {
+    g.tile_shift[a] = 0;
+    while ((1 << g.tile_shift[a]) < g.tile[a]) g.tile_shift[a]++;
+  }
   g.row_pitch = 2 * ctx->H;
   for (int a = 0; a < 3; a++) {
     g.bmin[a] = bmin[a];
@@ -594,6 +599,8 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
     HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (flags & 4)
+      return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");
     if (flags & 2)
       return set_err(ctx, PSH_ERR_MESH, "inconsistent interlaced cell of a particle");
     if (!(flags & 1)) return PSH_OK;
@@ -964,16 +971,30 @@ extern "C" const char *psh_last_error(const psh_ctx *ctx) {
 // Raw box sums of the field of `hp` (several parts: the field of their sum):
 // sum |F|^2 alias mult L_l(mu) per pole and bin (src/multipole.c:778-898),
 // after the box check of every particle (src/genr_mesh.c:517-533).
-int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
-  int rc;
+// The box of a simulation comes from the configuration, so the check of
+// every coordinate against it rides on the count pass (flag bit 4); only a
+// failed check runs the separate min/max pass, for the reference's message.
+int sim_box_error(psh_ctx *ctx, const std::vector<PshDevPart> &hp) {
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  ev_record(ctx, E_START);
+  int rc;
   for (auto &p : hp)
     if ((rc = bounds_device(ctx, p.xyzw, p.n, mn, mx))) return rc;
   if ((rc = def_box(ctx, mn, mx))) return rc;
+  return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");
+}
+
+int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
+  int rc;
+  ev_record(ctx, E_START);
+  size_t ntot = 0;
+  for (auto &p : hp) ntot += p.n;
+  if (ntot == 0) return sim_box_error(ctx, hp);   // "invalid coordinate" as the reference
+  const double zero[3] = {0, 0, 0};
+  if ((rc = def_box(ctx, zero, zero))) return rc;
   if ((rc = ensure_geometry(ctx))) return rc;
-  if ((rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr)))
-    return rc;
+  rc = run_assign(ctx, hp, 1, ctx->meshA, ctx->conf.intlace ? ctx->meshB : nullptr);
+  if (rc == PSH_ERR_CATA) return sim_box_error(ctx, hp);
+  if (rc) return rc;
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
   if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
@@ -1164,6 +1185,8 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     if ((rc = grow_records(ctx))) return rc;
     return set_err(ctx, PSH_ERR_MESH, "assignment record buffer overflow (grown; rerun)");
   }
+  if (flags & 4)
+    return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");
   if (flags & 2) return set_err(ctx, PSH_ERR_MESH, "inconsistent interlaced cell of a particle");
   const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
   for (int m = 0; m < nmock; m++) {

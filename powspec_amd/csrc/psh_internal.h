@@ -66,7 +66,9 @@ struct PshAssignGeom {
   int scheme;           // 0..3
   int intlace;          // second (half-cell shifted) grid
   int wide;             // 64-bit fixed-point tiles (weights of both signs)
-  int tile[3];          // tile extent per axis
+  int check_box;        // count pass flags (bit 4) coordinates outside [bmin, bmin + L)
+  int tile[3];          // tile extent per axis (a power of two)
+  int tile_shift[3];    // log2(tile)
   int ntile[3];         // Ng / tile
   int row_pitch;        // floats per mesh row (2H)
   double bmin[3];       // box corner of the plain grid

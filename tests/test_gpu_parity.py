@@ -129,11 +129,26 @@ def test_run_host_dropin_sim(psh, oracle):
 
 
 def test_box_errors(psh):
-    d = _uniform(100, [100.0] * 3, 2)
-    d[3, 1] = 100.0   # not smaller than BOX_SIZE
-    ctx = psh.Context(psh.make_conf(gsize=16, bsize=100.0, kbin=0.05))
+    """The box check of src/genr_mesh.c:517-533 (folded into the count pass)
+    rejects x >= L and x < 0 with the reference's messages, in plain and
+    batched runs, and leaves the context usable."""
+    good = _uniform(100, [100.0] * 3, 2)
+    ctx = psh.Context(psh.make_conf(gsize=16, bsize=100.0, kbin=0.05, intlace=True))
+    for axis, val, msg in ((1, 100.0, "y coordinate not smaller than BOX_SIZE"),
+                           (2, -1e-9, "z coordinate below 0")):
+        d = good.copy()
+        d[3, axis] = val
+        with pytest.raises(psh.PshError) as e:
+            ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), 100.0)
+        assert e.value.code == psh.ERR_CATA
+        assert msg in str(e.value)
+    ref = ctx.run_device(psh.DeviceArray.from_numpy(good), len(good), 100.0)
+    assert np.isfinite(ref["pl"][0]).all()
+    bctx = psh.Context(psh.make_conf(gsize=16, bsize=100.0, kbin=0.05), nmock_max=2)
+    d = np.concatenate([good, good])
+    d[150, 0] = 100.0
     with pytest.raises(psh.PshError) as e:
-        ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), 100.0)
+        bctx.run_batch(psh.DeviceArray.from_numpy(d), [0, 100, 200], [100.0, 100.0])
     assert e.value.code == psh.ERR_CATA
 
 

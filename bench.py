@@ -260,6 +260,68 @@ def run_survey(P, steps=3):
             "fft_transforms": 2 + (5 + 9)}
 
 
+def run_ingest(P, data, tile, reps=3):
+    """Catalogue ingest of the C2 catalogue (x, y, z float64 columns, unit
+    weights): binary file -> HBM and host columns -> HBM through the pinned
+    pipeline, and a stream of catalogues ingested while the previous one is
+    processed, against ingest-then-process in sequence.  The file is written
+    to the box's /tmp just before and read back from the page cache, so the
+    file rate is the host-memory -> PCIe -> HBM rate, not a disk rate."""
+    import tempfile
+    n = len(data)
+    cols = {k: np.ascontiguousarray(data[:, a]) for a, k in enumerate("xyz")}
+    out = [P.DeviceArray.empty((n, 4)) for _ in range(2)]
+    ing = P.Ingest()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "c2.pscat")
+        P.cat_write(path, cols)
+        ing.start_file(path, out[0], n)
+        ing.wait()
+        t_file, t_host = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ing.start_file(path, out[0], n)
+            r = ing.wait()
+            t_file.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ing.start_host(cols, out[1], n)
+            ing.wait()
+            t_host.append(time.perf_counter() - t0)
+        nbytes = r["bytes"]
+        # streaming: catalogue k+1 ingested while catalogue k is processed
+        ctx = P.Context(P.make_conf(**c2_conf(P)))
+        ctx.set_tiles(tile)
+        ctx.run_device(out[0], n, float(n))
+        ncat = 4
+        t0 = time.perf_counter()
+        for k in range(ncat):
+            ing.start_file(path, out[k % 2], n)
+            ing.wait()
+            ctx.run_device(out[k % 2], n, float(n))
+        t_seq = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ing.start_file(path, out[0], n)
+        ing.wait()
+        for k in range(ncat):
+            if k + 1 < ncat:
+                ing.start_file(path, out[(k + 1) % 2], n)
+            ctx.run_device(out[k % 2], n, float(n))
+            if k + 1 < ncat:
+                ing.wait()
+        t_ovl = time.perf_counter() - t0
+        ctx.close()
+    ing.close()
+    tf, th = min(t_file), min(t_host)
+    log(f"ingest: file {nbytes / tf / 1e9:.1f} GB/s, host columns {nbytes / th / 1e9:.1f} GB/s; "
+        f"{ncat} catalogues ingest+process: sequential {t_seq * 1e3:.1f} ms, "
+        f"overlapped {t_ovl * 1e3:.1f} ms")
+    return {"workload": f"C2 catalogue, {n} objects, x/y/z float64 columns ({nbytes / 1e9:.3f} GB)",
+            "file_GBps": round(nbytes / tf / 1e9, 2), "host_GBps": round(nbytes / th / 1e9, 2),
+            "file_ms": round(tf * 1e3, 2), "stream_catalogues": ncat,
+            "sequential_ms": round(t_seq * 1e3, 1), "overlapped_ms": round(t_ovl * 1e3, 1),
+            "overlapped_meshes_per_s": round(ncat / t_ovl, 2)}
+
+
 def cpu_baseline(c2_data):
     """FP64 oracle (OpenMP, own FFT) on a bounded sample of the C2 workload:
     N_g = 512 (1/8 of the cells) and every 8th particle (same particles per
@@ -296,6 +358,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-survey", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -318,6 +381,7 @@ def main():
     if not args.no_batched and args.nmock > 0:
         extra = run_batched(P, data, args.nmock)
     survey = None if args.no_survey else run_survey(P)
+    ingest = None if args.no_ingest else run_ingest(P, data, tile)
     cpu = None if args.no_cpu else cpu_baseline(data)
     line = {
         "metric": METRIC,
@@ -343,6 +407,7 @@ def main():
         "roofline": rl,
         "batched": extra or None,
         "survey": survey,
+        "ingest": ingest,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

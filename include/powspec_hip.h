@@ -357,6 +357,77 @@ typedef struct {
 int psh_autotune(const psh_conf *conf, const psh_data *d_data, const size_t *offsets,
     size_t npart, int nmock, const char *cache_path, psh_tune *out);
 
+/* ---- catalogue ingest ----------------------------------------------------
+ * Replaces the catalogue reader on the way into genr_mesh: read_cata
+ * (src/read_cata.h:85, src/read_cata.c:100-183) -> read_ascii_data
+ * (io/read_file.h, io/read_ascii.c:533-1142), whose per-object weight rule
+ * and sums (io/read_ascii.c:875-903) the ingest applies on the device:
+ *   survey: w = w_comp * w_fkp, sums w_comp, w^2 and w_comp w_fkp^2 n(z)
+ *           (wdata/wrand, sumw2, sumw2n of read_cata);
+ *   box:    w = w_comp, sum w_comp.
+ * Absent w_comp and w_fkp columns read as 1, an absent n(z) as 0.  Columns
+ * are float32 or float64 (all alike); the output is device psh_data.
+ * The source is a binary SoA catalogue file (this library's format below)
+ * or the caller's host columns.  Reading, the DMA over PCIe and the
+ * conversion are pipelined through two pinned buffers on the ingest's own
+ * stream and host thread: psh_ingest_file / psh_ingest_host return at once
+ * and psh_ingest_wait completes, so the caller's context can work on the
+ * previous catalogue meanwhile.  ASCII parsing, libast column expressions
+ * and selection criteria are not part of this path. */
+#define PSH_COL_X 1u
+#define PSH_COL_Y 2u
+#define PSH_COL_Z 4u
+#define PSH_COL_WCOMP 8u
+#define PSH_COL_WFKP 16u
+#define PSH_COL_NZ 32u
+#define PSH_CAT_MAGIC "PSHCAT01"
+
+/* Binary catalogue file: this 64-byte header, then each present column
+ * (bit order x, y, z, w_comp, w_fkp, nz) as n contiguous values of `dtype`
+ * bytes, little endian. */
+typedef struct {
+  char magic[8];          /* PSH_CAT_MAGIC (8 bytes, no terminator)    */
+  uint64_t n;             /* objects                                   */
+  uint32_t dtype;         /* 4 (float32) or 8 (float64)                */
+  uint32_t cols;          /* PSH_COL_* mask; x, y and z are required   */
+  uint8_t reserved[40];
+} psh_cat_header;
+
+typedef struct {
+  uint64_t n;             /* objects written to the device             */
+  double sumw;            /* sum of w_comp                             */
+  double sumw2;           /* sum of (w_comp w_fkp)^2 (surveys)         */
+  double sumw2n;          /* sum of w_comp w_fkp^2 n(z) (surveys)      */
+  double bytes;           /* column bytes moved host -> device         */
+  double seconds_read;    /* host time spent reading/copying chunks    */
+  double seconds;         /* wall time of the whole ingest             */
+} psh_ingest_result;
+
+typedef struct psh_ingest psh_ingest;
+
+/* Pinned staging for chunks of `chunk_objects` objects (0: 2^21) on the
+ * current device. */
+int psh_ingest_create(size_t chunk_objects, psh_ingest **out);
+int psh_ingest_destroy(psh_ingest *ing);
+
+/* Write columns col_ptrs[c] (for each bit c of cols) as a binary catalogue. */
+int psh_cat_write(const char *path, uint64_t n, int dtype, unsigned cols,
+    const void *const col_ptrs[6]);
+
+/* Start ingesting a binary catalogue file / host columns into d_out (device,
+ * room for cap objects).  sim != 0 selects the box weight rule.  One ingest
+ * in flight per object; finish it with psh_ingest_wait. */
+int psh_ingest_file(psh_ingest *ing, const char *path, int sim, psh_data *d_out,
+    size_t cap);
+int psh_ingest_host(psh_ingest *ing, uint64_t n, int dtype, unsigned cols,
+    const void *const col_ptrs[6], int sim, psh_data *d_out, size_t cap);
+
+/* Wait for the ingest in flight; fills *res (may be NULL). */
+int psh_ingest_wait(psh_ingest *ing, psh_ingest_result *res);
+
+/* Text of the last ingest error. */
+const char *psh_ingest_error(const psh_ingest *ing);
+
 #ifdef __cplusplus
 }
 #endif

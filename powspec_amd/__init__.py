@@ -69,7 +69,9 @@ EXPORTS = [
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
     "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device", "psh_mlp_param_count",
-    "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train",
+    "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
+    "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
+    "psh_ingest_wait", "psh_ingest_error",
 ]
 
 
@@ -184,6 +186,14 @@ def load():
     L.psh_powspec_destroy.argtypes = [C.POINTER(Pk)]
     L.psh_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(d), C.POINTER(d),
                                 C.POINTER(i), C.POINTER(i)]
+    L.psh_ingest_create.argtypes = [sz, C.POINTER(vp)]
+    L.psh_ingest_destroy.argtypes = [vp]
+    L.psh_cat_write.argtypes = [C.c_char_p, C.c_uint64, i, C.c_uint, C.POINTER(vp)]
+    L.psh_ingest_file.argtypes = [vp, C.c_char_p, i, vp, sz]
+    L.psh_ingest_host.argtypes = [vp, C.c_uint64, i, C.c_uint, C.POINTER(vp), i, vp, sz]
+    L.psh_ingest_wait.argtypes = [vp, vp]
+    L.psh_ingest_error.argtypes = [vp]
+    L.psh_ingest_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -245,6 +255,10 @@ class DeviceArray:
         rc = load().psh_device_malloc(C.byref(self.ptr), max(1, self.nbytes))
         if rc != 0:
             raise PshError(rc, f"device allocation of {nbytes} bytes failed")
+
+    @classmethod
+    def empty(cls, shape, dtype=np.float64) -> "DeviceArray":
+        return cls(int(np.prod(shape)) * np.dtype(dtype).itemsize)
 
     @classmethod
     def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
@@ -503,3 +517,101 @@ def gen_sky(n, seed, ra=(0.0, 60.0), dec=(-30.0, 30.0), zmu=0.55, zsigma=0.08, z
     if rc:
         raise PshError(rc, "psh_gen_sky failed")
     return d, nz
+
+
+# ---- catalogue ingest (psh_ingest_*) ---------------------------------------
+
+COL_X, COL_Y, COL_Z, COL_WCOMP, COL_WFKP, COL_NZ = 1, 2, 4, 8, 16, 32
+_COL_NAMES = ("x", "y", "z", "w_comp", "w_fkp", "nz")
+CAT_MAGIC = b"PSHCAT01"
+
+
+class CatHeader(C.Structure):
+    """psh_cat_header: the 64-byte header of a binary catalogue file."""
+
+    _fields_ = [("magic", C.c_char * 8), ("n", C.c_uint64), ("dtype", C.c_uint32),
+                ("cols", C.c_uint32), ("reserved", C.c_uint8 * 40)]
+
+
+class IngestResult(C.Structure):
+    """psh_ingest_result."""
+
+    _fields_ = [("n", C.c_uint64), ("sumw", C.c_double), ("sumw2", C.c_double),
+                ("sumw2n", C.c_double), ("bytes", C.c_double), ("seconds_read", C.c_double),
+                ("seconds", C.c_double)]
+
+
+def _columns(cols: dict, dtype):
+    """Contiguous arrays of the given dtype for the named columns; (mask,
+    pointer array, keep-alive list)."""
+    mask, ptrs, keep = 0, (C.c_void_p * 6)(), []
+    n = None
+    for c, name in enumerate(_COL_NAMES):
+        if cols.get(name) is None:
+            continue
+        a = np.ascontiguousarray(cols[name], dtype=dtype)
+        if n is None:
+            n = len(a)
+        elif len(a) != n:
+            raise ValueError("columns of different lengths")
+        mask |= 1 << c
+        ptrs[c] = a.ctypes.data
+        keep.append(a)
+    return mask, ptrs, keep, n or 0
+
+
+def cat_write(path, cols: dict, dtype=np.float64):
+    """Write a binary catalogue from columns {"x": ..., "y": ..., "z": ...,
+    optional "w_comp", "w_fkp", "nz"}."""
+    mask, ptrs, keep, n = _columns(cols, dtype)
+    rc = load().psh_cat_write(str(path).encode(), C.c_uint64(n), np.dtype(dtype).itemsize,
+                              C.c_uint(mask), ptrs)
+    if rc:
+        raise PshError(rc, f"psh_cat_write failed for {path}")
+
+
+class Ingest:
+    """psh_ingest: pinned double-buffered catalogue upload on its own stream
+    and host thread.  start_file / start_host return at once; wait() gives the
+    psh_ingest_result as a dict."""
+
+    def __init__(self, chunk_objects=0):
+        self.L = load()
+        self.h = C.c_void_p()
+        rc = self.L.psh_ingest_create(C.c_size_t(chunk_objects), C.byref(self.h))
+        if rc:
+            raise PshError(rc, "psh_ingest_create failed")
+        self._keep = None
+
+    def _check(self, rc):
+        if rc:
+            raise PshError(rc, self.L.psh_ingest_error(self.h).decode())
+
+    def start_file(self, path, out: "DeviceArray", cap, sim=True):
+        self._check(self.L.psh_ingest_file(self.h, str(path).encode(), int(bool(sim)),
+                                           out.ptr, C.c_size_t(cap)))
+
+    def start_host(self, cols: dict, out: "DeviceArray", cap, sim=True, dtype=np.float64):
+        mask, ptrs, keep, n = _columns(cols, dtype)
+        self._keep = (keep, ptrs)          # alive until wait()
+        self._check(self.L.psh_ingest_host(self.h, C.c_uint64(n), np.dtype(dtype).itemsize,
+                                           C.c_uint(mask), ptrs, int(bool(sim)), out.ptr,
+                                           C.c_size_t(cap)))
+
+    def wait(self):
+        r = IngestResult()
+        rc = self.L.psh_ingest_wait(self.h, C.byref(r))
+        self._keep = None
+        self._check(rc)
+        return {f: getattr(r, f) for f, _ in IngestResult._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.psh_ingest_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

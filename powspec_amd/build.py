@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpowspec_hip.so")
 
 SOURCES = ["k_assign.hip", "k_fft.hip", "k_bin.hip", "k_cnvt.hip", "k_gen.hip",
-           "k_mlp.hip", "psh_api.hip", "psh_output.cpp"]
+           "k_mlp.hip", "psh_api.hip", "psh_ingest.hip", "psh_output.cpp"]
 ARCH = os.environ.get("PSH_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -51,6 +51,10 @@ int main(void) {
   printf("psh_cnvt %zu\n", sizeof(psh_cnvt));
   printf("psh_box_spec %zu\npsh_sky_spec %zu\n", sizeof(psh_box_spec), sizeof(psh_sky_spec));
   printf("psh_tune %zu\n", sizeof(psh_tune));
+  printf("psh_cat_header %zu\npsh_ingest_result %zu\n", sizeof(psh_cat_header),
+         sizeof(psh_ingest_result));
+  F(psh_cat_header, n); F(psh_cat_header, dtype); F(psh_cat_header, cols);
+  F(psh_ingest_result, sumw2n); F(psh_ingest_result, seconds);
   F(psh_tune, ms); F(psh_tune, ncand);
   F(psh_box_spec, seed); F(psh_sky_spec, nbar0); F(psh_sky_spec, seed);
   F(psh_cnvt, zsp); F(psh_cnvt, nsp); F(psh_cnvt, rcnvt);
@@ -93,6 +97,12 @@ def test_struct_layout_matches_header():
     assert v["psh_box_spec"][0] == C.sizeof(P.BoxSpec)
     assert v["psh_sky_spec"][0] == C.sizeof(P.SkySpec)
     assert v["psh_tune"][0] == C.sizeof(P.Tune)
+    assert v["psh_cat_header"][0] == C.sizeof(P.CatHeader) == 64
+    assert v["psh_ingest_result"][0] == C.sizeof(P.IngestResult)
+    for f in ("n", "dtype", "cols"):
+        assert v[f"psh_cat_header.{f}"][0] == getattr(P.CatHeader, f).offset
+    for f in ("sumw2n", "seconds"):
+        assert v[f"psh_ingest_result.{f}"][0] == getattr(P.IngestResult, f).offset
     assert v["psh_mesh_meta"][0] == C.sizeof(P.MeshMeta)
 
 

@@ -310,8 +310,23 @@ def run_ingest(P, data, tile, reps=3):
                 ing.wait()
         t_ovl = time.perf_counter() - t0
         ctx.close()
+        # ASCII (the reference's input format): 2e6 lines of x y z, %.10f
+        na = min(n, 2_000_000)
+        apath = os.path.join(d, "c2.txt")
+        np.savetxt(apath, data[:na, :3], fmt="%.10f")
+        abytes = os.path.getsize(apath)
+        t_ascii = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ing.start_ascii(apath, out[0], n, comment="#")
+            ra = ing.wait()
+            t_ascii.append(time.perf_counter() - t0)
+        assert ra["n"] == na
     ing.close()
     tf, th = min(t_file), min(t_host)
+    ta = min(t_ascii)
+    log(f"ingest ASCII: {na} lines, {abytes / 1e6:.0f} MB in {ta * 1e3:.1f} ms "
+        f"({abytes / ta / 1e9:.2f} GB/s, {na / ta / 1e6:.1f} M lines/s)")
     log(f"ingest: file {nbytes / tf / 1e9:.1f} GB/s, host columns {nbytes / th / 1e9:.1f} GB/s; "
         f"{ncat} catalogues ingest+process: sequential {t_seq * 1e3:.1f} ms, "
         f"overlapped {t_ovl * 1e3:.1f} ms")
@@ -319,7 +334,10 @@ def run_ingest(P, data, tile, reps=3):
             "file_GBps": round(nbytes / tf / 1e9, 2), "host_GBps": round(nbytes / th / 1e9, 2),
             "file_ms": round(tf * 1e3, 2), "stream_catalogues": ncat,
             "sequential_ms": round(t_seq * 1e3, 1), "overlapped_ms": round(t_ovl * 1e3, 1),
-            "overlapped_meshes_per_s": round(ncat / t_ovl, 2)}
+            "overlapped_meshes_per_s": round(ncat / t_ovl, 2),
+            "ascii": {"lines": na, "bytes": abytes, "ms": round(ta * 1e3, 2),
+                      "GBps": round(abytes / ta / 1e9, 3),
+                      "lines_per_s": round(na / ta, 1)}}
 
 
 def cpu_baseline(c2_data):

@@ -422,6 +422,35 @@ int psh_ingest_file(psh_ingest *ing, const char *path, int sim, psh_data *d_out,
 int psh_ingest_host(psh_ingest *ing, uint64_t n, int dtype, unsigned cols,
     const void *const col_ptrs[6], int sim, psh_data *d_out, size_t cap);
 
+/* ASCII catalogues as the reference reads them (io/read_ascii.c:533-1142,
+ * CONF keys DATA_SKIP, DATA_COMMENT, DATA_FORMATTER, DATA_POSITION,
+ * DATA_WT_COMP, DATA_WT_FKP, DATA_NZ), parsed on the device: the first
+ * `skip` lines are dropped, then lines whose first non-blank character is
+ * `comment` (0: none) and blank lines; each remaining line is read with the
+ * formatter's conversions as whitespace-separated tokens.  Conversions:
+ * %lf %le %lg %lE %lG %lF (values), %d %i %ld %li %u %lu (integer values),
+ * %*<conversion> (skipped; numeric ones must parse) and %*s %*c %*[...]
+ * (any token).  Columns are 1-based among the conversions that are not
+ * suppressed, like `$n` in the reference's expressions; only plain column
+ * references are supported (no libast arithmetic, no selection).  Numbers
+ * are converted exactly (the correctly rounded double, as strtod); a line
+ * that does not match stops the ingest with PSH_ERR_FILE naming its line. */
+typedef struct {
+  long skip;              /* header lines                               */
+  char comment;           /* comment character, 0: none                 */
+  const char *fmtr;       /* e.g. "%lf %lf %lf %*s %lf"                 */
+  int pos[3];             /* columns of x, y, z (RA, Dec, z for surveys) */
+  int wcomp, wfkp, nz;    /* columns, 0: absent                          */
+} psh_ascii_fmt;
+
+int psh_ingest_ascii(psh_ingest *ing, const char *path, const psh_ascii_fmt *fmt,
+    int sim, psh_data *d_out, size_t cap);
+
+/* The ASCII ingest's decimal conversion on the host: the correctly rounded
+ * double of the whole token s[0..len) (PSH_ERR_ARG when it is not a
+ * supported decimal number). */
+int psh_parse_double(const char *s, size_t len, double *out);
+
 /* Wait for the ingest in flight; fills *res (may be NULL). */
 int psh_ingest_wait(psh_ingest *ing, psh_ingest_result *res);
 

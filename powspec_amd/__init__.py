@@ -71,7 +71,7 @@ EXPORTS = [
     "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device", "psh_mlp_param_count",
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
-    "psh_ingest_wait", "psh_ingest_error",
+    "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
 ]
 
 
@@ -194,6 +194,8 @@ def load():
     L.psh_ingest_wait.argtypes = [vp, vp]
     L.psh_ingest_error.argtypes = [vp]
     L.psh_ingest_error.restype = C.c_char_p
+    L.psh_ingest_ascii.argtypes = [vp, C.c_char_p, C.POINTER(AsciiFmt), i, vp, sz]
+    L.psh_parse_double.argtypes = [C.c_char_p, sz, C.POINTER(d)]
     _lib = L
     return L
 
@@ -570,6 +572,40 @@ def cat_write(path, cols: dict, dtype=np.float64):
         raise PshError(rc, f"psh_cat_write failed for {path}")
 
 
+class AsciiFmt(C.Structure):
+    """psh_ascii_fmt: DATA_SKIP, DATA_COMMENT, DATA_FORMATTER and the columns
+    of DATA_POSITION / DATA_WT_COMP / DATA_WT_FKP / DATA_NZ."""
+
+    _fields_ = [("skip", C.c_long), ("comment", C.c_char), ("fmtr", C.c_char_p),
+                ("pos", C.c_int * 3), ("wcomp", C.c_int), ("wfkp", C.c_int), ("nz", C.c_int)]
+
+
+def _column(v) -> int:
+    """Column number from an int or a plain libast reference "$n" / "${n}"."""
+    if v is None:
+        return 0
+    if isinstance(v, (int, np.integer)):
+        return int(v)
+    t = str(v).strip()
+    if t.startswith("${") and t.endswith("}"):
+        t = t[2:-1]
+    elif t.startswith("$"):
+        t = t[1:]
+    if not t.isdigit():
+        raise ValueError(f"only plain column references are supported, got {v!r}")
+    return int(t)
+
+
+def parse_double(token: str) -> float:
+    """The ASCII ingest's decimal conversion (host build of the device code)."""
+    b = token.encode()
+    out = C.c_double()
+    rc = load().psh_parse_double(b, len(b), C.byref(out))
+    if rc:
+        raise PshError(rc, f"not a supported decimal number: {token!r}")
+    return out.value
+
+
 class Ingest:
     """psh_ingest: pinned double-buffered catalogue upload on its own stream
     and host thread.  start_file / start_host return at once; wait() gives the
@@ -597,6 +633,19 @@ class Ingest:
         self._check(self.L.psh_ingest_host(self.h, C.c_uint64(n), np.dtype(dtype).itemsize,
                                            C.c_uint(mask), ptrs, int(bool(sim)), out.ptr,
                                            C.c_size_t(cap)))
+
+    def start_ascii(self, path, out: "DeviceArray", cap, fmtr="%lf %lf %lf", pos=(1, 2, 3),
+                    wcomp=None, wfkp=None, nz=None, skip=0, comment="#", sim=True):
+        f = AsciiFmt()
+        f.skip = int(skip)
+        f.comment = (comment or "\0").encode()[:1]
+        self._keep = fmtr.encode()
+        f.fmtr = self._keep
+        for a in range(3):
+            f.pos[a] = _column(pos[a])
+        f.wcomp, f.wfkp, f.nz = _column(wcomp), _column(wfkp), _column(nz)
+        self._check(self.L.psh_ingest_ascii(self.h, str(path).encode(), C.byref(f),
+                                            int(bool(sim)), out.ptr, C.c_size_t(cap)))
 
     def wait(self):
         r = IngestResult()

@@ -126,3 +126,94 @@ def test_ingest_overlaps_processing(psh, tmp_path):
         for l in range(2):   # FP64 bin sums: equal up to their summation order
             a, b = got[k]["pl"][0][l], ref["pl"][0][l]
             assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+
+
+def _ascii_catalogue(n, seed):
+    """Lines in the reference's ASCII layout with varied number spellings:
+    RA Dec z name w_comp w_fkp nz, plus a header, comments, blank lines,
+    tabs and CRLF endings.  Returns (text, expected columns, header lines)."""
+    rng = np.random.default_rng(seed)
+    spell = [lambda v: repr(float(v)), lambda v: f"{v:.10g}", lambda v: f"{v:.6e}",
+             lambda v: f"{v:+.12f}", lambda v: f"{v:.17E}"]
+    header = ["# ra dec z name wc wfkp nz", "some header text"]
+    lines = list(header)
+    rows = []
+    for i in range(n):
+        vals = [rng.uniform(0, 360), rng.uniform(-90, 90), rng.uniform(0.1, 1.2),
+                rng.uniform(0.5, 1.5), rng.uniform(0.05, 1.0), rng.uniform(1e-5, 5e-4)]
+        toks = [spell[int(rng.integers(0, len(spell)))](v) for v in vals]
+        rows.append([float(t) for t in toks])      # correctly rounded, as strtod
+        sep = "\t" if i % 7 == 0 else " "
+        ln = sep.join(toks[:3] + [f"gal{i}"] + toks[3:])
+        if i % 11 == 0:
+            ln = "   " + ln + "   "
+        lines.append(ln + ("\r" if i % 13 == 0 else ""))
+        if i % 17 == 0:
+            lines.append("# a comment line")
+        if i % 19 == 0:
+            lines.append("   ")
+    return "\n".join(lines), np.array(rows), len(header)
+
+
+@pytest.mark.parametrize("chunk", [1000, 1 << 20])
+def test_ascii_ingest_matches(psh, tmp_path, chunk):
+    n = 20_003
+    text, rows, nhead = _ascii_catalogue(n, 21)
+    path = tmp_path / "cat.txt"
+    path.write_text(text)                     # no newline after the last line
+    out = psh.DeviceArray.empty((n, 4))
+    ing = psh.Ingest(chunk_objects=chunk)     # 1000 objects: 48 KB chunks
+    ing.start_ascii(path, out, n, fmtr="%lf %lf %lf %*s %lf %lf %lf",
+                    pos=("$1", "$2", "${3}"), wcomp="$4", wfkp="$5", nz="$6",
+                    skip=nhead, comment="#", sim=False)
+    res = ing.wait()
+    cols = {"x": rows[:, 0], "y": rows[:, 1], "z": rows[:, 2], "w_comp": rows[:, 3],
+            "w_fkp": rows[:, 4], "nz": rows[:, 5]}
+    _check(psh, out.to_numpy((n, 4)), res, cols, np.float64, False)
+
+
+def test_ascii_matches_binary(psh, tmp_path):
+    from powspec_amd import synth
+    n = 50_000
+    d = synth.uniform_box(n, 1000.0, seed=5)
+    path = tmp_path / "box.txt"
+    path.write_text("\n".join(f"{float(x)!r} {y:.17g} {z:.17e}" for x, y, z in d[:, :3])
+                    + "\n")
+    a = psh.DeviceArray.empty((n, 4))
+    ing = psh.Ingest()
+    ing.start_ascii(path, a, n, comment=None)
+    ra = ing.wait()
+    assert np.array_equal(a.to_numpy((n, 4)), d)
+    assert ra["n"] == n and ra["sumw"] == n
+
+
+def test_ascii_errors(psh, tmp_path):
+    ing = psh.Ingest(chunk_objects=1000)
+    out = psh.DeviceArray.empty((100, 4))
+    lines = [f"{i}.5 2 3" for i in range(60)]
+    lines[41] = "1.5 2 abc"
+    p = tmp_path / "bad.txt"
+    p.write_text("# head\n" + "\n".join(lines) + "\n")
+    ing.start_ascii(p, out, 100, skip=1)
+    with pytest.raises(psh.PshError) as e:
+        ing.wait()
+    assert e.value.code == psh.ERR_FILE and "line 43 " in str(e.value)
+    lines[41] = "1.5 2"                          # missing column
+    p.write_text("\n".join(lines) + "\n")
+    ing.start_ascii(p, out, 100)
+    with pytest.raises(psh.PshError) as e:
+        ing.wait()
+    assert "line 42 " in str(e.value)
+    lines[41] = "1.5 2 3"
+    p.write_text("\n".join(lines) + "\n")
+    ing.start_ascii(p, out, 50)                   # capacity
+    with pytest.raises(psh.PshError) as e:
+        ing.wait()
+    assert e.value.code == psh.ERR_ARG
+    with pytest.raises(psh.PshError):             # unsupported formatter
+        ing.start_ascii(p, out, 100, fmtr="%f %f %f")
+    with pytest.raises(psh.PshError):             # column beyond the formatter
+        ing.start_ascii(p, out, 100, pos=(1, 2, 4))
+    ing.start_ascii(p, out, 100)
+    r = ing.wait()
+    assert r["n"] == 60

@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(kLdsThreads, 8)   // 8 waves per SIMD: two wor
 assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts,
                   size_t ntotal, uint32_t *__restrict__ counts,
                   uint32_t *__restrict__ cursors, Rec *__restrict__ recs, size_t rec_cap,
-                  int *__restrict__ flags) {
+                  int *__restrict__ flags, uint32_t *__restrict__ whist) {
   extern __shared__ uint32_t hist[];
   const int W = window_width(g.scheme);
   const uint32_t tpm = (uint32_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
@@ -370,53 +370,67 @@ assign_bucket_lds(PshAssignGeom g, const DevPart *__restrict__ parts, int nparts
   const UPart P0 = uniform_part(parts, nparts, c0);
   const int mock0 = P0.mock;
   const uint32_t mbase0 = (uint32_t)mock0 * tpm;
-  for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) hist[i] = 0;
-  __syncthreads();
+  uint32_t *wh = whist ? whist + (size_t)blockIdx.x * tpm : nullptr;
   uint32_t tiles[8];
   Rec rec;
-  // every lane of a wave runs the same number of iterations (the chunk is a
-  // multiple of the workgroup size), so the run aggregation sees whole waves
-  static_assert(kChunk % kLdsThreads == 0, "chunk of whole workgroup sweeps");
-  // pass A: histogram of this mock's pairs (count: others go to global)
-  {
-    // software pipeline: the next particle's loads are in flight while the
-    // current one is processed
-    size_t gi = c0 + threadIdx.x;
-    bool cur_live = gi < c1;
-    Loaded cur = load_particle(parts, nparts, P0, gi, cur_live);
-#pragma unroll 1
-    for (int it = 0; it < kChunk / kLdsThreads; it++) {
-      const size_t gn = gi + kLdsThreads;
-      const bool nxt_live = it + 1 < kChunk / kLdsThreads && gn < c1;
-      const Loaded nxt = load_particle(parts, nparts, P0, gn, nxt_live);
-      int nt = 0;
-      int mock = mock0;
-      if (cur_live) {
-        nt = particle_tiles<false, !FILL>(g, cur.r, cur.wscale, W, tiles, rec, flags);
-        mock = cur.mock;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const bool act = k < nt;
-        if (!__ballot(act)) break;
-        const bool here = act && mock == mock0;
-        run_increment<false>(hist, here ? tiles[k] : 0u, here);
-        if (!FILL && act && !here) atomicAdd(&counts[(uint32_t)mock * tpm + tiles[k]], 1u);
-      }
-      cur = nxt;
-      cur_live = nxt_live;
-      gi = gn;
+  if (FILL && wh) {
+    // the count pass left this workgroup's histogram in HBM: reserve the
+    // workgroup's range of each bucket without another pass over the chunk
+    for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) {
+      const uint32_t v = wh[i];
+      hist[i] = v ? atomicAdd(&cursors[mbase0 + i], v) : 0u;
     }
-  }
-  __syncthreads();
-  if (!FILL) {
+    __syncthreads();
+  } else {
+    for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) hist[i] = 0;
+    __syncthreads();
+    // every lane of a wave runs the same number of iterations (the chunk is a
+    // multiple of the workgroup size), so the run aggregation sees whole waves
+    static_assert(kChunk % kLdsThreads == 0, "chunk of whole workgroup sweeps");
+    // pass A: histogram of this mock's pairs (count: others go to global)
+    {
+      // software pipeline: the next particle's loads are in flight while the
+      // current one is processed
+      size_t gi = c0 + threadIdx.x;
+      bool cur_live = gi < c1;
+      Loaded cur = load_particle(parts, nparts, P0, gi, cur_live);
+#pragma unroll 1
+      for (int it = 0; it < kChunk / kLdsThreads; it++) {
+        const size_t gn = gi + kLdsThreads;
+        const bool nxt_live = it + 1 < kChunk / kLdsThreads && gn < c1;
+        const Loaded nxt = load_particle(parts, nparts, P0, gn, nxt_live);
+        int nt = 0;
+        int mock = mock0;
+        if (cur_live) {
+          nt = particle_tiles<false, !FILL>(g, cur.r, cur.wscale, W, tiles, rec, flags);
+          mock = cur.mock;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const bool act = k < nt;
+          if (!__ballot(act)) break;
+          const bool here = act && mock == mock0;
+          run_increment<false>(hist, here ? tiles[k] : 0u, here);
+          if (!FILL && act && !here) atomicAdd(&counts[(uint32_t)mock * tpm + tiles[k]], 1u);
+        }
+        cur = nxt;
+        cur_live = nxt_live;
+        gi = gn;
+      }
+    }
+    __syncthreads();
+    if (!FILL) {
+      for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x) {
+        const uint32_t v = hist[i];
+        if (wh) wh[i] = v;
+        if (v) atomicAdd(&counts[mbase0 + i], v);
+      }
+      return;
+    }
     for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x)
-      if (hist[i]) atomicAdd(&counts[mbase0 + i], hist[i]);
-    return;
-  }
-  for (uint32_t i = threadIdx.x; i < tpm; i += blockDim.x)
-    if (hist[i]) hist[i] = atomicAdd(&cursors[mbase0 + i], hist[i]);
-  __syncthreads();
+      if (hist[i]) hist[i] = atomicAdd(&cursors[mbase0 + i], hist[i]);
+    __syncthreads();
+  }   // without a stored histogram
   // pass B: records
   float wmax = 0.f;
   {
@@ -802,6 +816,14 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
 
 size_t psh_assign_record_bytes() { return sizeof(Rec); }
 
+size_t psh_assign_whist_words(const PshAssignGeom &g, size_t ntotal) {
+  const size_t tpm = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+  if (tpm > (size_t)kLdsTiles || ntotal == 0) return 0;
+  const size_t blocks = (ntotal + kChunk - 1) / kChunk;
+  const size_t words = blocks * tpm;
+  return words <= ((size_t)1 << 30) ? words : 0;   // at most 4 GB of histograms
+}
+
 hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s) {
   unsigned blocks = (unsigned)((n + 255) / 256);
   if (blocks > 4096) blocks = 4096;
@@ -831,7 +853,8 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
                           uint32_t *counts, uint32_t *starts, uint32_t *cursors,
                           void *records, size_t rec_cap, float *meshA,
                           float *meshB, size_t mstride, uint32_t *d_scan_tmp,
-                          int *d_flags, hipStream_t s, hipEvent_t ev_deposit) {
+                          int *d_flags, hipStream_t s, hipEvent_t ev_deposit,
+                          uint32_t *whist) {
   if (nparts < 1) return hipErrorInvalidValue;
   const size_t ntiles = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2] * nmock;
   const int nsb = (int)((ntiles + kScanBlock - 1) / kScanBlock);
@@ -855,7 +878,7 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
     if (lds)
       hipLaunchKernelGGL((assign_bucket_lds<false>), dim3(cblocks), dim3(kLdsThreads),
                          lds_bytes, s, g, d_parts, nparts, ntotal, counts, cursors, recs,
-                         rec_cap, d_flags);
+                         rec_cap, d_flags, whist);
     else
       hipLaunchKernelGGL((assign_bucket<false>), dim3(pblocks), dim3(256), 0, s, g,
                          d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,
@@ -872,7 +895,7 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
     if (lds)
       hipLaunchKernelGGL((assign_bucket_lds<true>), dim3(cblocks), dim3(kLdsThreads),
                          lds_bytes, s, g, d_parts, nparts, ntotal, counts, cursors, recs,
-                         rec_cap, d_flags);
+                         rec_cap, d_flags, whist);
     else
       hipLaunchKernelGGL((assign_bucket<true>), dim3(pblocks), dim3(256), 0, s, g,
                          d_parts, nparts, ntotal, counts, cursors, recs, rec_cap,

@@ -63,6 +63,8 @@ struct psh_ctx {
   PshAssignGeom ag{};
   size_t ntiles_mock = 0;
   uint32_t *counts = nullptr, *starts = nullptr, *cursors = nullptr;
+  uint32_t *whist = nullptr;   // per-workgroup tile histograms of the count pass
+  size_t whist_cap = 0, whist_words = 0;
   uint32_t *scan_tmp = nullptr;
   size_t tiles_cap = 0;
   PshDevPart *d_parts = nullptr;
@@ -475,10 +477,20 @@ int def_box(psh_ctx *ctx, const double mn[3], const double mx[3]) {
   return set_box(ctx, L, bmin);
 }
 
+// A captured graph holds the scratch pointers of its capture: any
+// reallocation drops it (it is captured again on the next batched call).
+void drop_graph(psh_ctx *ctx) {
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  ctx->gexec = nullptr;
+  ctx->graph = nullptr;
+}
+
 int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   int rc;
   const size_t ntiles = ctx->ntiles_mock * nmock;
   if (ntiles + 1 > ctx->tiles_cap) {
+    drop_graph(ctx);
     if ((rc = dev_alloc(ctx, &ctx->counts, ntiles + 1))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->starts, ntiles + 1))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->cursors, ntiles + 1))) return rc;
@@ -487,11 +499,23 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
     ctx->tiles_cap = ntiles + 1;
   }
   if (nparts > ctx->parts_cap) {
+    drop_graph(ctx);
     if ((rc = dev_alloc(ctx, &ctx->d_parts, (size_t)nparts))) return rc;
     ctx->parts_cap = nparts;
   }
   if (!ctx->d_flags) {
     if ((rc = dev_alloc(ctx, &ctx->d_flags, 4))) return rc;
+  }
+  ctx->whist_words = psh_assign_whist_words(ctx->ag, ntotal);
+  if (ctx->whist_words > ctx->whist_cap) {
+    drop_graph(ctx);
+    if (ctx->whist) (void)hipFree(ctx->whist);
+    ctx->whist = nullptr;
+    ctx->whist_cap = 0;
+    if (hipMalloc((void **)&ctx->whist, sizeof(uint32_t) * ctx->whist_words) == hipSuccess)
+      ctx->whist_cap = ctx->whist_words;
+    else
+      ctx->whist_words = 0;   // fill recounts instead
   }
   // expected (particle, tile) pairs: prod_a (1 + (W' - 1) / T_a), W' the
   // union window; 1.5x margin, grown on overflow
@@ -500,6 +524,7 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   for (int a = 0; a < 3; a++) dup *= 1.0 + (double)(W - 1) / ctx->ag.tile[a];
   size_t need = (size_t)(ntotal * dup * 1.5) + 4096;
   if (need > ctx->rec_cap) {
+    drop_graph(ctx);
     if (ctx->records) {
       (void)hipFree(ctx->records);
       ctx->records = nullptr;
@@ -511,6 +536,7 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
 }
 
 int grow_records(psh_ctx *ctx) {
+  drop_graph(ctx);
   size_t need = ctx->rec_cap * 2;
   if (ctx->records) (void)hipFree(ctx->records);
   ctx->records = nullptr;
@@ -593,7 +619,8 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
                                ctx->counts, ctx->starts, ctx->cursors, ctx->records,
                                ctx->rec_cap, meshA, meshB, ctx->mstride, ctx->scan_tmp,
                                ctx->d_flags, ctx->stream,
-                               ctx->profiling ? ctx->tm.ev[E_DEPOSIT] : nullptr));
+                               ctx->profiling ? ctx->tm.ev[E_DEPOSIT] : nullptr,
+                               ctx->whist_words ? ctx->whist : nullptr));
     ev_record(ctx, E_ASSIGNED);
     int flags = 0;
     HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
@@ -933,7 +960,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
                   ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
                   ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_gcnt, ctx->d_gkm,
                   ctx->d_glcnt, ctx->d_gpart_cnt, ctx->d_gpart_km, ctx->d_gpart_lcnt,
-                  ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat};
+                  ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat, ctx->whist};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   for (int i = 0; i < E_COUNT; i++)
@@ -1147,7 +1174,8 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
       hipError_t e2 = psh_assign_run(ctx->ag, ctx->d_parts, nmock, ntotal, nmock, ctx->counts,
                                      ctx->starts, ctx->cursors, ctx->records, ctx->rec_cap,
                                      ctx->meshA, mB, ctx->mstride, ctx->scan_tmp,
-                                     ctx->d_flags, ctx->stream);
+                                     ctx->d_flags, ctx->stream, nullptr,
+                                     ctx->whist_words ? ctx->whist : nullptr);
       int rc2 = (e1 == hipSuccess && e2 == hipSuccess) ? run_sim_fft_bin(ctx, nmock, ctx->d_bins)
                                                      : PSH_ERR_DEVICE;
       ctx->profiling = prof;

@@ -99,7 +99,12 @@ hipError_t psh_assign_run(const PshAssignGeom &g, const PshDevPart *d_parts,
     int nparts, size_t ntotal, int nmock, uint32_t *counts, uint32_t *starts,
     uint32_t *cursors, void *records, size_t rec_cap, float *meshA,
     float *meshB, size_t mstride, uint32_t *d_scan_tmp, int *d_flags,
-    hipStream_t s, hipEvent_t ev_deposit = nullptr);
+    hipStream_t s, hipEvent_t ev_deposit = nullptr, uint32_t *whist = nullptr);
+
+// uint32 words of the per-workgroup tile histograms that let the fill pass
+// skip its recount (0 when the LDS bucketing does not apply or they would
+// exceed 4 GB); whist passed to psh_assign_run must hold that many.
+size_t psh_assign_whist_words(const PshAssignGeom &g, size_t ntotal);
 
 // ---------------------------------------------------------------------------
 // FFT (k_fft.hip)

@@ -55,7 +55,7 @@ def stage_bytes(Ng, nmesh, npart):
     }
 
 
-def run_c2(P, steps, warmup):
+def run_c2(P, steps, warmup, fixed_tile=None):
     t = time.time()
     # C2 recipe on the GPU (same Philox streams as powspec_amd.synth.clustered_box)
     dev, n = P.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
@@ -66,8 +66,12 @@ def run_c2(P, steps, warmup):
     # deposit tiles tuned on this catalogue before the timed region (a planning
     # step: candidates timed on the same data; results do not depend on it)
     t = time.time()
-    tile, tms, ncand = P.autotune(P.make_conf(**conf), nmock=1, data=dev, offsets=[0, n])
-    log(f"C2 tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
+    if fixed_tile:
+        tile = tuple(fixed_tile)
+        log(f"C2 tiles: {tile} (given)")
+    else:
+        tile, tms, ncand = P.autotune(P.make_conf(**conf), nmock=1, data=dev, offsets=[0, n])
+        log(f"C2 tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
     ctx = P.Context(P.make_conf(**conf))
     ctx.set_tiles(tile)
     L = P.load()
@@ -373,6 +377,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--tile", default=None,
+                    help="C2 deposit tile 'tx,ty,tz' instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -389,7 +395,8 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     arch = P.device_arch()
-    data, res, ms_step, stages, n, tile = run_c2(P, args.steps, args.warmup)
+    fixed = [int(v) for v in args.tile.split(",")] if args.tile else None
+    data, res, ms_step, stages, n, tile = run_c2(P, args.steps, args.warmup, fixed)
     rl = roofline(stages, 1024, 2, n)
     extra = {}
     if not args.no_batched and args.nmock > 0:

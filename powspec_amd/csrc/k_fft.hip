@@ -416,25 +416,42 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   static_assert(PSH_PAD == 16 && RS::XSPAN < (1ull << 32), "lane offsets are 32-bit");
   const size_t ntiles = (size_t)N * ntz;
   const size_t t0 = xcd_tile(blockIdx.x, gridDim.x);
-  for (size_t tile = t0; tile < ntiles; tile += gridDim.x) {
-    const int y = (int)(tile / ntz), q = (int)(tile - (size_t)y * ntz);
-    const size_t base = mock * a.mstride_c + (size_t)y * H + (size_t)q * C;
-    const int ncv = min(C, M + 1 - q * C);
-    // columns q*C + c < H always lie inside the padded row; those past
-    // kz = M are read and discarded
-    auto ld_field = [&](const float2 *f) {
-      return [=](int r, int cc, int j) -> float2 {
-        const float2 x = *row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES,
-                                               (uint32_t)j * ES32 + cc);
-        return cc < ncv ? x : make_float2(0.f, 0.f);
-      };
+  // tile -> (y, kz block q); rows of the block start at base
+  auto tile_base = [&](size_t t, int &y, int &q) -> size_t {
+    y = (int)(t / ntz);
+    q = (int)(t - (size_t)y * ntz);
+    return mock * a.mstride_c + (size_t)y * H + (size_t)q * C;
+  };
+  // columns q*C + c < H always lie inside the padded row; those past
+  // kz = M are read and discarded
+  auto ld_field = [&](const float2 *f, size_t base, int ncv) {
+    return [=](int r, int cc, int j) -> float2 {
+      const float2 x = *row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES,
+                                             (uint32_t)j * ES32 + cc);
+      return cc < ncv ? x : make_float2(0.f, 0.f);
     };
-    float2 va[V];
-    float2 vb[INTLACE ? V : 1];
+  };
+  // The next tile's rows are loaded into va/vb as soon as phase 1 has
+  // consumed them, so their HBM latency hides behind phase 2 (one block per
+  // CU: nothing else would overlap it).
+  float2 va[V];
+  float2 vb[INTLACE ? V : 1];
+  auto load_tile = [&](size_t t) {
+    int ty, tq;
+    const size_t tb = tile_base(t, ty, tq);
+    const int tn = min(C, M + 1 - tq * C);
+    pass_load_rows<N, C, T, true>(va, ld_field(a.A, tb, tn));
+    if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
+  };
+  if (t0 < ntiles) load_tile(t0);
+  for (size_t tile = t0; tile < ntiles; tile += gridDim.x) {
+    int y, q;
+    const size_t base = tile_base(tile, y, q);
+    const int ncv = min(C, M + 1 - q * C);
     if (INTLACE)
-      transform_rows2<N, C, T, true, false, 1>(va, vb, lds, twl, ld_field(a.A), ld_field(a.B));
+      Passes2<N, C, T, 0, true, false, 1>::run(va, vb, lds, twl);
     else
-      transform_rows<N, C, T, true, false, 1>(va, lds, twl, ld_field(a.A));
+      Passes<N, C, T, 0, true, false, 1>::run(va, lds, twl);
     const int c = opaque_tid<T>() % C;   // the thread's column in every pass
     const bool colok = c < ncv;
     const int kz = min(q * C + c, M);
@@ -472,6 +489,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       }
     }
     __syncthreads();
+    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
 
     // phase 2: column c, rows i = threadIdx.x / C + k T / C
     if (colok) {

@@ -719,8 +719,11 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   // gcd(S, cnt) = 1 (a permutation): neighbouring lanes take records far
   // apart in fill order, i.e. far apart in the (usually space-ordered)
   // catalogue, so their windows rarely share LDS banks.
+  // (a bucket of at most one record per thread is taken in fill order: the
+  // permutation's gcd search and 64-bit remainders would cost more than the
+  // bank conflicts they avoid)
   uint32_t S = 1u;
-  if (cnt > 2) {
+  if (cnt > blockDim.x) {
     S = ((uint32_t)((float)cnt * 0.618034f)) | 1u;
     if (S >= cnt) S = 1u;
     for (;;) {
@@ -738,8 +741,11 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
       }
     }
   }
-  uint32_t x = cnt ? (uint32_t)(((uint64_t)threadIdx.x * S) % cnt) : 0u;
-  const uint32_t D = cnt ? (uint32_t)(((uint64_t)blockDim.x * S) % cnt) : 0u;
+  uint32_t x = threadIdx.x, D = blockDim.x;
+  if (S != 1u) {
+    x = (uint32_t)(((uint64_t)threadIdx.x * S) % cnt);
+    D = (uint32_t)(((uint64_t)blockDim.x * S) % cnt);
+  }
   for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
     const Rec r = recs[s0 + x];
     x += D;

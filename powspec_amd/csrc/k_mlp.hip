@@ -97,13 +97,115 @@ __device__ inline void forward(const float *P, const float *X, int s0, int B, Wa
 
 constexpr int kWaves = 4;
 
-__global__ void __launch_bounds__(64 * kWaves) mlp_forward_kernel(const float *P, const float *X,
-                                                                  int B, float *Y) {
-  __shared__ WaveBufs wb[kWaves];
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int s0 = (blockIdx.x * kWaves + wv) * 16;
-  forward(P, X, s0, B, wb[wv]);
-  if (l < 16 && s0 + l < B) Y[s0 + l] = wb[wv].y[l * OUT];
+// ---- inference: weights resident in LDS as bf16 MFMA fragments -------------
+//
+// A workgroup converts the FP32 weights once into bf16 B fragments (for each
+// 16x16 tile (kt, nt): lane l holds W[kt 16 + 4(l/16) + j][nt 16 + l%16],
+// j = 0..3, as 8 contiguous bytes) and its waves then loop over tiles of 16
+// samples.  Activations stay bf16 in a padded [16][H + 8] row block per wave;
+// bias and ReLU are applied to the accumulators in registers.  The bf16
+// roundings are the same as in forward(): every operand rounded to nearest
+// even once, FP32 accumulation.
+constexpr int kInferWaves = 8;
+constexpr int ROWP = H + 8;   // padded bf16 row (bank spread of the A reads)
+constexpr int FRAG1 = IN * H, FRAG2 = H * H, FRAG4 = H * OUT;
+
+struct InferLds {
+  short w1[FRAG1], w2[FRAG2], w3[FRAG2], w4[FRAG4];
+  float b[3 * H + OUT];
+  short act[kInferWaves][2][16 * ROWP];
+};
+
+template <int K, int N>
+__device__ inline void pack_frags(const float *W, short *F) {
+  for (int e = threadIdx.x; e < K * N; e += blockDim.x) {
+    const int tile = e >> 8, l = (e >> 2) & 63, j = e & 3;
+    const int kt = tile / (N / 16), nt = tile % (N / 16);
+    F[e] = bf16(W[(kt * 16 + 4 * (l >> 4) + j) * N + nt * 16 + (l & 15)]);
+  }
+}
+
+// acc tiles of C = A W for the wave's 16 rows: A bf16 rows [16][ROWP]
+// (or, for the input layer, FP32 rows of X), W fragments F.
+template <int K, int N, bool RELU>
+__device__ inline void infer_layer(const short *A, const short *F, const float *bias,
+                                   short *out) {
+  const int l = (int)(threadIdx.x & 63), r = l & 15, q = l >> 4;
+#pragma unroll
+  for (int nt = 0; nt < N / 16; nt++) {
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < K / 16; kt++) {
+      const s4 a = *reinterpret_cast<const s4 *>(A + r * ROWP + kt * 16 + 4 * q);
+      const s4 b = *reinterpret_cast<const s4 *>(F + (kt * (N / 16) + nt) * 256 + 4 * l);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const float bn = bias[nt * 16 + r];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      float v = acc[i] + bn;
+      if (RELU) v = fmaxf(v, 0.f);
+      out[(4 * q + i) * ROWP + nt * 16 + r] = bf16(v);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * kInferWaves)
+mlp_infer_kernel(const float *__restrict__ P, const float *__restrict__ X, int B,
+                 float *__restrict__ Y) {
+  extern __shared__ __align__(16) unsigned char infer_raw[];
+  InferLds &S = *reinterpret_cast<InferLds *>(infer_raw);
+  pack_frags<IN, H>(P + oW1, S.w1);
+  pack_frags<H, H>(P + oW2, S.w2);
+  pack_frags<H, H>(P + oW3, S.w3);
+  pack_frags<H, OUT>(P + oW4, S.w4);
+  for (int e = threadIdx.x; e < 3 * H + OUT; e += blockDim.x)
+    S.b[e] = e < H ? P[ob1 + e] : e < 2 * H ? P[ob2 + e - H] : e < 3 * H ? P[ob3 + e - 2 * H]
+                                                                          : P[ob4 + e - 3 * H];
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, q = l >> 4;
+  short *a0 = S.act[wv][0], *a1 = S.act[wv][1];
+  const int ntile = (B + 15) / 16;
+  for (int t = blockIdx.x * kInferWaves + wv; t < ntile; t += gridDim.x * kInferWaves) {
+    const int s0 = t * 16;
+    // input rows -> bf16 [16][ROWP] (columns past IN unused)
+#pragma unroll
+    for (int kt = 0; kt < IN / 16; kt++) {
+      const int s = s0 + r;
+      float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (s < B) x = *reinterpret_cast<const float4 *>(X + (size_t)s * IN + kt * 16 + 4 * q);
+      short *d = a0 + r * ROWP + kt * 16 + 4 * q;
+      d[0] = bf16(x.x);
+      d[1] = bf16(x.y);
+      d[2] = bf16(x.z);
+      d[3] = bf16(x.w);
+    }
+    __builtin_amdgcn_wave_barrier();
+    infer_layer<IN, H, true>(a0, S.w1, S.b, a1);
+    __builtin_amdgcn_wave_barrier();
+    infer_layer<H, H, true>(a1, S.w2, S.b + H, a0);
+    __builtin_amdgcn_wave_barrier();
+    infer_layer<H, H, true>(a0, S.w3, S.b + 2 * H, a1);
+    __builtin_amdgcn_wave_barrier();
+    // output layer: only column 0 is read (lanes with r == 0 hold rows 4q..4q+3)
+    {
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < H / 16; kt++) {
+        const s4 a = *reinterpret_cast<const s4 *>(a1 + r * ROWP + kt * 16 + 4 * q);
+        const s4 b = *reinterpret_cast<const s4 *>(S.w4 + kt * 256 + 4 * l);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+      if (r == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int s = s0 + 4 * q + i;
+          if (s < B) Y[s] = acc[i] + S.b[3 * H];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // dW[m][n] += sum_s A[s][m] D[s][n] over the wave's 16 samples (M = rows of
@@ -199,7 +301,21 @@ extern "C" int psh_mlp_param_count(void) { return NPAR; }
 extern "C" int psh_mlp_forward(const float *d_params, const float *d_X, int B, float *d_Y) {
   if (!d_params || !d_X || !d_Y || B < 0) return PSH_ERR_ARG;
   if (B == 0) return PSH_OK;
-  hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
+  // persistent: at most two workgroups per CU, each packing the weights once
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int ntile = (B + 15) / 16;
+  int grid = (ntile + kInferWaves - 1) / kInferWaves;
+  if (grid > 2 * ncu) grid = 2 * ncu;
+  const size_t lds = sizeof(InferLds);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)mlp_infer_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    attr = true;
+  }
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, nullptr,
                      d_params, d_X, B, d_Y);
   return hipDeviceSynchronize() == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
 }

@@ -266,3 +266,33 @@ def evaluate(params, spectra, kny, kmax_frac=0.8):
     """Worst errors over all catalogues: (corrected, Jing-corrected)."""
     m, j = errors(params, spectra, kny, kmax_frac)
     return float(m.max()), float(j.max())
+
+
+def _bench():
+    """Kernel workload for profiling: inference on 2^20 feature rows (5 calls),
+    training on 10240 rows (batch 256, 50 epochs), one gradient over 65536
+    rows; prints the wall time of each part."""
+    import time
+    rng = np.random.default_rng(0)
+    B = 1 << 20
+    X = np.zeros((B, IN), np.float32)
+    X[:, :NFEAT] = rng.normal(0, 1, (B, NFEAT))
+    p = init_params(0)
+    t = time.perf_counter()
+    for _ in range(5):
+        gpu_forward(p, X)
+    print(f"forward 5 x {B} rows: {time.perf_counter() - t:.3f} s (incl. transfers)")
+    Xt = X[:10240]
+    T = (0.1 * np.sin(Xt[:, 0])).astype(np.float32)
+    t = time.perf_counter()
+    _, loss = gpu_train(p, Xt, T, epochs=50, batch=256, lr=1e-3)
+    print(f"train 10240 rows x 50 epochs: {time.perf_counter() - t:.3f} s, loss {loss:.3e}")
+    gpu_grad(p, X[:65536], np.zeros(65536, np.float32))
+
+
+if __name__ == "__main__":
+    import sys
+    if sys.argv[1:] == ["bench"]:
+        _bench()
+    else:
+        print("usage: python -m powspec_amd.mlp bench")

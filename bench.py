@@ -344,7 +344,24 @@ def run_ingest(P, data, tile, reps=3):
                       "lines_per_s": round(na / ta, 1)}}
 
 
-def cpu_baseline(c2_data):
+def parity_vs_oracle(gpu, ref, poles=(0, 2, 4)):
+    """Per-bin error of the GPU P_l against the FP64 oracle on the same
+    catalogue: mode counts compared exactly, |dP_l| / S_b with S_b = |P_0| +
+    shot the bin's un-subtracted power, and |dP_0| / |P_0|."""
+    shot = ref["shot"][0]
+    has = ref["cnt"] > 0
+    S = np.abs(ref["pl"][0][0]) + shot
+    out = {"cnt_exact": bool(np.array_equal(gpu["cnt"], ref["cnt"])),
+           "nbin": int(len(ref["cnt"]))}
+    for li, ell in enumerate(poles):
+        d = np.abs(gpu["pl"][0][li] - ref["pl"][0][li])[has] / S[has]
+        out[f"max_dP{ell}_over_Sb"] = float(d.max())
+    p0 = ref["pl"][0][0][has]
+    out["max_rel_dP0"] = float(np.max(np.abs(gpu["pl"][0][0][has] - p0) / np.abs(p0)))
+    return out
+
+
+def cpu_baseline(c2_data, gpu_res=None):
     """FP64 oracle (OpenMP, own FFT) on the whole C2 workload: every C2
     particle, N_g = 1024, PCS interlaced, l = 0,2,4, dk = 0.005 -- one mesh
     (about 10-20 s on the host cores, ~40 GB of host memory)."""
@@ -358,7 +375,11 @@ def cpu_baseline(c2_data):
     ta, tf, tb, tt = r["timings"]
     threads = O.lib().pso_num_threads()
     log(f"cpu oracle C2: {wall:.1f} s wall (assign {ta:.1f}, fft {tf:.1f}, bin {tb:.1f})")
+    par = parity_vs_oracle(gpu_res, r) if gpu_res is not None else None
+    if par:
+        log("C2 vs FP64 oracle: " + ", ".join(f"{k}={v}" for k, v in par.items()))
     return {"value": round(1.0 / wall, 5), "unit": "meshes/s", "cores": threads,
+            "parity_vs_oracle": par,
             "kind": "port",
             "sample": f"oracle/psoracle.c (FP64, OpenMP {threads} threads, own radix-2 FFT) "
                       f"on the full C2 workload ({len(c2_data)} particles, N_g=1024, PCS "
@@ -403,7 +424,7 @@ def main():
         extra = run_batched(P, data, args.nmock)
     survey = None if args.no_survey else run_survey(P)
     ingest = None if args.no_ingest else run_ingest(P, data, tile)
-    cpu = None if args.no_cpu else cpu_baseline(data)
+    cpu = None if args.no_cpu else cpu_baseline(data, res)
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_step, 3),

@@ -423,3 +423,17 @@ def test_survey_device_entry(psh, oracle):
                                 sums["alpha"], sums["shot"], sums["norm"])
     ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
     _check(got, ref, [0, 2, 4], sums["shot"], sums["norm"])
+
+
+def test_c2_full_size_parity(psh, oracle):
+    """The C2 workload at its full size (2.0e7 clustered particles, PCS,
+    N_g = 1024, interlaced, l = 0,2,4, dk = 0.005) against the FP64 oracle:
+    mode counts bit-exact, every P_l within the bar of the smaller tests."""
+    dev, n = psh.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
+    data = dev.to_numpy((n, 4))
+    kw = dict(gsize=1024, assign=3, intlace=True, bsize=1000.0, kbin=0.005, poles=[0, 2, 4])
+    got = psh.Context(psh.make_conf(**kw)).run_device(dev, n, float(n))
+    dev.free()
+    ref = oracle.run(oracle.make_conf(**kw), [data])
+    worst = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"C2 full size: {n} particles, {len(ref['cnt'])} bins, worst |dP_l|/S_b {worst:.2e}")

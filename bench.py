@@ -182,7 +182,7 @@ def run_batched(P, data, nmock, seed=1000):
     ctx.set_tiles(tile)
     L = P.load()
     ws = [float(s) for s in sizes]
-    ctx.run_batch(dev, off, ws, graph=True)           # capture + warm-up
+    bres = ctx.run_batch(dev, off, ws, graph=True)    # capture + warm-up
     L.psh_device_synchronize()
     reps = 3
     t0 = time.perf_counter()
@@ -204,16 +204,27 @@ def run_batched(P, data, nmock, seed=1000):
         single.run_device(View(ptr0 + 32 * int(off[m])), int(sizes[m]), ws[m])
     L.psh_device_synchronize()
     t0 = time.perf_counter()
+    plain = []
     for m in range(nmock):
-        single.run_device(View(ptr0 + 32 * int(off[m])), int(sizes[m]), ws[m])
+        plain.append(single.run_device(View(ptr0 + 32 * int(off[m])), int(sizes[m]), ws[m]))
     L.psh_device_synchronize()
     t_plain = time.perf_counter() - t0
+    # batched == plain per mock up to the FP32/FP64 summation order:
+    # |dP_l| / S_b, S_b = |P_0| + shot
+    worst = 0.0
+    for m in range(nmock):
+        S = np.abs(plain[m]["pl"][0][0]) + 1000.0 ** 3 / sizes[m]
+        for li in range(3):
+            d = np.abs(bres[m]["pl"][0][li] - plain[m]["pl"][0][li]) / S
+            worst = max(worst, float(np.max(d)))
+    log(f"batched vs plain: worst |dP_l|/S_b over {nmock} mocks {worst:.2e}")
     log(f"batched: {nmock / t_batch:.1f} meshes/s, plain: {nmock / t_plain:.1f} meshes/s")
     return {"workload": f"{nmock} clustered mocks, N_p in [2e5,3e6] (total {len(allp)}), "
                         "TSC N_g=256 interlaced, l=0,2,4, dk=0.01",
             "batched_meshes_per_s": round(nmock / t_batch, 1),
             "plain_meshes_per_s": round(nmock / t_plain, 1),
             "batched_over_plain": round(t_plain / t_batch, 2),
+            "batched_vs_plain_max_dP_over_Sb": worst,
             "tile": list(tile),
             "batched_particles_per_s": round(len(allp) / t_batch, 1)}
 

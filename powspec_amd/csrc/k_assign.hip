@@ -672,9 +672,12 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
 template <int TOP>
 __device__ __forceinline__ int tile_exponent(const int *flags, uint32_t cnt) {
   const float maxw = __uint_as_float((unsigned)flags[1]);
-  const double bound = (double)maxw * (double)cnt;
+  const double bound = (double)maxw * (double)cnt * (1.0 + 1e-7);
   if (!(bound > 0.0)) return TOP;
-  int s = TOP - (int)ceil(log2(bound * (1.0 + 1e-7)));
+  // ceil(log2(bound)) exactly: bound = m 2^e with m in [0.5, 1)
+  int e;
+  const double m = frexp(bound, &e);
+  const int s = TOP - (m == 0.5 ? e - 1 : e);
   return s < -100 ? -100 : (s > 100 ? 100 : s);
 }
 
@@ -685,25 +688,28 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                float *__restrict__ meshA, float *__restrict__ meshB,
                size_t mstride, const int *__restrict__ flags, size_t ntotal) {
   using AT = typename Acc<WIDE>::T;
-  extern __shared__ long long ldsraw[];
+  extern __shared__ __align__(16) long long ldsraw[];
   AT *ldsq = reinterpret_cast<AT *>(ldsraw);
   const int T[3] = {g.tile[0], g.tile[1], g.tile[2]};
   const int tcells = T[0] * T[1] * T[2];
   AT *ldsA = ldsq;
   AT *ldsB = ldsq + tcells;
-  const size_t tiles_per_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
-  const size_t tile = blockIdx.x;
-  const int mock = (int)(tile / tiles_per_mock);
-  size_t tl = tile - (size_t)mock * tiles_per_mock;
-  const int tzi = (int)(tl % g.ntile[2]);
-  tl /= g.ntile[2];
-  const int tyi = (int)(tl % g.ntile[1]);
-  const int txi = (int)(tl / g.ntile[1]);
+  // tile counts per axis are powers of two (Ng and the tile extents are)
+  const int sz = __builtin_ctz(g.ntile[2]), sy = __builtin_ctz(g.ntile[1]);
+  const int stpm = sz + sy + __builtin_ctz(g.ntile[0]);
+  const uint32_t tile = blockIdx.x;
+  const int mock = (int)(tile >> stpm);
+  const uint32_t tl = tile & ((1u << stpm) - 1u);
+  const int tzi = (int)(tl & (g.ntile[2] - 1));
+  const int tyi = (int)((tl >> sz) & (g.ntile[1] - 1));
+  const int txi = (int)(tl >> (sz + sy));
   const int T0[3] = {txi * T[0], tyi * T[1], tzi * T[2]};
   (void)ntotal;
-  for (int i = threadIdx.x; i < tcells; i += blockDim.x) {
-    ldsA[i] = 0;
-    if (INTLACE) ldsB[i] = 0;
+  {
+    // both grids are contiguous in LDS: clear them with 16-byte stores
+    uint4 *z4 = reinterpret_cast<uint4 *>(ldsq);
+    const int n4 = (int)(sizeof(AT) * tcells * (INTLACE ? 2 : 1) / 16);
+    for (int i = threadIdx.x; i < n4; i += blockDim.x) z4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
 
@@ -773,10 +779,11 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   float *outA = meshA + (size_t)mock * mstride;
   float *outB = INTLACE ? meshB + (size_t)mock * mstride : nullptr;
   const int nrow = T[0] * T[1];
-  const int q4 = T[2] >> 2;   // T[2] is a multiple of 4 (>= 4)
+  const int q4s = g.tile_shift[2] - 2;   // T[2] = 4 * 2^q4s (T[2] >= 4)
+  const int q4 = 1 << q4s;
   for (int i = threadIdx.x; i < nrow * q4; i += blockDim.x) {
-    const int row = i / q4, q = i - row * q4;
-    const int lx = row / T[1], ly = row - lx * T[1];
+    const int row = i >> q4s, q = i & (q4 - 1);
+    const int lx = row >> g.tile_shift[1], ly = row & (T[1] - 1);
     const size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
                         T0[2] + 4 * q;
     const int li = row * T[2] + 4 * q;

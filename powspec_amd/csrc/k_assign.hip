@@ -682,7 +682,7 @@ __device__ __forceinline__ int tile_exponent(const int *flags, uint32_t cnt) {
 }
 
 template <int SCHEME, bool INTLACE, bool WIDE>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                const Rec *__restrict__ recs, size_t rec_cap,
                float *__restrict__ meshA, float *__restrict__ meshB,
@@ -725,9 +725,8 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   // gcd(S, cnt) = 1 (a permutation): neighbouring lanes take records far
   // apart in fill order, i.e. far apart in the (usually space-ordered)
   // catalogue, so their windows rarely share LDS banks.
-  // (a bucket of at most one record per thread is taken in fill order: the
-  // permutation's gcd search and 64-bit remainders would cost more than the
-  // bank conflicts they avoid)
+  // (a bucket of at most one record per thread skips the search: the gcd
+  // loop and 64-bit remainders would cost more than they save)
   uint32_t S = 1u;
   if (cnt > blockDim.x) {
     S = ((uint32_t)((float)cnt * 0.618034f)) | 1u;
@@ -747,15 +746,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
       }
     }
   }
-  uint32_t x = threadIdx.x, D = blockDim.x;
-  if (S != 1u) {
-    x = (uint32_t)(((uint64_t)threadIdx.x * S) % cnt);
-    D = (uint32_t)(((uint64_t)blockDim.x * S) % cnt);
-  }
-  for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-    const Rec r = recs[s0 + x];
-    x += D;
-    if (x >= cnt) x -= cnt;
+  auto deposit_rec = [&](const Rec &r) {
     int b[3] = {(int)(r.bxy & 0xffffu), (int)(r.bxy >> 16), (int)(r.bz & 0xffffu)};
     const float ws = r.w * scale;
     deposit_one<SCHEME, WIDE>(ldsA, T, T0, g.Ng, b, r.f, ws);
@@ -771,6 +762,22 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
         fs[a] = r.f[a] + 0.5f - (float)d;
       }
       deposit_one<SCHEME, WIDE>(ldsB, T, T0, g.Ng, bs, fs, ws);
+    }
+  };
+  if (cnt <= blockDim.x) {
+    // at most one record per thread: lane l of wave w takes record
+    // l * waves + w (a bijection of [0, blockDim)), so a wave's lanes hold
+    // records `waves` apart in fill order rather than neighbours
+    const uint32_t j = (threadIdx.x & 63u) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (j < cnt) deposit_rec(recs[s0 + j]);
+  } else {
+    uint32_t x = (uint32_t)(((uint64_t)threadIdx.x * S) % cnt);
+    const uint32_t D = (uint32_t)(((uint64_t)blockDim.x * S) % cnt);
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const Rec r = recs[s0 + x];
+      x += D;
+      if (x >= cnt) x -= cnt;
+      deposit_rec(r);
     }
   }
   __syncthreads();
@@ -820,7 +827,12 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-  hipLaunchKernelGGL(k, dim3((unsigned)ntiles), dim3(256), lds, s, g, starts, recs, rec_cap,
+  // one thread per 16 cells, 256..1024: a large tile fills its CU alone (its
+  // LDS admits no second workgroup), so it brings its own waves
+  const size_t tcells = (size_t)g.tile[0] * g.tile[1] * g.tile[2];
+  unsigned threads = 256;
+  while (threads < 1024 && threads * 16 < tcells) threads *= 2;
+  hipLaunchKernelGGL(k, dim3((unsigned)ntiles), dim3(threads), lds, s, g, starts, recs, rec_cap,
                      meshA, meshB, mstride, flags, ntotal);
   return hipGetLastError();
 }

@@ -284,7 +284,6 @@ __device__ __forceinline__ void pass_store(const float2 *v, ST st) {
 // one component at a time.
 template <int N, int C, int T, int P, bool COLMAP>
 __device__ __forceinline__ void exchange(float2 *v, float *lds) {
-  using G0 = PassGeom<N, C, T, P, COLMAP>;
   using G1 = PassGeom<N, C, T, P + 1, COLMAP>;
   constexpr int V = G1::NB * G1::R;
   float tmp[V];

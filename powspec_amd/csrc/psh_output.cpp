@@ -20,7 +20,11 @@ struct File {
   }
   template <class... A>
   void put(const char *fmt, A... a) {
-    if (ok && fprintf(fp, fmt, a...) < 0) ok = false;
+    if constexpr (sizeof...(A) == 0) {
+      if (ok && fputs(fmt, fp) < 0) ok = false;
+    } else {
+      if (ok && fprintf(fp, fmt, a...) < 0) ok = false;
+    }
   }
 };
 

@@ -176,6 +176,14 @@ struct Lds {
   __device__ static __forceinline__ int idx(int c, int n) {
     return COLMAP ? n * C + c + ((n >> 3) << 3) : c * LN + n + (n >> 4);
   }
+  // idx(c, n0 + d) - idx(c, n0) for the element groups the passes touch:
+  // a butterfly's elements n0 + r*S (n0 mod S < S, S and the radix powers of
+  // two) never carry across a padding block (16 rows, or 8 columns), so the
+  // difference is this constant and an element's address is one per-thread
+  // base plus an immediate offset.
+  __device__ static constexpr int off(int d) {
+    return COLMAP ? d * C + ((d >> 3) << 3) : d + (d >> 4);
+  }
 };
 
 // threadIdx.x hidden from loop-invariant code motion.  In persistent kernels
@@ -284,35 +292,48 @@ __device__ __forceinline__ void pass_store(const float2 *v, ST st) {
 // one component at a time.
 template <int N, int C, int T, int P, bool COLMAP>
 __device__ __forceinline__ void exchange(float2 *v, float *lds) {
+  using G0 = PassGeom<N, C, T, P, COLMAP>;
   using G1 = PassGeom<N, C, T, P + 1, COLMAP>;
+  using L = Lds<N, C, COLMAP>;
   constexpr int V = G1::NB * G1::R;
   float tmp[V];
-  pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
-    lds[Lds<N, C, COLMAP>::idx(c, n)] = x.x;
-  });
+  // outputs of pass P: element base + r NS of line c
+  auto store = [&](bool im) {
+#pragma unroll
+    for (int q = 0; q < G0::NB; q++) {
+      int c, j;
+      if (!G0::map(q, c, j)) continue;
+      const int k = j & (G0::NS - 1);
+      float *row = lds + L::idx(c, (j / G0::NS) * G0::NS * G0::R + k);
+#pragma unroll
+      for (int r = 0; r < G0::R; r++) {
+        const float2 x = v[q * G0::R + r];
+        row[L::off(r * G0::NS)] = im ? x.y : x.x;
+      }
+    }
+  };
+  // inputs of pass P+1: element j + r NBF of line c
+  auto load = [&](bool im) {
+#pragma unroll
+    for (int q = 0; q < G1::NB; q++) {
+      int c, j;
+      const bool ok = G1::map(q, c, j);
+      const float *row = lds + L::idx(c, j);
+#pragma unroll
+      for (int r = 0; r < G1::R; r++) {
+        const float x = ok ? row[L::off(r * G1::NBF)] : 0.f;
+        if (im) v[q * G1::R + r] = make_float2(tmp[q * G1::R + r], x);
+        else tmp[q * G1::R + r] = x;
+      }
+    }
+  };
+  store(false);
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < G1::NB; q++) {
-    int c, j;
-    bool ok = G1::map(q, c, j);
-#pragma unroll
-    for (int r = 0; r < G1::R; r++)
-      tmp[q * G1::R + r] = ok ? lds[Lds<N, C, COLMAP>::idx(c, j + r * G1::NBF)] : 0.f;
-  }
+  load(false);
   __syncthreads();
-  pass_store<N, C, T, P, COLMAP>(v, [&](int c, int n, float2 x) {
-    lds[Lds<N, C, COLMAP>::idx(c, n)] = x.y;
-  });
+  store(true);
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < G1::NB; q++) {
-    int c, j;
-    bool ok = G1::map(q, c, j);
-#pragma unroll
-    for (int r = 0; r < G1::R; r++)
-      v[q * G1::R + r] = make_float2(
-          tmp[q * G1::R + r], ok ? lds[Lds<N, C, COLMAP>::idx(c, j + r * G1::NBF)] : 0.f);
-  }
+  load(true);
   __syncthreads();
 }
 

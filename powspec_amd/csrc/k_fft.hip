@@ -485,7 +485,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           v = F.x * F.x + F.y * F.y;
         }
         if (EPI == PSH_EPI_SURVEY0) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
-        lds[L::idx(c, i)] = v;
+        lds[L::idx(c, j) + L::off(r * G::NS)] = v;   // i = j + r NS
       }
     }
     __syncthreads();
@@ -506,11 +506,13 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         kdy = (float)a.ax.kd[1][y];
         kdz = (float)a.ax.kd[2][kz];
       }
+      const int i0 = opaque_tid<T>() / C;
+      const float *vrow = lds + L::idx(c, i0);
 #pragma unroll 4
       for (int k = 0; k < N * C / T; k++) {
-        const int i = opaque_tid<T>() / C + k * (T / C);
+        const int i = i0 + k * (T / C);
         const float4 xt = xtab[i];
-        const float v = lds[L::idx(c, i)];
+        const float v = vrow[L::off(k * (T / C))];
         const float k2f = xt.y + ksqyz;
         const float rs = rsqrtf(k2f);
         int kb = bin_fast(b, k2f * rs);

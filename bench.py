@@ -367,8 +367,9 @@ def parity_vs_oracle(gpu, ref, poles=(0, 2, 4)):
     for li, ell in enumerate(poles):
         d = np.abs(gpu["pl"][0][li] - ref["pl"][0][li])[has] / S[has]
         out[f"max_dP{ell}_over_Sb"] = float(d.max())
-    p0 = ref["pl"][0][0][has]
-    out["max_rel_dP0"] = float(np.max(np.abs(gpu["pl"][0][0][has] - p0) / np.abs(p0)))
+    nz = has & (ref["pl"][0][0] != 0)   # bins whose P_0 is exactly 0 have no relative error
+    p0 = ref["pl"][0][0][nz]
+    out["max_rel_dP0"] = float(np.max(np.abs(gpu["pl"][0][0][nz] - p0) / np.abs(p0)))
     return out
 
 

@@ -319,15 +319,17 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
 
 // x pass: up to 1024 threads (16 waves, 128 VGPRs).  The plain kernel takes
 // 16-column tiles (128-byte row segments) and 16 values per thread.  The
-// interlaced kernel transforms both fields pass by pass together with 8
-// values per thread and field, which bounds its tile at 1024 threads x 8 / N
-// columns: 16 up to N = 512, 8 above (64-byte half lines, completed by the
-// neighbouring tile on the same XCD, see xcd_tile).  With T a multiple of C,
-// a thread works on the same column (kz) in every pass and in the binning
+// interlaced kernel transforms both fields pass by pass together, also in
+// 16-column tiles up to N = 1024 (16 values per thread and field there; the
+// register limit spills a few values outside the butterflies, which costs
+// far less than the 50 % of extra HBM reads of 8-column tiles: 5.05 -> 4.25
+// ms at N = 1024); at N = 2048 8 columns (64-byte half lines, completed by
+// the neighbouring tile on the same XCD, see xcd_tile).  With T a multiple of
+// C, a thread works on the same column (kz) in every pass and in the binning
 // loop.
 template <int N, bool IL>
 struct XCfg {
-  static constexpr int C = IL ? (N <= 512 ? 16 : 8) : (N >= 2048 ? 8 : 16);
+  static constexpr int C = IL ? (N <= 1024 ? 16 : 8) : (N >= 2048 ? 8 : 16);
   // up to N = 256 the first pass's radix-16 butterflies set the values per
   // thread whatever T is: T = its butterfly count leaves no thread idle
   static constexpr bool SMALL = IL && N <= 256;

@@ -345,10 +345,10 @@ struct XCfg {
 // LDS of the x pass besides the bins: ksq_x (FP64), xtab, twiddles, exchange
 // tile.  The twiddles are read from LDS: read from global memory they are
 // invariant over the persistent tile loop and stay live in registers.
-template <int N>
+template <int N, bool IL>
 constexpr size_t xbin_static_lds() {
   return sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
-         sizeof(float) * Lds<N, 16, true>::SIZE;
+         sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
 }
 
 // Bin of a mode from its FP32 |k|, kmodf: -1 outside the bins, -2 when
@@ -634,7 +634,8 @@ struct XBin {
     const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
-    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) + xbin_static_lds<N>();
+    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) +
+                         (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
     const dim3 g(grid, nmock), b(il ? XCfg<N, true>::T : XCfg<N, false>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
 #define PSH_XB(E, I)                                                          \

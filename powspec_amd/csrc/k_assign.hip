@@ -625,22 +625,30 @@ struct Acc<true> {
   }
 };
 
-// Window axes a >= branch_axes are deposited without branches: a window
-// cell outside the tile gets weight 0 and the address of cell 0 along that
-// axis (adding the integer 0 there is exact), so the inner loops carry no
-// per-cell exec-mask updates.  Axes below keep a branch per plane or row,
-// which skips whole planes or rows of cells outside the tile.  Tiles are
-// longest along z and shortest along x, so windows rarely leave them along
-// z and the zero additions are few.  Measured on MI355X: z and y without
-// branches for CIC and TSC, z alone for PCS (whose 4-cell rows make a
-// skipped row worth more); both cut the deposit by 6-25 %.
+// Window axes x and y: an axis a >= branch_axes is deposited without
+// branches (a window cell outside the tile gets weight 0 and the address of
+// cell 0 along that axis; adding the integer 0 there is exact), an axis
+// below keeps a branch per plane or row, which skips whole planes or rows
+// outside the tile.  Measured on MI355X: y without branches for CIC and TSC,
+// with them for PCS (whose 4-cell rows make a skipped row worth more).
+// Axis z: the tile's rows are padded by W cells on each side, so a window
+// row is W consecutive cells from one base address (constant LDS offsets,
+// no per-cell mask or address arithmetic); cells in the padding are
+// dropped, their tile deposits them from its own record.  A record's tile
+// meets the union of the plain and the shifted window, so the plain
+// window's first cell b lies in [T0 - W, T0 + T[2]) and the shifted one's
+// in [T0 - W + 1, T0 + T[2]] (up to a wrap by Ng): W cells of padding on
+// each side hold every window cell outside the tile.  When T[2] + W >= Ng
+// (one tile, or two tiles on a tiny grid) a window offset is ambiguous
+// modulo Ng, and z goes the branch-free way of x and y instead (ZPAD false).
 template <int SCHEME>
 constexpr int branch_axes() { return SCHEME == 3 ? 2 : 1; }
 
-// Accumulate one particle's window into an LDS tile of fixed-point cells.
-template <int SCHEME, bool WIDE>
+// Accumulate one particle's window into an LDS tile of fixed-point cells
+// (rows of P2 = T[2] + 2 W cells, the tile's first z cell at W).
+template <int SCHEME, bool WIDE, bool ZPAD>
 __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const int *T,
-                                            const int *T0, int Ng, const int *b,
+                                            const int *T0, int Ng, int P2, const int *b,
                                             const float *f, float w) {
   constexpr int W = SCHEME + 1;
   constexpr int kBranchAxes = branch_axes<SCHEME>();
@@ -651,6 +659,7 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
   for (int a = 0; a < 3; a++) {
     window_weights(SCHEME, f[a], wt[a]);
     m[a] = 0;
+    if (ZPAD && a == 2) break;
 #pragma unroll
     for (int n = 0; n < W; n++) {
       int c = b[a] + n;
@@ -666,22 +675,30 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
       }
     }
   }
+  // ZPAD: offset in a padded row of the window's first z cell, b - T0 + W in
+  // [0, T[2] + W] (cells up to T[2] + 2W - 1 < P2).  Windows wrapped by Ng
+  // (tile at z = 0, b in [Ng - W, Ng)) give b + W in [Ng, Ng + W): with
+  // T[2] + W < Ng the mask maps them to b - Ng + W and nothing else moves.
+  const int zoff = ZPAD ? ((b[2] - T0[2] + W) & (Ng - 1)) : W;
   if (SCHEME == 3) w *= (1.0f / 216.0f);
+  // row offsets: one integer multiply per window cell of x and of y
+  int ry[4];
+#pragma unroll
+  for (int j = 0; j < W; j++) ry[j] = l[1][j] * P2 + zoff;
+  const int PP = T[1] * P2;
 #pragma unroll
   for (int i = 0; i < W; i++) {
     if (kBranchAxes > 0 && !(m[0] >> i & 1)) continue;
     const float wx = wt[0][i] * w;
-    const int rx = l[0][i] * T[1];
+    const int rx = l[0][i] * PP;
 #pragma unroll
     for (int j = 0; j < W; j++) {
       if (kBranchAxes > 1 && !(m[1] >> j & 1)) continue;
       const float wxy = wx * wt[1][j];
-      const int rxy = (rx + l[1][j]) * T[2];
+      typename Acc<WIDE>::T *row = lds + rx + ry[j];
 #pragma unroll
-      for (int k = 0; k < W; k++) {
-        if (kBranchAxes > 2 && !(m[2] >> k & 1)) continue;
-        Acc<WIDE>::add(&lds[rxy + l[2][k]], wxy * wt[2][k]);
-      }
+      for (int k = 0; k < W; k++)
+        Acc<WIDE>::add(ZPAD ? row + k : row + l[2][k], wxy * wt[2][k]);
     }
   }
 }
@@ -711,9 +728,11 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   extern __shared__ __align__(16) long long ldsraw[];
   AT *ldsq = reinterpret_cast<AT *>(ldsraw);
   const int T[3] = {g.tile[0], g.tile[1], g.tile[2]};
-  const int tcells = T[0] * T[1] * T[2];
+  const int P2 = psh_tile_pitch(T[2], SCHEME);
+  const int gcells = (int)psh_tile_cells(T, SCHEME);   // per grid, padded
+  const bool zpad = T[2] + SCHEME + 1 < g.Ng;
   AT *ldsA = ldsq;
-  AT *ldsB = ldsq + tcells;
+  AT *ldsB = ldsq + gcells;
   // tile counts per axis are powers of two (Ng and the tile extents are)
   const int sz = __builtin_ctz(g.ntile[2]), sy = __builtin_ctz(g.ntile[1]);
   const int stpm = sz + sy + __builtin_ctz(g.ntile[0]);
@@ -728,7 +747,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   {
     // both grids are contiguous in LDS: clear them with 16-byte stores
     uint4 *z4 = reinterpret_cast<uint4 *>(ldsq);
-    const int n4 = (int)(sizeof(AT) * tcells * (INTLACE ? 2 : 1) / 16);
+    const int n4 = (int)(sizeof(AT) * gcells * (INTLACE ? 2 : 1) / 16);
     for (int i = threadIdx.x; i < n4; i += blockDim.x) z4[i] = make_uint4(0u, 0u, 0u, 0u);
   }
   __syncthreads();
@@ -769,7 +788,11 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   auto deposit_rec = [&](const Rec &r) {
     int b[3] = {(int)(r.bxy & 0xffffu), (int)(r.bxy >> 16), (int)(r.bz & 0xffffu)};
     const float ws = r.w * scale;
-    deposit_one<SCHEME, WIDE>(ldsA, T, T0, g.Ng, b, r.f, ws);
+    auto dep = [&](AT *lds, const int *bb, const float *ff) {
+      if (zpad) deposit_one<SCHEME, WIDE, true>(lds, T, T0, g.Ng, P2, bb, ff, ws);
+      else deposit_one<SCHEME, WIDE, false>(lds, T, T0, g.Ng, P2, bb, ff, ws);
+    };
+    dep(ldsA, b, r.f);
     if (INTLACE) {
       const uint32_t dl = r.bz >> 16;
       int bs[3];
@@ -781,7 +804,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
         if (bs[a] >= g.Ng) bs[a] -= g.Ng;
         fs[a] = r.f[a] + 0.5f - (float)d;
       }
-      deposit_one<SCHEME, WIDE>(ldsB, T, T0, g.Ng, bs, fs, ws);
+      dep(ldsB, bs, fs);
     }
   };
   if (cnt <= blockDim.x) {
@@ -817,7 +840,7 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
     const int lx = row >> g.tile_shift[1], ly = row & (T[1] - 1);
     const size_t gidx = ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * g.row_pitch +
                         T0[2] + 4 * q;
-    const int li = row * T[2] + 4 * q;
+    const int li = row * P2 + SCHEME + 1 + 4 * q;
     float4 va;
     va.x = (float)ldsA[li] * inv;
     va.y = (float)ldsA[li + 1] * inv;
@@ -841,8 +864,8 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
                           size_t rec_cap, float *meshA, float *meshB,
                           size_t mstride, const int *flags, size_t ntotal,
                           hipStream_t s) {
-  const size_t lds = (g.wide ? sizeof(long long) : sizeof(int)) * (size_t)g.tile[0] *
-                     g.tile[1] * g.tile[2] * (g.intlace ? 2 : 1);
+  const size_t lds = (g.wide ? sizeof(long long) : sizeof(int)) *
+                     psh_tile_cells(g.tile, SCHEME) * (g.intlace ? 2 : 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   auto k = g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true>
                                : assign_deposit<SCHEME, false, true>)

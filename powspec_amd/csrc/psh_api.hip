@@ -158,14 +158,17 @@ int set_err(psh_ctx *ctx, int code, const char *fmt, ...) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// Deposit tile shapes the kernels accept: powers of two, rows of at least 4
-// cells along z, LDS of both grids within 160 KB at `cell_bytes` per cell
+// Deposit tile shapes the kernels accept: powers of two, at least the
+// window width W = scheme + 1 along every axis (a window plus the shifted
+// grid's extra cell then meets at most two tiles per axis, which the
+// bucketing assumes), rows of at least 4 cells along z, LDS of both grids
+// (z rows padded for the window) within 160 KB at `cell_bytes` per cell
 // (4: 32-bit fixed point of boxes, 8: 64-bit of surveys).
-bool tiles_ok(const int t[3], int intlace, int cell_bytes) {
+bool tiles_ok(const int t[3], int scheme, int intlace, int cell_bytes) {
   for (int a = 0; a < 3; a++)
-    if (!is_pow2(t[a]) || t[a] > 2048) return false;
+    if (!is_pow2(t[a]) || t[a] > 2048 || t[a] < scheme + 1) return false;
   return t[2] >= 4 &&
-         (size_t)t[0] * t[1] * t[2] * cell_bytes * (intlace ? 2 : 1) <= 160 * 1024;
+         psh_tile_cells(t, scheme) * cell_bytes * (intlace ? 2 : 1) <= 160 * 1024;
 }
 
 // src/multipole.c:46-100: per-axis alias correction, series branch for
@@ -415,7 +418,7 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   if (const char *env = getenv("PSH_TILE")) {
     int t[3] = {0, 0, 0};
     if (sscanf(env, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 &&
-        tiles_ok(t, ctx->conf.intlace, ctx->survey ? 8 : 4))
+        tiles_ok(t, ctx->conf.assign, ctx->conf.intlace, ctx->survey ? 8 : 4))
       for (int a = 0; a < 3; a++) tl[a] = t[a];
   }
   if (ctx->tile_req[0])
@@ -604,7 +607,7 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
     ntotal += p.n;
     if (p.wscale < 0.f) ctx->ag.wide = 1;   // data - alpha * randoms
   }
-  if (ctx->ag.wide && !tiles_ok(ctx->ag.tile, ctx->conf.intlace, 8))
+  if (ctx->ag.wide && !tiles_ok(ctx->ag.tile, ctx->conf.assign, ctx->conf.intlace, 8))
     return set_err(ctx, PSH_ERR_UNSUPPORTED,
                    "tile %dx%dx%d too large for 64-bit accumulation of weights of both signs",
                    ctx->ag.tile[0], ctx->ag.tile[1], ctx->ag.tile[2]);
@@ -1554,7 +1557,7 @@ extern "C" int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, s
 
 extern "C" int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]) {
   if (!ctx || !tile) return set_err(ctx, PSH_ERR_ARG, "null argument");
-  if (!tiles_ok(tile, ctx->conf.intlace, ctx->survey ? 8 : 4))
+  if (!tiles_ok(tile, ctx->conf.assign, ctx->conf.intlace, ctx->survey ? 8 : 4))
     return set_err(ctx, PSH_ERR_ARG, "invalid tile shape %dx%dx%d", tile[0], tile[1], tile[2]);
   for (int a = 0; a < 3; a++) ctx->tile_req[a] = tile[a];
   if (ctx->box_set) {   // re-derive the assignment geometry with the new tiles
@@ -1604,7 +1607,7 @@ extern "C" int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path) {
                &t[0], &t[1], &t[2], &ms) != 10)
       continue;
     if (arch != a || Ng != ctx->Ng || scheme != ctx->conf.assign ||
-        il != ctx->conf.intlace || nm != cls || !tiles_ok(t, il, ctx->survey ? 8 : 4))
+        il != ctx->conf.intlace || nm != cls || !tiles_ok(t, scheme, il, ctx->survey ? 8 : 4))
       continue;
     if (ms < best_ms) {   // the latest measurement of equal speed wins ties
       best_ms = ms;
@@ -1675,7 +1678,7 @@ extern "C" int psh_autotune(const psh_conf *conf, const psh_data *d_data, const 
         const int t[3] = {opts[a], opts[b], zopts[z]};
         const size_t cells = (size_t)t[0] * t[1] * t[2];
         if (cells < 2048 || cells > 16384 || t[0] > t[1] || t[0] > c.gsize ||
-            t[1] > c.gsize || t[2] > c.gsize || !tiles_ok(t, c.intlace, conf->issim ? 4 : 8))
+            t[1] > c.gsize || t[2] > c.gsize || !tiles_ok(t, c.assign, c.intlace, conf->issim ? 4 : 8))
           continue;
         if ((rc = psh_ctx_set_tiles(ctx, t))) break;
         double times[4];

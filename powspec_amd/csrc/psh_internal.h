@@ -76,6 +76,15 @@ struct PshAssignGeom {
   double L[3];          // box size
 };
 
+// Deposit tile in LDS, per grid: rows along z padded by the window width
+// W = scheme + 1 on each side (window cells past the tile's z range land
+// there and are dropped; the neighbouring tile deposits them from its own
+// record), the grid's cell count rounded up to 4 (16-byte clears).
+__host__ __device__ inline int psh_tile_pitch(int tz, int scheme) { return tz + 2 * (scheme + 1); }
+__host__ __device__ inline size_t psh_tile_cells(const int t[3], int scheme) {
+  return ((size_t)t[0] * t[1] * psh_tile_pitch(t[2], scheme) + 3) & ~(size_t)3;
+}
+
 // One catalogue part of a (possibly batched) assignment; the table lives in
 // device memory (uploaded before any graph capture).
 struct PshDevPart {

@@ -9,8 +9,8 @@ from test_gpu_parity import _check, _uniform
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("tile", [(4, 4, 128), (8, 8, 64), (16, 16, 16), (32, 32, 16),
-                                  (4, 32, 64)])
+@pytest.mark.parametrize("tile", [(4, 4, 128), (8, 8, 64), (16, 16, 16), (32, 16, 16),
+                                  (4, 32, 64), (4, 4, 8)])
 def test_tile_shapes_keep_parity(psh, oracle, tile):
     Ng, L, n = 64, 500.0, 60000
     d = _uniform(n, [L] * 3, 51, weights=True)
@@ -27,6 +27,13 @@ def test_invalid_tiles_rejected(psh):
     for bad in [(3, 8, 32), (8, 8, 2), (64, 64, 64)]:
         with pytest.raises(psh.PshError) as e:
             ctx.set_tiles(bad)
+        assert e.value.code == psh.ERR_ARG
+    # PCS: tiles narrower than the 4-cell window, or whose padded z rows
+    # (4 cells each side) overflow the LDS
+    pcs = psh.Context(psh.make_conf(gsize=64, assign=3, bsize=500.0, kbin=0.02, intlace=True))
+    for bad in [(2, 8, 32), (8, 2, 32), (32, 32, 16)]:
+        with pytest.raises(psh.PshError) as e:
+            pcs.set_tiles(bad)
         assert e.value.code == psh.ERR_ARG
 
 

@@ -607,21 +607,26 @@ __device__ __forceinline__ void window_weights(int scheme, float f, float *w) {
 //    where large-scale modes are small differences of large sums.
 template <bool WIDE>
 struct Acc;
+// add(p, a, b, half) accumulates a * b rounded to an integer, half being
+// 0.5 with the sign of the product.  32-bit: one FMA and a truncating
+// conversion, trunc(a b + half) = a b rounded half away from zero (the
+// rounding of a b + half in FP32 can move a product of 2^23 or more by one
+// unit, a relative 1.2e-7, the FP32 precision the tile carries anyway).
 template <>
 struct Acc<false> {
   using T = int;
   static constexpr int TOP = 30;
-  __device__ static __forceinline__ void add(int *p, float v) {
-    atomicAdd(p, __float2int_rn(v));
+  __device__ static __forceinline__ void add(int *p, float a, float b, float half) {
+    atomicAdd(p, (int)__builtin_fmaf(a, b, half));
   }
 };
 template <>
 struct Acc<true> {
   using T = long long;
   static constexpr int TOP = 62;
-  __device__ static __forceinline__ void add(long long *p, float v) {
+  __device__ static __forceinline__ void add(long long *p, float a, float b, float) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p),
-              (unsigned long long)__float2ll_rn(v));
+              (unsigned long long)__float2ll_rn(a * b));
   }
 };
 
@@ -681,6 +686,7 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
   // T[2] + W < Ng the mask maps them to b - Ng + W and nothing else moves.
   const int zoff = ZPAD ? ((b[2] - T0[2] + W) & (Ng - 1)) : W;
   if (SCHEME == 3) w *= (1.0f / 216.0f);
+  const float half = __builtin_copysignf(0.5f, w);   // window weights are >= 0
   // row offsets: one integer multiply per window cell of x and of y
   int ry[4];
 #pragma unroll
@@ -698,7 +704,7 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
       typename Acc<WIDE>::T *row = lds + rx + ry[j];
 #pragma unroll
       for (int k = 0; k < W; k++)
-        Acc<WIDE>::add(ZPAD ? row + k : row + l[2][k], wxy * wt[2][k]);
+        Acc<WIDE>::add(ZPAD ? row + k : row + l[2][k], wxy, wt[2][k], half);
     }
   }
 }

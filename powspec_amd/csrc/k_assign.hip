@@ -724,7 +724,19 @@ __device__ __forceinline__ int tile_exponent(const int *flags, uint32_t cnt) {
   return s < -100 ? -100 : (s > 100 ? 100 : s);
 }
 
-template <int SCHEME, bool INTLACE, bool WIDE>
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+// Four consecutive mesh cells; non-temporal when the meshes are much larger
+// than the caches (a template parameter: under a runtime flag the compiler
+// merges both stores into an ordinary one).
+template <bool NT>
+__device__ __forceinline__ void store4(float *p, float4 v) {
+  const v4f_t x = {v.x, v.y, v.z, v.w};
+  if constexpr (NT) __builtin_nontemporal_store(x, reinterpret_cast<v4f_t *>(p));
+  else *reinterpret_cast<v4f_t *>(p) = x;
+}
+
+template <int SCHEME, bool INTLACE, bool WIDE, bool NT>
 __global__ void __launch_bounds__(1024)
 assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                const Rec *__restrict__ recs, size_t rec_cap,
@@ -852,14 +864,14 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
     va.y = (float)ldsA[li + 1] * inv;
     va.z = (float)ldsA[li + 2] * inv;
     va.w = (float)ldsA[li + 3] * inv;
-    *reinterpret_cast<float4 *>(outA + gidx) = va;
+    store4<NT>(outA + gidx, va);
     if (INTLACE) {
       float4 vb;
       vb.x = (float)ldsB[li] * inv;
       vb.y = (float)ldsB[li + 1] * inv;
       vb.z = (float)ldsB[li + 2] * inv;
       vb.w = (float)ldsB[li + 3] * inv;
-      *reinterpret_cast<float4 *>(outB + gidx) = vb;
+      store4<NT>(outB + gidx, vb);
     }
   }
 }
@@ -873,10 +885,15 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
   const size_t lds = (g.wide ? sizeof(long long) : sizeof(int)) *
                      psh_tile_cells(g.tile, SCHEME) * (g.intlace ? 2 : 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  auto k = g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true>
-                               : assign_deposit<SCHEME, false, true>)
-                  : (g.intlace ? assign_deposit<SCHEME, true, false>
-                               : assign_deposit<SCHEME, false, false>);
+  auto k = g.stream
+               ? (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, true>
+                                      : assign_deposit<SCHEME, false, true, true>)
+                         : (g.intlace ? assign_deposit<SCHEME, true, false, true>
+                                      : assign_deposit<SCHEME, false, false, true>))
+               : (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, false>
+                                      : assign_deposit<SCHEME, false, true, false>)
+                         : (g.intlace ? assign_deposit<SCHEME, true, false, false>
+                                      : assign_deposit<SCHEME, false, false, false>));
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);

@@ -112,6 +112,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
              const float2 *__restrict__ tw, YlmDev ylm) {
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
+  constexpr bool NT = stream_mesh<N>();
   __shared__ float lds[Lds<M, C, false>::SIZE];
   const size_t line0 = (size_t)blockIdx.x * C;
   const size_t lpm = (size_t)N * N;
@@ -122,7 +123,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     if (line >= nlines) return make_float2(0.f, 0.f);
     size_t mesh = line / lpm, rowi = line - mesh * lpm;
     const float *row = in + mesh * mstride_f + rowi * (size_t)(2 * H);
-    float2 z = *reinterpret_cast<const float2 *>(row + 2 * n);
+    float2 z = mesh_load<NT>(reinterpret_cast<const float2 *>(row + 2 * n));
     if (YLM) {
       int i = (int)(rowi / N), j = (int)(rowi - (size_t)i * N);
       z.x *= ylm_r_weight(ylm, i, j, 2 * n);
@@ -167,10 +168,10 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     float2 WO = cmul(tw[k], O);
     size_t mesh = line / lpm, rowi = line - mesh * lpm;
     float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
-    orow[k] = cadd(E, WO);
+    mesh_store<NT>(orow + k, cadd(E, WO));
     if (M - k != k) {
       float2 d = csub(E, WO);
-      orow[M - k] = make_float2(d.x, -d.y);
+      mesh_store<NT>(orow + (M - k), make_float2(d.x, -d.y));
     }
   }
 }
@@ -259,6 +260,7 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
            size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
   constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
+  constexpr bool NT = stream_mesh<N>();
   extern __shared__ float lds[];
   const size_t tile = blockIdx.x;
   const size_t per_mesh = (size_t)N * ntz;
@@ -273,14 +275,14 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
   // are read and discarded
   float2 v[V];
   transform_rows<N, C, T, true, INV, 1>(v, lds, tw, [&](int r, int c, int j) -> float2 {
-    const float2 x = *row_elem<RS::BYTE32>(in + base + (size_t)(r * RS::LOAD) * ES,
-                                           (uint32_t)j * (uint32_t)ES + c);
+    const float2 x = mesh_load<NT>(row_elem<RS::BYTE32>(in + base + (size_t)(r * RS::LOAD) * ES,
+                                                         (uint32_t)j * (uint32_t)ES + c));
     return c < ncv ? x : make_float2(0.f, 0.f);
   });
   store_last_rows<N, C, T, true>(v, [&](int r, int c, int j, float2 x) {
     if (c < ncv)
-      *row_elem<RS::BYTE32>(out + base + (size_t)(r * RS::STORE) * ES,
-                            (uint32_t)j * (uint32_t)ES + c) = x;
+      mesh_store<NT>(row_elem<RS::BYTE32>(out + base + (size_t)(r * RS::STORE) * ES,
+                                          (uint32_t)j * (uint32_t)ES + c), x);
   });
 }
 
@@ -428,8 +430,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   // kz = M are read and discarded
   auto ld_field = [&](const float2 *f, size_t base, int ncv) {
     return [=](int r, int cc, int j) -> float2 {
-      const float2 x = *row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES,
-                                             (uint32_t)j * ES32 + cc);
+      const float2 x = mesh_load<stream_mesh<N>()>(
+          row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES, (uint32_t)j * ES32 + cc));
       return cc < ncv ? x : make_float2(0.f, 0.f);
     };
   };

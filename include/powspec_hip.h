@@ -328,12 +328,15 @@ int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, in
 /* ---- tuning ----------------------------------------------------------------- */
 /* No counterpart in the reference (its FFTW plans use FFTW_ESTIMATE): the
  * deposit tile shape sets the bucketing duplication, the deposit's LDS
- * footprint and which bucketing kernel runs; results do not depend on it
- * (integer accumulation). */
+ * footprint and which bucketing kernel runs.  Results depend on it only
+ * through each tile's fixed-point scale (integer accumulation; differences
+ * at the 1e-9 level relative to a tile's largest cell). */
 
-/* Tiles of a context (cells per axis, powers of two, z >= 4, both grids in
- * 64-bit fixed point within 160 KB of LDS); overrides the default 8x16x32
- * and any cache.  PSH_ERR_ARG when invalid. */
+/* Tiles of a context: cells per axis, powers of two, each at least the
+ * window width (scheme + 1), z >= 4, and both grids' LDS tiles within
+ * 160 KB (z rows padded by the window width on each side; 4-byte cells for
+ * boxes, 8-byte for surveys).  Overrides the default 8x16x32 and any cache.
+ * PSH_ERR_ARG when invalid. */
 int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
 
 /* Applies the fastest entry of a tuning cache for the context's device

@@ -321,7 +321,9 @@ int psh_mlp_forward(const float *d_params, const float *d_X, int B, float *d_Y);
 int psh_mlp_grad(const float *d_params, const float *d_X, const float *d_T, int B,
     float *d_grads, float *loss);
 /* Adam (beta 0.9, 0.999) over `epochs` passes of contiguous mini-batches of
- * `batch` samples; *loss_out receives the final loss over all samples. */
+ * `batch` samples; *loss_out receives the final loss over all samples.
+ * Gradients are summed in 64-bit fixed point, so equal inputs give equal
+ * parameters on every run (psh_mlp_grad likewise). */
 int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, int batch,
     int epochs, float lr, unsigned seed, float *loss_out);
 

@@ -479,6 +479,13 @@ __device__ __forceinline__ void mesh_store(float2 *p, float2 x) {
   }
 }
 
+// Non-temporal 16-byte store (two complex values).
+__device__ __forceinline__ void mesh_store4(float4 *p, float4 x) {
+  typedef float v4f_t __attribute__((ext_vector_type(4)));
+  const v4f_t v = {x.x, x.y, x.z, x.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4f_t *>(p));
+}
+
 // Row strides of the split access.
 template <int N, int C, int T, bool COLMAP>
 struct RowSplit {

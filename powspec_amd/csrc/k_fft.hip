@@ -113,7 +113,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
   constexpr bool NT = stream_mesh<N>();
-  __shared__ float lds[Lds<M, C, false>::SIZE];
+  __shared__ __align__(16) float lds[Lds<M, C, false>::SIZE];
   const size_t line0 = (size_t)blockIdx.x * C;
   const size_t lpm = (size_t)N * N;
   float2 v[V];
@@ -154,24 +154,83 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     lds[Lds<M, C, false>::idx(c, n)] = x.y;
   });
   __syncthreads();
+  // staged rows: measured on MI355X, z pass at N = 512 0.267 -> 0.255 ms per
+  // mesh pair; at N = 1024 the extra registers (70 -> 86 VGPRs, 7 -> 5 waves
+  // per SIMD) cost more than the whole lines save (4.17 -> 4.33 ms)
+  constexpr bool STAGE = NT && N <= 512;
+  if constexpr (!STAGE) {
 #pragma unroll
-  for (int it = 0; it < NI; it++) {
-    int item = threadIdx.x + it * T;
-    int c = item / NP, k = item - c * NP;
-    if (item >= C * NP) continue;
-    size_t line = line0 + c;
-    if (line >= nlines) continue;
-    float ik = lds[Lds<M, C, false>::idx(c, k)];
-    float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
-    float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
-    float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
-    float2 WO = cmul(tw[k], O);
-    size_t mesh = line / lpm, rowi = line - mesh * lpm;
-    float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
-    mesh_store<NT>(orow + k, cadd(E, WO));
-    if (M - k != k) {
-      float2 d = csub(E, WO);
-      mesh_store<NT>(orow + (M - k), make_float2(d.x, -d.y));
+    for (int it = 0; it < NI; it++) {
+      int item = threadIdx.x + it * T;
+      int c = item / NP, k = item - c * NP;
+      if (item >= C * NP) continue;
+      size_t line = line0 + c;
+      if (line >= nlines) continue;
+      float ik = lds[Lds<M, C, false>::idx(c, k)];
+      float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+      float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
+      float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
+      float2 WO = cmul(tw[k], O);
+      size_t mesh = line / lpm, rowi = line - mesh * lpm;
+      float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
+      mesh_store<NT>(orow + k, cadd(E, WO));
+      if (M - k != k) {
+        float2 d = csub(E, WO);
+        mesh_store<NT>(orow + (M - k), make_float2(d.x, -d.y));
+      }
+    }
+  } else {
+    // The pairs X[k], X[M-k] land in two places of a row, so stores straight
+    // from the pairs leave partial 128-byte lines, which non-temporal stores
+    // send to HBM unmerged.  The rows are staged in LDS instead (half the
+    // block's rows at a time, zeros in the padding) and written as whole
+    // lines of 16-byte stores.
+    constexpr int HH = M + PSH_PAD;      // complex values per stored row
+    constexpr int HALF = C / 2;
+    static_assert(C % 2 == 0 && HALF * HH * 2 <= Lds<M, C, false>::SIZE,
+                  "half of the block's rows fit the exchange tile");
+    static_assert(HH % 2 == 0, "rows of whole float4 pairs");
+    float ik[NI], imk[NI];
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+      const int item = threadIdx.x + it * T;
+      const int c = item / NP, k = item - c * NP;
+      if (item < C * NP) {
+        ik[it] = lds[Lds<M, C, false>::idx(c, k)];
+        imk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+      }
+    }
+    float2 *stage = reinterpret_cast<float2 *>(lds);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      __syncthreads();   // the tile's previous contents are consumed
+#pragma unroll
+      for (int it = 0; it < NI; it++) {
+        const int item = threadIdx.x + it * T;
+        const int c = item / NP, k = item - c * NP;
+        if (item >= C * NP || c / HALF != h) continue;
+        const float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik[it] - imk[it]));
+        const float2 O = make_float2(0.5f * (ik[it] + imk[it]), -0.5f * (rk[it] - rmk[it]));
+        const float2 WO = cmul(tw[k], O);
+        float2 *srow = stage + (c - h * HALF) * HH;
+        srow[k] = cadd(E, WO);
+        if (M - k != k) {
+          const float2 d = csub(E, WO);
+          srow[M - k] = make_float2(d.x, -d.y);
+        }
+        if (k >= 1 && k < PSH_PAD) srow[M + k] = make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      constexpr int Q = HH / 2;           // float4 (two complex values) per row
+      const float4 *src = reinterpret_cast<const float4 *>(stage);
+      for (int i = threadIdx.x; i < HALF * Q; i += T) {
+        const int r = i / Q, q = i - r * Q;
+        const size_t line = line0 + h * HALF + r;
+        if (line >= nlines) continue;
+        const size_t mesh = line / lpm, rowi = line - mesh * lpm;
+        float4 *orow = reinterpret_cast<float4 *>(out + mesh * (mstride_f / 2) + rowi * (size_t)H);
+        mesh_store4(orow + q, src[i]);
+      }
     }
   }
 }

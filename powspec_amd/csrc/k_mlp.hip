@@ -208,31 +208,44 @@ mlp_infer_kernel(const float *__restrict__ P, const float *__restrict__ X, int B
   }
 }
 
+// Gradients are summed over the waves of a batch in 64-bit fixed point
+// (value * 2^40): integer addition is associative, so a training run is
+// reproducible bit for bit whatever order the waves finish in (FP32 atomics
+// would make every run a different trajectory).  |gradient| < 2^23 fits;
+// parts below 2^-41 are dropped, far below what Adam's eps = 1e-8 resolves.
+constexpr float kGradScale = 1099511627776.0f;           // 2^40
+constexpr double kGradUnscale = 1.0 / 1099511627776.0;   // 2^-40
+
+__device__ __forceinline__ void grad_add(long long *g, float v) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(g),
+            (unsigned long long)__float2ll_rn(v * kGradScale));
+}
+
 // dW[m][n] += sum_s A[s][m] D[s][n] over the wave's 16 samples (M = rows of
 // the weight), via MFMA with K = 16 samples; db[n] += sum_s D[s][n]
 template <int M, int N>
-__device__ inline void weight_grad(const float *A, int lda, const float *D, int ldd, float *G,
-                                   float *gb, float *tmp) {
+__device__ inline void weight_grad(const float *A, int lda, const float *D, int ldd, long long *G,
+                                   long long *gb, float *tmp) {
   const int l = (int)(threadIdx.x & 63);
 #pragma unroll
   for (int mt = 0; mt < M / 16; mt++) {
     // tmp(r, n) = sum_s A[s][mt*16 + r] D[s][n]
     wave_mma<16, N>(A + mt * 16, 1, lda, D, ldd, 1, tmp, N, 1);
     __builtin_amdgcn_wave_barrier();
-    for (int e = l; e < 16 * N; e += 64) atomicAdd(&G[(mt * 16 + e / N) * N + e % N], tmp[e]);
+    for (int e = l; e < 16 * N; e += 64) grad_add(&G[(mt * 16 + e / N) * N + e % N], tmp[e]);
     __builtin_amdgcn_wave_barrier();
   }
   for (int n = l; n < N; n += 64) {
     float s = 0.f;
     for (int r = 0; r < 16; r++) s += D[r * ldd + n];
-    atomicAdd(&gb[n], s);
+    grad_add(&gb[n], s);
   }
 }
 
 // loss = sum (y - t)^2 / B_total over the batch, gradient accumulated in G
 __global__ void __launch_bounds__(64 * kWaves) mlp_grad_kernel(const float *P, const float *X,
                                                                const float *T, int B,
-                                                               float inv_total, float *G,
+                                                               float inv_total, long long *G,
                                                                float *loss) {
   __shared__ WaveBufs wb[kWaves];
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -276,20 +289,26 @@ __global__ void __launch_bounds__(64 * kWaves) mlp_grad_kernel(const float *P, c
   weight_grad<IN, H>(w.x, IN, w.d1, H, G + oW1, G + ob1, w.t);
 }
 
-__global__ void adam_kernel(float *P, float *G, float *m, float *v, int n, float lr, float b1,
+__global__ void adam_kernel(float *P, long long *G, float *m, float *v, int n, float lr, float b1,
                             float b2, float eps, float c1, float c2) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float g = G[i];
+    const float g = (float)((double)G[i] * kGradUnscale);
     m[i] = b1 * m[i] + (1.f - b1) * g;
     v[i] = b2 * v[i] + (1.f - b2) * g * g;
     P[i] -= lr * (m[i] * c1) / (sqrtf(v[i] * c2) + eps);
-    G[i] = 0.f;
+    G[i] = 0;
   }
 }
 
-__global__ void zero_kernel(float *p, int n) {
+template <typename V>
+__global__ void zero_kernel(V *p, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    p[i] = 0.f;
+    p[i] = V(0);
+}
+
+__global__ void grad_out_kernel(const long long *G, float *out, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = (float)((double)G[i] * kGradUnscale);
 }
 
 unsigned blocks_for(int B) { return (unsigned)((B + 16 * kWaves - 1) / (16 * kWaves)); }
@@ -324,14 +343,21 @@ extern "C" int psh_mlp_grad(const float *d_params, const float *d_X, const float
                             float *d_grads, float *loss) {
   if (!d_params || !d_X || !d_T || !d_grads || B < 1) return PSH_ERR_ARG;
   float *d_loss = nullptr;
+  long long *G = nullptr;
   if (hipMalloc(&d_loss, sizeof(float)) != hipSuccess) return PSH_ERR_MEMORY;
-  hipLaunchKernelGGL(zero_kernel, dim3(64), dim3(256), 0, nullptr, d_grads, NPAR);
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
+  if (hipMalloc(&G, sizeof(long long) * NPAR) != hipSuccess) {
+    (void)hipFree(d_loss);
+    return PSH_ERR_MEMORY;
+  }
+  hipLaunchKernelGGL(zero_kernel<long long>, dim3(64), dim3(256), 0, nullptr, G, NPAR);
+  hipLaunchKernelGGL(zero_kernel<float>, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
   hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
-                     d_params, d_X, d_T, B, 1.f / (float)B, d_grads, d_loss);
+                     d_params, d_X, d_T, B, 1.f / (float)B, G, d_loss);
+  hipLaunchKernelGGL(grad_out_kernel, dim3(64), dim3(256), 0, nullptr, G, d_grads, NPAR);
   float h = 0.f;
   const bool ok = hipMemcpy(&h, d_loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
   (void)hipFree(d_loss);
+  (void)hipFree(G);
   if (loss) *loss = h;
   return ok ? PSH_OK : PSH_ERR_DEVICE;
 }
@@ -340,9 +366,15 @@ extern "C" int psh_mlp_train(float *d_params, const float *d_X, const float *d_T
                              int batch, int epochs, float lr, unsigned seed, float *loss_out) {
   if (!d_params || !d_X || !d_T || B < 1 || batch < 16 || epochs < 0) return PSH_ERR_ARG;
   float *buf = nullptr;
-  if (hipMalloc(&buf, sizeof(float) * (3 * NPAR + 1)) != hipSuccess) return PSH_ERR_MEMORY;
-  float *G = buf, *m = buf + NPAR, *v = buf + 2 * NPAR, *d_loss = buf + 3 * NPAR;
-  hipLaunchKernelGGL(zero_kernel, dim3(64), dim3(256), 0, nullptr, buf, 3 * NPAR + 1);
+  long long *G = nullptr;
+  if (hipMalloc(&buf, sizeof(float) * (2 * NPAR + 1)) != hipSuccess) return PSH_ERR_MEMORY;
+  if (hipMalloc(&G, sizeof(long long) * NPAR) != hipSuccess) {
+    (void)hipFree(buf);
+    return PSH_ERR_MEMORY;
+  }
+  float *m = buf, *v = buf + NPAR, *d_loss = buf + 2 * NPAR;
+  hipLaunchKernelGGL(zero_kernel<float>, dim3(64), dim3(256), 0, nullptr, buf, 2 * NPAR + 1);
+  hipLaunchKernelGGL(zero_kernel<long long>, dim3(64), dim3(256), 0, nullptr, G, NPAR);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   int step = 0;
   // mini-batches are contiguous slices of a sample order the caller shuffles
@@ -362,12 +394,13 @@ extern "C" int psh_mlp_train(float *d_params, const float *d_X, const float *d_T
     }
   }
   // final loss over the whole set
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
+  hipLaunchKernelGGL(zero_kernel<float>, dim3(1), dim3(64), 0, nullptr, d_loss, 1);
   hipLaunchKernelGGL(mlp_grad_kernel, dim3(blocks_for(B)), dim3(64 * kWaves), 0, nullptr,
                      d_params, d_X, d_T, B, 1.f / (float)B, G, d_loss);
   float h = 0.f;
   const bool ok = hipMemcpy(&h, d_loss, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
   (void)hipFree(buf);
+  (void)hipFree(G);
   if (loss_out) *loss_out = h;
   return ok ? PSH_OK : PSH_ERR_DEVICE;
 }

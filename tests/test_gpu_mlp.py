@@ -55,11 +55,22 @@ KNY = np.pi * 512 / 1000.0
 
 @pytest.fixture(scope="module")
 def trained(psh):
-    """The network trained on 64 clustered boxes (CIC, N_g = 512) over an
-    8 x 8 design of power-law amplitude [5e3, 2e4] and index [-2, -1]."""
+    """The ensemble of networks trained on 64 clustered boxes (CIC,
+    N_g = 512) over an 8 x 8 design of power-law amplitude [5e3, 2e4] and
+    index [-2, -1]."""
     X, T, _ = mlp.dataset(psh, range(5000, 5064), design=mlp.grid_design(8), **C5)
-    p, loss = mlp.train(X, T)
+    p, loss = mlp.train_ensemble(X, T)
     return p, loss
+
+
+def test_training_is_reproducible(psh):
+    """Fixed-point gradient sums: two runs from one seed give identical
+    parameters, whatever order the waves add their gradients in."""
+    X, T = _data(2048, 8)
+    p0 = mlp.init_params(9)
+    a, la = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
+    b, lb = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
+    assert np.array_equal(a, b)
 
 
 def test_alias_correction_reference_spectrum(psh, trained):

@@ -448,14 +448,20 @@ __device__ __forceinline__ float2 *row_elem(float2 *rowbase, uint32_t off) {
   return const_cast<float2 *>(row_elem<BYTE32>(static_cast<const float2 *>(rowbase), off));
 }
 
-// Mesh traffic of the column, z and x passes.  From N = 512 a mesh (537 MB
-// per grid) is twice the 256 MB Infinity Cache and every element is read
-// and written once per pass, so the loads and stores are non-temporal (the
-// `nt` bit): measured on MI355X at N = 1024 the y pass drops from 4.1 to
-// 3.4 ms and the whole path by 4 %.  Smaller meshes stay cached between
-// passes and keep ordinary accesses (non-temporal ones cost 2 % at N = 256).
+// Mesh traffic of the column, z and x passes.  A launch whose meshes
+// outgrow the 256 MB Infinity Cache reads and writes every element once, so
+// its loads and stores are non-temporal (the `nt` bit): measured on MI355X
+// at N = 1024 the y pass drops from 4.1 to 3.4 ms and the whole path by 4 %.
+// From N = 512 one mesh (537 MB per grid) is that large; smaller meshes
+// stream when a batch of them is.  Meshes that fit stay cached between
+// passes and keep ordinary accesses (non-temporal ones cost 2 % for one
+// N = 256 pair).
 template <int N>
 constexpr bool stream_mesh() { return N >= 512; }
+
+constexpr size_t kStreamBytes = (size_t)256 << 20;
+
+inline bool stream_bytes(size_t bytes) { return bytes > kStreamBytes; }
 
 typedef float v2f_t __attribute__((ext_vector_type(2)));
 

@@ -885,7 +885,13 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
   const size_t lds = (g.wide ? sizeof(long long) : sizeof(int)) *
                      psh_tile_cells(g.tile, SCHEME) * (g.intlace ? 2 : 1);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  auto k = g.stream
+  // non-temporal mesh stores when the launch's meshes outgrow the 256 MB
+  // Infinity Cache (every cell is written once and read by the next pass
+  // only after the whole launch)
+  const bool stream = sizeof(float) * (size_t)g.tile[0] * g.tile[1] * g.tile[2] * ntiles *
+                          (g.intlace ? 2 : 1) >
+                      ((size_t)256 << 20);
+  auto k = stream
                ? (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, true>
                                       : assign_deposit<SCHEME, false, true, true>)
                          : (g.intlace ? assign_deposit<SCHEME, true, false, true>

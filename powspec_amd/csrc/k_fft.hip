@@ -18,6 +18,7 @@
 #include "psh_internal.h"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 using namespace pshfft;
@@ -105,14 +106,13 @@ __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
 // ---------------------------------------------------------------------------
 // z pass: real rows -> half-complex rows
 // ---------------------------------------------------------------------------
-template <int N, bool YLM>
+template <int N, bool YLM, bool NT>
 __global__ void __launch_bounds__(256)
 z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
              size_t nlines, size_t mstride_f, int H,
              const float2 *__restrict__ tw, YlmDev ylm) {
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
-  constexpr bool NT = stream_mesh<N>();
   __shared__ __align__(16) float lds[Lds<M, C, false>::SIZE];
   const size_t line0 = (size_t)blockIdx.x * C;
   const size_t lpm = (size_t)N * N;
@@ -157,7 +157,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
   // staged rows: measured on MI355X, z pass at N = 512 0.267 -> 0.255 ms per
   // mesh pair; at N = 1024 the extra registers (70 -> 86 VGPRs, 7 -> 5 waves
   // per SIMD) cost more than the whole lines save (4.17 -> 4.33 ms)
-  constexpr bool STAGE = NT && N <= 512;
+  constexpr bool STAGE = NT && N == 512;
   if constexpr (!STAGE) {
 #pragma unroll
     for (int it = 0; it < NI; it++) {
@@ -313,13 +313,12 @@ z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
 // ---------------------------------------------------------------------------
 // strided column passes (y: OS = N*H, ES = H; x: OS = H, ES = N*H)
 // ---------------------------------------------------------------------------
-template <int N, bool INV>
+template <int N, bool INV, bool NT>
 __global__ void __launch_bounds__(ColCfg<N>::T)
 col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
            size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
   constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
-  constexpr bool NT = stream_mesh<N>();
   extern __shared__ float lds[];
   const size_t tile = blockIdx.x;
   const size_t per_mesh = (size_t)N * ntz;
@@ -447,7 +446,7 @@ __device__ __forceinline__ int bin_exact(const PshBins &b, double ksqx, double k
 //     Legendre weights, accumulated in per-workgroup FP64 LDS bins
 //     (src/multipole.c:147-177, src/mp_template.c:107-139).
 // xtab[i] = {alias_x, ksq_x, phase_x.re, phase_x.im} in LDS.
-template <int N, int EPI, bool INTLACE>
+template <int N, int EPI, bool INTLACE, bool NT>
 __global__ void __launch_bounds__((XCfg<N, INTLACE>::T), (XCfg<N, INTLACE>::WAVES))
 x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   constexpr int C = XCfg<N, INTLACE>::C, T = XCfg<N, INTLACE>::T, M = N / 2;
@@ -489,7 +488,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   // kz = M are read and discarded
   auto ld_field = [&](const float2 *f, size_t base, int ncv) {
     return [=](int r, int cc, int j) -> float2 {
-      const float2 x = mesh_load<stream_mesh<N>()>(
+      const float2 x = mesh_load<NT>(
           row_elem<RS::BYTE32>(f + base + (size_t)(r * RS::LOAD) * ES, (uint32_t)j * ES32 + cc));
       return cc < ncv ? x : make_float2(0.f, 0.f);
     };
@@ -644,13 +643,20 @@ struct ZR2C {
     const size_t nlines = (size_t)N * N * nmesh;
     const unsigned blocks = (unsigned)((nlines + ZCfg<N>::C - 1) / ZCfg<N>::C);
     const int H = N / 2 + PSH_PAD;
-    if (y.ell > 0)
-      hipLaunchKernelGGL((z_r2c_kernel<N, true>), dim3(blocks), dim3(ZCfg<N>::T), 0,
-                         s, in, out, nlines, mstride, H, tw, y);
+    auto go = [&](auto ntc) {
+      constexpr bool NT = decltype(ntc)::value;
+      if (y.ell > 0)
+        hipLaunchKernelGGL((z_r2c_kernel<N, true, NT>), dim3(blocks), dim3(ZCfg<N>::T), 0,
+                           s, in, out, nlines, mstride, H, tw, y);
+      else
+        hipLaunchKernelGGL((z_r2c_kernel<N, false, NT>), dim3(blocks), dim3(ZCfg<N>::T), 0,
+                           s, in, out, nlines, mstride, H, tw, y);
+      return hipGetLastError();
+    };
+    if constexpr (stream_mesh<N>()) return go(std::true_type{});
     else
-      hipLaunchKernelGGL((z_r2c_kernel<N, false>), dim3(blocks), dim3(ZCfg<N>::T), 0,
-                         s, in, out, nlines, mstride, H, tw, y);
-    return hipGetLastError();
+      return stream_bytes(sizeof(float) * mstride * nmesh) ? go(std::true_type{})
+                                                           : go(std::false_type{});
   }
 };
 
@@ -678,13 +684,20 @@ struct Col {
     const unsigned blocks = (unsigned)((size_t)N * ntz * nmesh);
     constexpr size_t shmem = ColCfg<N>::LDS;
     static_assert(shmem <= 160 * 1024, "column tile exceeds the LDS");
-    auto k = inverse ? col_kernel<N, true> : col_kernel<N, false>;
-    if (shmem > 64 * 1024)
-      (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)shmem);
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N>::T), shmem, s, in, out, ntz, OS, ES,
-                       mstride / 2, tw);
-    return hipGetLastError();
+    auto go = [&](auto ntc) {
+      constexpr bool NT = decltype(ntc)::value;
+      auto k = inverse ? col_kernel<N, true, NT> : col_kernel<N, false, NT>;
+      if (shmem > 64 * 1024)
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shmem);
+      hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N>::T), shmem, s, in, out, ntz, OS, ES,
+                         mstride / 2, tw);
+      return hipGetLastError();
+    };
+    if constexpr (stream_mesh<N>()) return go(std::true_type{});
+    else
+      return stream_bytes(sizeof(float) * mstride * nmesh) ? go(std::true_type{})
+                                                           : go(std::false_type{});
   }
 };
 
@@ -699,24 +712,31 @@ struct XBin {
                          (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
     const dim3 g(grid, nmock), b(il ? XCfg<N, true>::T : XCfg<N, false>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
+    auto go = [&](auto ntc) {
+      constexpr bool NT = decltype(ntc)::value;
 #define PSH_XB(E, I)                                                          \
   {                                                                              \
-    auto k = x_bin_kernel<N, E, I>;                                              \
+    auto k = x_bin_kernel<N, E, I, NT>;                                          \
     if (shmem > 64 * 1024)                                                       \
       (void)hipFuncSetAttribute((const void *)k,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                 (int)shmem);                                     \
-    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                                \
+    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                        \
     return hipGetLastError();                                                    \
   }
-    if (epi == PSH_EPI_SIM) {
-      if (il) PSH_XB(PSH_EPI_SIM, true) else PSH_XB(PSH_EPI_SIM, false)
-    } else if (epi == PSH_EPI_SURVEY0) {
-      if (il) PSH_XB(PSH_EPI_SURVEY0, true) else PSH_XB(PSH_EPI_SURVEY0, false)
-    } else {
-      PSH_XB(PSH_EPI_SURVEYL, false)
-    }
+      if (epi == PSH_EPI_SIM) {
+        if (il) PSH_XB(PSH_EPI_SIM, true) else PSH_XB(PSH_EPI_SIM, false)
+      } else if (epi == PSH_EPI_SURVEY0) {
+        if (il) PSH_XB(PSH_EPI_SURVEY0, true) else PSH_XB(PSH_EPI_SURVEY0, false)
+      } else {
+        PSH_XB(PSH_EPI_SURVEYL, false)
+      }
 #undef PSH_XB
+    };
+    // the x pass reads nmock fields (two when interlaced) of mstride_c values
+    const size_t bytes = sizeof(float2) * d.mstride_c * (size_t)nmock * (il ? 2 : 1);
+    if constexpr (stream_mesh<N>()) return go(std::true_type{});
+    else return stream_bytes(bytes) ? go(std::true_type{}) : go(std::false_type{});
   }
 };
 

@@ -430,10 +430,6 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
     while ((1 << g.tile_shift[a]) < g.tile[a]) g.tile_shift[a]++;
   }
   g.row_pitch = 2 * ctx->H;
-  // meshes of 537 MB and more per grid outgrow the Infinity Cache: the
-  // deposit's single write of every cell goes out non-temporal, as the FFT
-  // passes' accesses do from the same size
-  g.stream = ctx->Ng >= 512 ? 1 : 0;
   for (int a = 0; a < 3; a++) {
     g.bmin[a] = bmin[a];
     g.smin[a] = bmin[a] - 0.5 * L[a] / ctx->Ng;   // src/genr_mesh.c:817-818

@@ -71,7 +71,6 @@ struct PshAssignGeom {
   int tile_shift[3];    // log2(tile)
   int ntile[3];         // Ng / tile
   int row_pitch;        // floats per mesh row (2H)
-  int stream;           // non-temporal mesh stores (meshes of Ng >= 512)
   double bmin[3];       // box corner of the plain grid
   double smin[3];       // box corner of the shifted grid
   double L[3];          // box size

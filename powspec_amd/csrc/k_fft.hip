@@ -363,6 +363,7 @@ struct XBinDev {
   // for the requested pole l; bit l of oddmask set when ell is odd
   float legc[PSH_MAX_NL][4];
   int oddmask;
+  int even024;   // the poles are exactly 0, 2, 4 (the common request)
 };
 
 // Bin of a mode from |k|^2, in the reference's operation order
@@ -534,7 +535,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         if (INTLACE) {
           const float4 xt = xtab[i];
           const float2 t = cmul(cmul(make_float2(xt.z, xt.w), pyz), vb[qq * G::R + r]);
-          F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
+          // the simulation epilogue takes |F_A + t|^2 and applies the
+          // (1/2)^2 with the per-column weight in phase 2
+          if (EPI == PSH_EPI_SIM) F = make_float2(F.x + t.x, F.y + t.y);
+          else F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
         }
         // element (i = j + r NS, c): row base uniform, lane offset j ES + c
         const size_t rbase = base + (size_t)(r * RS::STORE) * ES;
@@ -558,7 +562,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       const float ksqyz = a.ax.ksqf[1][y] + a.ax.ksqf[2][kz];
       const float dyz = a.ax.kvf[1][y] * b.losf[1] + a.ax.kvf[2][kz] * b.losf[2];
       const bool single = (kz == 0) || (kz == M);
-      const float wyz = a.ax.alf[1][y] * a.ax.alf[2][kz] * (single ? 1.f : 2.f);
+      const float wyz = a.ax.alf[1][y] * a.ax.alf[2][kz] *
+                        ((single ? 1.f : 2.f) * (INTLACE && EPI == PSH_EPI_SIM ? 0.25f : 1.f));
       const double ksqy = a.ax.ksq[1][y], ksqz = a.ax.ksq[2][kz];
       const bool dcrow = (EPI == PSH_EPI_SIM) && y == 0 && kz == 0;
       // the two half-spaces cancel odd poles off the kz = 0, M planes
@@ -586,6 +591,16 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           const float kvx = (float)(i <= M ? i : i - N) * a.vec0f;
           const float mu = (kvx * b.losf[0] + dyz) * rs;
           const float x2 = mu * mu;
+          if (a.even024) {
+            // L_0 = 1, L_2 and L_4 by Horner in mu^2: the generic loop's
+            // arithmetic without its zero coefficients
+            const float p2 = __builtin_fmaf(1.5f, x2, -0.5f);
+            const float p4 = __builtin_fmaf(__builtin_fmaf(4.375f, x2, -3.75f), x2, 0.375f);
+            atomicAdd(&bins[kb], (double)w);
+            atomicAdd(&bins[b.nbin + kb], (double)(w * p2));
+            atomicAdd(&bins[2 * b.nbin + kb], (double)(w * p4));
+            continue;
+          }
 #pragma unroll
           for (int l = 0; l < PSH_MAX_NL; l++) {
             if (l >= b.nl) break;
@@ -816,5 +831,7 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
     for (int t = 0; t < 4; t++) d.legc[l][t] = kLeg[ell][t];
     if (ell & 1) d.oddmask |= 1 << l;
   }
+  d.even024 = a.bins.nl == 3 && a.bins.poles[0] == 0 && a.bins.poles[1] == 2 &&
+              a.bins.poles[2] == 4;
   return dispatch_n<XBin>(Ng, d, a.epi, a.B != nullptr, a.grid, a.nmock, tw, s);
 }

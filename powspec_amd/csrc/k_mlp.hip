@@ -211,14 +211,20 @@ mlp_infer_kernel(const float *__restrict__ P, const float *__restrict__ X, int B
 // Gradients are summed over the waves of a batch in 64-bit fixed point
 // (value * 2^40): integer addition is associative, so a training run is
 // reproducible bit for bit whatever order the waves finish in (FP32 atomics
-// would make every run a different trajectory).  |gradient| < 2^23 fits;
-// parts below 2^-41 are dropped, far below what Adam's eps = 1e-8 resolves.
+// would make every run a different trajectory).  Parts below 2^-41 are
+// dropped, far below what Adam's eps = 1e-8 resolves.  A wave's
+// contribution is clamped to |g| <= 2^16 (NaN to the bound) so that the
+// conversion is always defined; the 64-bit sum holds 2^7 saturated
+// contributions, and those of a converging run are orders of magnitude
+// smaller (the loss gradient carries the 1/B of the mean).
 constexpr float kGradScale = 1099511627776.0f;           // 2^40
 constexpr double kGradUnscale = 1.0 / 1099511627776.0;   // 2^-40
+constexpr float kGradClamp = 65536.0f;                   // 2^16
 
 __device__ __forceinline__ void grad_add(long long *g, float v) {
+  const float c = fminf(fmaxf(v, -kGradClamp), kGradClamp);   // NaN -> clamp bound
   atomicAdd(reinterpret_cast<unsigned long long *>(g),
-            (unsigned long long)__float2ll_rn(v * kGradScale));
+            (unsigned long long)__float2ll_rn(c * kGradScale));
 }
 
 // dW[m][n] += sum_s A[s][m] D[s][n] over the wave's 16 samples (M = rows of

@@ -174,6 +174,29 @@ def test_batch_matches_plain(psh):
             assert np.all(np.abs(r["pl"][0] - p["pl"][0]) <= 1e-5 * scale)
 
 
+def test_batch_streaming_matches_plain(psh):
+    """A batch whose meshes outgrow the 256 MB Infinity Cache (16 interlaced
+    mocks at N_g = 128: 335 MB of meshes) takes the non-temporal loads and
+    stores; one mock alone (21 MB) takes the ordinary ones.  Both give the
+    same spectra."""
+    Ng, L = 128, 1000.0
+    rng = np.random.default_rng(12)
+    sizes = rng.integers(2000, 6000, 16)
+    cats = [_uniform(int(s), [L] * 3, 300 + i) for i, s in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    kw = dict(gsize=Ng, assign=2, intlace=True, bsize=L, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw), nmock_max=len(cats))
+    batch = ctx.run_batch(psh.DeviceArray.from_numpy(np.concatenate(cats)), off,
+                          [float(v) for v in sizes], graph=False)
+    single = psh.Context(psh.make_conf(**kw))
+    for m in (0, 7, 15):
+        c = cats[m]
+        p = single.run_device(psh.DeviceArray.from_numpy(c), len(c), float(len(c)))
+        assert np.array_equal(batch[m]["cnt"], p["cnt"])
+        scale = np.abs(p["pl"][0][0]) + 1e9 / len(c)
+        assert np.all(np.abs(batch[m]["pl"][0] - p["pl"][0]) <= 1e-5 * scale)
+
+
 def _survey_cats(nrand, seed, nz=3e-4, p0=1e4):
     """Randoms uniform in a shell sector (r in [500, 900], RA in [0, 60) deg,
     Dec in [-20, 20) deg); data drawn from them with a modulated selection.

@@ -448,6 +448,36 @@ def test_survey_device_entry(psh, oracle):
     _check(got, ref, [0, 2, 4], sums["shot"], sums["norm"])
 
 
+@pytest.mark.parametrize("intlace", [False, True])
+def test_sim_ng512_streaming_parity(psh, oracle, intlace):
+    """N_g = 512 (the learned-correction size): meshes of 537 MB per grid take
+    the non-temporal path, and the z pass stages its output rows in LDS."""
+    dev, n = psh.gen_clustered_box(ng=256, L=1000.0, nbar=0.01, f=0.8, seed=5)
+    data = dev.to_numpy((n, 4))
+    kw = dict(gsize=512, assign=1, intlace=intlace, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
+    got = psh.Context(psh.make_conf(**kw)).run_device(dev, n, float(n))
+    dev.free()
+    ref = oracle.run(oracle.make_conf(**kw), [data])
+    worst = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"Ng 512 intlace={intlace}: {n} particles, worst |dP_l|/S_b {worst:.2e}")
+
+
+def test_survey_ng512_parity(psh, oracle):
+    """Survey estimator at N_g = 512 (l = 0, 2: the Y_2m-weighted z passes on
+    the staged, non-temporal path) against the oracle."""
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(400000, 61)
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    d = np.column_stack([dp, wd])
+    r = np.column_stack([rp, wr])
+    kw = dict(gsize=512, assign=1, intlace=True, issim=False, kbin=0.01, poles=[0, 2])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_survey_device(psh.DeviceArray.from_numpy(d), len(d),
+                                psh.DeviceArray.from_numpy(r), len(r),
+                                sums["alpha"], sums["shot"], sums["norm"])
+    ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
+    _check(got, ref, [0, 2], sums["shot"], sums["norm"])
+
+
 def test_c2_full_size_parity(psh, oracle):
     """The C2 workload at its full size (2.0e7 clustered particles, PCS,
     N_g = 1024, interlaced, l = 0,2,4, dk = 0.005) against the FP64 oracle:

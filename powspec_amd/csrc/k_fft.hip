@@ -800,6 +800,32 @@ int psh_xbin_nacc(const PshBins &b, int epi) {
   return epi == PSH_EPI_SIM ? b.nl * b.nbin : b.nbin;
 }
 
+namespace {
+template <int N>
+struct XBinLds {
+  static size_t run(bool il, int nacc) {
+    return sizeof(double) * ((nacc + 1) & ~1) +
+           (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
+  }
+};
+}  // namespace
+
+size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc) {
+  switch (Ng) {
+    case 4: return XBinLds<4>::run(il, nacc);
+    case 8: return XBinLds<8>::run(il, nacc);
+    case 16: return XBinLds<16>::run(il, nacc);
+    case 32: return XBinLds<32>::run(il, nacc);
+    case 64: return XBinLds<64>::run(il, nacc);
+    case 128: return XBinLds<128>::run(il, nacc);
+    case 256: return XBinLds<256>::run(il, nacc);
+    case 512: return XBinLds<512>::run(il, nacc);
+    case 1024: return XBinLds<1024>::run(il, nacc);
+    case 2048: return XBinLds<2048>::run(il, nacc);
+    default: return 0;
+  }
+}
+
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
                          hipStream_t s) {
   XBinDev d{};

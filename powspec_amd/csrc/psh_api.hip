@@ -300,6 +300,19 @@ int make_bins(psh_ctx *ctx) {
   b.kedge0f = (float)b.kedge0;
   b.invdkf = (float)(1.0 / c.kbin);
   b.k0dk = (float)fabs(b.kedge0 / c.kbin);
+  // the fused x pass keeps each workgroup's bin sums in LDS next to its
+  // tables and exchange tile (simulation boxes: npole sums per bin; survey
+  // fields: one, the l > 0 fields without interlacing)
+  const int nacc = psh_xbin_nacc(b, c.issim ? PSH_EPI_SIM : PSH_EPI_SURVEY0);
+  const size_t lds = psh_xbin_lds_bytes(ctx->Ng, c.intlace != 0, nacc);
+  if (lds > 160 * 1024) {
+    const size_t fixed = psh_xbin_lds_bytes(ctx->Ng, c.intlace != 0, 0);
+    const long maxacc = fixed < 160 * 1024 ? (long)((160 * 1024 - fixed) / sizeof(double)) : 0;
+    return set_err(ctx, PSH_ERR_UNSUPPORTED,
+                   "%d k bins x %d sums exceed the LDS of the fused x pass at GRID_SIZE %d "
+                   "(at most %ld sums: widen BIN_SIZE or lower MAX_K)",
+                   nbin, nacc / nbin, ctx->Ng, maxacc);
+  }
   return PSH_OK;
 }
 

@@ -166,6 +166,9 @@ struct PshXBinArgs {
   int grid;                // number of persistent workgroups per mock
 };
 int psh_xbin_nacc(const PshBins &b, int epi);
+// LDS bytes of the fused x pass for a grid size, interlacing and number of
+// accumulators (0 for an unsupported grid size); at most 160 KB run.
+size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc);
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
     hipStream_t s);
 

@@ -37,6 +37,21 @@ def test_invalid_tiles_rejected(psh):
         assert e.value.code == psh.ERR_ARG
 
 
+def test_too_many_bins_rejected(psh):
+    """The fused x pass keeps a workgroup's bin sums in LDS: a binning that
+    does not fit is refused when the context is made, with the limit in the
+    message, and a coarser one at the same grid size runs."""
+    with pytest.raises(psh.PshError) as e:
+        psh.Context(psh.make_conf(gsize=256, bsize=1000.0, kbin=1e-4, intlace=True,
+                                  poles=[0, 2, 4]))
+    assert e.value.code == psh.ERR_UNSUPPORTED and "at most" in str(e.value)
+    ctx = psh.Context(psh.make_conf(gsize=256, bsize=1000.0, kbin=1e-3, intlace=True,
+                                    poles=[0, 2, 4]))
+    d = _uniform(20000, [1000.0] * 3, 54)
+    res = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
+    assert res["nbin"] == 804
+
+
 def test_autotune_and_cache(psh, tmp_path):
     cache = tmp_path / "tune.txt"
     conf = psh.make_conf(gsize=64, assign=2, intlace=True, bsize=500.0, kbin=0.02)

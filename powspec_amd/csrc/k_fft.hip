@@ -118,12 +118,24 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
   const size_t lpm = (size_t)N * N;
   float2 v[V];
 
+  // Line bases.  When every butterfly of the first pass of a line falls in
+  // one wave (NBF a multiple of 64: N >= 1024), a wave's line is uniform:
+  // its row base is formed once in scalar registers and each element is a
+  // 32-bit byte offset from it (SGPR base + VGPR offset loads), instead of
+  // 64-bit address arithmetic per element.
+  constexpr bool UNIFORM = PassGeom<M, C, T, 0, false>::NBF % 64 == 0 && T % 64 == 0;
+  auto row_of = [&](size_t line, size_t &rowi) -> const float2 * {
+    const size_t mesh = line / lpm;
+    rowi = line - mesh * lpm;
+    return reinterpret_cast<const float2 *>(in + mesh * mstride_f + rowi * (size_t)(2 * H));
+  };
   transform<M, C, T, false, false, 2>(v, lds, tw, [&](int c, int n) -> float2 {
-    size_t line = line0 + c;
+    const int cu = UNIFORM ? __builtin_amdgcn_readfirstlane(c) : c;
+    const size_t line = line0 + cu;
     if (line >= nlines) return make_float2(0.f, 0.f);
-    size_t mesh = line / lpm, rowi = line - mesh * lpm;
-    const float *row = in + mesh * mstride_f + rowi * (size_t)(2 * H);
-    float2 z = mesh_load<NT>(reinterpret_cast<const float2 *>(row + 2 * n));
+    size_t rowi;
+    const float2 *row = row_of(line, rowi);
+    float2 z = UNIFORM ? mesh_load<NT>(row_elem<true>(row, (uint32_t)n)) : mesh_load<NT>(row + n);
     if (YLM) {
       int i = (int)(rowi / N), j = (int)(rowi - (size_t)i * N);
       z.x *= ylm_r_weight(ylm, i, j, 2 * n);
@@ -135,18 +147,38 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
   // post-process: X[k] = E + W^k O, X[M-k] = conj(E - W^k O)
   constexpr int NP = M / 2 + 1;
   constexpr int NI = (C * NP + T - 1) / T;
-  float rk[NI], rmk[NI];
+  // line by line (N >= 1024): k = thread + u T over [0, M/2) for one line at
+  // a time, so a line's output row is a uniform base again; X[M/2] of each
+  // line is formed by one thread at the end
+  constexpr bool LINEWISE = UNIFORM && (M / 2) % T == 0;
+  constexpr int KPT = LINEWISE ? (M / 2) / T : 1;
+  constexpr int NR = LINEWISE ? C * KPT : NI;
+  float rk[NR], rmk[NR];
+  float rmid = 0.f;
   store_last<M, C, T, false>(v, [&](int c, int n, float2 x) {
     lds[Lds<M, C, false>::idx(c, n)] = x.x;
   });
   __syncthreads();
+  if constexpr (LINEWISE) {
+    const int t = opaque_tid<T>();
 #pragma unroll
-  for (int it = 0; it < NI; it++) {
-    int item = threadIdx.x + it * T;
-    int c = item / NP, k = item - c * NP;
-    if (item < C * NP) {
-      rk[it] = lds[Lds<M, C, false>::idx(c, k)];
-      rmk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+    for (int c = 0; c < C; c++)
+#pragma unroll
+      for (int u = 0; u < KPT; u++) {
+        const int k = t + u * T;
+        rk[c * KPT + u] = lds[Lds<M, C, false>::idx(c, k)];
+        rmk[c * KPT + u] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+      }
+    if (t < C) rmid = lds[Lds<M, C, false>::idx(t, M / 2)];
+  } else {
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+      int item = threadIdx.x + it * T;
+      int c = item / NP, k = item - c * NP;
+      if (item < C * NP) {
+        rk[it] = lds[Lds<M, C, false>::idx(c, k)];
+        rmk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+      }
     }
   }
   __syncthreads();
@@ -154,6 +186,46 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     lds[Lds<M, C, false>::idx(c, n)] = x.y;
   });
   __syncthreads();
+  if constexpr (LINEWISE) {
+    const int t = opaque_tid<T>();
+    float2 twk[KPT];
+#pragma unroll
+    for (int u = 0; u < KPT; u++) twk[u] = tw[t + u * T];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const size_t line = line0 + c;
+      if (line >= nlines) break;   // uniform: lines past the end are all at the end
+      size_t rowi;
+      float2 *orow = out + (line / lpm) * (mstride_f / 2);
+      rowi = line - (line / lpm) * lpm;
+      orow += rowi * (size_t)H;
+#pragma unroll
+      for (int u = 0; u < KPT; u++) {
+        const int k = t + u * T;
+        const float ik = lds[Lds<M, C, false>::idx(c, k)];
+        const float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+        const float r0 = rk[c * KPT + u], r1 = rmk[c * KPT + u];
+        const float2 E = make_float2(0.5f * (r0 + r1), 0.5f * (ik - imk));
+        const float2 O = make_float2(0.5f * (ik + imk), -0.5f * (r0 - r1));
+        const float2 WO = cmul(twk[u], O);
+        mesh_store<NT>(row_elem<true>(orow, (uint32_t)k), cadd(E, WO));
+        const float2 d = csub(E, WO);   // k < M/2: M - k != k
+        mesh_store<NT>(row_elem<true>(orow, (uint32_t)(M - k)), make_float2(d.x, -d.y));
+      }
+    }
+    if (t < C && line0 + t < nlines) {   // X[M/2] = conj of itself's pair: one value per line
+      const size_t line = line0 + t;
+      const size_t mesh = line / lpm, rowi = line - mesh * lpm;
+      float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
+      constexpr int k = M / 2;
+      const float ik = lds[Lds<M, C, false>::idx(t, k)];
+      const float2 E = make_float2(0.5f * (rmid + rmid), 0.5f * (ik - ik));
+      const float2 O = make_float2(0.5f * (ik + ik), -0.5f * (rmid - rmid));
+      const float2 WO = cmul(tw[k], O);
+      mesh_store<NT>(orow + k, cadd(E, WO));
+    }
+    return;
+  }
   // staged rows: measured on MI355X, z pass at N = 512 0.267 -> 0.255 ms per
   // mesh pair; at N = 1024 the extra registers (70 -> 86 VGPRs, 7 -> 5 waves
   // per SIMD) cost more than the whole lines save (4.17 -> 4.33 ms)

@@ -42,7 +42,21 @@ def c2_conf(P):
                 poles=[0, 2, 4])
 
 
-def stage_bytes(Ng, nmesh, npart):
+def x_live_fraction(Ng, L, kedge_max, C=16):
+    """Share of the stored modes the fused x pass reads: it skips the (k_y,
+    k_z-tile) columns of C k_z values whose nearest mode lies at or past the
+    last bin edge (same test as the kernel, for a cubic box of side L)."""
+    kf = 2 * np.pi / L
+    M = Ng // 2
+    ky = np.fft.fftfreq(Ng, 1.0 / Ng) * kf
+    q = np.arange((M + 1 + C - 1) // C)
+    kz0 = q * C * kf
+    live = np.sqrt(ky[:, None] ** 2 + kz0[None, :] ** 2) < kedge_max
+    cols = np.minimum(C, M + 1 - q * C)
+    return float((live * cols[None, :]).sum() / (Ng * (M + 1)))
+
+
+def stage_bytes(Ng, nmesh, npart, x_live=1.0):
     """Algorithmic (compulsory) HBM bytes per launch of each stage."""
     ntot = float(Ng) ** 3
     nc = float(Ng) * Ng * (Ng // 2 + 1)
@@ -50,7 +64,7 @@ def stage_bytes(Ng, nmesh, npart):
         "deposit": nmesh * 4 * ntot,                    # each mesh written once
         "z_r2c": nmesh * (4 * ntot + 8 * nc),            # read real, write half-complex
         "y_fft": nmesh * 16 * nc,                        # read + write complex
-        "x_fft_bin": nmesh * 8 * nc,                     # read complex; bins negligible
+        "x_fft_bin": nmesh * 8 * nc * x_live,            # read the columns with binned modes
         "assign_total": npart * 32 + nmesh * 4 * ntot,   # particles once + meshes once
     }
 
@@ -118,8 +132,8 @@ def measured_traffic(kernel):
     return None
 
 
-def roofline(stages, Ng, nmesh, npart):
-    b = stage_bytes(Ng, nmesh, npart)
+def roofline(stages, Ng, nmesh, npart, x_live=1.0):
+    b = stage_bytes(Ng, nmesh, npart, x_live)
     cands = {k: stages[k] for k in ("deposit", "z_r2c", "y_fft", "x_fft_bin")}
     dom = max(cands, key=lambda k: cands[k])
     per_launch = b[dom]
@@ -430,7 +444,7 @@ def main():
     arch = P.device_arch()
     fixed = [int(v) for v in args.tile.split(",")] if args.tile else None
     data, res, ms_step, stages, n, tile = run_c2(P, args.steps, args.warmup, fixed)
-    rl = roofline(stages, 1024, 2, n)
+    rl = roofline(stages, 1024, 2, n, x_live_fraction(1024, 1000.0, float(res["kedge"][-1])))
     extra = {}
     if not args.no_batched and args.nmock > 0:
         extra = run_batched(P, data, args.nmock)

@@ -578,8 +578,28 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     pass_load_rows<N, C, T, true>(va, ld_field(a.A, tb, tn));
     if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
   };
-  if (t0 < ntiles) load_tile(t0);
-  for (size_t tile = t0; tile < ntiles; tile += gridDim.x) {
+  // A tile whose nearest mode (k_x = 0, lowest k_z) already lies at or past
+  // the last bin edge has no mode in the bins (|k|^2 = (k_x^2 + k_y^2) + k_z^2
+  // only grows with |k_x| and k_z, in FP64 as in the binning): its x
+  // transform is skipped.  With k_max = k_Ny that is about 1 - pi/4 of the
+  // (k_y, k_z) tiles.  Not for the survey l = 0 pass, whose stored field is
+  // transformed back in full, nor for log bins.
+  auto live = [&](size_t t) -> bool {
+    if (EPI == PSH_EPI_SURVEY0 || b.logscale) return true;
+    int ty, tq;
+    (void)tile_base(t, ty, tq);
+    const int kz0 = tq * C;
+    double m2 = a.ax.ksq[1][ty];
+    if (kz0 > 0) m2 = m2 + a.ax.ksq[2][kz0];
+    return sqrt(m2) < b.kedgeN;
+  };
+  auto next_live = [&](size_t t) -> size_t {
+    while (t < ntiles && !live(t)) t += gridDim.x;
+    return t;
+  };
+  size_t tile = next_live(t0);
+  if (tile < ntiles) load_tile(tile);
+  for (; tile < ntiles;) {
     int y, q;
     const size_t base = tile_base(tile, y, q);
     const int ncv = min(C, M + 1 - q * C);
@@ -627,7 +647,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       }
     }
     __syncthreads();
-    if (tile + gridDim.x < ntiles) load_tile(tile + gridDim.x);
+    const size_t next = next_live(tile + gridDim.x);
+    if (next < ntiles) load_tile(next);
 
     // phase 2: column c, rows i = threadIdx.x / C + k T / C
     if (colok) {
@@ -699,6 +720,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       }
     }
     __syncthreads();
+    tile = next;
   }
   double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc;
   for (int i = threadIdx.x; i < a.nacc; i += T) out[i] = bins[i];

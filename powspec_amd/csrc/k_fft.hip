@@ -388,7 +388,8 @@ z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
 template <int N, bool INV, bool NT>
 __global__ void __launch_bounds__(ColCfg<N>::T)
 col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
-           size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw) {
+           size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw,
+           const int *__restrict__ ylive, int cxs) {
   constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
   constexpr int V = max_values<N, C, T, true>();
   extern __shared__ float lds[];
@@ -409,8 +410,17 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
                                                          (uint32_t)j * (uint32_t)ES + c));
     return c < ncv ? x : make_float2(0.f, 0.f);
   });
+  // outputs k_y = i' (i = j + r STORE) past the column's live bound are
+  // never read (the fused x pass skips their tiles): not stored
+  int kylim = M;
+  if (ylive) {
+    const int cc = min(opaque_tid<T>() % C, ncv - 1);   // the thread's column in every pass
+    kylim = ylive[(q * C + cc) >> cxs];
+  }
   store_last_rows<N, C, T, true>(v, [&](int r, int c, int j, float2 x) {
-    if (c < ncv)
+    const int i = j + r * RS::STORE;
+    const int ay = i <= M ? i : N - i;
+    if (c < ncv && ay <= kylim)
       mesh_store<NT>(row_elem<RS::BYTE32>(out + base + (size_t)(r * RS::STORE) * ES,
                                           (uint32_t)j * (uint32_t)ES + c), x);
   });
@@ -436,6 +446,7 @@ struct XBinDev {
   float legc[PSH_MAX_NL][4];
   int oddmask;
   int even024;   // the poles are exactly 0, 2, 4 (the common request)
+  const int *ylive;   // per k_z tile, the largest |k_y| with binned modes, or nullptr
 };
 
 // Bin of a mode from |k|^2, in the reference's operation order
@@ -579,19 +590,18 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
   };
   // A tile whose nearest mode (k_x = 0, lowest k_z) already lies at or past
-  // the last bin edge has no mode in the bins (|k|^2 = (k_x^2 + k_y^2) + k_z^2
-  // only grows with |k_x| and k_z, in FP64 as in the binning): its x
-  // transform is skipped.  With k_max = k_Ny that is about 1 - pi/4 of the
-  // (k_y, k_z) tiles.  Not for the survey l = 0 pass, whose stored field is
-  // transformed back in full, nor for log bins.
+  // the last bin edge has no mode in the bins: its x transform is skipped.
+  // ylive[q] is the largest |k_y| index of a live tile in k_z tile q (made on
+  // the host with the binning's FP64 |k|, which only grows with |k_x|, |k_y|
+  // and k_z); with k_max = k_Ny about 1 - pi/4 of the (k_y, k_z) tiles are
+  // skipped.  nullptr (the survey l = 0 pass, whose stored field is
+  // transformed back in full, and log bins): every tile.
   auto live = [&](size_t t) -> bool {
-    if (EPI == PSH_EPI_SURVEY0 || b.logscale) return true;
+    if (!a.ylive) return true;
     int ty, tq;
     (void)tile_base(t, ty, tq);
-    const int kz0 = tq * C;
-    double m2 = a.ax.ksq[1][ty];
-    if (kz0 > 0) m2 = m2 + a.ax.ksq[2][kz0];
-    return sqrt(m2) < b.kedgeN;
+    const int ay = ty <= M ? ty : N - ty;
+    return ay <= a.ylive[tq];
   };
   auto next_live = [&](size_t t) -> size_t {
     while (t < ntiles && !live(t)) t += gridDim.x;
@@ -784,8 +794,13 @@ struct ZC2R {
 template <int N>
 struct Col {
   static hipError_t run(const float2 *in, float2 *out, int nmesh, size_t mstride,
-                        int axis, const float2 *tw, int inverse, hipStream_t s) {
+                        int axis, const float2 *tw, int inverse, hipStream_t s,
+                        const int *ylive) {
     constexpr int C = ColCfg<N>::C;
+    constexpr int CX = XCfg<N, true>::C;
+    static_assert(CX == XCfg<N, false>::C && (CX & (CX - 1)) == 0, "one x tile width");
+    int cxs = 0;
+    while ((1 << cxs) < CX) cxs++;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
     const size_t OS = axis == 1 ? (size_t)N * H : (size_t)H;
@@ -800,7 +815,7 @@ struct Col {
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shmem);
       hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N>::T), shmem, s, in, out, ntz, OS, ES,
-                         mstride / 2, tw);
+                         mstride / 2, tw, ylive, cxs);
       return hipGetLastError();
     };
     if constexpr (stream_mesh<N>()) return go(std::true_type{});
@@ -879,20 +894,23 @@ hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh, size_t mstride_floats
 }
 
 hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
-                     const float2 *tw, int inverse, hipStream_t s) {
+                     const float2 *tw, int inverse, hipStream_t s, const int *ylive) {
   return dispatch_n<Col>(Ng, (const float2 *)data, data, nmesh, mstride_floats, 1, tw,
-                         inverse, s);
+                         inverse, s, inverse ? nullptr : ylive);
 }
 
 hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
                      size_t mstride_floats, const float2 *tw, int inverse,
                      hipStream_t s) {
-  return dispatch_n<Col>(Ng, in, out, nmesh, mstride_floats, 0, tw, inverse, s);
+  return dispatch_n<Col>(Ng, in, out, nmesh, mstride_floats, 0, tw, inverse, s,
+                         (const int *)nullptr);
 }
 
 int psh_xbin_nacc(const PshBins &b, int epi) {
   return epi == PSH_EPI_SIM ? b.nl * b.nbin : b.nbin;
 }
+
+int psh_xbin_tile_cols(int Ng) { return Ng >= 2048 ? XCfg<2048, true>::C : XCfg<1024, true>::C; }
 
 namespace {
 template <int N>
@@ -937,6 +955,7 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.m = a.m;
   d.ynorm = a.ynorm;
   d.vec0f = a.vec0f;
+  d.ylive = a.ylive;
   // Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference) as
   // polynomials in mu^2
   static const float kLeg[7][4] = {

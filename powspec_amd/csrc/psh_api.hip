@@ -84,6 +84,7 @@ struct psh_ctx {
   PshBins bins{};
   double *d_tab = nullptr;
   float *d_rpos = nullptr;
+  int *d_ylive = nullptr;   // per x-pass k_z tile, the largest live |k_y| (nullptr: all)
   PshAxisTables ax{};
   bool geom_valid = false;
   std::vector<unsigned long long> cnt;
@@ -360,6 +361,35 @@ int make_tables(psh_ctx *ctx) {
     ctx->ax.ksqf[a] = dtf + (size_t)(3 * a + 1) * N;
     ctx->ax.alf[a] = dtf + (size_t)(3 * a + 2) * N;
   }
+  // Live k_y bound per tile of the fused x pass (k_z columns [q CX, q CX +
+  // CX)): the largest |k_y| index for which the tile's nearest mode (k_x = 0,
+  // k_z = q CX) lies before the last bin edge, with the binning's own FP64
+  // |k| = sqrt((k_x^2 + k_y^2) + k_z^2) (src/multipole.c:147-158), which
+  // only grows with |k_x|, |k_y| and k_z.  The y pass stores no output past
+  // it and the x pass skips the tiles; -1 when no k_y is live.  Log bins
+  // keep every tile.
+  if (ctx->conf.logscale) {
+    if (ctx->d_ylive) (void)hipFree(ctx->d_ylive);
+    ctx->d_ylive = nullptr;
+  } else {
+    const int CX = psh_xbin_tile_cols(N);
+    const int ntzx = (N / 2 + 1 + CX - 1) / CX;
+    const double *ksqy = h.data() + (size_t)(4 * 1 + 1) * N;
+    const double *ksqz = h.data() + (size_t)(4 * 2 + 1) * N;
+    std::vector<int> yl(ntzx, -1);
+    for (int q = 0; q < ntzx; q++) {
+      const int kz0 = q * CX;
+      for (int ky = 0; ky <= N / 2; ky++) {
+        double m2 = ksqy[ky];
+        if (kz0 > 0) m2 = m2 + ksqz[kz0];
+        if (!(sqrt(m2) < ctx->bins.kedgeN)) break;
+        yl[q] = ky;
+      }
+    }
+    if ((rc = dev_alloc(ctx, &ctx->d_ylive, yl.size()))) return rc;
+    HIPCHK(ctx, hipMemcpy(ctx->d_ylive, yl.data(), sizeof(int) * yl.size(),
+                          hipMemcpyHostToDevice));
+  }
   if (ctx->survey) {
     // r-space positions (i + smin_g) * L with smin_g = bmin * Ng / L
     // (src/genr_mesh.c:913-914, src/mp_template.c:71-83)
@@ -406,27 +436,16 @@ int ensure_geometry(psh_ctx *ctx) {
   return PSH_OK;
 }
 
-int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
-  bool same = ctx->box_set;
-  for (int a = 0; a < 3; a++)
-    if (ctx->L[a] != L[a] || ctx->bmin[a] != bmin[a]) same = false;
-  if (same) return PSH_OK;
-  for (int a = 0; a < 3; a++) {
-    ctx->L[a] = L[a];
-    ctx->bmin[a] = bmin[a];
-  }
-  int rc;
-  if ((rc = make_bins(ctx))) return rc;
-  if ((rc = make_tables(ctx))) return rc;
-  // assignment geometry
+// Assignment geometry of the context's box and deposit tiles.
+void set_assign_geom(psh_ctx *ctx) {
   PshAssignGeom &g = ctx->ag;
   g.Ng = ctx->Ng;
   g.scheme = ctx->conf.assign;
   g.intlace = ctx->conf.intlace;
   g.check_box = ctx->conf.issim ? 1 : 0;   // simulation boxes: [0, L) per axis
-  // 32-bit fixed-point LDS tiles (<= 64 KB): 8x16x32 cells per grid by
-  // default (measured best of 7 shapes for PCS interlaced at Ng = 1024);
-  // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4)
+  // 32-bit fixed-point LDS tiles: 8x16x32 cells per grid by default;
+  // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4), and so do
+  // psh_ctx_set_tiles and a tuning cache
   int tl[3] = {8, 16, 32};
   if (const char *env = getenv("PSH_TILE")) {
     int t[3] = {0, 0, 0};
@@ -444,11 +463,26 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   }
   g.row_pitch = 2 * ctx->H;
   for (int a = 0; a < 3; a++) {
-    g.bmin[a] = bmin[a];
-    g.smin[a] = bmin[a] - 0.5 * L[a] / ctx->Ng;   // src/genr_mesh.c:817-818
-    g.L[a] = L[a];
+    g.bmin[a] = ctx->bmin[a];
+    g.smin[a] = ctx->bmin[a] - 0.5 * ctx->L[a] / ctx->Ng;   // src/genr_mesh.c:817-818
+    g.L[a] = ctx->L[a];
   }
   ctx->ntiles_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
+}
+
+int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
+  bool same = ctx->box_set;
+  for (int a = 0; a < 3; a++)
+    if (ctx->L[a] != L[a] || ctx->bmin[a] != bmin[a]) same = false;
+  if (same) return PSH_OK;
+  for (int a = 0; a < 3; a++) {
+    ctx->L[a] = L[a];
+    ctx->bmin[a] = bmin[a];
+  }
+  int rc;
+  if ((rc = make_bins(ctx))) return rc;
+  if ((rc = make_tables(ctx))) return rc;
+  set_assign_geom(ctx);
   ctx->box_set = true;
   ctx->geom_valid = false;
   return PSH_OK;
@@ -662,8 +696,9 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   if (B)
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
   ev_record(ctx, E_ZDONE);
-  HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
-  if (B) HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream));
+  HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
+  if (B)
+    HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
   ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, nmock);
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
@@ -681,6 +716,7 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   a.partials = ctx->partials;
   a.grid = grid;
   a.vec0f = (float)(2 * M_PI / ctx->L[0]);
+  a.ylive = ctx->d_ylive;
   HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
   ev_record(ctx, E_XDONE);
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, nmock, grid, nacc, out_bins, ctx->stream));
@@ -802,9 +838,10 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
       y.norm = ylm_norm(ell, m);
       for (int ax = 0; ax < 3; ax++) y.rpos[ax] = ctx->d_rpos + (size_t)ax * N;
       HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream));
-      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;
+      b.ylive = ctx->d_ylive;   // S is read by this pass only
       b.A = S;
       b.B = nullptr;
       b.store = nullptr;
@@ -974,7 +1011,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   void *bufs[] = {ctx->meshA, ctx->meshB, ctx->meshS, ctx->counts, ctx->starts, ctx->cursors,
                   ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
-                  ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_gcnt, ctx->d_gkm,
+                  ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
                   ctx->d_glcnt, ctx->d_gpart_cnt, ctx->d_gpart_km, ctx->d_gpart_lcnt,
                   ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat, ctx->whist};
   for (void *p : bufs)
@@ -1573,10 +1610,11 @@ extern "C" int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]) {
   if (!tiles_ok(tile, ctx->conf.assign, ctx->conf.intlace, ctx->survey ? 8 : 4))
     return set_err(ctx, PSH_ERR_ARG, "invalid tile shape %dx%dx%d", tile[0], tile[1], tile[2]);
   for (int a = 0; a < 3; a++) ctx->tile_req[a] = tile[a];
-  if (ctx->box_set) {   // re-derive the assignment geometry with the new tiles
-    double L[3] = {ctx->L[0], ctx->L[1], ctx->L[2]};
-    double bmin[3] = {ctx->bmin[0], ctx->bmin[1], ctx->bmin[2]};
-    return set_box(ctx, L, bmin);
+  if (ctx->box_set) {
+    // re-derive the assignment geometry with the new tiles (the box, bins
+    // and tables stay); a captured batch graph holds the old tiles
+    drop_graph(ctx);
+    set_assign_geom(ctx);
   }
   return PSH_OK;
 }

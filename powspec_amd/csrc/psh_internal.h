@@ -140,8 +140,11 @@ hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh,
     size_t mstride_floats, const float2 *tw, float scale, hipStream_t s);
 
 // Complex FFT along y (axis 1) in place; dir +1 inverse, -1 forward.
+// ylive (forward y pass of fields that only the fused x pass reads, or
+// nullptr): per x-pass tile of k_z columns, the largest |k_y| index whose
+// tile holds binned modes; outputs past it are not stored.
 hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
-    const float2 *tw, int inverse, hipStream_t s);
+    const float2 *tw, int inverse, hipStream_t s, const int *ylive = nullptr);
 
 // Complex FFT along x (axis 0) from `in` to `out` (may alias); no epilogue.
 hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
@@ -164,8 +167,12 @@ struct PshXBinArgs {
   const float2 *phase;     // interlace phase tables [3][Ng] exp(i pi i'/Ng)
   double *partials;        // [grid][nacc] per workgroup partial sums
   int grid;                // number of persistent workgroups per mock
+  const int *ylive;        // per k_z tile, the largest |k_y| with binned modes
+                           // (tiles past it are skipped), or nullptr: all tiles
 };
 int psh_xbin_nacc(const PshBins &b, int epi);
+// k_z columns per tile of the fused x pass at a grid size.
+int psh_xbin_tile_cols(int Ng);
 // LDS bytes of the fused x pass for a grid size, interlacing and number of
 // accumulators (0 for an unsupported grid size); at most 160 KB run.
 size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc);

@@ -43,9 +43,10 @@ def c2_conf(P):
 
 
 def x_live_fraction(Ng, L, kedge_max, C=16):
-    """Share of the stored modes the fused x pass reads: it skips the (k_y,
-    k_z-tile) columns of C k_z values whose nearest mode lies at or past the
-    last bin edge (same test as the kernel, for a cubic box of side L)."""
+    """Share of the stored modes the fused x pass reads (and the y pass
+    writes): the (k_y, k_z-tile) columns of C k_z values whose nearest mode
+    lies before the last bin edge (same test as the library, for a cubic box
+    of side L)."""
     kf = 2 * np.pi / L
     M = Ng // 2
     ky = np.fft.fftfreq(Ng, 1.0 / Ng) * kf
@@ -63,7 +64,7 @@ def stage_bytes(Ng, nmesh, npart, x_live=1.0):
     return {
         "deposit": nmesh * 4 * ntot,                    # each mesh written once
         "z_r2c": nmesh * (4 * ntot + 8 * nc),            # read real, write half-complex
-        "y_fft": nmesh * 16 * nc,                        # read + write complex
+        "y_fft": nmesh * 8 * nc * (1 + x_live),          # read all, write the columns the x pass reads
         "x_fft_bin": nmesh * 8 * nc * x_live,            # read the columns with binned modes
         "assign_total": npart * 32 + nmesh * 4 * ntot,   # particles once + meshes once
     }

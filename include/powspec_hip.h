@@ -341,6 +341,11 @@ int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, in
  * PSH_ERR_ARG when invalid. */
 int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
 
+/* The deposit tiles in use (each axis clipped to GRID_SIZE): the shape set by
+ * psh_ctx_set_tiles, a tuning cache, PSH_TILE or the default.  PSH_ERR_ARG
+ * before the context has a box (surveys: before their first run). */
+int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]);
+
 /* Applies the fastest entry of a tuning cache for the context's device
  * architecture, grid size, scheme, interlacing and batch size (rounded down
  * to a power of two).  PSH_ERR_FILE / PSH_ERR_ARG when none applies. */

@@ -68,6 +68,7 @@ EXPORTS = [
     "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
+    "psh_ctx_get_tiles",
     "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device", "psh_mlp_param_count",
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
@@ -165,6 +166,7 @@ def load():
     L.psh_ctx_destroy.argtypes = [vp]
     L.psh_ctx_set_box.argtypes = [vp, C.POINTER(d), C.POINTER(d)]
     L.psh_ctx_nbin.argtypes = [vp]
+    L.psh_ctx_get_tiles.argtypes = [vp, C.POINTER(C.c_int)]
     L.psh_ctx_run_host.argtypes = [vp, C.POINTER(Cata), C.POINTER(Pk)]
     L.psh_ctx_define_box.argtypes = [vp, C.POINTER(Cata)]
     L.psh_ctx_run_device.argtypes = [vp, vp, sz, d, C.POINTER(Pk)]
@@ -356,6 +358,12 @@ class Context:
 
     def set_tiles(self, tile):
         self._check(self.L.psh_ctx_set_tiles(self.h, (C.c_int * 3)(*tile)))
+
+    def tiles(self):
+        """The deposit tiles in use (clipped to the grid size)."""
+        t = (C.c_int * 3)()
+        self._check(self.L.psh_ctx_get_tiles(self.h, t))
+        return tuple(t)
 
     def load_tuning(self, cache_path):
         self._check(self.L.psh_ctx_load_tuning(self.h, str(cache_path).encode()))

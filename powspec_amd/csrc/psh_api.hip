@@ -1605,6 +1605,14 @@ extern "C" int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, s
 // deposit tile tuning
 // ===========================================================================
 
+extern "C" int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]) {
+  if (!ctx || !tile) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  if (!ctx->box_set)
+    return set_err(const_cast<psh_ctx *>(ctx), PSH_ERR_ARG, "the context has no box yet");
+  for (int a = 0; a < 3; a++) tile[a] = ctx->ag.tile[a];
+  return PSH_OK;
+}
+
 extern "C" int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]) {
   if (!ctx || !tile) return set_err(ctx, PSH_ERR_ARG, "null argument");
   if (!tiles_ok(tile, ctx->conf.assign, ctx->conf.intlace, ctx->survey ? 8 : 4))

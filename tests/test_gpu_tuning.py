@@ -17,6 +17,7 @@ def test_tile_shapes_keep_parity(psh, oracle, tile):
     kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.02, poles=[0, 2, 4])
     ctx = psh.Context(psh.make_conf(**kw))
     ctx.set_tiles(tile)
+    assert ctx.tiles() == tuple(min(t, Ng) for t in tile)   # the shape is the one in use
     got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, d[:, 3].sum())
     ref = oracle.run(oracle.make_conf(**kw), [d])
     _check(got, ref, [0, 2, 4], ref["shot"][0])
@@ -65,6 +66,7 @@ def test_autotune_and_cache(psh, tmp_path):
     assert line[0].startswith("gfx950") and [int(v) for v in line[6:9]] == list(tile)
     ctx = psh.Context(conf)
     ctx.load_tuning(str(cache))
+    assert ctx.tiles() == tuple(min(t, 64) for t in tile)
     d = _uniform(20000, [500.0] * 3, 52)
     res = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
     assert res["nbin"] > 0

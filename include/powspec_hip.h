@@ -337,7 +337,7 @@ int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, in
 /* Tiles of a context: cells per axis, powers of two, each at least the
  * window width (scheme + 1), z >= 4, and both grids' LDS tiles within
  * 160 KB (z rows padded by the window width on each side; 4-byte cells for
- * boxes, 8-byte for surveys).  Overrides the default 8x16x32 and any cache.
+ * boxes, 8-byte for surveys).  Overrides the default 8x16x64 and any cache.
  * PSH_ERR_ARG when invalid. */
 int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
 

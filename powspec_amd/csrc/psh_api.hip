@@ -443,10 +443,12 @@ void set_assign_geom(psh_ctx *ctx) {
   g.scheme = ctx->conf.assign;
   g.intlace = ctx->conf.intlace;
   g.check_box = ctx->conf.issim ? 1 : 0;   // simulation boxes: [0, L) per axis
-  // 32-bit fixed-point LDS tiles: 8x16x32 cells per grid by default;
+  // 8x16x64 cells per grid by default (measured on MI355X against 8x16x32:
+  // 2-10 % faster for every scheme, grid size and interlacing timed; it fits
+  // the LDS for every scheme with 64-bit survey cells too);
   // PSH_TILE="tx,ty,tz" overrides (powers of two, tz >= 4), and so do
   // psh_ctx_set_tiles and a tuning cache
-  int tl[3] = {8, 16, 32};
+  int tl[3] = {8, 16, 64};
   if (const char *env = getenv("PSH_TILE")) {
     int t[3] = {0, 0, 0};
     if (sscanf(env, "%d,%d,%d", &t[0], &t[1], &t[2]) == 3 &&

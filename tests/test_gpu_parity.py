@@ -69,6 +69,21 @@ def test_sim_grid_sizes(psh, oracle, Ng):
     _check(got, ref, [0, 1, 2, 3, 4, 6], ref["shot"][0])
 
 
+@pytest.mark.parametrize("kmax", [0.3, 0.6])
+def test_sim_kmax_below_nyquist(psh, oracle, kmax):
+    """k_max below k_Ny: most (k_y, k_z) tiles hold no binned mode, the y pass
+    stores only the outputs the x pass reads and the x pass skips the rest."""
+    Ng, L, n = 128, 500.0, 40000
+    d = _uniform(n, [L] * 3, 57, weights=True)
+    kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.01, kmax=kmax,
+              poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), n, d[:, 3].sum())
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    assert got["nbin"] == round(kmax / 0.01)
+    _check(got, ref, [0, 2, 4], ref["shot"][0])
+
+
 def test_sim_config1_exact_setup(psh, oracle):
     """L=1000, 1e6 uniform particles, CIC, Ng=256, no interlacing, l=0,2,4,
     dk=0.01 (the reference's CPU-runnable case)."""

@@ -400,6 +400,13 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
   const int o = (int)(rem / ntz), q = (int)(rem - (size_t)o * ntz);
   const size_t base = mesh * mstride_c + o * OS + (size_t)q * C;
   const int ncv = min(C, M + 1 - q * C);
+  if (ylive) {
+    // a tile of k_z columns with no live k_y at all (k_z past k_max) is
+    // never read by the x pass: nothing to transform
+    bool any = false;
+    for (int t = (q * C) >> cxs; t <= (q * C + ncv - 1) >> cxs; t++) any |= ylive[t] >= 0;
+    if (!any) return;
+  }
   using RS = RowSplit<N, C, T, true>;
   static_assert(PSH_PAD == 16 && RS::XSPAN < (1ull << 32), "lane offsets are 32-bit");
   // columns q*C + c < H always lie inside the padded row; those past kz = M

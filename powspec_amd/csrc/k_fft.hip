@@ -110,7 +110,7 @@ template <int N, bool YLM, bool NT>
 __global__ void __launch_bounds__(256)
 z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
              size_t nlines, size_t mstride_f, int H,
-             const float2 *__restrict__ tw, YlmDev ylm) {
+             const float2 *__restrict__ tw, YlmDev ylm, int kzlim) {
   constexpr int M = N / 2, C = ZCfg<N>::C, T = ZCfg<N>::T;
   constexpr int V = max_values<M, C, T, false>();
   __shared__ __align__(16) float lds[Lds<M, C, false>::SIZE];
@@ -208,12 +208,13 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
         const float2 E = make_float2(0.5f * (r0 + r1), 0.5f * (ik - imk));
         const float2 O = make_float2(0.5f * (ik + imk), -0.5f * (r0 - r1));
         const float2 WO = cmul(twk[u], O);
-        mesh_store<NT>(row_elem<true>(orow, (uint32_t)k), cadd(E, WO));
+        if (k <= kzlim) mesh_store<NT>(row_elem<true>(orow, (uint32_t)k), cadd(E, WO));
         const float2 d = csub(E, WO);   // k < M/2: M - k != k
-        mesh_store<NT>(row_elem<true>(orow, (uint32_t)(M - k)), make_float2(d.x, -d.y));
+        if (M - k <= kzlim)
+          mesh_store<NT>(row_elem<true>(orow, (uint32_t)(M - k)), make_float2(d.x, -d.y));
       }
     }
-    if (t < C && line0 + t < nlines) {   // X[M/2] = conj of itself's pair: one value per line
+    if (t < C && line0 + t < nlines && M / 2 <= kzlim) {   // X[M/2]: one value per line
       const size_t line = line0 + t;
       const size_t mesh = line / lpm, rowi = line - mesh * lpm;
       float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
@@ -245,8 +246,8 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       float2 WO = cmul(tw[k], O);
       size_t mesh = line / lpm, rowi = line - mesh * lpm;
       float2 *orow = out + mesh * (mstride_f / 2) + rowi * (size_t)H;
-      mesh_store<NT>(orow + k, cadd(E, WO));
-      if (M - k != k) {
+      if (k <= kzlim) mesh_store<NT>(orow + k, cadd(E, WO));
+      if (M - k != k && M - k <= kzlim) {
         float2 d = csub(E, WO);
         mesh_store<NT>(orow + (M - k), make_float2(d.x, -d.y));
       }
@@ -301,7 +302,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
         if (line >= nlines) continue;
         const size_t mesh = line / lpm, rowi = line - mesh * lpm;
         float4 *orow = reinterpret_cast<float4 *>(out + mesh * (mstride_f / 2) + rowi * (size_t)H);
-        mesh_store4(orow + q, src[i]);
+        if (2 * q <= kzlim) mesh_store4(orow + q, src[i]);
       }
     }
   }
@@ -765,7 +766,7 @@ hipError_t dispatch_n(int N, Args &&...args) {
 template <int N>
 struct ZR2C {
   static hipError_t run(const float *in, float2 *out, int nmesh, size_t mstride,
-                        const float2 *tw, const YlmDev &y, hipStream_t s) {
+                        const float2 *tw, const YlmDev &y, hipStream_t s, int kzlim) {
     const size_t nlines = (size_t)N * N * nmesh;
     const unsigned blocks = (unsigned)((nlines + ZCfg<N>::C - 1) / ZCfg<N>::C);
     const int H = N / 2 + PSH_PAD;
@@ -773,10 +774,10 @@ struct ZR2C {
       constexpr bool NT = decltype(ntc)::value;
       if (y.ell > 0)
         hipLaunchKernelGGL((z_r2c_kernel<N, true, NT>), dim3(blocks), dim3(ZCfg<N>::T), 0,
-                           s, in, out, nlines, mstride, H, tw, y);
+                           s, in, out, nlines, mstride, H, tw, y, kzlim);
       else
         hipLaunchKernelGGL((z_r2c_kernel<N, false, NT>), dim3(blocks), dim3(ZCfg<N>::T), 0,
-                           s, in, out, nlines, mstride, H, tw, y);
+                           s, in, out, nlines, mstride, H, tw, y, kzlim);
       return hipGetLastError();
     };
     if constexpr (stream_mesh<N>()) return go(std::true_type{});
@@ -886,13 +887,13 @@ hipError_t psh_fft_twiddles(int N, float2 **d_tw) {
 
 hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
                          size_t mstride_floats, const float2 *tw,
-                         const PshYlmR &ylm, hipStream_t s) {
+                         const PshYlmR &ylm, hipStream_t s, int kzlim) {
   YlmDev y{};
   y.ell = ylm.ell;
   y.m = ylm.m;
   y.norm = ylm.norm;
   for (int a = 0; a < 3; a++) y.rpos[a] = ylm.rpos[a];
-  return dispatch_n<ZR2C>(Ng, in, out, nmesh, mstride_floats, tw, y, s);
+  return dispatch_n<ZR2C>(Ng, in, out, nmesh, mstride_floats, tw, y, s, kzlim);
 }
 
 hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh, size_t mstride_floats,

@@ -85,6 +85,7 @@ struct psh_ctx {
   double *d_tab = nullptr;
   float *d_rpos = nullptr;
   int *d_ylive = nullptr;   // per x-pass k_z tile, the largest live |k_y| (nullptr: all)
+  int kzlim = 1 << 30;      // last k_z column of a live x-pass tile (z pass stores up to it)
   PshAxisTables ax{};
   bool geom_valid = false;
   std::vector<unsigned long long> cnt;
@@ -368,6 +369,7 @@ int make_tables(psh_ctx *ctx) {
   // only grows with |k_x|, |k_y| and k_z.  The y pass stores no output past
   // it and the x pass skips the tiles; -1 when no k_y is live.  Log bins
   // keep every tile.
+  ctx->kzlim = 1 << 30;
   if (ctx->conf.logscale) {
     if (ctx->d_ylive) (void)hipFree(ctx->d_ylive);
     ctx->d_ylive = nullptr;
@@ -386,6 +388,10 @@ int make_tables(psh_ctx *ctx) {
         yl[q] = ky;
       }
     }
+    // the z pass stores k_z up to the last column of the last live tile
+    ctx->kzlim = -1;
+    for (int q = 0; q < ntzx; q++)
+      if (yl[q] >= 0) ctx->kzlim = std::min(N / 2, (q + 1) * CX - 1);
     if ((rc = dev_alloc(ctx, &ctx->d_ylive, yl.size()))) return rc;
     HIPCHK(ctx, hipMemcpy(ctx->d_ylive, yl.data(), sizeof(int) * yl.size(),
                           hipMemcpyHostToDevice));
@@ -694,9 +700,11 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   PshYlmR noy{};
   float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
   float2 *B = ctx->conf.intlace ? reinterpret_cast<float2 *>(ctx->meshB) : nullptr;
-  HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
+  HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy, ctx->stream,
+                            ctx->kzlim));
   if (B)
-    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream));
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream,
+                              ctx->kzlim));
   ev_record(ctx, E_ZDONE);
   HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
   if (B)
@@ -839,7 +847,8 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
       y.m = m;
       y.norm = ylm_norm(ell, m);
       for (int ax = 0; ax < 3; ax++) y.rpos[ax] = ctx->d_rpos + (size_t)ax * N;
-      HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream));
+      HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream,
+                                ctx->kzlim));
       HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;

@@ -131,9 +131,11 @@ struct PshYlmR {
   const float *rpos[3];  // per-axis (i + smin_g) * L, device arrays of Ng
 };
 // tw: twiddle table of N = Ng entries.
+// kzlim: outputs past k_z = kzlim are not stored (fields that only the
+// fused x pass reads, whose tiles past it hold no binned mode).
 hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
     size_t mstride_floats, const float2 *tw, const PshYlmR &ylm,
-    hipStream_t s);
+    hipStream_t s, int kzlim = 1 << 30);
 
 // Inverse c2r along z in place, output scaled by `scale`.
 hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh,

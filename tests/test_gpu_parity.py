@@ -69,11 +69,13 @@ def test_sim_grid_sizes(psh, oracle, Ng):
     _check(got, ref, [0, 1, 2, 3, 4, 6], ref["shot"][0])
 
 
-@pytest.mark.parametrize("kmax", [0.3, 0.6])
-def test_sim_kmax_below_nyquist(psh, oracle, kmax):
-    """k_max below k_Ny: most (k_y, k_z) tiles hold no binned mode, the y pass
-    stores only the outputs the x pass reads and the x pass skips the rest."""
-    Ng, L, n = 128, 500.0, 40000
+@pytest.mark.parametrize("Ng,kmax", [(128, 0.3), (128, 0.6), (512, 0.3), (1024, 0.3)])
+def test_sim_kmax_below_nyquist(psh, oracle, Ng, kmax):
+    """k_max below k_Ny: most (k_y, k_z) tiles hold no binned mode; the z pass
+    stores k_z only up to the last live tile, the y pass transforms and stores
+    only what the x pass reads, and the x pass skips the rest (N_g = 512 and
+    1024 take the staged and the line-by-line z pass)."""
+    L, n = 500.0, 40000
     d = _uniform(n, [L] * 3, 57, weights=True)
     kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.01, kmax=kmax,
               poles=[0, 2, 4])

@@ -478,11 +478,16 @@ void set_assign_geom(psh_ctx *ctx) {
   ctx->ntiles_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
 }
 
+void drop_graph(psh_ctx *ctx);
+
 int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   bool same = ctx->box_set;
   for (int a = 0; a < 3; a++)
     if (ctx->L[a] != L[a] || ctx->bmin[a] != bmin[a]) same = false;
   if (same) return PSH_OK;
+  // a captured batch graph holds the tables and geometry of the old box,
+  // which are about to be reallocated
+  drop_graph(ctx);
   for (int a = 0; a < 3; a++) {
     ctx->L[a] = L[a];
     ctx->bmin[a] = bmin[a];

@@ -191,6 +191,29 @@ def test_batch_matches_plain(psh):
             assert np.all(np.abs(r["pl"][0] - p["pl"][0]) <= 1e-5 * scale)
 
 
+def test_batch_graph_after_box_change(psh):
+    """A captured batch graph is dropped when the box changes: the replay
+    after psh_ctx_set_box uses the new box's tables, as a fresh context."""
+    Ng = 32
+    rng = np.random.default_rng(13)
+    sizes = rng.integers(2000, 5000, 3)
+    cats = [_uniform(int(v), [800.0] * 3, 400 + i) for i, v in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    dev = psh.DeviceArray.from_numpy(np.concatenate(cats))
+    w = [float(v) for v in sizes]
+    kw = dict(gsize=Ng, assign=2, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw), nmock_max=len(cats))
+    ctx.run_batch(dev, off, w, graph=True)
+    ctx.set_box(800.0)
+    got = ctx.run_batch(dev, off, w, graph=True)
+    ref = psh.Context(psh.make_conf(**dict(kw, bsize=800.0)), nmock_max=len(cats))
+    want = ref.run_batch(dev, off, w, graph=False)
+    for m in range(len(cats)):
+        assert np.array_equal(got[m]["cnt"], want[m]["cnt"])
+        scale = np.abs(want[m]["pl"][0][0]) + 1e9 / sizes[m]
+        assert np.all(np.abs(got[m]["pl"][0] - want[m]["pl"][0]) <= 1e-5 * scale)
+
+
 def test_batch_streaming_matches_plain(psh):
     """A batch whose meshes outgrow the 256 MB Infinity Cache (16 interlaced
     mocks at N_g = 128: 335 MB of meshes) takes the non-temporal loads and

@@ -611,12 +611,15 @@ int grow_records(psh_ctx *ctx) {
 int ensure_partials(psh_ctx *ctx, int nmock, int grid, int nacc, int slots) {
   int rc;
   size_t need = (size_t)nmock * grid * nacc;
+  // a captured batch graph holds these buffers: drop it before reallocating
   if (need > ctx->partials_cap) {
+    drop_graph(ctx);
     if ((rc = dev_alloc(ctx, &ctx->partials, need))) return rc;
     ctx->partials_cap = need;
   }
   size_t needb = (size_t)nmock * nacc * slots;
   if (needb > ctx->bins_cap) {
+    drop_graph(ctx);
     if ((rc = dev_alloc(ctx, &ctx->d_bins, needb))) return rc;
     ctx->bins_cap = needb;
   }

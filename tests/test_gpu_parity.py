@@ -214,6 +214,32 @@ def test_batch_graph_after_box_change(psh):
         assert np.all(np.abs(got[m]["pl"][0] - want[m]["pl"][0]) <= 1e-5 * scale)
 
 
+def test_batch_graph_after_plain_run(psh):
+    """A plain run between two graph replays may grow the partial-sum and
+    bin buffers that the captured batch graph holds; the replay after it
+    still matches eager runs (the graph is dropped on reallocation)."""
+    Ng = 32
+    rng = np.random.default_rng(14)
+    sizes = rng.integers(2000, 5000, 3)
+    cats = [_uniform(int(v), [1000.0] * 3, 500 + i) for i, v in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    dev = psh.DeviceArray.from_numpy(np.concatenate(cats))
+    w = [float(v) for v in sizes]
+    kw = dict(gsize=Ng, assign=1, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw), nmock_max=len(cats))
+    first = ctx.run_batch(dev, off, w, graph=True)
+    one = ctx.run_device(psh.DeviceArray.from_numpy(cats[0]), len(cats[0]), w[0])
+    again = ctx.run_batch(dev, off, w, graph=True)
+    eager = ctx.run_batch(dev, off, w, graph=False)
+    for m in range(len(cats)):
+        for r in (first[m], again[m]):
+            assert np.array_equal(r["cnt"], eager[m]["cnt"])
+            scale = np.abs(eager[m]["pl"][0][0]) + 1e9 / sizes[m]
+            assert np.all(np.abs(r["pl"][0] - eager[m]["pl"][0]) <= 1e-5 * scale)
+    assert np.all(np.abs(one["pl"][0] - eager[0]["pl"][0])
+                  <= 1e-5 * (np.abs(eager[0]["pl"][0][0]) + 1e9 / sizes[0]))
+
+
 def test_batch_streaming_matches_plain(psh):
     """A batch whose meshes outgrow the 256 MB Infinity Cache (16 interlaced
     mocks at N_g = 128: 335 MB of meshes) takes the non-temporal loads and

@@ -268,7 +268,10 @@ int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n,
 
 /* Batched mocks: nmock catalogues concatenated in d_data (device) with
  * offsets[nmock+1] (host) and weight_sum[nmock] (host); out[nmock].
- * graph != 0 replays the launch sequence as a captured hipGraph. */
+ * graph != 0 replays the launch sequence as a captured hipGraph.  The
+ * graph is recaptured when the batch's shape changes, and dropped whenever
+ * the context reallocates a buffer it holds (psh_ctx_set_box,
+ * psh_ctx_set_tiles, larger scratch for a later run). */
 int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     const size_t *offsets, const double *weight_sum, int nmock, int graph,
     psh_pk *out);

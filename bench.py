@@ -37,7 +37,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def c2_conf(P):
+def box_conf(P):
     return dict(gsize=1024, assign=P.PCS, intlace=True, bsize=1000.0, kbin=0.005,
                 poles=[0, 2, 4])
 
@@ -70,23 +70,23 @@ def stage_bytes(Ng, nmesh, npart, x_live=1.0):
     }
 
 
-def run_c2(P, steps, warmup, fixed_tile=None):
+def run_box(P, steps, warmup, fixed_tile=None):
     t = time.time()
-    # C2 recipe on the GPU (same Philox streams as powspec_amd.synth.clustered_box)
+    # clustered N_g=1024 box catalogue generated on the GPU (same Philox streams as powspec_amd.synth.clustered_box)
     dev, n = P.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
     P.load().psh_device_synchronize()
     data = dev.to_numpy((n, 4))   # host copy for the CPU baseline and the mocks
-    log(f"C2 catalogue: {n} particles generated in {time.time() - t:.1f} s")
-    conf = c2_conf(P)
+    log(f"box catalogue: {n} particles generated in {time.time() - t:.1f} s")
+    conf = box_conf(P)
     # deposit tiles tuned on this catalogue before the timed region (a planning
     # step: candidates timed on the same data; results do not depend on it)
     t = time.time()
     if fixed_tile:
         tile = tuple(fixed_tile)
-        log(f"C2 tiles: {tile} (given)")
+        log(f"deposit tiles: {tile} (given)")
     else:
         tile, tms, ncand = P.autotune(P.make_conf(**conf), nmock=1, data=dev, offsets=[0, n])
-        log(f"C2 tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
+        log(f"deposit tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
     ctx = P.Context(P.make_conf(**conf))
     ctx.set_tiles(tile)
     L = P.load()
@@ -105,11 +105,11 @@ def run_c2(P, steps, warmup, fixed_tile=None):
     t1 = time.perf_counter()
     ctx.set_profiling(False)
     ms_step = (t1 - t0) / steps * 1e3
-    log(f"C2: {ms_step:.3f} ms/step over {steps} steps")
+    log(f"box N_g=1024: {ms_step:.3f} ms/step over {steps} steps")
     ms = np.mean(np.array(reps), axis=0)
     stages = {"assign_total": ms[0], "zy_fft": ms[1], "x_fft_bin": ms[2], "total": ms[3],
               "deposit": ms[4], "z_r2c": ms[5], "y_fft": ms[6], "bucket": ms[7]}
-    log("C2 stages (ms, mean over the timed steps): " +
+    log("box stages (ms, mean over the timed steps): " +
         ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
     return data, res, ms_step, stages, n, tile
 
@@ -122,7 +122,7 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
     None when no summary for it is present."""
-    path = os.path.join(ROOT, "profiles", "c2_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", "box1024_pcs_hbm_traffic.md")
     try:
         with open(path) as f:
             for ln in f:
@@ -245,7 +245,7 @@ def run_batched(P, data, nmock, seed=1000):
 
 
 def run_survey(P, steps=3):
-    """Survey workload (C4 recipe): sky cap RA in [0, 60) deg, Dec in [-30, 30)
+    """Survey workload: sky cap RA in [0, 60) deg, Dec in [-30, 30)
     deg (uniform sin Dec), redshift from a Gaussian n(z) (0.55, 0.08)
     truncated to [0.4, 0.7], 5e6 data + 1e8 randoms generated on the GPU,
     w_fkp = 1/(1 + n(z) 1e4), converted to comoving coordinates on the GPU
@@ -291,7 +291,7 @@ def run_survey(P, steps=3):
 
 
 def run_ingest(P, data, tile, reps=3):
-    """Catalogue ingest of the C2 catalogue (x, y, z float64 columns, unit
+    """Catalogue ingest of the clustered box catalogue (x, y, z float64 columns, unit
     weights): binary file -> HBM and host columns -> HBM through the pinned
     pipeline, and a stream of catalogues ingested while the previous one is
     processed, against ingest-then-process in sequence.  The file is written
@@ -303,7 +303,7 @@ def run_ingest(P, data, tile, reps=3):
     out = [P.DeviceArray.empty((n, 4)) for _ in range(2)]
     ing = P.Ingest()
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        path = os.path.join(d, "c2.pscat")
+        path = os.path.join(d, "box.pscat")
         P.cat_write(path, cols)
         ing.start_file(path, out[0], n)
         ing.wait()
@@ -319,7 +319,7 @@ def run_ingest(P, data, tile, reps=3):
             t_host.append(time.perf_counter() - t0)
         nbytes = r["bytes"]
         # streaming: catalogue k+1 ingested while catalogue k is processed
-        ctx = P.Context(P.make_conf(**c2_conf(P)))
+        ctx = P.Context(P.make_conf(**box_conf(P)))
         ctx.set_tiles(tile)
         ctx.run_device(out[0], n, float(n))
         ncat = 4
@@ -342,7 +342,7 @@ def run_ingest(P, data, tile, reps=3):
         ctx.close()
         # ASCII (the reference's input format): 2e6 lines of x y z, %.10f
         na = min(n, 2_000_000)
-        apath = os.path.join(d, "c2.txt")
+        apath = os.path.join(d, "box.txt")
         np.savetxt(apath, data[:na, :3], fmt="%.10f")
         abytes = os.path.getsize(apath)
         t_ascii = []
@@ -360,7 +360,7 @@ def run_ingest(P, data, tile, reps=3):
     log(f"ingest: file {nbytes / tf / 1e9:.1f} GB/s, host columns {nbytes / th / 1e9:.1f} GB/s; "
         f"{ncat} catalogues ingest+process: sequential {t_seq * 1e3:.1f} ms, "
         f"overlapped {t_ovl * 1e3:.1f} ms")
-    return {"workload": f"C2 catalogue, {n} objects, x/y/z float64 columns ({nbytes / 1e9:.3f} GB)",
+    return {"workload": f"clustered box catalogue, {n} objects, x/y/z float64 columns ({nbytes / 1e9:.3f} GB)",
             "file_GBps": round(nbytes / tf / 1e9, 2), "host_GBps": round(nbytes / th / 1e9, 2),
             "file_ms": round(tf * 1e3, 2), "stream_catalogues": ncat,
             "sequential_ms": round(t_seq * 1e3, 1), "overlapped_ms": round(t_ovl * 1e3, 1),
@@ -388,8 +388,8 @@ def parity_vs_oracle(gpu, ref, poles=(0, 2, 4)):
     return out
 
 
-def cpu_baseline(c2_data, gpu_res=None):
-    """FP64 oracle (OpenMP, own FFT) on the whole C2 workload: every C2
+def cpu_baseline(box_data, gpu_res=None):
+    """FP64 oracle (OpenMP, own FFT) on the whole clustered-box workload: every
     particle, N_g = 1024, PCS interlaced, l = 0,2,4, dk = 0.005 -- one mesh
     (about 10-20 s on the host cores, ~40 GB of host memory)."""
     from oracle import oracle as O
@@ -397,22 +397,22 @@ def cpu_baseline(c2_data, gpu_res=None):
     conf = O.make_conf(gsize=1024, assign=3, intlace=True, bsize=1000.0, kbin=0.005,
                        poles=[0, 2, 4])
     t = time.perf_counter()
-    r = O.run(conf, [c2_data])
+    r = O.run(conf, [box_data])
     wall = time.perf_counter() - t
     ta, tf, tb, tt = r["timings"]
     threads = O.lib().pso_num_threads()
-    log(f"cpu oracle C2: {wall:.1f} s wall (assign {ta:.1f}, fft {tf:.1f}, bin {tb:.1f})")
+    log(f"cpu oracle, box: {wall:.1f} s wall (assign {ta:.1f}, fft {tf:.1f}, bin {tb:.1f})")
     par = parity_vs_oracle(gpu_res, r) if gpu_res is not None else None
     if par:
-        log("C2 vs FP64 oracle: " + ", ".join(f"{k}={v}" for k, v in par.items()))
+        log("box vs FP64 oracle: " + ", ".join(f"{k}={v}" for k, v in par.items()))
     return {"value": round(1.0 / wall, 5), "unit": "meshes/s", "cores": threads,
             "parity_vs_oracle": par,
             "kind": "port",
             "sample": f"oracle/psoracle.c (FP64, OpenMP {threads} threads, own radix-2 FFT) "
-                      f"on the full C2 workload ({len(c2_data)} particles, N_g=1024, PCS "
+                      f"on the full clustered-box workload ({len(box_data)} particles, N_g=1024, PCS "
                       f"interlaced, l=0,2,4): one mesh in {wall:.2f} s (assign {ta:.2f}, "
                       f"FFT {tf:.2f}, binning {tb:.2f})",
-            "particles_per_s": round(len(c2_data) / wall, 1)}
+            "particles_per_s": round(len(box_data) / wall, 1)}
 
 
 def main():
@@ -426,7 +426,7 @@ def main():
     ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--tile", default=None,
-                    help="C2 deposit tile 'tx,ty,tz' instead of autotuning (profiling runs)")
+                    help="deposit tile of the N_g=1024 box 'tx,ty,tz' instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -444,7 +444,7 @@ def main():
         sys.exit(1)
     arch = P.device_arch()
     fixed = [int(v) for v in args.tile.split(",")] if args.tile else None
-    data, res, ms_step, stages, n, tile = run_c2(P, args.steps, args.warmup, fixed)
+    data, res, ms_step, stages, n, tile = run_box(P, args.steps, args.warmup, fixed)
     rl = roofline(stages, 1024, 2, n, x_live_fraction(1024, 1000.0, float(res["kedge"][-1])))
     extra = {}
     if not args.no_batched and args.nmock > 0:

@@ -175,7 +175,7 @@ def features(k, p_plain_raw, scheme, kf, kny, shot):
     return X
 
 
-# ---- training data: plain vs interlaced spectra of clustered boxes (C5) ---
+# ---- training data: plain vs interlaced spectra of clustered boxes, N_g = 512 ---
 
 def mock_spectra(P, seed, A, nidx, Ng=512, ng_gen=512, L=1000.0, nbar=0.02, scheme=1,
                  kbin=0.01):

@@ -49,7 +49,7 @@ def test_training_fits_smooth_target(psh):
     assert mlp.ref_loss(p, Xh, Th) < 0.1 * l0   # generalises
 
 
-C5 = dict(Ng=512, ng_gen=512, L=1000.0, nbar=0.02, kbin=0.01)
+BOX512 = dict(Ng=512, ng_gen=512, L=1000.0, nbar=0.02, kbin=0.01)
 KNY = np.pi * 512 / 1000.0
 
 
@@ -58,7 +58,7 @@ def trained(psh):
     """The ensemble of networks trained on 64 clustered boxes (CIC,
     N_g = 512) over an 8 x 8 design of power-law amplitude [5e3, 2e4] and
     index [-2, -1]."""
-    X, T, _ = mlp.dataset(psh, range(5000, 5064), design=mlp.grid_design(8), **C5)
+    X, T, _ = mlp.dataset(psh, range(5000, 5064), design=mlp.grid_design(8), **BOX512)
     p, loss = mlp.train_ensemble(X, T)
     return p, loss
 
@@ -78,7 +78,7 @@ def test_alias_correction_reference_spectrum(psh, trained):
     corrected plain monopole is within 0.5 % of the interlaced one for
     k <= 0.8 k_Ny, ten times closer than the Jing-corrected plain one."""
     p, loss = trained
-    _, _, held = mlp.dataset(psh, range(6100, 6104), design=[(1e4, -1.5)] * 4, **C5)
+    _, _, held = mlp.dataset(psh, range(6100, 6104), design=[(1e4, -1.5)] * 4, **BOX512)
     e_mlp, e_jing = mlp.errors(p, held, KNY)
     print(f"train loss {loss:.2e}; worst error per box: mlp {np.round(e_mlp, 4)}, "
           f"jing {np.round(e_jing, 4)}")
@@ -92,7 +92,7 @@ def test_alias_correction_family(psh, trained):
     factor of ten in the median (its worst case is set by the realisation
     scatter of the highest-variance lognormal fields)."""
     p, _ = trained
-    _, _, held = mlp.dataset(psh, range(6000, 6008), **C5)
+    _, _, held = mlp.dataset(psh, range(6000, 6008), **BOX512)
     e_mlp, e_jing = mlp.errors(p, held, KNY)
     print(f"worst error per box: mlp {np.round(e_mlp, 4)}, jing {np.round(e_jing, 4)}")
     assert np.all(e_mlp < e_jing)

@@ -544,8 +544,8 @@ def test_survey_ng512_parity(psh, oracle):
     _check(got, ref, [0, 2], sums["shot"], sums["norm"])
 
 
-def test_c2_full_size_parity(psh, oracle):
-    """The C2 workload at its full size (2.0e7 clustered particles, PCS,
+def test_box1024_full_size_parity(psh, oracle):
+    """The clustered-box workload at its full size (2.0e7 clustered particles, PCS,
     N_g = 1024, interlaced, l = 0,2,4, dk = 0.005) against the FP64 oracle:
     mode counts bit-exact, every P_l within the bar of the smaller tests."""
     dev, n = psh.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
@@ -555,4 +555,4 @@ def test_c2_full_size_parity(psh, oracle):
     dev.free()
     ref = oracle.run(oracle.make_conf(**kw), [data])
     worst = _check(got, ref, [0, 2, 4], ref["shot"][0])
-    print(f"C2 full size: {n} particles, {len(ref['cnt'])} bins, worst |dP_l|/S_b {worst:.2e}")
+    print(f"box N_g=1024 full size: {n} particles, {len(ref['cnt'])} bins, worst |dP_l|/S_b {worst:.2e}")

@@ -188,12 +188,12 @@ def test_alias_corr_negative_frequencies_use_series(oracle):
 # ---- k bins ---------------------------------------------------------------------
 
 def test_kbins_closed_form(oracle):
-    # C1 geometry: kNy = pi*256/1000 = 0.80425, dk=0.01 -> round(80.4)=80 bins
+    # N_g = 256, L = 1000 geometry: kNy = pi*256/1000 = 0.80425, dk=0.01 -> round(80.4)=80 bins
     nb, kedge, kc = oracle.kbins(256, [1000.0] * 3, False, 0.0, -1.0, 0.01)
     assert nb == 80
     assert np.allclose(kedge, 0.01 * np.arange(81))
     assert np.allclose(kc, 0.01 * np.arange(80) + 0.005)
-    # C2 geometry: kNy = 3.21699, dk = 0.005 -> round(643.4) = 643
+    # N_g = 1024, L = 1000 geometry: kNy = 3.21699, dk = 0.005 -> round(643.4) = 643
     nb, _, _ = oracle.kbins(1024, [1000.0] * 3, False, 0.0, -1.0, 0.005)
     assert nb == 643
     # trimming: kNy = pi*16/100 = 0.50265; dk 0.2 -> round(2.51)=3 but 0.6>kNy -> 2

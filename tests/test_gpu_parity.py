@@ -240,6 +240,26 @@ def test_batch_graph_after_plain_run(psh):
                   <= 1e-5 * (np.abs(eager[0]["pl"][0][0]) + 1e9 / sizes[0]))
 
 
+def test_batch_graph_new_offsets(psh):
+    """Replaying a captured batch graph with the same data, mock count and
+    total size but other per-mock offsets uses the new split."""
+    Ng = 32
+    allp = _uniform(12000, [1000.0] * 3, 600)
+    dev = psh.DeviceArray.from_numpy(allp)
+    kw = dict(gsize=Ng, assign=3, intlace=False, bsize=1000.0, kbin=0.01, poles=[0, 2])
+    ctx = psh.Context(psh.make_conf(**kw), nmock_max=3)
+    ref = psh.Context(psh.make_conf(**kw))   # eager runs elsewhere keep ctx's graph
+    for off in ([0, 3000, 7000, 12000], [0, 5000, 6000, 12000]):
+        w = [float(off[m + 1] - off[m]) for m in range(3)]
+        got = ctx.run_batch(dev, np.asarray(off), w, graph=True)
+        for m in range(3):
+            one = psh.DeviceArray.from_numpy(allp[off[m]:off[m + 1]])
+            want = ref.run_device(one, off[m + 1] - off[m], w[m])
+            assert np.array_equal(got[m]["cnt"], want["cnt"])
+            scale = np.abs(want["pl"][0][0]) + 1e9 / w[m]
+            assert np.all(np.abs(got[m]["pl"][0] - want["pl"][0]) <= 1e-5 * scale)
+
+
 def test_batch_streaming_matches_plain(psh):
     """A batch whose meshes outgrow the 256 MB Infinity Cache (16 interlaced
     mocks at N_g = 128: 335 MB of meshes) takes the non-temporal loads and

@@ -299,6 +299,18 @@ int psh_ctx_set_profiling(psh_ctx *ctx, int on);
  * ctx is NULL). */
 const char *psh_last_error(const psh_ctx *ctx);
 
+/* Fault injection for tests of the error paths (no counterpart in the
+ * reference, which exit()s on allocation failure inside parallel regions,
+ * src/multipole.c:121-126).  PSH_FAULT_ALLOC: value n > 0 makes the n-th
+ * device allocation of a context from now on fail (PSH_ERR_MEMORY), n < 0
+ * every one, 0 none.  PSH_FAULT_GRAPH: value != 0 makes the instantiation of
+ * a batch graph fail (PSH_ERR_DEVICE).  The environment variables
+ * PSH_FAULT_ALLOC and PSH_FAULT_GRAPH set the same at load.  Library calls
+ * return the error code and never exit. */
+#define PSH_FAULT_ALLOC 1
+#define PSH_FAULT_GRAPH 2
+int psh_fault_inject(int what, long value);
+
 /* ---- device helpers (for hosts without their own HIP runtime bindings) ---- */
 int psh_device_count(void);
 int psh_device_malloc(void **ptr, size_t bytes);

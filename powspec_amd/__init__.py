@@ -21,6 +21,7 @@ OK = 0
 ERR_MEMORY, ERR_ARG, ERR_FILE, ERR_CATA, ERR_MESH, ERR_PK = -1, -2, -3, -11, -13, -14
 ERR_CNVT = -12
 ERR_DEVICE, ERR_UNSUPPORTED = -20, -21
+FAULT_ALLOC, FAULT_GRAPH = 1, 2
 
 
 class Conf(C.Structure):
@@ -73,6 +74,7 @@ EXPORTS = [
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
+    "psh_fault_inject",
 ]
 
 
@@ -198,8 +200,18 @@ def load():
     L.psh_ingest_error.restype = C.c_char_p
     L.psh_ingest_ascii.argtypes = [vp, C.c_char_p, C.POINTER(AsciiFmt), i, vp, sz]
     L.psh_parse_double.argtypes = [C.c_char_p, sz, C.POINTER(d)]
+    L.psh_fault_inject.argtypes = [i, C.c_long]
     _lib = L
     return L
+
+
+def fault_inject(what: int, value: int) -> None:
+    """psh_fault_inject: FAULT_ALLOC n > 0 fails the n-th device allocation
+    of a context from now on, n < 0 every one, 0 none; FAULT_GRAPH != 0 fails
+    batch-graph instantiation."""
+    rc = load().psh_fault_inject(int(what), int(value))
+    if rc:
+        raise PshError(rc, "psh_fault_inject failed")
 
 
 def device_count() -> int:

@@ -874,15 +874,19 @@ struct XBin {
 
 }  // namespace
 
-hipError_t psh_fft_twiddles(int N, float2 **d_tw) {
+hipError_t psh_fft_twiddles_fill(int N, float2 *d_tw) {
   std::vector<float2> h(N);
   for (int m = 0; m < N; m++) {
     double ang = -2.0 * M_PI * (double)m / (double)N;
     h[m] = make_float2((float)cos(ang), (float)sin(ang));
   }
+  return hipMemcpy(d_tw, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
+}
+
+hipError_t psh_fft_twiddles(int N, float2 **d_tw) {
   hipError_t e = hipMalloc((void **)d_tw, sizeof(float2) * N);
   if (e != hipSuccess) return e;
-  return hipMemcpy(*d_tw, h.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
+  return psh_fft_twiddles_fill(N, *d_tw);
 }
 
 hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,

@@ -17,7 +17,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,38 @@
 namespace {
 
 std::string g_last_error;
+
+// Fault injection for tests of the error paths (no counterpart in the
+// reference, whose allocation failures exit(), src/multipole.c:121-126):
+// g_fault_alloc > 0 counts down the device allocations of contexts; the one
+// that brings it to 0 fails as if the device were out of memory.  A negative
+// value makes every allocation fail; 0 disables.  g_fault_graph != 0 makes
+// graph instantiation fail.  Set from PSH_FAULT_ALLOC / PSH_FAULT_GRAPH at
+// load or with psh_fault_inject.
+std::atomic<long> g_fault_alloc{0};
+std::atomic<int> g_fault_graph{0};
+std::once_flag g_fault_env_once;
+
+void fault_env_init() {
+  std::call_once(g_fault_env_once, [] {
+    if (const char *e = getenv("PSH_FAULT_ALLOC")) g_fault_alloc = atol(e);
+    if (const char *e = getenv("PSH_FAULT_GRAPH")) g_fault_graph = atoi(e) != 0;
+  });
+}
+
+bool fault_alloc_now() {
+  fault_env_init();
+  long v = g_fault_alloc.load();
+  while (v > 0) {
+    if (g_fault_alloc.compare_exchange_weak(v, v - 1)) return v == 1;
+  }
+  return v < 0;
+}
+
+bool fault_graph_now() {
+  fault_env_init();
+  return g_fault_graph.load() != 0;
+}
 
 // Stage events of one run (recorded only when profiling is on).
 enum {
@@ -49,6 +83,11 @@ struct Timer {
 struct psh_ctx {
   psh_conf conf{};
   int dev = 0;
+  // Device memory of an allocation-free run: every buffer of the context is
+  // carved from this region (a bump allocator, 256-byte aligned; freeing is
+  // a no-op) instead of hipMalloc'd.  nullptr: ordinary allocations.
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   int Ng = 0, M = 0, H = 0;
   size_t mstride = 0;            // floats per mesh
   int nmock_max = 1;
@@ -249,20 +288,46 @@ int validate_conf(const psh_conf *c, psh_ctx *ctx) {
   return PSH_OK;
 }
 
+// Every device buffer of a context comes from ctx_malloc and goes back with
+// ctx_free: hipMalloc/hipFree, or the context's arena.
+int ctx_malloc(psh_ctx *ctx, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 1;
+  if (fault_alloc_now())
+    return set_err(ctx, PSH_ERR_MEMORY,
+                   "device allocation of %zu bytes failed: injected fault", bytes);
+  if (ctx && ctx->arena) {
+    const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+    if (off > ctx->arena_cap || bytes > ctx->arena_cap - off)
+      return set_err(ctx, PSH_ERR_MEMORY,
+                     "the run needs more than the %zu bytes of the static device scratch "
+                     "(%zu in use, %zu more requested)",
+                     ctx->arena_cap, off, bytes);
+    *p = ctx->arena + off;
+    ctx->arena_used = off + bytes;
+    return PSH_OK;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(ctx, PSH_ERR_MEMORY, "device allocation of %zu bytes failed: %s", bytes,
+                   hipGetErrorString(e));
+  }
+  return PSH_OK;
+}
+
+void ctx_free(psh_ctx *ctx, void *p) {
+  if (!p || (ctx && ctx->arena)) return;
+  (void)hipFree(p);
+}
+
 template <class T>
 int dev_alloc(psh_ctx *ctx, T **p, size_t count) {
   if (*p) {
-    (void)hipFree(*p);
+    ctx_free(ctx, *p);
     *p = nullptr;
   }
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return set_err(ctx, PSH_ERR_MEMORY, "device allocation of %zu bytes failed: %s",
-                   sizeof(T) * count, hipGetErrorString(e));
-  }
-  return PSH_OK;
+  return ctx_malloc(ctx, (void **)p, sizeof(T) * count);
 }
 
 // k bins: src/multipole.c:334-394.
@@ -371,7 +436,7 @@ int make_tables(psh_ctx *ctx) {
   // keep every tile.
   ctx->kzlim = 1 << 30;
   if (ctx->conf.logscale) {
-    if (ctx->d_ylive) (void)hipFree(ctx->d_ylive);
+    ctx_free(ctx, ctx->d_ylive);
     ctx->d_ylive = nullptr;
   } else {
     const int CX = psh_xbin_tile_cols(N);
@@ -486,8 +551,11 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
     if (ctx->L[a] != L[a] || ctx->bmin[a] != bmin[a]) same = false;
   if (same) return PSH_OK;
   // a captured batch graph holds the tables and geometry of the old box,
-  // which are about to be reallocated
+  // which are about to be reallocated; until the new box is complete the
+  // context has none (a failed set-up is redone in full by the next call)
   drop_graph(ctx);
+  ctx->box_set = false;
+  ctx->geom_valid = false;
   for (int a = 0; a < 3; a++) {
     ctx->L[a] = L[a];
     ctx->bmin[a] = bmin[a];
@@ -497,7 +565,6 @@ int set_box(psh_ctx *ctx, const double L[3], const double bmin[3]) {
   if ((rc = make_tables(ctx))) return rc;
   set_assign_geom(ctx);
   ctx->box_set = true;
-  ctx->geom_valid = false;
   return PSH_OK;
 }
 
@@ -552,8 +619,11 @@ void drop_graph(psh_ctx *ctx) {
 int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   int rc;
   const size_t ntiles = ctx->ntiles_mock * nmock;
+  // each capacity is cleared before its buffers are replaced and set only
+  // once they all exist, so a failed allocation leaves no stale capacity
   if (ntiles + 1 > ctx->tiles_cap) {
     drop_graph(ctx);
+    ctx->tiles_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->counts, ntiles + 1))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->starts, ntiles + 1))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->cursors, ntiles + 1))) return rc;
@@ -563,6 +633,7 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   }
   if (nparts > ctx->parts_cap) {
     drop_graph(ctx);
+    ctx->parts_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->d_parts, (size_t)nparts))) return rc;
     ctx->parts_cap = nparts;
   }
@@ -572,13 +643,17 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   ctx->whist_words = psh_assign_whist_words(ctx->ag, ntotal);
   if (ctx->whist_words > ctx->whist_cap) {
     drop_graph(ctx);
-    if (ctx->whist) (void)hipFree(ctx->whist);
+    ctx_free(ctx, ctx->whist);
     ctx->whist = nullptr;
     ctx->whist_cap = 0;
-    if (hipMalloc((void **)&ctx->whist, sizeof(uint32_t) * ctx->whist_words) == hipSuccess)
+    // optional: without the histograms the fill pass recounts
+    const std::string keep = ctx->err;
+    if (ctx_malloc(ctx, (void **)&ctx->whist, sizeof(uint32_t) * ctx->whist_words) == PSH_OK) {
       ctx->whist_cap = ctx->whist_words;
-    else
-      ctx->whist_words = 0;   // fill recounts instead
+    } else {
+      ctx->whist_words = 0;
+      ctx->err = keep;
+    }
   }
   // expected (particle, tile) pairs: prod_a (1 + (W' - 1) / T_a), W' the
   // union window; 1.5x margin, grown on overflow
@@ -588,11 +663,10 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
   size_t need = (size_t)(ntotal * dup * 1.5) + 4096;
   if (need > ctx->rec_cap) {
     drop_graph(ctx);
-    if (ctx->records) {
-      (void)hipFree(ctx->records);
-      ctx->records = nullptr;
-    }
-    HIPCHK(ctx, hipMalloc(&ctx->records, need * psh_assign_record_bytes()));
+    ctx_free(ctx, ctx->records);
+    ctx->records = nullptr;
+    ctx->rec_cap = 0;
+    if ((rc = ctx_malloc(ctx, &ctx->records, need * psh_assign_record_bytes()))) return rc;
     ctx->rec_cap = need;
   }
   return PSH_OK;
@@ -600,10 +674,12 @@ int ensure_assign_scratch(psh_ctx *ctx, int nmock, int nparts, size_t ntotal) {
 
 int grow_records(psh_ctx *ctx) {
   drop_graph(ctx);
-  size_t need = ctx->rec_cap * 2;
-  if (ctx->records) (void)hipFree(ctx->records);
+  const size_t need = ctx->rec_cap * 2;
+  ctx_free(ctx, ctx->records);
   ctx->records = nullptr;
-  HIPCHK(ctx, hipMalloc(&ctx->records, need * psh_assign_record_bytes()));
+  ctx->rec_cap = 0;
+  int rc;
+  if ((rc = ctx_malloc(ctx, &ctx->records, need * psh_assign_record_bytes()))) return rc;
   ctx->rec_cap = need;
   return PSH_OK;
 }
@@ -614,12 +690,14 @@ int ensure_partials(psh_ctx *ctx, int nmock, int grid, int nacc, int slots) {
   // a captured batch graph holds these buffers: drop it before reallocating
   if (need > ctx->partials_cap) {
     drop_graph(ctx);
+    ctx->partials_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->partials, need))) return rc;
     ctx->partials_cap = need;
   }
   size_t needb = (size_t)nmock * nacc * slots;
   if (needb > ctx->bins_cap) {
     drop_graph(ctx);
+    ctx->bins_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->d_bins, needb))) return rc;
     ctx->bins_cap = needb;
   }
@@ -953,16 +1031,15 @@ int stage_host(psh_ctx *ctx, const psh_data *h, size_t n, size_t offset_records)
 // context layer
 // ===========================================================================
 
-extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out) {
-  if (!out) return set_err(nullptr, PSH_ERR_ARG, "null output pointer");
-  *out = nullptr;
-  psh_ctx *ctx = new (std::nothrow) psh_ctx();
-  if (!ctx) return set_err(nullptr, PSH_ERR_MEMORY, "out of host memory");
+namespace {
+
+// Set up a context for conf on the current device.  With ctx->arena set the
+// buffers come from the arena and the work runs on the null stream (no
+// stream or event is created); otherwise on a stream of its own.  On error
+// ctx->err holds the message and the caller destroys the context.
+int ctx_init(psh_ctx *ctx, const psh_conf *conf, int nmock_max) {
   int rc = validate_conf(conf, ctx);
-  if (rc) {
-    delete ctx;
-    return rc;
-  }
+  if (rc) return rc;
   ctx->conf = *conf;
   ctx->nmock_max = nmock_max < 1 ? 1 : nmock_max;
   ctx->Ng = conf->gsize;
@@ -971,30 +1048,22 @@ extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out
   ctx->mstride = (size_t)ctx->Ng * ctx->Ng * 2 * ctx->H;
   ctx->survey = !conf->issim;
   ctx->ellmax = conf->poles[conf->npole - 1];
-  if (ctx->survey && ctx->nmock_max > 1) {
-    set_err(ctx, PSH_ERR_UNSUPPORTED, "batched mocks are supported for simulation boxes");
-    delete ctx;
-    return PSH_ERR_UNSUPPORTED;
-  }
+  if (ctx->survey && ctx->nmock_max > 1)
+    return set_err(ctx, PSH_ERR_UNSUPPORTED, "batched mocks are supported for simulation boxes");
   if (hipGetDevice(&ctx->dev) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-    set_err(ctx, PSH_ERR_DEVICE, "no usable HIP device");
-    delete ctx;
-    return PSH_ERR_DEVICE;
+      (!ctx->arena &&
+       hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)) {
+    ctx->stream = nullptr;
+    return set_err(ctx, PSH_ERR_DEVICE, "no usable HIP device");
   }
   const size_t nm = (size_t)ctx->nmock_max;
   if ((rc = dev_alloc(ctx, &ctx->meshA, ctx->mstride * nm)) ||
       ((conf->intlace || ctx->survey) && (rc = dev_alloc(ctx, &ctx->meshB, ctx->mstride * nm))) ||
-      (ctx->survey && ctx->ellmax > 0 && (rc = dev_alloc(ctx, &ctx->meshS, ctx->mstride)))) {
-    std::string e = ctx->err;
-    psh_ctx_destroy(ctx);
-    set_err(nullptr, rc, "%s", e.c_str());
+      (ctx->survey && ctx->ellmax > 0 && (rc = dev_alloc(ctx, &ctx->meshS, ctx->mstride))))
     return rc;
-  }
-  if (psh_fft_twiddles(ctx->Ng, &ctx->tw) != hipSuccess) {
-    psh_ctx_destroy(ctx);
-    return set_err(nullptr, PSH_ERR_DEVICE, "twiddle table upload failed");
-  }
+  if ((rc = dev_alloc(ctx, &ctx->tw, (size_t)ctx->Ng))) return rc;
+  if (psh_fft_twiddles_fill(ctx->Ng, ctx->tw) != hipSuccess)
+    return set_err(ctx, PSH_ERR_DEVICE, "twiddle table upload failed");
   {
     std::vector<float2> ph(ctx->Ng);
     for (int i = 0; i < ctx->Ng; i++) {
@@ -1002,22 +1071,32 @@ extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out
       double a = M_PI * s / ctx->Ng;
       ph[i] = make_float2((float)cos(a), (float)sin(a));
     }
-    if (hipMalloc((void **)&ctx->phase, sizeof(float2) * ctx->Ng) != hipSuccess ||
-        hipMemcpy(ctx->phase, ph.data(), sizeof(float2) * ctx->Ng, hipMemcpyHostToDevice) !=
-            hipSuccess) {
-      psh_ctx_destroy(ctx);
-      return set_err(nullptr, PSH_ERR_DEVICE, "phase table upload failed");
-    }
+    if ((rc = dev_alloc(ctx, &ctx->phase, (size_t)ctx->Ng))) return rc;
+    if (hipMemcpy(ctx->phase, ph.data(), sizeof(float2) * ctx->Ng, hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return set_err(ctx, PSH_ERR_DEVICE, "phase table upload failed");
   }
-  for (int i = 0; i < E_COUNT; i++) (void)hipEventCreate(&ctx->tm.ev[i]);
+  if (!ctx->arena)
+    for (int i = 0; i < E_COUNT; i++) (void)hipEventCreate(&ctx->tm.ev[i]);
   if (conf->issim) {
     double bmin[3] = {0, 0, 0};
-    if ((rc = set_box(ctx, conf->bsize, bmin))) {
-      std::string e = ctx->err;
-      psh_ctx_destroy(ctx);
-      set_err(nullptr, rc, "%s", e.c_str());
-      return rc;
-    }
+    if ((rc = set_box(ctx, conf->bsize, bmin))) return rc;
+  }
+  return PSH_OK;
+}
+
+}  // namespace
+
+extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out) {
+  if (!out) return set_err(nullptr, PSH_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  psh_ctx *ctx = new (std::nothrow) psh_ctx();
+  if (!ctx) return set_err(nullptr, PSH_ERR_MEMORY, "out of host memory");
+  int rc = ctx_init(ctx, conf, nmock_max);
+  if (rc) {
+    const std::string e = ctx->err;
+    psh_ctx_destroy(ctx);
+    return set_err(nullptr, rc, "%s", e.c_str());
   }
   *out = ctx;
   return PSH_OK;
@@ -1025,7 +1104,7 @@ extern "C" int psh_ctx_create(const psh_conf *conf, int nmock_max, psh_ctx **out
 
 extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
   if (!ctx) return PSH_OK;
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->stream || ctx->arena) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   void *bufs[] = {ctx->meshA, ctx->meshB, ctx->meshS, ctx->counts, ctx->starts, ctx->cursors,
@@ -1033,8 +1112,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
                   ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
                   ctx->d_glcnt, ctx->d_gpart_cnt, ctx->d_gpart_km, ctx->d_gpart_lcnt,
                   ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat, ctx->whist};
-  for (void *p : bufs)
-    if (p) (void)hipFree(p);
+  for (void *p : bufs) ctx_free(ctx, p);
   for (int i = 0; i < E_COUNT; i++)
     if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1065,6 +1143,19 @@ extern "C" int psh_ctx_timings(const psh_ctx *ctx, double ms[8]) {
 extern "C" const char *psh_last_error(const psh_ctx *ctx) {
   if (ctx) return ctx->err.c_str();
   return g_last_error.c_str();
+}
+
+extern "C" int psh_fault_inject(int what, long value) {
+  fault_env_init();
+  if (what == PSH_FAULT_ALLOC) {
+    g_fault_alloc = value;
+    return PSH_OK;
+  }
+  if (what == PSH_FAULT_GRAPH) {
+    g_fault_graph = value != 0;
+    return PSH_OK;
+  }
+  return set_err(nullptr, PSH_ERR_ARG, "unknown fault kind %d", what);
 }
 
 // Raw box sums of the field of `hp` (several parts: the field of their sum):
@@ -1209,6 +1300,11 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   if (ctx->survey) return set_err(ctx, PSH_ERR_ARG, "batched runs are for simulation boxes");
   if (nmock < 1 || nmock > ctx->nmock_max)
     return set_err(ctx, PSH_ERR_ARG, "nmock %d outside [1, %d]", nmock, ctx->nmock_max);
+  for (int m = 0; m < nmock; m++)
+    if (offsets[m + 1] < offsets[m])
+      return set_err(ctx, PSH_ERR_ARG, "offsets must be non-decreasing (mock %d)", m);
+  if (graph && ctx->arena)
+    return set_err(ctx, PSH_ERR_ARG, "no graph capture on the null stream");
   int rc;
   if ((rc = ensure_geometry(ctx))) return rc;
   std::vector<PshDevPart> hp(nmock);
@@ -1222,53 +1318,68 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, nmock, ntotal))) return rc;
   float *mB = ctx->conf.intlace ? ctx->meshB : nullptr;
-  ev_record(ctx, E_START);
-  if (!graph) {
-    if ((rc = run_assign(ctx, hp, nmock, ctx->meshA, mB))) return rc;
-
-    if ((rc = run_sim_fft_bin(ctx, nmock, ctx->d_bins))) return rc;
-  } else {
-    // capture the launch sequence once per (data pointer, nmock, size);
-    // the part table is uploaded before capture
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_parts, hp.data(), sizeof(PshDevPart) * nmock,
-                               hipMemcpyHostToDevice, ctx->stream));
-    if (!ctx->gexec || ctx->graph_nmock != nmock || ctx->graph_ntotal != ntotal ||
-        ctx->graph_data != d_data) {
-      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-      ctx->gexec = nullptr;
-      ctx->graph = nullptr;
-      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-      HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      bool prof = ctx->profiling;
-      ctx->profiling = false;
-      hipError_t e1 = psh_fill_zero_u32((uint32_t *)ctx->d_flags, 4, ctx->stream);
-      hipError_t e2 = psh_assign_run(ctx->ag, ctx->d_parts, nmock, ntotal, nmock, ctx->counts,
-                                     ctx->starts, ctx->cursors, ctx->records, ctx->rec_cap,
-                                     ctx->meshA, mB, ctx->mstride, ctx->scan_tmp,
-                                     ctx->d_flags, ctx->stream, nullptr,
-                                     ctx->whist_words ? ctx->whist : nullptr);
-      int rc2 = (e1 == hipSuccess && e2 == hipSuccess) ? run_sim_fft_bin(ctx, nmock, ctx->d_bins)
-                                                     : PSH_ERR_DEVICE;
-      ctx->profiling = prof;
-      hipError_t e3 = hipStreamEndCapture(ctx->stream, &ctx->graph);
-      if (rc2 || e3 != hipSuccess)
-        return set_err(ctx, PSH_ERR_DEVICE, "graph capture failed");
-      HIPCHK(ctx, hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
-      ctx->graph_nmock = nmock;
-      ctx->graph_ntotal = ntotal;
-      ctx->graph_data = d_data;
-    }
-    HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
-  }
-  ev_record(ctx, E_END);
   std::vector<double> raw((size_t)nmock * nacc);
   int flags = 0;
-  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
-                             hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  // a replay whose records overflow the buffer is rerun after the buffer
+  // grows (the graph, which holds the old buffer, is captured again), as the
+  // eager path's run_assign does
+  for (int attempt = 0;; attempt++) {
+    ev_record(ctx, E_START);
+    if (!graph) {
+      if ((rc = run_assign(ctx, hp, nmock, ctx->meshA, mB))) return rc;
+      if ((rc = run_sim_fft_bin(ctx, nmock, ctx->d_bins))) return rc;
+    } else {
+      // capture the launch sequence once per (data pointer, nmock, size);
+      // the part table is uploaded before capture
+      HIPCHK(ctx, hipMemcpyAsync(ctx->d_parts, hp.data(), sizeof(PshDevPart) * nmock,
+                                 hipMemcpyHostToDevice, ctx->stream));
+      if (!ctx->gexec || ctx->graph_nmock != nmock || ctx->graph_ntotal != ntotal ||
+          ctx->graph_data != d_data) {
+        drop_graph(ctx);
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        bool prof = ctx->profiling;
+        ctx->profiling = false;
+        hipError_t e1 = psh_fill_zero_u32((uint32_t *)ctx->d_flags, 4, ctx->stream);
+        hipError_t e2 = psh_assign_run(ctx->ag, ctx->d_parts, nmock, ntotal, nmock, ctx->counts,
+                                       ctx->starts, ctx->cursors, ctx->records, ctx->rec_cap,
+                                       ctx->meshA, mB, ctx->mstride, ctx->scan_tmp,
+                                       ctx->d_flags, ctx->stream, nullptr,
+                                       ctx->whist_words ? ctx->whist : nullptr);
+        int rc2 = (e1 == hipSuccess && e2 == hipSuccess)
+                      ? run_sim_fft_bin(ctx, nmock, ctx->d_bins)
+                      : PSH_ERR_DEVICE;
+        ctx->profiling = prof;
+        hipError_t e3 = hipStreamEndCapture(ctx->stream, &ctx->graph);
+        if (rc2 || e3 != hipSuccess) {
+          drop_graph(ctx);
+          return set_err(ctx, PSH_ERR_DEVICE, "graph capture failed");
+        }
+        hipError_t e4 = fault_graph_now()
+                            ? hipErrorOutOfMemory
+                            : hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0);
+        if (e4 != hipSuccess) {
+          ctx->gexec = nullptr;
+          drop_graph(ctx);
+          return set_err(ctx, PSH_ERR_DEVICE, "graph instantiation failed: %s",
+                         hipGetErrorString(e4));
+        }
+        ctx->graph_nmock = nmock;
+        ctx->graph_ntotal = ntotal;
+        ctx->graph_data = d_data;
+      }
+      HIPCHK(ctx, hipGraphLaunch(ctx->gexec, ctx->stream));
+    }
+    ev_record(ctx, E_END);
+    HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * raw.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&flags, ctx->d_flags, sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!(flags & 1)) break;
+    if (attempt == 3) return set_err(ctx, PSH_ERR_MESH, "assignment record buffer overflow");
+    if ((rc = grow_records(ctx))) return rc;
+  }
   if (graph) {
     if (ctx->profiling) {
       float t;
@@ -1277,13 +1388,6 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     }
   } else {
     ev_collect(ctx);
-  }
-  if (flags & 1) {
-    // record overflow inside the graph: grow and recapture next call
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-    ctx->gexec = nullptr;
-    if ((rc = grow_records(ctx))) return rc;
-    return set_err(ctx, PSH_ERR_MESH, "assignment record buffer overflow (grown; rerun)");
   }
   if (flags & 4)
     return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");

@@ -119,8 +119,10 @@ size_t psh_assign_whist_words(const PshAssignGeom &g, size_t ntotal);
 // FFT (k_fft.hip)
 // ---------------------------------------------------------------------------
 
-// Twiddle table exp(-2 pi i m / N), m = 0..N-1, float2, from FP64.
+// Twiddle table exp(-2 pi i m / N), m = 0..N-1, float2, from FP64:
+// allocated (psh_fft_twiddles) or written into d_tw's N entries.
 hipError_t psh_fft_twiddles(int N, float2 **d_tw);
+hipError_t psh_fft_twiddles_fill(int N, float2 *d_tw);
 
 // Forward r2c along z, in place or out of place, for nmesh meshes laid out
 // contiguously (mesh stride mstride floats).  ylm: optional Y_lm(r) weight

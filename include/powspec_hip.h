@@ -311,6 +311,29 @@ const char *psh_last_error(const psh_ctx *ctx);
 #define PSH_FAULT_GRAPH 2
 int psh_fault_inject(int what, long value);
 
+/* ---- allocation-free run ---------------------------------------------------- */
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc -- the simulation-box run of psh_ctx_run_device (one
+ * catalogue of n device records in [0, BOX_SIZE), auto power spectrum,
+ * weight_sum = sum of w) without a context: every intermediate buffer
+ * (meshes, tables, tile records, partial sums) is carved from one file-scope
+ * device array of PSH_STATIC_SCRATCH_BYTES that each GPU the library runs on
+ * holds until the process ends.  The call makes no device allocation (no
+ * hipMalloc, hipFree or allocating library routine) and creates no stream:
+ * it runs on the null stream of the current device and returns when the
+ * results are on the host.  On entry out->nbin is the capacity of out's
+ * arrays in bins (PSH_ERR_ARG when the run has more; psh_ctx_nbin of a
+ * context with the same conf gives the count).  A workload whose buffers
+ * exceed the array returns PSH_ERR_MEMORY with the size in the message (at
+ * GRID_SIZE 128 interlaced, about 1e6 particles fit).  Calls on one GPU
+ * share the array and the last call's contents stay in it: calls that
+ * overlap on one GPU (other host threads) race. */
+#define PSH_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int psh_run_device_static(const psh_conf *conf, const psh_data *d_data, size_t n,
+    double weight_sum, psh_pk *out);
+
 /* ---- device helpers (for hosts without their own HIP runtime bindings) ---- */
 int psh_device_count(void);
 int psh_device_malloc(void **ptr, size_t bytes);

@@ -74,7 +74,7 @@ EXPORTS = [
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
-    "psh_fault_inject",
+    "psh_fault_inject", "psh_run_device_static",
 ]
 
 
@@ -201,6 +201,7 @@ def load():
     L.psh_ingest_ascii.argtypes = [vp, C.c_char_p, C.POINTER(AsciiFmt), i, vp, sz]
     L.psh_parse_double.argtypes = [C.c_char_p, sz, C.POINTER(d)]
     L.psh_fault_inject.argtypes = [i, C.c_long]
+    L.psh_run_device_static.argtypes = [C.POINTER(Conf), vp, sz, d, C.POINTER(Pk)]
     _lib = L
     return L
 
@@ -469,6 +470,22 @@ class Context:
         res["shot"] = [vals["shot"][0], vals["shot"][1]]
         res["norm"] = [vals["norm"][0], vals["norm"][1]]
         return res
+
+
+def run_device_static(conf: Conf, d_data: DeviceArray, n: int, weight_sum: float, nbin: int):
+    """psh_run_device_static: a box run with no device allocation, every
+    intermediate buffer in the library's static device scratch; `nbin` is
+    the capacity of the result arrays."""
+    arr = _PkArrays(nbin, conf.npole)
+    arr.pk.nbin = nbin
+    L = load()
+    rc = L.psh_run_device_static(C.byref(conf), d_data.ptr, C.c_size_t(n),
+                                 C.c_double(weight_sum), C.byref(arr.pk))
+    if rc:
+        raise PshError(rc, L.psh_last_error(None).decode())
+    res = arr.result()
+    nb = arr.pk.nbin
+    return {k: (v[:nb] if k in ("k", "km", "cnt") else v) for k, v in res.items()}
 
 
 # ---- synthetic catalogues on the GPU (psh_gen_*) ---------------------------

@@ -293,9 +293,6 @@ int validate_conf(const psh_conf *c, psh_ctx *ctx) {
 int ctx_malloc(psh_ctx *ctx, void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 1;
-  if (fault_alloc_now())
-    return set_err(ctx, PSH_ERR_MEMORY,
-                   "device allocation of %zu bytes failed: injected fault", bytes);
   if (ctx && ctx->arena) {
     const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
     if (off > ctx->arena_cap || bytes > ctx->arena_cap - off)
@@ -307,6 +304,9 @@ int ctx_malloc(psh_ctx *ctx, void **p, size_t bytes) {
     ctx->arena_used = off + bytes;
     return PSH_OK;
   }
+  if (fault_alloc_now())
+    return set_err(ctx, PSH_ERR_MEMORY,
+                   "device allocation of %zu bytes failed: injected fault", bytes);
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
@@ -1225,6 +1225,70 @@ extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n
   const double norm = weight_sum * weight_sum / vol;
   fill_pk_common(ctx, out);
   if (out->pl[0]) normalise_sim(ctx, raw.data(), shot, norm, out->pl[0]);
+  return PSH_OK;
+}
+
+// ===========================================================================
+// allocation-free entry point
+// ===========================================================================
+
+// Device scratch of psh_run_device_static: one array per GPU, held by the
+// process from the library's load to its end.  Host code never passes its
+// name to a kernel or a copy (on the host it only names the symbol): the
+// device address comes from hipGetSymbolAddress, once per device, below.
+static __device__ __align__(256) char scratch[PSH_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (dev).
+int scratch_address(int dev, char **out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(nullptr, PSH_ERR_DEVICE, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(nullptr, PSH_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                     hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PSH_OK;
+}
+
+}  // namespace
+
+extern "C" int psh_run_device_static(const psh_conf *conf, const psh_data *d_data, size_t n,
+                                     double weight_sum, psh_pk *out) {
+  if (!conf || !out || (n && !d_data)) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  if (!conf->issim)
+    return set_err(nullptr, PSH_ERR_ARG, "psh_run_device_static is for simulation boxes");
+  if (conf->ndata != 1 || conf->iscross)
+    return set_err(nullptr, PSH_ERR_ARG, "psh_run_device_static takes one catalogue");
+  int dev = 0, rc;
+  if (hipGetDevice(&dev) != hipSuccess) return set_err(nullptr, PSH_ERR_DEVICE, "no HIP device");
+  char *base = nullptr;
+  if ((rc = scratch_address(dev, &base))) return rc;
+  psh_ctx *ctx = new (std::nothrow) psh_ctx();
+  if (!ctx) return set_err(nullptr, PSH_ERR_MEMORY, "out of host memory");
+  ctx->arena = base;
+  ctx->arena_cap = PSH_STATIC_SCRATCH_BYTES;
+  const int cap = out->nbin;
+  rc = ctx_init(ctx, conf, 1);
+  if (!rc && ctx->nbin > cap)
+    rc = set_err(ctx, PSH_ERR_ARG, "the output holds %d bins, the run has %d", cap, ctx->nbin);
+  if (!rc) rc = psh_ctx_run_device(ctx, d_data, n, weight_sum, out);
+  if (rc) {
+    const std::string e = ctx->err;
+    psh_ctx_destroy(ctx);
+    return set_err(nullptr, rc, "%s", e.c_str());
+  }
+  psh_ctx_destroy(ctx);
   return PSH_OK;
 }
 

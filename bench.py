@@ -70,7 +70,7 @@ def stage_bytes(Ng, nmesh, npart, x_live=1.0):
     }
 
 
-def run_box(P, steps, warmup, fixed_tile=None):
+def run_box(P, steps, warmup, fixed_tile=None, fixed_ysched=None):
     t = time.time()
     # clustered N_g=1024 box catalogue generated on the GPU (same Philox streams as powspec_amd.synth.clustered_box)
     dev, n = P.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=2)
@@ -87,8 +87,16 @@ def run_box(P, steps, warmup, fixed_tile=None):
     else:
         tile, tms, ncand = P.autotune(P.make_conf(**conf), nmock=1, data=dev, offsets=[0, n])
         log(f"deposit tiles: {tile} ({tms:.2f} ms, {ncand} candidates, {time.time() - t:.1f} s)")
+    if fixed_ysched is not None:
+        ysched = int(fixed_ysched)
+        log(f"y-pass schedule: {ysched} (given)")
+    else:
+        ysched, yms, yn, y0 = P.autotune_fft(P.make_conf(**conf), 1)
+        log(f"y-pass schedule: {ysched} ({yms:.3f} ms per launch vs {y0:.3f} ms for "
+            f"schedule 0, {yn} candidates)")
     ctx = P.Context(P.make_conf(**conf))
     ctx.set_tiles(tile)
+    ctx.set_fft_schedule(ysched)
     L = P.load()
     for _ in range(warmup):
         res = ctx.run_device(dev, n, float(n))
@@ -111,7 +119,7 @@ def run_box(P, steps, warmup, fixed_tile=None):
               "deposit": ms[4], "z_r2c": ms[5], "y_fft": ms[6], "bucket": ms[7]}
     log("box stages (ms, mean over the timed steps): " +
         ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
-    return data, res, ms_step, stages, n, tile
+    return data, res, ms_step, stages, n, tile, ysched
 
 
 KERNELS = {"deposit": "assign_deposit", "z_r2c": "z_r2c_kernel", "y_fft": "col_kernel",
@@ -170,31 +178,55 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "launches": reps}), flush=True)
 
 
-def run_batched(P, data, nmock, seed=1000):
+def gen_mocks(P, nmock, seed=1000):
+    """The batched-mock workload: nmock independent clustered boxes (the
+    clustered recipe on a 256^3 generator grid, L = 1000 Mpc/h, f = 0.8),
+    mock m with its own seed (seed + m) and an expected particle count drawn
+    from U[2e5, 3e6], generated on the GPU and concatenated in one device
+    buffer.  Returns (DeviceArray, offsets, counts)."""
+    sizes = np.random.default_rng(seed).integers(200_000, 3_000_001, nmock)
+    mocks, counts = [], []
+    for m in range(nmock):
+        dev, n = P.gen_clustered_box(ng=256, L=1000.0, f=0.8, seed=seed + m,
+                                     npart=float(sizes[m]))
+        mocks.append(dev)
+        counts.append(n)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    allp = P.DeviceArray(32 * int(off[-1]))
+    for m, dev in enumerate(mocks):
+        P.memcpy_d2d(allp, 32 * int(off[m]), dev, 32 * counts[m])
+        dev.free()
+    P.load().psh_device_synchronize()
+    return allp, off, np.array(counts)
+
+
+def batched_ceiling(counts, Ng=256, nmesh=2):
+    """HBM ceiling of the batched workload in meshes/s: compulsory bytes per
+    mock (particles read once at 16 B, each mesh written once, one r2c
+    pass reading real and writing half-complex, binning reading it) at the
+    8 TB/s peak."""
+    ntot = float(Ng) ** 3
+    nc = float(Ng) * Ng * (Ng // 2 + 1)
+    per = np.asarray(counts, float) * 16 + nmesh * (4 * ntot + (4 * ntot + 8 * nc) + 8 * nc)
+    return len(counts) / (per.sum() / (HBM_PEAK_GBS * 1e9))
+
+
+def run_batched(P, nmock, seed=1000):
     """Many small clustered mocks (TSC, N_g = 256, interlaced): one batched
     launch sequence replayed as a hipGraph vs one plain run per mock."""
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(200_000, 3_000_001, nmock)
-    parts = []
-    for m in range(nmock):
-        r = np.random.default_rng(seed + m)
-        idx = r.integers(0, len(data), sizes[m])
-        shift = r.uniform(0, 1000.0, 3)
-        d = data[idx].copy()
-        d[:, :3] = np.minimum(np.mod(d[:, :3] + shift, 1000.0), 1000.0 * (1 - 1e-15))
-        parts.append(d)
-    allp = np.concatenate(parts)
-    del parts
-    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    log(f"batched mocks: {nmock} mocks, {len(allp)} particles")
-    dev = P.DeviceArray.from_numpy(allp)
+    t = time.time()
+    dev, off, sizes = gen_mocks(P, nmock, seed)
+    log(f"batched mocks: {nmock} mocks, {int(off[-1])} particles generated in "
+        f"{time.time() - t:.1f} s")
     kw = dict(gsize=256, assign=P.TSC, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
     t = time.time()
     tile, tms, ncand = P.autotune(P.make_conf(**kw), nmock=nmock, data=dev, offsets=off)
     log(f"batched tiles: {tile} ({tms:.1f} ms per batch, {ncand} candidates, "
         f"{time.time() - t:.1f} s)")
+    ysched = P.autotune_fft(P.make_conf(**kw), nmock)[0]
     ctx = P.Context(P.make_conf(**kw), nmock_max=nmock)
     ctx.set_tiles(tile)
+    ctx.set_fft_schedule(ysched)
     L = P.load()
     ws = [float(s) for s in sizes]
     bres = ctx.run_batch(dev, off, ws, graph=True)    # capture + warm-up
@@ -208,6 +240,7 @@ def run_batched(P, data, nmock, seed=1000):
     # plain: one mock per launch sequence
     single = P.Context(P.make_conf(**kw))
     single.set_tiles(tile)
+    single.set_fft_schedule(P.autotune_fft(P.make_conf(**kw), 1)[0])
     ptr0 = dev.ptr.value
 
     class View:  # device pointer into the concatenated buffer
@@ -233,15 +266,75 @@ def run_batched(P, data, nmock, seed=1000):
             d = np.abs(bres[m]["pl"][0][li] - plain[m]["pl"][0][li]) / S
             worst = max(worst, float(np.max(d)))
     log(f"batched vs plain: worst |dP_l|/S_b over {nmock} mocks {worst:.2e}")
-    log(f"batched: {nmock / t_batch:.1f} meshes/s, plain: {nmock / t_plain:.1f} meshes/s")
-    return {"workload": f"{nmock} clustered mocks, N_p in [2e5,3e6] (total {len(allp)}), "
+    ceil = batched_ceiling(sizes)
+    log(f"batched: {nmock / t_batch:.1f} meshes/s ({nmock / t_batch / ceil:.3f} of the "
+        f"{ceil:.0f} meshes/s HBM ceiling), plain: {nmock / t_plain:.1f} meshes/s")
+    return {"workload": f"{nmock} independent clustered mocks (seeds {seed}..{seed + nmock - 1}, "
+                        f"expected N_p in [2e5,3e6], total {int(off[-1])}), "
                         "TSC N_g=256 interlaced, l=0,2,4, dk=0.01",
             "batched_meshes_per_s": round(nmock / t_batch, 1),
             "plain_meshes_per_s": round(nmock / t_plain, 1),
             "batched_over_plain": round(t_plain / t_batch, 2),
             "batched_vs_plain_max_dP_over_Sb": worst,
-            "tile": list(tile),
-            "batched_particles_per_s": round(len(allp) / t_batch, 1)}
+            "tile": list(tile), "y_schedule": ysched,
+            "hbm_ceiling_meshes_per_s": round(ceil, 1),
+            "batched_frac_of_ceiling": round(nmock / t_batch / ceil, 4),
+            "batched_particles_per_s": round(int(off[-1]) / t_batch, 1)}
+
+
+def run_alias(P, steps=10, warmup=2, params_path=None):
+    """Learned alias correction: a held-out clustered box of the training
+    family (P = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2), 512^3 generator grid,
+    nbar = 0.02, seed 6200), CIC, N_g = 512, l = 0,2,4, dk = 0.01; the plain
+    path with the library's correction stage (8 |mu| cells, the shipped
+    network ensemble) against the interlaced path.  Both timed with the
+    catalogue in HBM: HIP events from the run's first to its last kernel
+    (device) and wall time per call."""
+    from powspec_amd import mlp
+    params = mlp.load_params(params_path) if params_path else mlp.load_params()
+    if params is None:
+        log("alias correction: no trained networks in the package, skipped")
+        return None
+    dev, n = P.gen_clustered_box(ng=512, L=1000.0, nbar=0.02, f=0.8, seed=6200)
+    L = P.load()
+    out = {}
+    for name, il in (("plain_mlp", False), ("interlaced", True)):
+        conf = mlp.box_conf(P, 512, 1000.0, 0.01, P.CIC, il)
+        tile = P.autotune(conf, nmock=1, data=dev, offsets=[0, n])[0]
+        ctx = P.Context(conf)
+        ctx.set_tiles(tile)
+        if not il:
+            ctx.set_mu_cells(mlp.NMU)
+            ctx.set_alias_mlp(params)
+        for _ in range(warmup):
+            res = ctx.run_device(dev, n, float(n))
+        L.psh_device_synchronize()
+        ctx.set_profiling(True)
+        dev_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            res = ctx.run_device(dev, n, float(n))
+            dev_ms.append(ctx.timings()[3])
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        out[name] = (float(np.mean(dev_ms)), wall, res, tile)
+        ctx.close()
+    dev.free()
+    (dp, wp, rp, tp), (di, wi, ri, ti) = out["plain_mlp"], out["interlaced"]
+    kny = np.pi * 512 / 1000.0
+    sel = rp["k"] <= 0.8 * kny
+    shot = 1000.0 ** 3 / n
+    S = ri["pl"][0][0][sel] + shot
+    err = [float(np.max(np.abs(rp["pl"][0][l][sel] - ri["pl"][0][l][sel]) / S)) for l in range(3)]
+    log(f"alias correction: plain+MLP {dp:.3f} ms device / {wp:.3f} ms wall, interlaced "
+        f"{di:.3f} / {wi:.3f} ms (ratio {dp / di:.3f} device, {wp / wi:.3f} wall); worst "
+        f"|dP_l|/S_b for k <= 0.8 k_Ny vs interlaced: {np.round(err, 5).tolist()}")
+    return {"workload": f"held-out clustered box (seed 6200, {n} particles), CIC N_g=512, "
+                        "l=0,2,4, dk=0.01; plain + learned correction (8 |mu| cells, "
+                        f"{len(params)} networks) vs interlaced",
+            "plain_mlp_ms_device": round(dp, 4), "interlaced_ms_device": round(di, 4),
+            "plain_mlp_ms_wall": round(wp, 4), "interlaced_ms_wall": round(wi, 4),
+            "ratio_device": round(dp / di, 4), "ratio_wall": round(wp / wi, 4),
+            "max_dP_over_Sb_l024": err, "tiles": [list(tp), list(ti)]}
 
 
 def run_survey(P, steps=3):
@@ -370,22 +463,47 @@ def run_ingest(P, data, tile, reps=3):
                       "lines_per_s": round(na / ta, 1)}}
 
 
-def parity_vs_oracle(gpu, ref, poles=(0, 2, 4)):
+def parity_vs_oracle(gpu, ref, poles=(0, 2, 4), per_bin_path=None):
     """Per-bin error of the GPU P_l against the FP64 oracle on the same
     catalogue: mode counts compared exactly, |dP_l| / S_b with S_b = |P_0| +
-    shot the bin's un-subtracted power, and |dP_0| / |P_0|."""
+    shot the bin's un-subtracted power, and |dP_0| / |P_0|.  The per-bin
+    arrays go to per_bin_path (JSON); the line carries their quantiles."""
     shot = ref["shot"][0]
     has = ref["cnt"] > 0
     S = np.abs(ref["pl"][0][0]) + shot
     out = {"cnt_exact": bool(np.array_equal(gpu["cnt"], ref["cnt"])),
            "nbin": int(len(ref["cnt"]))}
+    per_bin = {"k": [float(v) for v in ref["k"]], "cnt": [int(v) for v in ref["cnt"]]}
     for li, ell in enumerate(poles):
-        d = np.abs(gpu["pl"][0][li] - ref["pl"][0][li])[has] / S[has]
+        dd = np.where(has, np.abs(gpu["pl"][0][li] - ref["pl"][0][li]) / S, 0.0)
+        d = dd[has]
         out[f"max_dP{ell}_over_Sb"] = float(d.max())
+        out[f"q50_q90_q99_dP{ell}_over_Sb"] = [float(np.quantile(d, q)) for q in (0.5, 0.9, 0.99)]
+        out[f"worst_bin_l{ell}"] = int(np.argmax(dd))
+        per_bin[f"dP{ell}_over_Sb"] = [float(f"{v:.3e}") for v in dd]
+    if per_bin_path:
+        try:
+            os.makedirs(os.path.dirname(per_bin_path), exist_ok=True)
+            with open(per_bin_path, "w") as f:
+                json.dump(per_bin, f)
+            out["per_bin_file"] = os.path.relpath(per_bin_path, ROOT)
+        except OSError:
+            pass
     nz = has & (ref["pl"][0][0] != 0)   # bins whose P_0 is exactly 0 have no relative error
     p0 = ref["pl"][0][0][nz]
     out["max_rel_dP0"] = float(np.max(np.abs(gpu["pl"][0][0][nz] - p0) / np.abs(p0)))
     return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(box_data, gpu_res=None):
@@ -402,10 +520,12 @@ def cpu_baseline(box_data, gpu_res=None):
     ta, tf, tb, tt = r["timings"]
     threads = O.lib().pso_num_threads()
     log(f"cpu oracle, box: {wall:.1f} s wall (assign {ta:.1f}, fft {tf:.1f}, bin {tb:.1f})")
-    par = parity_vs_oracle(gpu_res, r) if gpu_res is not None else None
+    par = (parity_vs_oracle(gpu_res, r, per_bin_path=os.path.join(
+        ROOT, "gpurun_out", "box1024_pcs_parity_per_bin.json")) if gpu_res is not None else None)
     if par:
         log("box vs FP64 oracle: " + ", ".join(f"{k}={v}" for k, v in par.items()))
     return {"value": round(1.0 / wall, 5), "unit": "meshes/s", "cores": threads,
+            "cpu_model": cpu_model(),
             "parity_vs_oracle": par,
             "kind": "port",
             "sample": f"oracle/psoracle.c (FP64, OpenMP {threads} threads, own radix-2 FFT) "
@@ -425,8 +545,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-survey", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-alias", action="store_true")
+    ap.add_argument("--alias-params", default=None,
+                    help="networks of the learned correction (default: the shipped ones)")
     ap.add_argument("--tile", default=None,
                     help="deposit tile of the N_g=1024 box 'tx,ty,tz' instead of autotuning (profiling runs)")
+    ap.add_argument("--ysched", default=None,
+                    help="y-pass schedule of the N_g=1024 box instead of tuning (profiling runs)")
+    ap.add_argument("--only", default=None, choices=["batched", "survey", "alias"],
+                    help="run one secondary workload alone, no JSON line (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -443,13 +570,21 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     arch = P.device_arch()
+    if args.only:
+        r = {"batched": lambda: run_batched(P, args.nmock),
+             "survey": lambda: run_survey(P),
+             "alias": lambda: run_alias(P, params_path=args.alias_params)}[args.only]()
+        log(json.dumps(r))
+        return
     fixed = [int(v) for v in args.tile.split(",")] if args.tile else None
-    data, res, ms_step, stages, n, tile = run_box(P, args.steps, args.warmup, fixed)
+    data, res, ms_step, stages, n, tile, ysched = run_box(P, args.steps, args.warmup, fixed,
+                                                          args.ysched)
     rl = roofline(stages, 1024, 2, n, x_live_fraction(1024, 1000.0, float(res["kedge"][-1])))
     extra = {}
     if not args.no_batched and args.nmock > 0:
-        extra = run_batched(P, data, args.nmock)
+        extra = run_batched(P, args.nmock)
     survey = None if args.no_survey else run_survey(P)
+    alias = None if args.no_alias else run_alias(P, params_path=args.alias_params)
     ingest = None if args.no_ingest else run_ingest(P, data, tile)
     cpu = None if args.no_cpu else cpu_baseline(data, res)
     line = {
@@ -470,12 +605,13 @@ def main():
                                "interlaced, l=0,2,4, dk=0.005, L=1000 Mpc/h",
                    "N_g": 1024, "N_p": n, "assign": "PCS", "interlaced": True,
                    "precision": "FP32 meshes/FFT, FP64 bin sums", "arch": arch,
-                   "tile": list(tile)},
+                   "tile": list(tile), "y_schedule": ysched},
         "particles_per_s": round(n / (ms_step * 1e-3), 1),
         "stage_ms": {k: round(float(v), 4) for k, v in stages.items()},
         "roofline": rl,
         "batched": extra or None,
         "survey": survey,
+        "alias_correction": alias,
         "ingest": ingest,
         "cpu_baseline": cpu,
     }

@@ -340,6 +340,7 @@ int psh_device_malloc(void **ptr, size_t bytes);
 int psh_device_free(void *ptr);
 int psh_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int psh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int psh_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int psh_device_synchronize(void);
 /* Name of the device architecture (e.g. "gfx950"). */
 int psh_device_arch(char *buf, int len);
@@ -365,6 +366,35 @@ int psh_mlp_grad(const float *d_params, const float *d_X, const float *d_T, int 
 int psh_mlp_train(float *d_params, const float *d_X, const float *d_T, int B, int batch,
     int epochs, float lr, unsigned seed, float *loss_out);
 
+/* Library stage of the learned correction (no counterpart in the reference,
+ * whose fallback is the Jing correction of src/multipole.c:71-99): boxes
+ * binned in |mu| cells, cell c of a k bin holding the modes with |mu| in
+ * [c / nmu, (c + 1) / nmu) (1 <= nmu <= 64; 1: no cells, the default).  Box
+ * runs then keep, besides P_l, the raw sums S[l][b][c] = sum over the
+ * cell's modes of |delta|^2 alias mult L_l(mu). */
+int psh_ctx_set_mu_cells(psh_ctx *ctx, int nmu);
+/* The raw cell sums of the last box run, [npole][nbin][nmu]. */
+int psh_ctx_cells(const psh_ctx *ctx, double *cells);
+/* The network's input rows of the last box run, [nbin * nmu][32] floats
+ * (row b nmu + c; zero rows for bins without modes or power): k_b / k_Ny,
+ * the cell's |mu| centre, the one-hot scheme, 16 band averages of
+ * log10(un-subtracted P_0 / shot) over [k_Ny / 4, k_Ny], the bin's own
+ * log10(un-subtracted P_0 / shot) and log10(nmu S_0[b][c] / sum_c S_0[b][c])
+ * clipped to [-2, 2]. */
+int psh_ctx_alias_rows(psh_ctx *ctx, float *X);
+/* Sets (params != NULL) or clears the learned correction of a plain
+ * (non-interlaced) box context with a monopole and |mu| cells: nnet
+ * networks, host params [nnet][psh_mlp_param_count()].  Its box runs
+ * (psh_ctx_run_device, psh_ctx_run_host and the drop-in layer, auto
+ * spectra) then compute on the device the rows above, the networks'
+ * outputs f_n and the corrected sums sum_c exp(mean_n f_n[b][c]) S[l][b][c]
+ * -- an estimate of the interlaced estimator's sums -- before the usual
+ * normalisation, so the shifted grid's mesh and FFTs are skipped.  The
+ * networks are trained on log(S_il / S_plain) of the monopole cell sums of
+ * interlaced and plain runs (powspec_amd/mlp.py); the interlaced estimator
+ * stays the fallback and the judge of the error. */
+int psh_ctx_set_alias_mlp(psh_ctx *ctx, const float *params, int nnet);
+
 /* ---- tuning ----------------------------------------------------------------- */
 /* No counterpart in the reference (its FFTW plans use FFTW_ESTIMATE): the
  * deposit tile shape sets the bucketing duplication, the deposit's LDS
@@ -384,10 +414,37 @@ int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
  * before the context has a box (surveys: before their first run). */
 int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]);
 
-/* Applies the fastest entry of a tuning cache for the context's device
+/* Applies the fastest entries of a tuning cache for the context's device
  * architecture, grid size, scheme, interlacing and batch size (rounded down
- * to a power of two).  PSH_ERR_FILE / PSH_ERR_ARG when none applies. */
+ * to a power of two): deposit tiles and the y-pass schedule.  PSH_ERR_FILE /
+ * PSH_ERR_ARG when none applies. */
 int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path);
+
+/* FFT schedules (no counterpart in the reference, whose FFTW plans use
+ * FFTW_ESTIMATE, src/fftw_define.h:64): the column (y) pass has schedules
+ * 0..5 -- columns per tile (16, 16, 16, 16, 32, 8), values per thread
+ * (32, 32, 32, 16, 32, 16) and radix plan (0: fewest passes of radix <= 16,
+ * larger first; 1: the same passes, smaller first; 2: one pass more) -- of
+ * which those whose LDS fits and that differ from schedule 0 apply at a
+ * grid size.  Results differ between schedules by FP32 rounding only.
+ * psh_ctx_set_fft_schedule refuses (PSH_ERR_ARG) one that does not apply. */
+int psh_ctx_set_fft_schedule(psh_ctx *ctx, int ysched);
+int psh_ctx_get_fft_schedule(const psh_ctx *ctx, int *ysched);
+
+typedef struct {
+  int ysched;      /* fastest y-pass schedule               */
+  double ms;       /* its median time per launch            */
+  int ncand;       /* schedules timed                       */
+  double ms_default; /* schedule 0's median time            */
+} psh_tune_fft;
+
+/* Times every applicable y-pass schedule on zeroed meshes of conf's grid
+ * size (nmock meshes per launch, the live-column pruning of conf's
+ * binning), appends "fft arch Ng intlace nmock_class y ysched ms" to
+ * cache_path (when not NULL; psh_ctx_load_tuning applies the fastest entry
+ * for the context) and returns the winner. */
+int psh_autotune_fft(const psh_conf *conf, int nmock, const char *cache_path,
+    psh_tune_fft *out);
 
 typedef struct {
   int tile[3];     /* fastest shape                          */

@@ -66,7 +66,7 @@ EXPORTS = [
     "psh_ctx_nbin", "psh_ctx_run_host", "psh_ctx_define_box", "psh_ctx_run_device",
     "psh_ctx_run_batch", "psh_ctx_timings", "psh_ctx_set_profiling", "psh_last_error",
     "psh_device_count", "psh_device_malloc", "psh_device_free", "psh_memcpy_h2d",
-    "psh_memcpy_d2h", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
+    "psh_memcpy_d2h", "psh_memcpy_d2d", "psh_device_synchronize", "psh_device_arch", "psh_write_pk",
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
     "psh_ctx_get_tiles",
@@ -74,7 +74,9 @@ EXPORTS = [
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
-    "psh_fault_inject", "psh_run_device_static",
+    "psh_fault_inject", "psh_run_device_static", "psh_ctx_set_mu_cells", "psh_ctx_cells",
+    "psh_ctx_alias_rows", "psh_ctx_set_alias_mlp", "psh_ctx_set_fft_schedule",
+    "psh_ctx_get_fft_schedule", "psh_autotune_fft",
 ]
 
 
@@ -82,6 +84,26 @@ class Tune(C.Structure):
     """psh_tune: result of psh_autotune."""
 
     _fields_ = [("tile", C.c_int * 3), ("ms", C.c_double), ("ncand", C.c_int)]
+
+
+class TuneFft(C.Structure):
+    """psh_tune_fft: result of psh_autotune_fft."""
+
+    _fields_ = [("ysched", C.c_int), ("ms", C.c_double), ("ncand", C.c_int),
+                ("ms_default", C.c_double)]
+
+
+def autotune_fft(conf, nmock=1, cache_path=None):
+    """Time the y-pass schedules for conf's grid size on nmock meshes;
+    returns (schedule, ms, candidates, ms of schedule 0) and appends the
+    winner to cache_path."""
+    t = TuneFft()
+    L = load()
+    rc = L.psh_autotune_fft(C.byref(conf), int(nmock),
+                            cache_path.encode() if cache_path else None, C.byref(t))
+    if rc:
+        raise PshError(rc, L.psh_last_error(None).decode())
+    return t.ysched, t.ms, t.ncand, t.ms_default
 
 
 def autotune(conf, npart=0, nmock=1, cache_path=None, data=None, offsets=None):
@@ -181,6 +203,7 @@ def load():
     L.psh_device_free.argtypes = [vp]
     L.psh_memcpy_h2d.argtypes = [vp, vp, sz]
     L.psh_memcpy_d2h.argtypes = [vp, vp, sz]
+    L.psh_memcpy_d2d.argtypes = [vp, vp, sz]
     L.psh_device_arch.argtypes = [C.c_char_p, i]
     L.psh_genr_mesh.argtypes = [C.POINTER(Conf), C.POINTER(Cata)]
     L.psh_genr_mesh.restype = vp
@@ -202,6 +225,13 @@ def load():
     L.psh_parse_double.argtypes = [C.c_char_p, sz, C.POINTER(d)]
     L.psh_fault_inject.argtypes = [i, C.c_long]
     L.psh_run_device_static.argtypes = [C.POINTER(Conf), vp, sz, d, C.POINTER(Pk)]
+    L.psh_ctx_set_mu_cells.argtypes = [vp, i]
+    L.psh_ctx_cells.argtypes = [vp, vp]
+    L.psh_ctx_alias_rows.argtypes = [vp, vp]
+    L.psh_ctx_set_alias_mlp.argtypes = [vp, vp, i]
+    L.psh_ctx_set_fft_schedule.argtypes = [vp, i]
+    L.psh_ctx_get_fft_schedule.argtypes = [vp, C.POINTER(i)]
+    L.psh_autotune_fft.argtypes = [C.POINTER(Conf), i, C.c_char_p, C.POINTER(TuneFft)]
     _lib = L
     return L
 
@@ -213,6 +243,17 @@ def fault_inject(what: int, value: int) -> None:
     rc = load().psh_fault_inject(int(what), int(value))
     if rc:
         raise PshError(rc, "psh_fault_inject failed")
+
+
+def memcpy_d2d(dst: "DeviceArray", dst_off: int, src: "DeviceArray", nbytes: int,
+               src_off: int = 0) -> None:
+    """Copy nbytes between device allocations (byte offsets)."""
+    if dst_off + nbytes > dst.nbytes or src_off + nbytes > src.nbytes:
+        raise ValueError("device copy out of range")
+    rc = load().psh_memcpy_d2d(C.c_void_p(dst.ptr.value + dst_off),
+                               C.c_void_p(src.ptr.value + src_off), C.c_size_t(nbytes))
+    if rc:
+        raise PshError(rc, "device to device copy failed")
 
 
 def device_count() -> int:
@@ -372,11 +413,46 @@ class Context:
     def set_tiles(self, tile):
         self._check(self.L.psh_ctx_set_tiles(self.h, (C.c_int * 3)(*tile)))
 
+    def set_fft_schedule(self, ysched):
+        self._check(self.L.psh_ctx_set_fft_schedule(self.h, int(ysched)))
+
+    def fft_schedule(self):
+        v = C.c_int()
+        self._check(self.L.psh_ctx_get_fft_schedule(self.h, C.byref(v)))
+        return v.value
+
     def tiles(self):
         """The deposit tiles in use (clipped to the grid size)."""
         t = (C.c_int * 3)()
         self._check(self.L.psh_ctx_get_tiles(self.h, t))
         return tuple(t)
+
+    def set_mu_cells(self, nmu):
+        self._check(self.L.psh_ctx_set_mu_cells(self.h, int(nmu)))
+        self.nmu = int(nmu)
+
+    def cells(self):
+        """Raw (pole, bin, |mu| cell) sums of the last box run."""
+        nmu = getattr(self, "nmu", 1)
+        out = np.zeros((self.conf.npole, self.nbin(), nmu))
+        self._check(self.L.psh_ctx_cells(self.h, out.ctypes.data))
+        return out
+
+    def alias_rows(self):
+        """The learned correction's input rows of the last box run."""
+        nmu = getattr(self, "nmu", 1)
+        X = np.zeros((self.nbin() * nmu, 32), np.float32)
+        self._check(self.L.psh_ctx_alias_rows(self.h, X.ctypes.data))
+        return X
+
+    def set_alias_mlp(self, params):
+        """Networks (one [NPAR] or an ensemble [n, NPAR]) of the learned
+        correction; None clears it."""
+        if params is None:
+            self._check(self.L.psh_ctx_set_alias_mlp(self.h, None, 0))
+            return
+        p = np.ascontiguousarray(np.atleast_2d(params), np.float32)
+        self._check(self.L.psh_ctx_set_alias_mlp(self.h, p.ctypes.data, len(p)))
 
     def load_tuning(self, cache_path):
         self._check(self.L.psh_ctx_load_tuning(self.h, str(cache_path).encode()))

@@ -41,13 +41,38 @@ __device__ __forceinline__ float2 rot(float2 a) {
 }
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n >> 1); }
-constexpr int npasses(int N) { return (ilog2(N) + 3) / 4; }
-constexpr int radix_bits(int N, int p) {
-  return ilog2(N) / npasses(N) + (p < ilog2(N) % npasses(N) ? 1 : 0);
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+constexpr int pow2_floor(int n) { return n < 2 ? 1 : 2 * pow2_floor(n / 2); }
+constexpr int pow2_ceil(int n) { return pow2_floor(n) < n ? 2 * pow2_floor(n) : pow2_floor(n); }
+// Lengths N = f 2^a with f in {1, 3, 5}: the power-of-two part and f.
+constexpr int pow2_part(int N) { return N & -N; }
+constexpr int odd_part(int N) { return N / pow2_part(N); }
+// Radix plans of a length-N transform (PL), for the power-of-two part:
+//   0  the fewest passes (radix <= 16), the larger radices first;
+//   1  the same passes, the smaller radices first;
+//   2  one pass more (smaller radices), the larger first;
+// an odd factor f = 3 or 5 adds one radix-f pass at the end (its stride
+// N / f is then a power of two, which keeps the padded LDS addresses of
+// the other passes separable).
+constexpr int npasses2(int P, int PL) {
+  return (ilog2(P) + 3) / 4 + (PL == 2 && (ilog2(P) + 3) / 4 < ilog2(P) ? 1 : 0);
 }
-constexpr int radix(int N, int p) { return 1 << radix_bits(N, p); }
-constexpr int ns_before(int N, int p) {
-  return p == 0 ? 1 : ns_before(N, p - 1) * radix(N, p - 1);
+constexpr int npasses(int N, int PL = 0) {
+  return npasses2(pow2_part(N), PL) + (odd_part(N) > 1 ? 1 : 0);
+}
+constexpr int radix_bits(int N, int p, int PL = 0) {
+  return ilog2(pow2_part(N)) / npasses2(pow2_part(N), PL) +
+         ((PL == 1 ? npasses2(pow2_part(N), PL) - 1 - p : p) <
+                  ilog2(pow2_part(N)) % npasses2(pow2_part(N), PL)
+              ? 1
+              : 0);
+}
+constexpr int radix(int N, int p, int PL = 0) {
+  return (odd_part(N) > 1 && p == npasses(N, PL) - 1) ? odd_part(N)
+                                                       : 1 << radix_bits(N, p, PL);
+}
+constexpr int ns_before(int N, int p, int PL = 0) {
+  return p == 0 ? 1 : ns_before(N, p - 1, PL) * radix(N, p - 1, PL);
 }
 
 // exp(-2 pi i m / 16) for a compile-time m (after unrolling)
@@ -132,6 +157,47 @@ __device__ __forceinline__ void dft<8, false>(float2 *v) { dft8<false>(v); }
 template <>
 __device__ __forceinline__ void dft<8, true>(float2 *v) { dft8<true>(v); }
 
+// 3-point DFT: X1,2 = x0 - (x1 + x2)/2 -+ i (sqrt 3 / 2)(x1 - x2) (forward)
+template <bool INV>
+__device__ __forceinline__ void dft3(float2 *v) {
+  constexpr float s3 = 0.86602540378443865f;
+  const float2 t1 = cadd(v[1], v[2]);
+  const float2 t2 = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+  const float2 t3 = rot<INV>(csub(v[1], v[2]));
+  v[0] = cadd(v[0], t1);
+  v[1] = make_float2(t2.x + s3 * t3.x, t2.y + s3 * t3.y);
+  v[2] = make_float2(t2.x - s3 * t3.x, t2.y - s3 * t3.y);
+}
+template <>
+__device__ __forceinline__ void dft<3, false>(float2 *v) { dft3<false>(v); }
+template <>
+__device__ __forceinline__ void dft<3, true>(float2 *v) { dft3<true>(v); }
+
+// 5-point DFT: with c_k = cos(2 pi k / 5), s_k = sin(2 pi k / 5),
+// t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3:
+// X1,4 = x0 + c1 t1 + c2 t2 -+ i (s1 t3 + s2 t4),
+// X2,3 = x0 + c2 t1 + c1 t2 -+ i (s2 t3 - s1 t4) (forward).
+template <bool INV>
+__device__ __forceinline__ void dft5(float2 *v) {
+  constexpr float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+  constexpr float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+  const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  const float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  const float2 b1 = rot<INV>(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  const float2 b2 = rot<INV>(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+template <>
+__device__ __forceinline__ void dft<5, false>(float2 *v) { dft5<false>(v); }
+template <>
+__device__ __forceinline__ void dft<5, true>(float2 *v) { dft5<true>(v); }
+
 // 16 = 4 x 4
 template <bool INV>
 __device__ __forceinline__ void dft16(float2 *v) {
@@ -211,10 +277,10 @@ __device__ __forceinline__ size_t xcd_tile(size_t b, size_t nblk) {
 }
 
 // Per-pass thread geometry.
-template <int N, int C, int T, int P, bool COLMAP>
+template <int N, int C, int T, int P, bool COLMAP, int PL = 0>
 struct PassGeom {
-  static constexpr int R = radix(N, P);
-  static constexpr int NS = ns_before(N, P);
+  static constexpr int R = radix(N, P, PL);
+  static constexpr int NS = ns_before(N, P, PL);
   static constexpr int NBF = N / R;           // butterflies per line
   static constexpr int NBT = C * NBF;         // butterflies in the tile
   static constexpr int NB = (NBT + T - 1) / T;// butterflies per thread
@@ -239,9 +305,9 @@ struct Plan {
 };
 
 // Load the inputs of pass P through `ld(c, n) -> float2`.
-template <int N, int C, int T, int P, bool COLMAP, class LD>
+template <int N, int C, int T, int P, bool COLMAP, class LD, int PL = 0>
 __device__ __forceinline__ void pass_load(float2 *v, LD ld) {
-  using G = PassGeom<N, C, T, P, COLMAP>;
+  using G = PassGeom<N, C, T, P, COLMAP, PL>;
 #pragma unroll
   for (int q = 0; q < G::NB; q++) {
     int c, j;
@@ -254,16 +320,16 @@ __device__ __forceinline__ void pass_load(float2 *v, LD ld) {
 
 // Twiddle and butterfly of pass P in registers.  tw: table of
 // exp(-2 pi i m / (N*TWS)) read with stride TWS.
-template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS>
+template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS, int PL = 0>
 __device__ __forceinline__ void pass_compute(float2 *v,
                                              const float2 *__restrict__ tw) {
-  using G = PassGeom<N, C, T, P, COLMAP>;
+  using G = PassGeom<N, C, T, P, COLMAP, PL>;
 #pragma unroll
   for (int q = 0; q < G::NB; q++) {
     int c, j;
     G::map(q, c, j);
     if (P > 0) {
-      const int k = j & (G::NS - 1);
+      const int k = (int)((unsigned)j % (unsigned)G::NS);
       constexpr int step = N / (G::NS * G::R);
 #pragma unroll
       for (int r = 1; r < G::R; r++)
@@ -274,15 +340,15 @@ __device__ __forceinline__ void pass_compute(float2 *v,
 }
 
 // Store the outputs of pass P through `st(c, n, value)`.
-template <int N, int C, int T, int P, bool COLMAP, class ST>
+template <int N, int C, int T, int P, bool COLMAP, class ST, int PL = 0>
 __device__ __forceinline__ void pass_store(const float2 *v, ST st) {
-  using G = PassGeom<N, C, T, P, COLMAP>;
+  using G = PassGeom<N, C, T, P, COLMAP, PL>;
 #pragma unroll
   for (int q = 0; q < G::NB; q++) {
     int c, j;
     if (!G::map(q, c, j)) continue;
-    const int k = j & (G::NS - 1);
-    const int base = (j / G::NS) * G::NS * G::R + k;
+    const int k = (int)((unsigned)j % (unsigned)G::NS);
+    const int base = (int)((unsigned)j / (unsigned)G::NS) * G::NS * G::R + k;
 #pragma unroll
     for (int r = 0; r < G::R; r++) st(c, base + r * G::NS, v[q * G::R + r]);
   }
@@ -290,12 +356,18 @@ __device__ __forceinline__ void pass_store(const float2 *v, ST st) {
 
 // Exchange the outputs of pass P into the inputs of pass P+1 through LDS,
 // one component at a time.
-template <int N, int C, int T, int P, bool COLMAP>
+template <int N, int C, int T, int P, bool COLMAP, int PL = 0>
 __device__ __forceinline__ void exchange(float2 *v, float *lds) {
-  using G0 = PassGeom<N, C, T, P, COLMAP>;
-  using G1 = PassGeom<N, C, T, P + 1, COLMAP>;
+  using G0 = PassGeom<N, C, T, P, COLMAP, PL>;
+  using G1 = PassGeom<N, C, T, P + 1, COLMAP, PL>;
   using L = Lds<N, C, COLMAP>;
   constexpr int V = G1::NB * G1::R;
+  // padded addresses of a butterfly's elements are one base plus constant
+  // offsets when the stride is a power of two (or a multiple of the padding
+  // block); otherwise each element's address is formed on its own
+  constexpr int BLK = COLMAP ? 8 : 16;
+  constexpr bool SEP0 = is_pow2(G0::NS) || G0::NS % BLK == 0;
+  constexpr bool SEP1 = is_pow2(G1::NBF) || G1::NBF % BLK == 0;
   float tmp[V];
   // outputs of pass P: element base + r NS of line c
   auto store = [&](bool im) {
@@ -303,12 +375,14 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
     for (int q = 0; q < G0::NB; q++) {
       int c, j;
       if (!G0::map(q, c, j)) continue;
-      const int k = j & (G0::NS - 1);
-      float *row = lds + L::idx(c, (j / G0::NS) * G0::NS * G0::R + k);
+      const int k = (int)((unsigned)j % (unsigned)G0::NS);
+      const int e0 = (int)((unsigned)j / (unsigned)G0::NS) * G0::NS * G0::R + k;
+      float *row = lds + L::idx(c, e0);
 #pragma unroll
       for (int r = 0; r < G0::R; r++) {
         const float2 x = v[q * G0::R + r];
-        row[L::off(r * G0::NS)] = im ? x.y : x.x;
+        if (SEP0) row[L::off(r * G0::NS)] = im ? x.y : x.x;
+        else lds[L::idx(c, e0 + r * G0::NS)] = im ? x.y : x.x;
       }
     }
   };
@@ -321,7 +395,8 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
       const float *row = lds + L::idx(c, j);
 #pragma unroll
       for (int r = 0; r < G1::R; r++) {
-        const float x = ok ? row[L::off(r * G1::NBF)] : 0.f;
+        const float x = ok ? (SEP1 ? row[L::off(r * G1::NBF)] : lds[L::idx(c, j + r * G1::NBF)])
+                           : 0.f;
         if (im) v[q * G1::R + r] = make_float2(tmp[q * G1::R + r], x);
         else tmp[q * G1::R + r] = x;
       }
@@ -338,14 +413,14 @@ __device__ __forceinline__ void exchange(float2 *v, float *lds) {
 }
 
 // Run passes P..last on values already loaded for pass P.
-template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS>
+template <int N, int C, int T, int P, bool COLMAP, bool INV, int TWS, int PL = 0>
 struct Passes {
   __device__ static __forceinline__ void run(float2 *v, float *lds,
                                              const float2 *__restrict__ tw) {
-    pass_compute<N, C, T, P, COLMAP, INV, TWS>(v, tw);
-    if constexpr (P + 1 < npasses(N)) {
-      exchange<N, C, T, P, COLMAP>(v, lds);
-      Passes<N, C, T, P + 1, COLMAP, INV, TWS>::run(v, lds, tw);
+    pass_compute<N, C, T, P, COLMAP, INV, TWS, PL>(v, tw);
+    if constexpr (P + 1 < npasses(N, PL)) {
+      exchange<N, C, T, P, COLMAP, PL>(v, lds);
+      Passes<N, C, T, P + 1, COLMAP, INV, TWS, PL>::run(v, lds, tw);
     }
   }
 };
@@ -370,9 +445,9 @@ __device__ __forceinline__ void store_last(const float2 *v, ST st) {
 // the address as a per-tile uniform row base (r) plus one per-lane offset
 // (c, j) shared by all values of the thread, which keeps 64-bit addresses
 // out of the registers.
-template <int N, int C, int T, bool COLMAP, class LD>
+template <int N, int C, int T, bool COLMAP, class LD, int PL = 0>
 __device__ __forceinline__ void pass_load_rows(float2 *v, LD ld) {
-  using G = PassGeom<N, C, T, 0, COLMAP>;
+  using G = PassGeom<N, C, T, 0, COLMAP, PL>;
 #pragma unroll
   for (int q = 0; q < G::NB; q++) {
     int c, j;
@@ -383,11 +458,11 @@ __device__ __forceinline__ void pass_load_rows(float2 *v, LD ld) {
   }
 }
 
-template <int N, int C, int T, bool COLMAP, bool INV, int TWS, class LD>
+template <int N, int C, int T, bool COLMAP, bool INV, int TWS, class LD, int PL = 0>
 __device__ __forceinline__ void transform_rows(float2 *v, float *lds,
                                                const float2 *__restrict__ tw, LD ld) {
-  pass_load_rows<N, C, T, COLMAP>(v, ld);
-  Passes<N, C, T, 0, COLMAP, INV, TWS>::run(v, lds, tw);
+  pass_load_rows<N, C, T, COLMAP, LD, PL>(v, ld);
+  Passes<N, C, T, 0, COLMAP, INV, TWS, PL>::run(v, lds, tw);
 }
 
 // Two fields of the same tile advanced pass by pass together (each exchanged
@@ -417,9 +492,9 @@ __device__ __forceinline__ void transform_rows2(float2 *va, float2 *vb, float *l
 }
 
 // Natural-order outputs of the last pass: element j + r*NS of line c.
-template <int N, int C, int T, bool COLMAP, class ST>
+template <int N, int C, int T, bool COLMAP, class ST, int PL = 0>
 __device__ __forceinline__ void store_last_rows(const float2 *v, ST st) {
-  using G = PassGeom<N, C, T, npasses(N) - 1, COLMAP>;
+  using G = PassGeom<N, C, T, npasses(N, PL) - 1, COLMAP, PL>;
   static_assert(G::NS * G::R == N, "last pass");
 #pragma unroll
   for (int q = 0; q < G::NB; q++) {
@@ -493,10 +568,10 @@ __device__ __forceinline__ void mesh_store4(float4 *p, float4 x) {
 }
 
 // Row strides of the split access.
-template <int N, int C, int T, bool COLMAP>
+template <int N, int C, int T, bool COLMAP, int PL = 0>
 struct RowSplit {
-  static constexpr int LOAD = PassGeom<N, C, T, 0, COLMAP>::NBF;
-  static constexpr int STORE = PassGeom<N, C, T, npasses(N) - 1, COLMAP>::NS;
+  static constexpr int LOAD = PassGeom<N, C, T, 0, COLMAP, PL>::NBF;
+  static constexpr int STORE = PassGeom<N, C, T, npasses(N, PL) - 1, COLMAP, PL>::NS;
   // largest per-lane row index (must keep j * ES + c within 32 bits)
   static constexpr int JMAX = (LOAD > STORE ? LOAD : STORE);
   // lane offsets of a column pass over an x stride of N (N/2 + pad) complex

@@ -35,9 +35,14 @@ using DevPart = PshDevPart;
 
 __device__ __forceinline__ int window_width(int scheme) { return scheme + 1; }
 
-// Periodic wrap into [0, Ng); Ng is a power of two (the context accepts no
-// other grid size), so two's complement masking is the modulo.
-__device__ __forceinline__ int wrap(int c, int Ng) { return c & (Ng - 1); }
+// Periodic wrap into [0, Ng).  Bases and window ends are at most a window
+// (or a grid) away from [0, Ng), so one compare settles nearly every call;
+// the remainder covers coordinates further out.
+__device__ __forceinline__ int wrap(int c, int Ng) {
+  if ((unsigned)c < (unsigned)Ng) return c;
+  const int r = c % Ng;
+  return r < 0 ? r + Ng : r;
+}
 
 // Base cell (leftmost window cell, unwrapped) and window fraction of a grid
 // coordinate x >= 0, following src/genr_mesh.c:57-66 (NGP), 94-104 (CIC),
@@ -683,8 +688,13 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
   // ZPAD: offset in a padded row of the window's first z cell, b - T0 + W in
   // [0, T[2] + W] (cells up to T[2] + 2W - 1 < P2).  Windows wrapped by Ng
   // (tile at z = 0, b in [Ng - W, Ng)) give b + W in [Ng, Ng + W): with
-  // T[2] + W < Ng the mask maps them to b - Ng + W and nothing else moves.
-  const int zoff = ZPAD ? ((b[2] - T0[2] + W) & (Ng - 1)) : W;
+  // T[2] + W < Ng subtracting Ng maps them to b - Ng + W and nothing else
+  // moves.
+  int zoff = W;
+  if (ZPAD) {
+    zoff = b[2] - T0[2] + W;
+    if (zoff >= Ng) zoff -= Ng;
+  }
   if (SCHEME == 3) w *= (1.0f / 216.0f);
   const float half = __builtin_copysignf(0.5f, w);   // window weights are >= 0
   // row offsets: one integer multiply per window cell of x and of y
@@ -751,15 +761,17 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   const bool zpad = T[2] + SCHEME + 1 < g.Ng;
   AT *ldsA = ldsq;
   AT *ldsB = ldsq + gcells;
-  // tile counts per axis are powers of two (Ng and the tile extents are)
-  const int sz = __builtin_ctz(g.ntile[2]), sy = __builtin_ctz(g.ntile[1]);
-  const int stpm = sz + sy + __builtin_ctz(g.ntile[0]);
+  // tile index -> (mock, x, y, z) tile; tile counts need not be powers of
+  // two (a grid of 3 * 2^k or 5 * 2^k cells has 3 or 5 times a power)
   const uint32_t tile = blockIdx.x;
-  const int mock = (int)(tile >> stpm);
-  const uint32_t tl = tile & ((1u << stpm) - 1u);
-  const int tzi = (int)(tl & (g.ntile[2] - 1));
-  const int tyi = (int)((tl >> sz) & (g.ntile[1] - 1));
-  const int txi = (int)(tl >> (sz + sy));
+  const uint32_t nz = (uint32_t)g.ntile[2], ny = (uint32_t)g.ntile[1];
+  const uint32_t tpm = (uint32_t)g.ntile[0] * ny * nz;
+  const int mock = (int)(tile / tpm);
+  const uint32_t tl = tile - (uint32_t)mock * tpm;
+  const uint32_t tyz = tl / nz;
+  const int tzi = (int)(tl - tyz * nz);
+  const int txi = (int)(tyz / ny);
+  const int tyi = (int)(tyz - (uint32_t)txi * ny);
   const int T0[3] = {txi * T[0], tyi * T[1], tzi * T[2]};
   (void)ntotal;
   {

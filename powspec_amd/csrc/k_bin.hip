@@ -6,7 +6,8 @@
 // (src/multipole.c:782-892), which depend on the box and bins only: it is run
 // once per geometry and cached.  The bin index of each mode is computed in
 // FP64 in the reference's operation order with contraction disabled, so the
-// integer counts are bit-identical to the reference's.
+// integer counts are bit-identical to the reference's; the FP64 sums are
+// added in a fixed order, so they are the same on every run.
 #include "psh_internal.h"
 
 #include <float.h>
@@ -34,32 +35,56 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
-__global__ void __launch_bounds__(256)
-geometry_kernel(int N, PshAxisTables ax, PshBins b,
-                unsigned long long *__restrict__ part_cnt,
-                double *__restrict__ part_km, double *__restrict__ part_lcnt) {
+// One workgroup per bin.  A mode (i, j, k) belongs to bin b when the
+// reference's FP64 expression puts it there (mode_bin above); the workgroup
+// of bin b visits every (i, j) row, bounds the k_z indices whose |k| can
+// fall in [kedge_b, kedge_b+1) -- widened by two indices against rounding --
+// and decides each candidate with mode_bin, so every mode is counted by
+// exactly one workgroup.  Threads take rows in a fixed stride and sum in
+// registers in row order; the block then adds the thread sums in a fixed
+// tree.  The results (cnt, km, lcnt) are therefore the same on every run.
+constexpr int kGeoThreads = 256;
+
+__global__ void __launch_bounds__(kGeoThreads)
+geometry_kernel(int N, PshAxisTables ax, PshBins b, unsigned long long *__restrict__ out_cnt,
+                double *__restrict__ out_km, double *__restrict__ out_lcnt) {
 #pragma clang fp contract(off)
-  extern __shared__ double sh[];
-  const int nb = b.nbin;
-  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(sh);
-  double *km = sh + nb;
-  double *lc = sh + 2 * nb;
-  for (int i = threadIdx.x; i < (2 + b.nl) * nb; i += blockDim.x) sh[i] = 0.0;
-  __syncthreads();
+  __shared__ double red[kGeoThreads];
+  __shared__ unsigned long long redc[kGeoThreads];
+  const int bin = blockIdx.x;
   const int M = N / 2;
+  // |k| range of the bin (log bins: the edges are 0.5 log10 |k|^2)
+  double klo = b.kedge0 + b.dk * bin, khi = b.kedge0 + b.dk * (bin + 1);
+  if (b.logscale) {
+    klo = pow(10.0, klo);
+    khi = pow(10.0, khi);
+  }
+  if (klo < 0) klo = 0;
+  const double klo2 = klo * klo, khi2 = khi * khi;
+  const double vz = sqrt(ax.ksq[2][1]);   // |k_z| of index 1
+  unsigned long long cnt = 0;
+  double km = 0.0;
+  double lc[PSH_MAX_NL];
+  for (int l = 0; l < PSH_MAX_NL; l++) lc[l] = 0.0;
   const size_t rows = (size_t)N * N;
-  for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+  for (size_t row = threadIdx.x; row < rows; row += kGeoThreads) {
     const int i = (int)(row / N), j = (int)(row - (size_t)i * N);
     const double kmij = ax.ksq[0][i] + ax.ksq[1][j];
-    for (int k = threadIdx.x; k <= M; k += blockDim.x) {
+    if (kmij > khi2 * (1 + 1e-12) + 1e-300) continue;
+    // candidate k_z indices: |k_z| in [sqrt(klo^2 - kxy^2), sqrt(khi^2 - kxy^2))
+    const double zlo = klo2 > kmij ? sqrt(klo2 - kmij) / vz : 0.0;
+    const double zhi = sqrt(fmax(khi2 - kmij, 0.0)) / vz;
+    int k0 = (int)floor(zlo) - 2, k1 = (int)ceil(zhi) + 2;
+    if (k0 < 0) k0 = 0;
+    if (k1 > M) k1 = M;
+    for (int k = k0; k <= k1; k++) {
       double kmod2 = kmij;
       if (k > 0) kmod2 = kmij + ax.ksq[2][k];
       double kmod;
-      int kb = mode_bin(kmod2, b, kmod);
-      if (kb < 0) continue;
+      if (mode_bin(kmod2, b, kmod) != bin) continue;
       const bool single = (k == 0) || (k == M);
-      atomicAdd(&cnt[kb], single ? 1ull : 2ull);
-      atomicAdd(&km[kb], single ? kmod : 2 * kmod);
+      cnt += single ? 1ull : 2ull;
+      km += single ? kmod : 2 * kmod;
       if (k == 0 && kmod2 == 0.0) continue;  // DC: no multipole weight
       // mu divides by |k| in both binnings (src/multipole.c:940, 966): with
       // log bins kmod is 0.5 log10 |k|^2, not |k|
@@ -69,18 +94,37 @@ geometry_kernel(int N, PshAxisTables ax, PshBins b,
       const double num = single ? 1.0 : 2.0;
       for (int l = 0; l < b.nl; l++) {
         if (!single && (b.poles[l] & 1)) continue;
-        atomicAdd(&lc[l * nb + kb], num * legendre(b.poles[l], mu));
+        lc[l] += num * legendre(b.poles[l], mu);
       }
     }
   }
+  // fixed-order tree over the threads
+  auto tree = [&](double v) -> double {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kGeoThreads / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+  };
+  redc[threadIdx.x] = cnt;
   __syncthreads();
-  const size_t o = (size_t)blockIdx.x;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    part_cnt[o * nb + i] = cnt[i];
-    part_km[o * nb + i] = km[i];
+  for (int h = kGeoThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) redc[threadIdx.x] += redc[threadIdx.x + h];
+    __syncthreads();
   }
-  for (int i = threadIdx.x; i < b.nl * nb; i += blockDim.x)
-    part_lcnt[o * b.nl * nb + i] = lc[i];
+  const double kms = tree(km);
+  if (threadIdx.x == 0) {
+    out_cnt[bin] = redc[0];
+    out_km[bin] = kms;
+  }
+  for (int l = 0; l < b.nl; l++) {
+    const double t = tree(lc[l]);
+    if (threadIdx.x == 0) out_lcnt[l * b.nbin + bin] = t;
+  }
 }
 
 // out[m][a] = sum over g of part[m][g][a].  A block takes 16 accumulators;
@@ -126,10 +170,13 @@ bounds_kernel(const double *__restrict__ xyzw, size_t n,
               double *__restrict__ partial) {
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
   double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double ws = 0.0;
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
        p += (size_t)gridDim.x * blockDim.x) {
     const double2 a = reinterpret_cast<const double2 *>(xyzw + 4 * p)[0];
-    const double z = xyzw[4 * p + 2];
+    const double2 zw = reinterpret_cast<const double2 *>(xyzw + 4 * p)[1];
+    const double z = zw.x;
+    ws += zw.y;
     mn[0] = fmin(mn[0], a.x);
     mx[0] = fmax(mx[0], a.x);
     mn[1] = fmin(mn[1], a.y);
@@ -137,11 +184,12 @@ bounds_kernel(const double *__restrict__ xyzw, size_t n,
     mn[2] = fmin(mn[2], z);
     mx[2] = fmax(mx[2], z);
   }
-  __shared__ double sh[6][256];
+  __shared__ double sh[7][256];
   for (int a = 0; a < 3; a++) {
     sh[a][threadIdx.x] = mn[a];
     sh[3 + a][threadIdx.x] = mx[a];
   }
+  sh[6][threadIdx.x] = ws;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
@@ -150,31 +198,23 @@ bounds_kernel(const double *__restrict__ xyzw, size_t n,
         sh[3 + a][threadIdx.x] =
             fmax(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + s]);
       }
+      sh[6][threadIdx.x] += sh[6][threadIdx.x + s];   // fixed tree: same sum every run
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+  static_assert(PSH_BOUNDS_STRIDE == 7, "bounds partial layout");
+  if (threadIdx.x < 7) partial[blockIdx.x * PSH_BOUNDS_STRIDE + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 }  // namespace
 
 hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
-                        unsigned long long *part_cnt, double *part_km,
-                        double *part_lcnt, int grid, unsigned long long *out_cnt,
-                        double *out_km, double *out_lcnt, hipStream_t s) {
-  const size_t shmem = sizeof(double) * (size_t)(2 + b.nl) * b.nbin;
-  if (shmem > 160 * 1024) return hipErrorInvalidValue;
-  if (shmem > 64 * 1024)
-    (void)hipFuncSetAttribute((const void *)geometry_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-  hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(256), shmem, s, Ng, ax, b,
-                     part_cnt, part_km, part_lcnt);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if ((e = launch_reduce<unsigned long long>(part_cnt, 1, grid, b.nbin, out_cnt, s)))
-    return e;
-  if ((e = launch_reduce<double>(part_km, 1, grid, b.nbin, out_km, s))) return e;
-  return launch_reduce<double>(part_lcnt, 1, grid, b.nl * b.nbin, out_lcnt, s);
+                        unsigned long long *out_cnt, double *out_km, double *out_lcnt,
+                        hipStream_t s) {
+  if (b.nbin < 1 || b.nl > PSH_MAX_NL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(geometry_kernel, dim3(b.nbin), dim3(kGeoThreads), 0, s, Ng, ax, b,
+                     out_cnt, out_km, out_lcnt);
+  return hipGetLastError();
 }
 
 hipError_t psh_reduce_partials(const double *partials, int nmock, int grid, int nacc,

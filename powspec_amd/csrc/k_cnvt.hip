@@ -218,21 +218,23 @@ extern "C" int psh_cnvt_device(const psh_cnvt *cv, psh_data *const *d_data, cons
   double zmin = DBL_MAX, zmax = -DBL_MAX;
   double *part = nullptr;
   const int pb = 1024;
-  if (hipMalloc(&part, sizeof(double) * 6 * pb) != hipSuccess) return PSH_ERR_MEMORY;
-  std::vector<double> h(6 * pb);
+  if (hipMalloc(&part, sizeof(double) * PSH_BOUNDS_STRIDE * pb) != hipSuccess)
+    return PSH_ERR_MEMORY;
+  std::vector<double> h(PSH_BOUNDS_STRIDE * pb);
   for (int a = 0; a < narr; a++) {
     if (!n[a]) continue;
     int blocks = (int)((n[a] + 255) / 256);
     if (blocks > pb) blocks = pb;
     if (psh_coord_bounds((const double *)d_data[a], n[a], part, blocks, s) != hipSuccess ||
-        hipMemcpy(h.data(), part, sizeof(double) * 6 * blocks, hipMemcpyDeviceToHost) !=
+        hipMemcpy(h.data(), part, sizeof(double) * PSH_BOUNDS_STRIDE * blocks,
+                  hipMemcpyDeviceToHost) !=
             hipSuccess) {
       (void)hipFree(part);
       return PSH_ERR_DEVICE;
     }
     for (int b = 0; b < blocks; b++) {
-      zmin = fmin(zmin, h[6 * b + 2]);
-      zmax = fmax(zmax, h[6 * b + 5]);
+      zmin = fmin(zmin, h[PSH_BOUNDS_STRIDE * b + 2]);
+      zmax = fmax(zmax, h[PSH_BOUNDS_STRIDE * b + 5]);
     }
   }
   (void)hipFree(part);

@@ -25,31 +25,45 @@ using namespace pshfft;
 
 namespace {
 
-template <int N>
+// Schedules of the column pass (y), chosen per workload by the tuner
+// (psh_autotune_fft): columns per tile, values per thread and radix plan.
+//   0  16 columns (one 128-byte line per row), 32 values, plan 0 (default)
+//   1  16 columns, 32 values, plan 1 (smaller radices first)
+//   2  16 columns, 32 values, plan 2 (one pass more)
+//   3  16 columns, 16 values (twice the threads), plan 0
+//   4  32 columns (two lines per row), 32 values, plan 0
+//   5  8 columns (half lines), 16 values, plan 0
+// 8-column tiles read at about 60 % of the bandwidth of 16-column ones at
+// N = 1024; the other shapes trade LDS, registers and waves per CU.
+constexpr int kColScheds = 6;
+template <int N, int S = 0>
 struct ColCfg {
-  // 16 columns: each row segment is one 128-byte line (8-column tiles reach
-  // about 60% of the read bandwidth of 16-column ones); 32 values per thread
-  static constexpr int C = 16;
-  static constexpr int T0 = C * N / 32;
-  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+  static constexpr int C = S == 4 ? 32 : (S == 5 ? 8 : 16);
+  static constexpr int VPT = (S == 3 || S == 5) ? 16 : 32;
+  static constexpr int PL = (S == 1 || S == 2) ? S : 0;
+  static constexpr int T0 = C * N / VPT;
+  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : pow2_ceil(T0));
   static constexpr size_t LDS = sizeof(float) * Lds<N, C, true>::SIZE;
+  static constexpr bool OK = LDS <= 160 * 1024 && (S != 2 || npasses(N, 2) > npasses(N, 0)) &&
+                             (S != 1 || radix(N, 0, 1) != radix(N, 0, 0));
 };
 
 template <int N>
 struct ZCfg {
   static constexpr int M = N / 2;
-  static constexpr int C = (4096 / M) > 0 ? 4096 / M : 1;
+  static constexpr int C = (4096 / M) > 0 ? pow2_floor(4096 / M) : 1;
   static constexpr int T = 256;
 };
 
 // Largest per-thread value count over the passes of a plan.
-template <int N, int C, int T, bool COLMAP, int P = 0>
+template <int N, int C, int T, bool COLMAP, int P = 0, int PL = 0>
 constexpr int max_values() {
-  if constexpr (P >= npasses(N)) {
+  if constexpr (P >= npasses(N, PL)) {
     return 1;
   } else {
-    constexpr int here = PassGeom<N, C, T, P, COLMAP>::NB * PassGeom<N, C, T, P, COLMAP>::R;
-    constexpr int rest = max_values<N, C, T, COLMAP, P + 1>();
+    constexpr int here =
+        PassGeom<N, C, T, P, COLMAP, PL>::NB * PassGeom<N, C, T, P, COLMAP, PL>::R;
+    constexpr int rest = max_values<N, C, T, COLMAP, P + 1, PL>();
     return here > rest ? here : rest;
   }
 }
@@ -101,6 +115,13 @@ __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
   float ir22 = rsqrtf(r2);
   float r22 = r2 * ir22;
   return ylm_eval(y.ell, y.m, y.norm, rz * ir3, r22 * ir3, rx * ir22, ry * ir22);
+}
+
+// M - k modulo M (the partner of element k of an M-point line).
+template <int M>
+__device__ __forceinline__ int mirror(int k) {
+  if constexpr (is_pow2(M)) return (M - k) & (M - 1);
+  else return k == 0 ? 0 : M - k;
 }
 
 // ---------------------------------------------------------------------------
@@ -167,7 +188,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       for (int u = 0; u < KPT; u++) {
         const int k = t + u * T;
         rk[c * KPT + u] = lds[Lds<M, C, false>::idx(c, k)];
-        rmk[c * KPT + u] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+        rmk[c * KPT + u] = lds[Lds<M, C, false>::idx(c, mirror<M>(k))];
       }
     if (t < C) rmid = lds[Lds<M, C, false>::idx(t, M / 2)];
   } else {
@@ -177,7 +198,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       int c = item / NP, k = item - c * NP;
       if (item < C * NP) {
         rk[it] = lds[Lds<M, C, false>::idx(c, k)];
-        rmk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+        rmk[it] = lds[Lds<M, C, false>::idx(c, mirror<M>(k))];
       }
     }
   }
@@ -203,7 +224,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       for (int u = 0; u < KPT; u++) {
         const int k = t + u * T;
         const float ik = lds[Lds<M, C, false>::idx(c, k)];
-        const float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+        const float imk = lds[Lds<M, C, false>::idx(c, mirror<M>(k))];
         const float r0 = rk[c * KPT + u], r1 = rmk[c * KPT + u];
         const float2 E = make_float2(0.5f * (r0 + r1), 0.5f * (ik - imk));
         const float2 O = make_float2(0.5f * (ik + imk), -0.5f * (r0 - r1));
@@ -240,7 +261,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       size_t line = line0 + c;
       if (line >= nlines) continue;
       float ik = lds[Lds<M, C, false>::idx(c, k)];
-      float imk = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+      float imk = lds[Lds<M, C, false>::idx(c, mirror<M>(k))];
       float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
       float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
       float2 WO = cmul(tw[k], O);
@@ -270,7 +291,7 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       const int c = item / NP, k = item - c * NP;
       if (item < C * NP) {
         ik[it] = lds[Lds<M, C, false>::idx(c, k)];
-        imk[it] = lds[Lds<M, C, false>::idx(c, (M - k) & (M - 1))];
+        imk[it] = lds[Lds<M, C, false>::idx(c, mirror<M>(k))];
       }
     }
     float2 *stage = reinterpret_cast<float2 *>(lds);
@@ -386,13 +407,14 @@ z_c2r_kernel(float2 *__restrict__ data, size_t nlines, size_t mstride_f, int H,
 // ---------------------------------------------------------------------------
 // strided column passes (y: OS = N*H, ES = H; x: OS = H, ES = N*H)
 // ---------------------------------------------------------------------------
-template <int N, bool INV, bool NT>
-__global__ void __launch_bounds__(ColCfg<N>::T)
+template <int N, bool INV, bool NT, int S>
+__global__ void __launch_bounds__((ColCfg<N, S>::T))
 col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
            size_t OS, size_t ES, size_t mstride_c, const float2 *__restrict__ tw,
            const int *__restrict__ ylive, int cxs) {
-  constexpr int C = ColCfg<N>::C, T = ColCfg<N>::T, M = N / 2;
-  constexpr int V = max_values<N, C, T, true>();
+  using CC = ColCfg<N, S>;
+  constexpr int C = CC::C, T = CC::T, M = N / 2, PL = CC::PL;
+  constexpr int V = max_values<N, C, T, true, 0, PL>();
   extern __shared__ float lds[];
   const size_t tile = blockIdx.x;
   const size_t per_mesh = (size_t)N * ntz;
@@ -408,16 +430,20 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
     for (int t = (q * C) >> cxs; t <= (q * C + ncv - 1) >> cxs; t++) any |= ylive[t] >= 0;
     if (!any) return;
   }
-  using RS = RowSplit<N, C, T, true>;
+  using RS = RowSplit<N, C, T, true, PL>;
   static_assert(PSH_PAD == 16 && RS::XSPAN < (1ull << 32), "lane offsets are 32-bit");
-  // columns q*C + c < H always lie inside the padded row; those past kz = M
-  // are read and discarded
+  // columns q*C + c < H always lie inside the padded row (the row holds
+  // M + 1 + 15 columns, so a tile of C columns starting at or before M ends
+  // inside it when C <= 16; 32-column tiles are clipped below); those past
+  // kz = M are read and discarded
   float2 v[V];
-  transform_rows<N, C, T, true, INV, 1>(v, lds, tw, [&](int r, int c, int j) -> float2 {
+  auto ld = [&](int r, int c, int j) -> float2 {
+    if (C > 16 && q * C + c >= M + 1 + 15) return make_float2(0.f, 0.f);
     const float2 x = mesh_load<NT>(row_elem<RS::BYTE32>(in + base + (size_t)(r * RS::LOAD) * ES,
                                                          (uint32_t)j * (uint32_t)ES + c));
     return c < ncv ? x : make_float2(0.f, 0.f);
-  });
+  };
+  transform_rows<N, C, T, true, INV, 1, decltype(ld), PL>(v, lds, tw, ld);
   // outputs k_y = i' (i = j + r STORE) past the column's live bound are
   // never read (the fused x pass skips their tiles): not stored
   int kylim = M;
@@ -425,13 +451,14 @@ col_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int ntz,
     const int cc = min(opaque_tid<T>() % C, ncv - 1);   // the thread's column in every pass
     kylim = ylive[(q * C + cc) >> cxs];
   }
-  store_last_rows<N, C, T, true>(v, [&](int r, int c, int j, float2 x) {
+  auto st = [&](int r, int c, int j, float2 x) {
     const int i = j + r * RS::STORE;
     const int ay = i <= M ? i : N - i;
     if (c < ncv && ay <= kylim)
       mesh_store<NT>(row_elem<RS::BYTE32>(out + base + (size_t)(r * RS::STORE) * ES,
                                           (uint32_t)j * (uint32_t)ES + c), x);
-  });
+  };
+  store_last_rows<N, C, T, true, decltype(st), PL>(v, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,7 +472,13 @@ struct XBinDev {
   PshBins bins;
   const float2 *phase;
   double *partials;
-  int nacc;
+  int nacc;        // accumulators of the launch's bin window (LDS)
+  int nacc_full;   // accumulators per workgroup row of partials (all bins)
+  int kb0, nbw;    // bin window [kb0, kb0 + nbw) of the launch
+  double xbound;   // (largest |contribution| / |v|) x N C, with margins
+  int losx0;       // the line of sight has no x component
+  int has_dc;      // replace the k = 0 mode by dcf
+  float dcf;
   int ell, m;
   float ynorm;
   float vec0f;   // 2 pi / L_x
@@ -481,12 +514,15 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
 // loop.
 template <int N, bool IL>
 struct XCfg {
-  static constexpr int C = IL ? (N <= 1024 ? 16 : 8) : (N >= 2048 ? 8 : 16);
+  static constexpr int C = N <= 1024 ? 16 : 8;
   // up to N = 256 the first pass's radix-16 butterflies set the values per
   // thread whatever T is: T = its butterfly count leaves no thread idle
   static constexpr bool SMALL = IL && N <= 256;
   static constexpr int T0 = SMALL ? C * N / radix(N, 0) : C * N / (IL ? 8 : 16);
-  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : T0);
+  // a power of two: grid sizes 3 2^k and 5 2^k round up, which leaves some
+  // threads idle in the power-of-two passes but keeps the values per
+  // thread (and the registers) of the power-of-two grids
+  static constexpr int T = T0 < 64 ? 64 : (T0 > 1024 ? 1024 : pow2_ceil(T0));
   // waves per SIMD the register allocation must allow: 4 (128 VGPRs) lets
   // workgroups share a CU and hide each other's barriers; the small
   // interlaced kernel holds 2 x 16 values per thread and gets 2 (256 VGPRs)
@@ -499,8 +535,25 @@ struct XCfg {
 // invariant over the persistent tile loop and stay live in registers.
 template <int N, bool IL>
 constexpr size_t xbin_static_lds() {
-  return sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
+  return sizeof(double) * 2 + sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
          sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
+}
+
+// LDS per accumulator of the x pass: the workgroup's FP64 sum and the
+// current tile's 64-bit integer sum.
+constexpr size_t kXbinAccBytes = sizeof(double) + sizeof(long long);
+constexpr size_t kXbinLdsMax = 160 * 1024;
+
+// Adds x * 2^-E, rounded to the nearest integer, to a 64-bit integer sum in
+// LDS.  sdown = 2^-E; |x| sdown < 2^51, so adding 1.5 * 2^52 leaves the
+// rounded integer in the low mantissa bits of the double (exact scaling,
+// one rounding) and the subtraction of the constant's bits recovers it.
+// Integer additions are associative: the tile's sums do not depend on the
+// order in which the lanes and waves arrive.
+__device__ __forceinline__ void acc_fixed(long long *ib, float x, double sdown) {
+  const double y = __builtin_fma((double)x, sdown, 6755399441055744.0);
+  const long long q = __double_as_longlong(y) - 0x4338000000000000LL;
+  atomicAdd(reinterpret_cast<unsigned long long *>(ib), (unsigned long long)q);
 }
 
 // Bin of a mode from its FP32 |k|, kmodf: -1 outside the bins, -2 when
@@ -534,9 +587,15 @@ __device__ __forceinline__ int bin_exact(const PshBins &b, double ksqx, double k
 //     (src/multipole.c:476-484); each mode reduced to one float -- |F|^2, or
 //     Re(F0 conj F) for l > 0 survey fields -- written to LDS in natural
 //     order, F itself stored for the l = 0 survey field;
-//  2. (LDS, one loop) bin, alias correction, half-space multiplicity and
-//     Legendre weights, accumulated in per-workgroup FP64 LDS bins
-//     (src/multipole.c:147-177, src/mp_template.c:107-139).
+//  2. (LDS, one loop over the pairs of modes k_x = +-i of the column, which
+//     share |k|, the bin and, for a line of sight without an x component,
+//     mu; only the alias factor differs, src/multipole.c:50-58) bin, alias
+//     correction, half-space multiplicity and Legendre weights
+//     (src/multipole.c:147-177, src/mp_template.c:107-139), accumulated as
+//     64-bit integers at a power-of-two scale set by the tile's largest
+//     |v|, so that every sum of the tile fits; after the tile each
+//     accumulator's integer is converted and added to its FP64 sum by the
+//     thread that owns it.  The sums are therefore the same on every run.
 // xtab[i] = {alias_x, ksq_x, phase_x.re, phase_x.im} in LDS.
 template <int N, int EPI, bool INTLACE, bool NT>
 __global__ void __launch_bounds__((XCfg<N, INTLACE>::T), (XCfg<N, INTLACE>::WAVES))
@@ -548,11 +607,17 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   extern __shared__ double dyn[];
   double *bins = dyn;
   const int nacc_pad = (a.nacc + 1) & ~1;
-  double *ksqx = bins + nacc_pad;
+  long long *ibins = reinterpret_cast<long long *>(bins + nacc_pad);
+  unsigned *tmaxp = reinterpret_cast<unsigned *>(ibins + nacc_pad);
+  double *ksqx = bins + 2 * nacc_pad + 2;
   float4 *xtab = reinterpret_cast<float4 *>(ksqx + N);
   float2 *twl = reinterpret_cast<float2 *>(xtab + N);
   float *lds = reinterpret_cast<float *>(twl + N);
-  for (int i = threadIdx.x; i < a.nacc; i += T) bins[i] = 0.0;
+  for (int i = threadIdx.x; i < a.nacc; i += T) {
+    bins[i] = 0.0;
+    ibins[i] = 0;
+  }
+  if (threadIdx.x == 0) *tmaxp = 0u;
   for (int i = threadIdx.x; i < N; i += T) {
     const float2 ph = a.phase[i];
     xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
@@ -632,6 +697,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     // phase 1
     float2 pyz = make_float2(1.f, 0.f);
     if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
+    float vmax = 0.f;
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
       const int bt = opaque_tid<T>() + qq * T;
@@ -650,6 +716,11 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           if (EPI == PSH_EPI_SIM) F = make_float2(F.x + t.x, F.y + t.y);
           else F = make_float2(0.5f * (F.x + t.x), 0.5f * (F.y + t.y));
         }
+        // the field's k = 0 mode is the sum of its weights (the windows sum
+        // to 1): data - alpha randoms cancel there, so the exact FP64 sum
+        // replaces the FP32 transform's rounding-dominated value
+        if (EPI == PSH_EPI_SURVEY0 && a.has_dc && y == 0 && kz == 0 && i == 0)
+          F = make_float2(a.dcf, 0.f);
         // element (i = j + r NS, c): row base uniform, lane offset j ES + c
         const size_t rbase = base + (size_t)(r * RS::STORE) * ES;
         const uint32_t lo = (uint32_t)j * ES32 + c;
@@ -660,15 +731,32 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         } else {
           v = F.x * F.x + F.y * F.y;
         }
-        if (EPI == PSH_EPI_SURVEY0) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
+        if (EPI == PSH_EPI_SURVEY0 && a.store) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
         lds[L::idx(c, j) + L::off(r * G::NS)] = v;   // i = j + r NS
+        vmax = fmaxf(vmax, fabsf(v));
       }
     }
+    // largest |v| of the tile (non-negative floats order as their bits)
+    if (vmax > 0.f) atomicMax(tmaxp, __float_as_uint(vmax));
     __syncthreads();
     const size_t next = next_live(tile + gridDim.x);
     if (next < ntiles) load_tile(next);
 
-    // phase 2: column c, rows i = threadIdx.x / C + k T / C
+    // phase 2: column c, pairs (i, N - i) with i = threadIdx.x / C + k T / C
+    // up to M.  The tile's integer sums take the scale 2^E with
+    // tmax xbound < 2^(E + 61): every contribution is below tmax xbound /
+    // (N C) and an accumulator takes at most N C of them, so no sum leaves
+    // 2^61 and each contribution stays below 2^51 (acc_fixed).
+    int E = 0;
+    {
+      const float tmax = __uint_as_float(*tmaxp);
+      if (tmax > 0.f) {
+        int e;
+        (void)frexp((double)tmax * a.xbound, &e);
+        E = e - 61;
+      }
+    }
+    const double sdown = ldexp(1.0, -E);
     if (colok) {
       const float ksqyz = a.ax.ksqf[1][y] + a.ax.ksqf[2][kz];
       const float dyz = a.ax.kvf[1][y] * b.losf[1] + a.ax.kvf[2][kz] * b.losf[2];
@@ -684,47 +772,99 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         kdy = (float)a.ax.kd[1][y];
         kdz = (float)a.ax.kd[2][kz];
       }
+      constexpr int S = T / C;   // rows between a thread's modes
       const int i0 = opaque_tid<T>() / C;
       const float *vrow = lds + L::idx(c, i0);
-#pragma unroll 4
-      for (int k = 0; k < N * C / T; k++) {
-        const int i = i0 + k * (T / C);
+#pragma unroll 2
+      for (int k = 0; k <= M / S; k++) {
+        const int i = i0 + k * S;
+        if (i > M) break;
         const float4 xt = xtab[i];
-        const float v = vrow[L::off(k * (T / C))];
+        const float v = vrow[L::off(k * S)];
         const float k2f = xt.y + ksqyz;
         const float rs = rsqrtf(k2f);
         int kb = bin_fast(b, k2f * rs);
         if (kb == -2) kb = bin_exact(b, ksqx[i], ksqy, ksqz, kz);
         if (dcrow && i == 0) kb = -1;   // DC mode skipped
-        if (kb < 0) continue;
+        kb -= a.kb0;
+        if (kb < 0 || kb >= a.nbw) continue;
+        // the partner k_x = -i (none for i = 0 and the Nyquist index M)
+        const bool pair = i != 0 && i != M;
+        float vp = 0.f, ap = 0.f;
+        if (pair) {
+          vp = lds[L::idx(c, N - i)];
+          ap = xtab[N - i].x;
+        }
         const float w = v * (xt.x * wyz);
+        const float wp = vp * (ap * wyz);
         if (EPI == PSH_EPI_SIM) {
-          const float kvx = (float)(i <= M ? i : i - N) * a.vec0f;
+          const float kvx = (float)i * a.vec0f;
           const float mu = (kvx * b.losf[0] + dyz) * rs;
-          const float x2 = mu * mu;
+          const float mup = (-kvx * b.losf[0] + dyz) * rs;
+          // accumulator of pole slot l, bin kb and |mu| cell c:
+          // (l nbw + kb) nmu + c (nmu = 1: no cells)
+          const int nmu = b.nmu;
+          const int stride = a.nbw * nmu;
+          int cm = 0, cp = 0;
+          if (nmu > 1) {
+            cm = min((int)(fabsf(mu) * (float)nmu), nmu - 1);
+            cp = min((int)(fabsf(mup) * (float)nmu), nmu - 1);
+          }
+          const int base = kb * nmu + cm;
           if (a.even024) {
-            // L_0 = 1, L_2 and L_4 by Horner in mu^2: the generic loop's
-            // arithmetic without its zero coefficients
+            // L_0 = 1, L_2 and L_4 by Horner in mu^2
+            const float x2 = mu * mu;
             const float p2 = __builtin_fmaf(1.5f, x2, -0.5f);
             const float p4 = __builtin_fmaf(__builtin_fmaf(4.375f, x2, -3.75f), x2, 0.375f);
-            atomicAdd(&bins[kb], (double)w);
-            atomicAdd(&bins[b.nbin + kb], (double)(w * p2));
-            atomicAdd(&bins[2 * b.nbin + kb], (double)(w * p4));
+            if (a.losx0) {   // both modes of the pair share mu, and so the cell
+              const float s0 = w + wp;
+              acc_fixed(&ibins[base], s0, sdown);
+              acc_fixed(&ibins[stride + base], s0 * p2, sdown);
+              acc_fixed(&ibins[2 * stride + base], s0 * p4, sdown);
+              continue;
+            }
+            const float y2 = mup * mup;
+            const float q2 = __builtin_fmaf(1.5f, y2, -0.5f);
+            const float q4 = __builtin_fmaf(__builtin_fmaf(4.375f, y2, -3.75f), y2, 0.375f);
+            if (cm == cp) {
+              acc_fixed(&ibins[base], w + wp, sdown);
+              acc_fixed(&ibins[stride + base], w * p2 + wp * q2, sdown);
+              acc_fixed(&ibins[2 * stride + base], w * p4 + wp * q4, sdown);
+            } else {
+              const int basep = kb * nmu + cp;
+              acc_fixed(&ibins[base], w, sdown);
+              acc_fixed(&ibins[stride + base], w * p2, sdown);
+              acc_fixed(&ibins[2 * stride + base], w * p4, sdown);
+              acc_fixed(&ibins[basep], wp, sdown);
+              acc_fixed(&ibins[stride + basep], wp * q2, sdown);
+              acc_fixed(&ibins[2 * stride + basep], wp * q4, sdown);
+            }
             continue;
           }
+          const float x2 = mu * mu, y2 = mup * mup;
+          const int basep = kb * nmu + cp;
 #pragma unroll
           for (int l = 0; l < PSH_MAX_NL; l++) {
             if (l >= b.nl) break;
             if (!((lmask >> l) & 1)) continue;
             const float *cf = a.legc[l];
             float pl = ((cf[3] * x2 + cf[2]) * x2 + cf[1]) * x2 + cf[0];
-            if ((a.oddmask >> l) & 1) pl *= mu;
-            atomicAdd(&bins[l * b.nbin + kb], (double)(w * pl));
+            float ql = ((cf[3] * y2 + cf[2]) * y2 + cf[1]) * y2 + cf[0];
+            if ((a.oddmask >> l) & 1) {
+              pl *= mu;
+              ql *= mup;
+            }
+            if (cm == cp) {
+              acc_fixed(&ibins[l * stride + base], w * pl + wp * ql, sdown);
+            } else {
+              acc_fixed(&ibins[l * stride + base], w * pl, sdown);
+              acc_fixed(&ibins[l * stride + basep], wp * ql, sdown);
+            }
           }
         } else if (EPI == PSH_EPI_SURVEY0) {
-          atomicAdd(&bins[kb], (double)w);
+          acc_fixed(&ibins[kb], w + wp, sdown);
         } else {  // Y_lm(k) weight; kx = ky = 0 is added unweighted (mp_template.c:121)
-          float yw = 1.f;
+          float yw = 1.f, ywp = 0.f;
           const float kdx = (float)a.ax.kd[0][i];
           const float k2 = kdx * kdx + kdy * kdy;
           if (k2 != 0.f) {
@@ -732,16 +872,38 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
             const float ik22 = rsqrtf(k2);
             yw = ylm_eval(a.ell, a.m, a.ynorm, kdz * ik3, k2 * ik22 * ik3, kdx * ik22,
                           kdy * ik22);
+            if (pair)
+              ywp = ylm_eval(a.ell, a.m, a.ynorm, kdz * ik3, k2 * ik22 * ik3, -kdx * ik22,
+                             kdy * ik22);
           }
-          atomicAdd(&bins[kb], (double)(w * yw));
+          acc_fixed(&ibins[kb], w * yw + wp * ywp, sdown);
         }
       }
     }
     __syncthreads();
+    // fold the tile's integer sums into the FP64 sums (each accumulator by
+    // the thread that also writes it out), then clear them for the next tile
+    const double sup = ldexp(1.0, E);
+    for (int q2 = threadIdx.x; q2 < a.nacc; q2 += T) {
+      const long long t = ibins[q2];
+      if (t) {
+        bins[q2] += (double)t * sup;
+        ibins[q2] = 0;
+      }
+    }
+    if (threadIdx.x == 0) *tmaxp = 0u;
+    __syncthreads();
     tile = next;
   }
-  double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc;
-  for (int i = threadIdx.x; i < a.nacc; i += T) out[i] = bins[i];
+  // the window's accumulators into the workgroup's row of all bins
+  double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc_full;
+  const int nmu = EPI == PSH_EPI_SIM ? b.nmu : 1;
+  const int stride = a.nbw * nmu;
+  for (int i = threadIdx.x; i < a.nacc; i += T) {
+    const int l = i / stride;
+    const int rem = i - l * stride;
+    out[(size_t)(l * b.nbin + a.kb0) * nmu + rem] = bins[i];
+  }
 }
 
 // ---- dispatch ------------------------------------------------------------------
@@ -759,6 +921,17 @@ hipError_t dispatch_n(int N, Args &&...args) {
     case 512: return F<512>::run(args...);
     case 1024: return F<1024>::run(args...);
     case 2048: return F<2048>::run(args...);
+    case 48: return F<48>::run(args...);
+    case 96: return F<96>::run(args...);
+    case 192: return F<192>::run(args...);
+    case 384: return F<384>::run(args...);
+    case 768: return F<768>::run(args...);
+    case 1536: return F<1536>::run(args...);
+    case 80: return F<80>::run(args...);
+    case 160: return F<160>::run(args...);
+    case 320: return F<320>::run(args...);
+    case 640: return F<640>::run(args...);
+    case 1280: return F<1280>::run(args...);
     default: return hipErrorInvalidValue;
   }
 }
@@ -801,35 +974,62 @@ struct ZC2R {
 
 template <int N>
 struct Col {
-  static hipError_t run(const float2 *in, float2 *out, int nmesh, size_t mstride,
-                        int axis, const float2 *tw, int inverse, hipStream_t s,
-                        const int *ylive) {
-    constexpr int C = ColCfg<N>::C;
-    constexpr int CX = XCfg<N, true>::C;
-    static_assert(CX == XCfg<N, false>::C && (CX & (CX - 1)) == 0, "one x tile width");
-    int cxs = 0;
-    while ((1 << cxs) < CX) cxs++;
-    const int H = N / 2 + PSH_PAD;
-    const int ntz = (N / 2 + 1 + C - 1) / C;
-    const size_t OS = axis == 1 ? (size_t)N * H : (size_t)H;
-    const size_t ES = axis == 1 ? (size_t)H : (size_t)N * H;
-    const unsigned blocks = (unsigned)((size_t)N * ntz * nmesh);
-    constexpr size_t shmem = ColCfg<N>::LDS;
-    static_assert(shmem <= 160 * 1024, "column tile exceeds the LDS");
-    auto go = [&](auto ntc) {
-      constexpr bool NT = decltype(ntc)::value;
-      auto k = inverse ? col_kernel<N, true, NT> : col_kernel<N, false, NT>;
-      if (shmem > 64 * 1024)
-        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shmem);
-      hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N>::T), shmem, s, in, out, ntz, OS, ES,
-                         mstride / 2, tw, ylive, cxs);
-      return hipGetLastError();
-    };
-    if constexpr (stream_mesh<N>()) return go(std::true_type{});
-    else
-      return stream_bytes(sizeof(float) * mstride * nmesh) ? go(std::true_type{})
-                                                           : go(std::false_type{});
+  template <int S>
+  static hipError_t run_s(const float2 *in, float2 *out, int nmesh, size_t mstride, int axis,
+                          const float2 *tw, int inverse, hipStream_t s, const int *ylive) {
+    if constexpr (!ColCfg<N, S>::OK) {
+      return hipErrorInvalidValue;
+    } else {
+      constexpr int C = ColCfg<N, S>::C;
+      constexpr int CX = XCfg<N, true>::C;
+      static_assert(CX == XCfg<N, false>::C && (CX & (CX - 1)) == 0, "one x tile width");
+      int cxs = 0;
+      while ((1 << cxs) < CX) cxs++;
+      const int H = N / 2 + PSH_PAD;
+      const int ntz = (N / 2 + 1 + C - 1) / C;
+      const size_t OS = axis == 1 ? (size_t)N * H : (size_t)H;
+      const size_t ES = axis == 1 ? (size_t)H : (size_t)N * H;
+      const unsigned blocks = (unsigned)((size_t)N * ntz * nmesh);
+      constexpr size_t shmem = ColCfg<N, S>::LDS;
+      static_assert(shmem <= 160 * 1024, "column tile exceeds the LDS");
+      auto go = [&](auto ntc) {
+        constexpr bool NT = decltype(ntc)::value;
+        auto k = inverse ? col_kernel<N, true, NT, S> : col_kernel<N, false, NT, S>;
+        if (shmem > 64 * 1024)
+          (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shmem);
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(ColCfg<N, S>::T), shmem, s, in, out, ntz, OS,
+                           ES, mstride / 2, tw, ylive, cxs);
+        return hipGetLastError();
+      };
+      if constexpr (stream_mesh<N>()) return go(std::true_type{});
+      else
+        return stream_bytes(sizeof(float) * mstride * nmesh) ? go(std::true_type{})
+                                                             : go(std::false_type{});
+    }
+  }
+  static hipError_t run(const float2 *in, float2 *out, int nmesh, size_t mstride, int axis,
+                        const float2 *tw, int inverse, hipStream_t s, const int *ylive,
+                        int sched) {
+    switch (sched) {
+      case 1: return run_s<1>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+      case 2: return run_s<2>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+      case 3: return run_s<3>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+      case 4: return run_s<4>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+      case 5: return run_s<5>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+      default: return run_s<0>(in, out, nmesh, mstride, axis, tw, inverse, s, ylive);
+    }
+  }
+  static int ok(int sched) {
+    switch (sched) {
+      case 0: return ColCfg<N, 0>::OK;
+      case 1: return ColCfg<N, 1>::OK;
+      case 2: return ColCfg<N, 2>::OK;
+      case 3: return ColCfg<N, 3>::OK;
+      case 4: return ColCfg<N, 4>::OK;
+      case 5: return ColCfg<N, 5>::OK;
+      default: return 0;
+    }
   }
 };
 
@@ -840,7 +1040,7 @@ struct XBin {
     const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
-    const size_t shmem = sizeof(double) * ((d.nacc + 1) & ~1) +
+    const size_t shmem = kXbinAccBytes * ((d.nacc + 1) & ~1) +
                          (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
     const dim3 g(grid, nmock), b(il ? XCfg<N, true>::T : XCfg<N, false>::T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
@@ -906,33 +1106,64 @@ hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh, size_t mstride_floats
 }
 
 hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
-                     const float2 *tw, int inverse, hipStream_t s, const int *ylive) {
+                     const float2 *tw, int inverse, hipStream_t s, const int *ylive,
+                     int sched) {
   return dispatch_n<Col>(Ng, (const float2 *)data, data, nmesh, mstride_floats, 1, tw,
-                         inverse, s, inverse ? nullptr : ylive);
+                         inverse, s, inverse ? nullptr : ylive, sched);
 }
 
 hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
                      size_t mstride_floats, const float2 *tw, int inverse,
                      hipStream_t s) {
   return dispatch_n<Col>(Ng, in, out, nmesh, mstride_floats, 0, tw, inverse, s,
-                         (const int *)nullptr);
+                         (const int *)nullptr, 0);
+}
+
+namespace {
+template <int N>
+struct ColOk {
+  static hipError_t run(int sched, int *ok) {
+    *ok = Col<N>::ok(sched);
+    return hipSuccess;
+  }
+};
+}  // namespace
+
+static_assert(kColScheds == PSH_FFT_Y_SCHEDS, "schedule count");
+
+int psh_fft_y_sched_ok(int Ng, int sched) {
+  int ok = 0;
+  if (sched < 0 || sched >= kColScheds) return 0;
+  return dispatch_n<ColOk>(Ng, sched, &ok) == hipSuccess && ok;
 }
 
 int psh_xbin_nacc(const PshBins &b, int epi) {
-  return epi == PSH_EPI_SIM ? b.nl * b.nbin : b.nbin;
+  return epi == PSH_EPI_SIM ? b.nl * b.nbin * b.nmu : b.nbin;
 }
 
-int psh_xbin_tile_cols(int Ng) { return Ng >= 2048 ? XCfg<2048, true>::C : XCfg<1024, true>::C; }
+int psh_xbin_tile_cols(int Ng) { return Ng > 1024 ? XCfg<2048, true>::C : XCfg<1024, true>::C; }
 
 namespace {
 template <int N>
 struct XBinLds {
   static size_t run(bool il, int nacc) {
-    return sizeof(double) * ((nacc + 1) & ~1) +
+    return kXbinAccBytes * ((nacc + 1) & ~1) +
            (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
   }
 };
 }  // namespace
+
+bool psh_grid_supported(int Ng) {
+  switch (Ng) {
+    case 4: case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024:
+    case 2048:
+    case 48: case 96: case 192: case 384: case 768: case 1536:
+    case 80: case 160: case 320: case 640: case 1280:
+      return true;
+    default:
+      return false;
+  }
+}
 
 size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc) {
   switch (Ng) {
@@ -946,8 +1177,27 @@ size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc) {
     case 512: return XBinLds<512>::run(il, nacc);
     case 1024: return XBinLds<1024>::run(il, nacc);
     case 2048: return XBinLds<2048>::run(il, nacc);
+    case 48: return XBinLds<48>::run(il, nacc);
+    case 96: return XBinLds<96>::run(il, nacc);
+    case 192: return XBinLds<192>::run(il, nacc);
+    case 384: return XBinLds<384>::run(il, nacc);
+    case 768: return XBinLds<768>::run(il, nacc);
+    case 1536: return XBinLds<1536>::run(il, nacc);
+    case 80: return XBinLds<80>::run(il, nacc);
+    case 160: return XBinLds<160>::run(il, nacc);
+    case 320: return XBinLds<320>::run(il, nacc);
+    case 640: return XBinLds<640>::run(il, nacc);
+    case 1280: return XBinLds<1280>::run(il, nacc);
     default: return 0;
   }
+}
+
+int psh_xbin_window_bins(int Ng, bool il, int per_bin) {
+  const size_t fixed = psh_xbin_lds_bytes(Ng, il, 0);
+  if (fixed == 0 || fixed >= kXbinLdsMax || per_bin < 1) return 0;
+  long nb = (long)((kXbinLdsMax - fixed) / kXbinAccBytes) / per_bin;
+  while (nb > 0 && psh_xbin_lds_bytes(Ng, il, (int)(nb * per_bin)) > kXbinLdsMax) nb--;
+  return nb > (1 << 30) ? (1 << 30) : (int)nb;
 }
 
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
@@ -962,7 +1212,28 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.bins = a.bins;
   d.phase = a.phase;
   d.partials = a.partials;
-  d.nacc = psh_xbin_nacc(a.bins, a.epi);
+  d.nacc_full = psh_xbin_nacc(a.bins, a.epi);
+  const int per_bin = a.epi == PSH_EPI_SIM ? a.bins.nl * a.bins.nmu : 1;
+  d.kb0 = a.nbw > 0 ? a.kb0 : 0;
+  d.nbw = a.nbw > 0 ? a.nbw : a.bins.nbin;
+  if (d.kb0 < 0 || d.kb0 + d.nbw > a.bins.nbin) return hipErrorInvalidValue;
+  d.nacc = per_bin * d.nbw;
+  // bound of |contribution| / |v| per mode pair, times N C (at least 1024):
+  // alias factors at their largest, multiplicity 2, the interlace 1/4 of
+  // the box epilogue, |L_l| or |Y_lm| (real form, l <= 6) below 1.01 and
+  // 2, two modes per pair, a 2 % margin for FP32 rounding
+  {
+    const int C = psh_xbin_tile_cols(Ng);
+    double w = (double)a.ax.almax[0] * a.ax.almax[1] * a.ax.almax[2] * 2.0;
+    if (a.epi == PSH_EPI_SIM && a.B) w *= 0.25;
+    w *= a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01;
+    w *= 2.0 * 1.02;
+    const double nc = (double)Ng * C;
+    d.xbound = w * (nc > 1024.0 ? nc : 1024.0);
+  }
+  d.losx0 = a.bins.losf[0] == 0.f;
+  d.has_dc = a.has_dc;
+  d.dcf = (float)a.dc;
   d.ell = a.ell;
   d.m = a.m;
   d.ynorm = a.ynorm;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "powspec_hip.h"
+#include "psh_internal.h"
 
 namespace {
 
@@ -322,6 +323,140 @@ unsigned blocks_for(int B) { return (unsigned)((B + 16 * kWaves - 1) / (16 * kWa
 }  // namespace
 
 extern "C" int psh_mlp_param_count(void) { return NPAR; }
+
+namespace {
+
+// Features of the learned alias correction, one row per (k bin, |mu| cell)
+// of a plain (non-interlaced) box run, from the run's own raw cell sums
+// S[l][b][c] (pole slot 0 the monopole):
+//   [0]      k_b / k_Ny (bin centre)
+//   [1]      centre of the |mu| cell, (c + 1/2) / nmu
+//   [2..5]   one-hot assignment scheme
+//   [6..21]  log10 of the un-subtracted monopole over the shot noise,
+//            averaged over 16 equal bands of [k_Ny / 4, k_Ny] (an empty band
+//            takes the populated bin nearest its centre): the spectrum's
+//            shape towards the Nyquist frequency and its level
+//   [22]     log10 of the bin's own un-subtracted monopole over the shot noise
+//   [23]     log10(nmu S_0[b][c] / sum_c S_0[b][c]), clipped to [-2, 2]: the
+//            cell's share of the bin (|mu| is uniform over directions)
+// Bins without modes or power get zero rows (and no correction).
+__global__ void __launch_bounds__(256)
+alias_features_kernel(const double *__restrict__ cells, const unsigned long long *__restrict__ cnt,
+                      int nbin, int nmu, double norm, double shot, double kny, double kedge0,
+                      double dk, int logscale, int scheme, float *__restrict__ X) {
+  __shared__ float summ[16];
+  const int t = threadIdx.x;
+  auto kcen = [&](int b) {
+    const double c = kedge0 + dk * (b + 0.5);
+    return logscale ? pow(10.0, c) : c;
+  };
+  auto pbin = [&](int b) -> double {   // un-subtracted monopole / shot, or 0
+    if (!cnt[b]) return 0.0;
+    double s = 0.0;
+    for (int c = 0; c < nmu; c++) s += cells[(size_t)b * nmu + c];
+    const double p = s / (norm * (double)cnt[b]) / shot;
+    return p > 0 ? p : 0.0;
+  };
+  if (t < 16) {
+    const double e0 = kny * (0.25 + 0.75 * t / 16.0), e1 = kny * (0.25 + 0.75 * (t + 1) / 16.0);
+    double sum = 0.0;
+    int n = 0;
+    int best = -1;
+    double bestd = 1e300;
+    const double mid = 0.5 * (e0 + e1);
+    for (int b = 0; b < nbin; b++) {
+      const double p = pbin(b);
+      if (p <= 0) continue;
+      const double k = kcen(b);
+      if (k >= e0 && k < e1) {
+        sum += p;
+        n++;
+      }
+      const double d = fabs(k - mid);
+      if (d < bestd) {
+        bestd = d;
+        best = b;
+      }
+    }
+    const double v = n ? sum / n : (best >= 0 ? pbin(best) : 1.0);
+    summ[t] = (float)log10(v);
+  }
+  __syncthreads();
+  for (int r = t; r < nbin * nmu; r += 256) {
+    const int b = r / nmu, c = r - b * nmu;
+    float *x = X + (size_t)r * IN;
+    for (int q = 0; q < IN; q++) x[q] = 0.f;
+    const double p = pbin(b);
+    if (p <= 0) continue;
+    double sb = 0.0;
+    for (int q = 0; q < nmu; q++) sb += cells[(size_t)b * nmu + q];
+    const double sc = cells[(size_t)b * nmu + c];
+    x[0] = (float)(kcen(b) / kny);
+    x[1] = (float)((c + 0.5) / nmu);
+    x[2 + (scheme & 3)] = 1.f;
+    for (int q = 0; q < 16; q++) x[6 + q] = summ[q];
+    x[22] = (float)log10(p);
+    double share = (sb > 0 && sc > 0) ? log10(nmu * sc / sb) : -2.0;
+    x[23] = (float)fmin(2.0, fmax(-2.0, share));
+  }
+}
+
+// Corrected raw sums: out[l][b] = sum_c exp(f[b][c]) S[l][b][c], f the mean
+// over the nnet networks' outputs (rows without features keep factor 1).
+__global__ void __launch_bounds__(256)
+alias_apply_kernel(const float *__restrict__ Y, int nnet, int rows, const float *__restrict__ X,
+                   const double *__restrict__ cells, int nl, int nbin, int nmu,
+                   double *__restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nl * nbin) return;
+  const int l = q / nbin, b = q - l * nbin;
+  double s = 0.0;
+  for (int c = 0; c < nmu; c++) {
+    const int r = b * nmu + c;
+    double f = 0.0;
+    if (X[(size_t)r * IN] != 0.f) {
+      for (int n = 0; n < nnet; n++) f += Y[(size_t)n * rows + r];
+      f /= nnet;
+    }
+    s += exp(f) * cells[((size_t)l * nbin + b) * nmu + c];
+  }
+  out[q] = s;
+}
+
+}  // namespace
+
+hipError_t psh_alias_features(const double *cells, const unsigned long long *cnt, int nbin,
+                              int nmu, double norm, double shot, double kny, double kedge0,
+                              double dk, int logscale, int scheme, float *X, hipStream_t s) {
+  hipLaunchKernelGGL(alias_features_kernel, dim3(1), dim3(256), 0, s, cells, cnt, nbin, nmu,
+                     norm, shot, kny, kedge0, dk, logscale, scheme, X);
+  return hipGetLastError();
+}
+
+hipError_t psh_alias_apply(const float *Y, int nnet, const float *X, const double *cells, int nl,
+                           int nbin, int nmu, double *out, hipStream_t s) {
+  const int n = nl * nbin;
+  hipLaunchKernelGGL(alias_apply_kernel, dim3((n + 255) / 256), dim3(256), 0, s, Y, nnet,
+                     nbin * nmu, X, cells, nl, nbin, nmu, out);
+  return hipGetLastError();
+}
+
+hipError_t psh_mlp_forward_async(const float *d_params, const float *d_X, int B, float *d_Y,
+                                 hipStream_t s) {
+  if (B <= 0) return hipSuccess;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int ntile = (B + 15) / 16;
+  int grid = (ntile + kInferWaves - 1) / kInferWaves;
+  if (grid > 2 * ncu) grid = 2 * ncu;
+  const size_t lds = sizeof(InferLds);
+  (void)hipFuncSetAttribute((const void *)mlp_infer_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, s, d_params,
+                     d_X, B, d_Y);
+  return hipGetLastError();
+}
 
 extern "C" int psh_mlp_forward(const float *d_params, const float *d_X, int B, float *d_Y) {
   if (!d_params || !d_X || !d_Y || B < 0) return PSH_ERR_ARG;

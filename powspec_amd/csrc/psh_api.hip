@@ -97,6 +97,7 @@ struct psh_ctx {
 
   float *meshA = nullptr, *meshB = nullptr, *meshS = nullptr;
   int tile_req[3] = {0, 0, 0};   // deposit tiles set by psh_ctx_set_tiles / tuning
+  int ysched = 0;                // schedule of the forward y passes (tuning)
 
   // assignment scratch
   PshAssignGeom ag{};
@@ -131,9 +132,18 @@ struct psh_ctx {
   std::vector<double> km, lcnt;
   unsigned long long *d_gcnt = nullptr;
   double *d_gkm = nullptr, *d_glcnt = nullptr;
-  unsigned long long *d_gpart_cnt = nullptr;
-  double *d_gpart_km = nullptr, *d_gpart_lcnt = nullptr;
-  int ggrid = 0;
+  // bins per launch of the fused x pass (its LDS holds a window of the bins)
+  int xwin_sim = 0, xwin_survey = 0;
+
+  // |mu| cells of box runs and the learned alias correction of plain boxes
+  int nmu = 1;
+  std::vector<double> cells;          // raw sums of the last box run [nl][nbin][nmu]
+  double last_norm = 0, last_shot = 0;
+  float *d_mlp = nullptr;             // [mlp_nnet][psh_mlp_param_count()]
+  int mlp_nnet = 0;
+  float *d_ax = nullptr, *d_ay = nullptr;   // feature rows, network outputs
+  double *d_acorr = nullptr;          // corrected raw sums [nl][nbin]
+  size_t arows_cap = 0;
 
   // binning
   double *partials = nullptr;
@@ -174,6 +184,7 @@ struct psh_mesh {
   size_t ndata[2] = {0, 0};
   double *d_rand[2] = {nullptr, nullptr};
   size_t nrand[2] = {0, 0};
+  double sw_d[2] = {0, 0}, sw_r[2] = {0, 0};   // weight sums of the catalogues
 };
 
 namespace {
@@ -264,9 +275,10 @@ int validate_conf(const psh_conf *c, psh_ctx *ctx) {
     return set_err(ctx, PSH_ERR_ARG, "invalid number of catalogues: %d", c->ndata);
   if (c->gsize <= 1 || c->gsize > 65536)
     return set_err(ctx, PSH_ERR_ARG, "invalid GRID_SIZE: %d", c->gsize);
-  if (!is_pow2(c->gsize) || c->gsize < 4 || c->gsize > 2048)
+  if (!psh_grid_supported(c->gsize))
     return set_err(ctx, PSH_ERR_UNSUPPORTED,
-                   "GRID_SIZE %d: this build supports powers of two from 4 to 2048",
+                   "GRID_SIZE %d: this build supports powers of two from 4 to 2048 "
+                   "and 3 * 2^k (48 to 1536) or 5 * 2^k (80 to 1280)",
                    c->gsize);
   if (c->assign < 0 || c->assign > 3)
     return set_err(ctx, PSH_ERR_ARG, "invalid PARTICLE_ASSIGN: %d", c->assign);
@@ -367,19 +379,17 @@ int make_bins(psh_ctx *ctx) {
   b.kedge0f = (float)b.kedge0;
   b.invdkf = (float)(1.0 / c.kbin);
   b.k0dk = (float)fabs(b.kedge0 / c.kbin);
+  b.nmu = c.issim ? ctx->nmu : 1;
   // the fused x pass keeps each workgroup's bin sums in LDS next to its
   // tables and exchange tile (simulation boxes: npole sums per bin; survey
-  // fields: one, the l > 0 fields without interlacing)
-  const int nacc = psh_xbin_nacc(b, c.issim ? PSH_EPI_SIM : PSH_EPI_SURVEY0);
-  const size_t lds = psh_xbin_lds_bytes(ctx->Ng, c.intlace != 0, nacc);
-  if (lds > 160 * 1024) {
-    const size_t fixed = psh_xbin_lds_bytes(ctx->Ng, c.intlace != 0, 0);
-    const long maxacc = fixed < 160 * 1024 ? (long)((160 * 1024 - fixed) / sizeof(double)) : 0;
+  // fields: one); more bins than fit are binned in windows of bins, one
+  // launch each (the reference has no bin limit, src/multipole.c:340-356)
+  ctx->xwin_sim = psh_xbin_window_bins(ctx->Ng, c.intlace != 0, c.npole * b.nmu);
+  ctx->xwin_survey = std::min(psh_xbin_window_bins(ctx->Ng, c.intlace != 0, 1),
+                              psh_xbin_window_bins(ctx->Ng, false, 1));
+  if ((c.issim ? ctx->xwin_sim : ctx->xwin_survey) < 1)
     return set_err(ctx, PSH_ERR_UNSUPPORTED,
-                   "%d k bins x %d sums exceed the LDS of the fused x pass at GRID_SIZE %d "
-                   "(at most %ld sums: widen BIN_SIZE or lower MAX_K)",
-                   nbin, nacc / nbin, ctx->Ng, maxacc);
-  }
+                   "the fused x pass has no LDS left for bin sums at GRID_SIZE %d", ctx->Ng);
   return PSH_OK;
 }
 
@@ -419,6 +429,9 @@ int make_tables(psh_ctx *ctx) {
   float *dtf = reinterpret_cast<float *>(ctx->d_tab + h.size());
   HIPCHK(ctx, hipMemcpy(dtf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
   for (int a = 0; a < 3; a++) {
+    float amax = 0.f;
+    for (int i = 0; i < N; i++) amax = std::max(amax, hf[(size_t)(3 * a + 2) * N + i]);
+    ctx->ax.almax[a] = amax;
     ctx->ax.kv[a] = ctx->d_tab + (size_t)(4 * a + 0) * N;
     ctx->ax.ksq[a] = ctx->d_tab + (size_t)(4 * a + 1) * N;
     ctx->ax.al[a] = ctx->d_tab + (size_t)(4 * a + 2) * N;
@@ -479,18 +492,12 @@ int make_tables(psh_ctx *ctx) {
 int ensure_geometry(psh_ctx *ctx) {
   if (ctx->geom_valid) return PSH_OK;
   const int nb = ctx->nbin, nl = ctx->conf.npole;
-  int grid = ctx->Ng * ctx->Ng;
-  if (grid > 2048) grid = 2048;
   int rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_gpart_cnt, (size_t)grid * nb))) return rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_gpart_km, (size_t)grid * nb))) return rc;
-  if ((rc = dev_alloc(ctx, &ctx->d_gpart_lcnt, (size_t)grid * nb * nl))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_gcnt, (size_t)nb))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_gkm, (size_t)nb))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_glcnt, (size_t)nb * nl))) return rc;
-  HIPCHK(ctx, psh_geometry(ctx->Ng, ctx->ax, ctx->bins, ctx->d_gpart_cnt,
-                           ctx->d_gpart_km, ctx->d_gpart_lcnt, grid, ctx->d_gcnt,
-                           ctx->d_gkm, ctx->d_glcnt, ctx->stream));
+  HIPCHK(ctx, psh_geometry(ctx->Ng, ctx->ax, ctx->bins, ctx->d_gcnt, ctx->d_gkm,
+                           ctx->d_glcnt, ctx->stream));
   ctx->cnt.assign(nb, 0);
   ctx->km.assign(nb, 0.0);
   ctx->lcnt.assign((size_t)nb * nl, 0.0);
@@ -528,7 +535,10 @@ void set_assign_geom(psh_ctx *ctx) {
   }
   if (ctx->tile_req[0])
     for (int a = 0; a < 3; a++) tl[a] = ctx->tile_req[a];
-  for (int a = 0; a < 3; a++) g.tile[a] = ctx->Ng < tl[a] ? ctx->Ng : tl[a];
+  // tiles are powers of two that divide the grid: at most its largest
+  // power-of-two factor (the grid itself when it is a power of two)
+  const int p2 = ctx->Ng & -ctx->Ng;
+  for (int a = 0; a < 3; a++) g.tile[a] = p2 < tl[a] ? p2 : tl[a];
   for (int a = 0; a < 3; a++) g.ntile[a] = ctx->Ng / g.tile[a];
   for (int a = 0; a < 3; a++) {
     g.tile_shift[a] = 0;
@@ -792,9 +802,11 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream,
                               ctx->kzlim));
   ev_record(ctx, E_ZDONE);
-  HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
+  HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
+                        ctx->ysched));
   if (B)
-    HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
+    HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
+                          ctx->ysched));
   ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, nmock);
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
@@ -813,7 +825,11 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   a.grid = grid;
   a.vec0f = (float)(2 * M_PI / ctx->L[0]);
   a.ylive = ctx->d_ylive;
-  HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  for (int kb0 = 0; kb0 < ctx->nbin; kb0 += ctx->xwin_sim) {
+    a.kb0 = kb0;
+    a.nbw = std::min(ctx->xwin_sim, ctx->nbin - kb0);
+    HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  }
   ev_record(ctx, E_XDONE);
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, nmock, grid, nacc, out_bins, ctx->stream));
   return PSH_OK;
@@ -871,7 +887,8 @@ int survey_slots(const psh_ctx *ctx) {
 // raw[0..nb) = sum |F0|^2 alias mult (src/multipole.c:579-660), then per pole
 // n >= 1 and m = -l..l: sum Re(F0 conj(Y_lm(k) S_lm)) with S_lm the transform
 // of Y_lm(r) times the field (src/mp_template.c:48-149).
-int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
+int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw,
+               double dc) {
   const int N = ctx->Ng;
   const bool il = ctx->conf.intlace;
   const int nb = ctx->nbin, nl = ctx->conf.npole;
@@ -888,11 +905,14 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
   if (il) {
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
-    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
-    HIPCHK(ctx, psh_fft_y(N, B, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream, nullptr,
+                          ctx->ysched));
+    HIPCHK(ctx, psh_fft_y(N, B, 1, ctx->mstride, ctx->tw, 0, ctx->stream, nullptr,
+                          ctx->ysched));
   } else {
     HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, A, 1, ctx->mstride, ctx->tw, noy, ctx->stream));
-    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream));
+    HIPCHK(ctx, psh_fft_y(N, A, 1, ctx->mstride, ctx->tw, 0, ctx->stream, nullptr,
+                          ctx->ysched));
   }
   ev_record(ctx, E_YDONE);
   const int grid = xgrid_for(ctx, 1);
@@ -911,7 +931,19 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
   a.partials = ctx->partials;
   a.grid = grid;
   a.vec0f = (float)(2 * M_PI / ctx->L[0]);
-  HIPCHK(ctx, psh_fft_x_bin(N, a, ctx->tw, ctx->stream));
+  a.has_dc = 1;
+  a.dc = dc;
+  // F0 is stored over its own input (A): the launch that stores it must be
+  // the last to read A, so the windows run from the last to the first and
+  // only the first window's launch (kb0 = 0) stores
+  const int nwin = (nb + ctx->xwin_survey - 1) / ctx->xwin_survey;
+  for (int wi = nwin - 1; wi >= 0; wi--) {
+    PshXBinArgs w = a;
+    w.kb0 = wi * ctx->xwin_survey;
+    w.nbw = std::min(ctx->xwin_survey, nb - w.kb0);
+    if (wi > 0) w.store = nullptr;
+    HIPCHK(ctx, psh_fft_x_bin(N, w, ctx->tw, ctx->stream));
+  }
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb, ctx->d_bins, ctx->stream));
   if (il) {
     // back to configuration space: B = c2r(F0) / Ntot (src/multipole.c:489-500)
@@ -935,9 +967,11 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
       for (int ax = 0; ax < 3; ax++) y.rpos[ax] = ctx->d_rpos + (size_t)ax * N;
       HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream,
                                 ctx->kzlim));
-      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive));
+      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
+                            ctx->ysched));
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;
+      b.has_dc = 0;
       b.ylive = ctx->d_ylive;   // S is read by this pass only
       b.A = S;
       b.B = nullptr;
@@ -946,7 +980,11 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
       b.ell = ell;
       b.m = m;
       b.ynorm = y.norm;
-      HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
+      for (int kb0 = 0; kb0 < nb; kb0 += ctx->xwin_survey) {
+        b.kb0 = kb0;
+        b.nbw = std::min(ctx->xwin_survey, nb - kb0);
+        HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
+      }
       HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb,
                                       ctx->d_bins + (size_t)slot * nb, ctx->stream));
     }
@@ -999,23 +1037,40 @@ void normalise_survey(const psh_ctx *ctx, const double *raw, double shot, double
   }
 }
 
-int bounds_device(psh_ctx *ctx, const double *d, size_t n, double mn[3], double mx[3]) {
+// Coordinate bounds of n device records folded into mn/mx; *wsum (may be
+// NULL) receives the sum of their weights (block sums added in block order).
+int bounds_device(psh_ctx *ctx, const double *d, size_t n, double mn[3], double mx[3],
+                  double *wsum = nullptr) {
+  if (wsum) *wsum = 0.0;
   if (n == 0) return PSH_OK;
   int blocks = (int)((n + 255) / 256);
   if (blocks > 1024) blocks = 1024;
   int rc;
-  if (!ctx->d_bounds && (rc = dev_alloc(ctx, &ctx->d_bounds, (size_t)1024 * 6))) return rc;
+  if (!ctx->d_bounds &&
+      (rc = dev_alloc(ctx, &ctx->d_bounds, (size_t)1024 * PSH_BOUNDS_STRIDE)))
+    return rc;
   HIPCHK(ctx, psh_coord_bounds(d, n, ctx->d_bounds, blocks, ctx->stream));
-  std::vector<double> h((size_t)blocks * 6);
+  std::vector<double> h((size_t)blocks * PSH_BOUNDS_STRIDE);
   HIPCHK(ctx, hipMemcpyAsync(h.data(), ctx->d_bounds, sizeof(double) * h.size(),
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  for (int b = 0; b < blocks; b++)
+  double ws = 0.0;
+  for (int b = 0; b < blocks; b++) {
     for (int a = 0; a < 3; a++) {
-      if (mn[a] > h[b * 6 + a]) mn[a] = h[b * 6 + a];
-      if (mx[a] < h[b * 6 + 3 + a]) mx[a] = h[b * 6 + 3 + a];
+      if (mn[a] > h[b * PSH_BOUNDS_STRIDE + a]) mn[a] = h[b * PSH_BOUNDS_STRIDE + a];
+      if (mx[a] < h[b * PSH_BOUNDS_STRIDE + 3 + a]) mx[a] = h[b * PSH_BOUNDS_STRIDE + 3 + a];
     }
+    ws += h[b * PSH_BOUNDS_STRIDE + 6];
+  }
+  if (wsum) *wsum = ws;
   return PSH_OK;
+}
+
+// Sum of the weights of n host records, in record order.
+double weight_sum_host(const psh_data *d, size_t n) {
+  double s = 0.0;
+  for (size_t p = 0; p < n; p++) s += d[p].w;
+  return s;
 }
 
 int stage_host(psh_ctx *ctx, const psh_data *h, size_t n, size_t offset_records) {
@@ -1110,8 +1165,8 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
   void *bufs[] = {ctx->meshA, ctx->meshB, ctx->meshS, ctx->counts, ctx->starts, ctx->cursors,
                   ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
                   ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
-                  ctx->d_glcnt, ctx->d_gpart_cnt, ctx->d_gpart_km, ctx->d_gpart_lcnt,
-                  ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat, ctx->whist};
+                  ctx->d_glcnt, ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat,
+                  ctx->whist, ctx->d_mlp, ctx->d_ax, ctx->d_ay, ctx->d_acorr};
   for (void *p : bufs) ctx_free(ctx, p);
   for (int i = 0; i < E_COUNT; i++)
     if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
@@ -1145,6 +1200,94 @@ extern "C" const char *psh_last_error(const psh_ctx *ctx) {
   return g_last_error.c_str();
 }
 
+extern "C" int psh_ctx_set_mu_cells(psh_ctx *ctx, int nmu) {
+  if (!ctx) return set_err(nullptr, PSH_ERR_ARG, "null context");
+  if (ctx->survey) return set_err(ctx, PSH_ERR_ARG, "|mu| cells are for simulation boxes");
+  if (nmu < 1 || nmu > 64) return set_err(ctx, PSH_ERR_ARG, "nmu %d outside [1, 64]", nmu);
+  if (nmu == ctx->nmu) return PSH_OK;
+  ctx->nmu = nmu;
+  ctx->cells.clear();
+  if (ctx->box_set) {
+    // the bins' accumulator layout changes: redo the binning set-up (the
+    // tables and geometry stay); a captured graph holds the old layout
+    drop_graph(ctx);
+    int rc = make_bins(ctx);
+    if (rc) {
+      ctx->box_set = false;
+      ctx->geom_valid = false;
+      return rc;
+    }
+  }
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_cells(const psh_ctx *ctx, double *cells) {
+  if (!ctx || !cells) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  if (ctx->cells.empty())
+    return set_err(const_cast<psh_ctx *>(ctx), PSH_ERR_ARG, "no box run with cells yet");
+  memcpy(cells, ctx->cells.data(), sizeof(double) * ctx->cells.size());
+  return PSH_OK;
+}
+
+namespace {
+
+int ensure_alias_rows(psh_ctx *ctx) {
+  const size_t rows = (size_t)ctx->nbin * ctx->bins.nmu;
+  int rc;
+  if (rows > ctx->arows_cap) {
+    ctx->arows_cap = 0;
+    if ((rc = dev_alloc(ctx, &ctx->d_ax, rows * 32))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_ay, rows * (size_t)std::max(ctx->mlp_nnet, 1)))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_acorr, (size_t)ctx->conf.npole * ctx->nbin))) return rc;
+    ctx->arows_cap = rows;
+  }
+  return PSH_OK;
+}
+
+}  // namespace
+
+extern "C" int psh_ctx_alias_rows(psh_ctx *ctx, float *X) {
+  if (!ctx || !X) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (ctx->cells.empty() || ctx->bins.nmu < 1 || ctx->conf.poles[0] != 0)
+    return set_err(ctx, PSH_ERR_ARG, "no box run with |mu| cells and a monopole yet");
+  int rc;
+  if ((rc = ensure_alias_rows(ctx))) return rc;
+  const int nb = ctx->nbin, nmu = ctx->bins.nmu;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->d_bins, ctx->cells.data(), sizeof(double) * nb * nmu,
+                             hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, psh_alias_features(ctx->d_bins, ctx->d_gcnt, nb, nmu, ctx->last_norm,
+                                 ctx->last_shot,
+                                 M_PI * ctx->Ng / std::max(ctx->L[0], std::max(ctx->L[1], ctx->L[2])),
+                                 ctx->bins.kedge0, ctx->bins.dk, ctx->bins.logscale,
+                                 ctx->conf.assign, ctx->d_ax, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(X, ctx->d_ax, sizeof(float) * 32 * nb * nmu,
+                             hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_set_alias_mlp(psh_ctx *ctx, const float *params, int nnet) {
+  if (!ctx) return set_err(nullptr, PSH_ERR_ARG, "null context");
+  if (!params || nnet <= 0) {
+    ctx->mlp_nnet = 0;
+    return PSH_OK;
+  }
+  if (ctx->survey || ctx->conf.intlace || ctx->conf.poles[0] != 0 || ctx->nmu < 2)
+    return set_err(ctx, PSH_ERR_ARG,
+                   "the learned alias correction is for plain (non-interlaced) boxes with a "
+                   "monopole and |mu| cells (psh_ctx_set_mu_cells)");
+  if (nnet > 64) return set_err(ctx, PSH_ERR_ARG, "at most 64 networks");
+  const size_t P = (size_t)psh_mlp_param_count();
+  int rc;
+  ctx->mlp_nnet = 0;
+  ctx->arows_cap = 0;   // d_ay holds one output row block per network
+  if ((rc = dev_alloc(ctx, &ctx->d_mlp, P * nnet))) return rc;
+  HIPCHK(ctx, hipMemcpy(ctx->d_mlp, params, sizeof(float) * P * nnet, hipMemcpyHostToDevice));
+  ctx->mlp_nnet = nnet;
+  if (ctx->box_set && (rc = ensure_alias_rows(ctx))) return rc;
+  return PSH_OK;
+}
+
 extern "C" int psh_fault_inject(int what, long value) {
   fault_env_init();
   if (what == PSH_FAULT_ALLOC) {
@@ -1173,7 +1316,13 @@ int sim_box_error(psh_ctx *ctx, const std::vector<PshDevPart> &hp) {
   return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");
 }
 
-int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw) {
+// Raw box sums of the field of `hp` per pole and bin (raw[nl][nbin]); the
+// per-cell sums (|mu| cells) stay in ctx->cells.  With a learned alias
+// correction set (plain grids) and `correct`, the sums are corrected on the
+// device: feature rows from the run's own cell sums (norm and shot of the
+// catalogue), the networks' outputs, and sum_c exp(f) S[l][b][c].
+int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw,
+            double norm = 0, double shot = 0, bool correct = false) {
   int rc;
   ev_record(ctx, E_START);
   size_t ntot = 0;
@@ -1188,11 +1337,36 @@ int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double>
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
   if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
+  const int nb = ctx->nbin, nl = ctx->conf.npole, nmu = ctx->bins.nmu;
+  const bool mlp = correct && ctx->mlp_nnet > 0 && !ctx->conf.intlace && nmu > 1;
+  ctx->last_norm = norm;
+  ctx->last_shot = shot;
+  if (mlp && (rc = ensure_alias_rows(ctx))) return rc;
+  if (mlp) {
+    const int rows = nb * nmu;
+    const int P = psh_mlp_param_count();
+    HIPCHK(ctx, psh_alias_features(ctx->d_bins, ctx->d_gcnt, nb, nmu, norm, shot,
+                                   M_PI * ctx->Ng / std::max(ctx->L[0], std::max(ctx->L[1], ctx->L[2])),
+                                   ctx->bins.kedge0, ctx->bins.dk, ctx->bins.logscale,
+                                   ctx->conf.assign, ctx->d_ax, ctx->stream));
+    for (int n = 0; n < ctx->mlp_nnet; n++)
+      HIPCHK(ctx, psh_mlp_forward_async(ctx->d_mlp + (size_t)n * P, ctx->d_ax, rows,
+                                        ctx->d_ay + (size_t)n * rows, ctx->stream));
+    HIPCHK(ctx, psh_alias_apply(ctx->d_ay, ctx->mlp_nnet, ctx->d_ax, ctx->d_bins, nl, nb, nmu,
+                                ctx->d_acorr, ctx->stream));
+  }
   ev_record(ctx, E_END);
-  raw.assign(nacc, 0.0);
-  HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_bins, sizeof(double) * nacc,
+  ctx->cells.assign(nacc, 0.0);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->cells.data(), ctx->d_bins, sizeof(double) * nacc,
                              hipMemcpyDeviceToHost, ctx->stream));
+  raw.assign((size_t)nl * nb, 0.0);
+  if (mlp)
+    HIPCHK(ctx, hipMemcpyAsync(raw.data(), ctx->d_acorr, sizeof(double) * nl * nb,
+                               hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!mlp)
+    for (size_t q = 0; q < raw.size(); q++)
+      for (int c = 0; c < nmu; c++) raw[q] += ctx->cells[q * nmu + c];
   ev_collect(ctx);
   return PSH_OK;
 }
@@ -1218,11 +1392,11 @@ extern "C" int psh_ctx_run_device(psh_ctx *ctx, const psh_data *d_data, size_t n
   std::vector<PshDevPart> hp(1);
   hp[0] = PshDevPart{(const double *)d_data, n, 0, 1.0f, 0};
   std::vector<double> raw;
-  if ((rc = sim_raw(ctx, hp, raw))) return rc;
   // shot noise and normalisation of a box (src/genr_mesh.c:904-910)
   const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
   const double shot = vol / weight_sum;
   const double norm = weight_sum * weight_sum / vol;
+  if ((rc = sim_raw(ctx, hp, raw, norm, shot, true))) return rc;
   fill_pk_common(ctx, out);
   if (out->pl[0]) normalise_sim(ctx, raw.data(), shot, norm, out->pl[0]);
   return PSH_OK;
@@ -1302,6 +1476,7 @@ struct CatDev {
   const double *data, *rand;   // device psh_data records
   size_t nd, nr;
   double wdata, alpha, shot, norm;
+  double sw_d, sw_r;           // sums of the records' weights (surveys)
 };
 
 int run_catalogues(psh_ctx *ctx, const CatDev *cats, int num, psh_pk *out) {
@@ -1314,9 +1489,10 @@ int run_catalogues(psh_ctx *ctx, const CatDev *cats, int num, psh_pk *out) {
     hp.push_back(PshDevPart{cats[c].data, cats[c].nd, 0, 1.0f, 0});
     if (ctx->survey) {
       hp.push_back(PshDevPart{cats[c].rand, cats[c].nr, cats[c].nd, (float)(-cats[c].alpha), 0});
-      if ((rc = survey_raw(ctx, hp, raw[c]))) return rc;
+      if ((rc = survey_raw(ctx, hp, raw[c], cats[c].sw_d - cats[c].alpha * cats[c].sw_r)))
+        return rc;
     } else {
-      if ((rc = sim_raw(ctx, hp, raw[c]))) return rc;
+      if ((rc = sim_raw(ctx, hp, raw[c], cats[c].norm, cats[c].shot, !cross))) return rc;
     }
   }
   if (cross) {
@@ -1330,7 +1506,9 @@ int run_catalogues(psh_ctx *ctx, const CatDev *cats, int num, psh_pk *out) {
         start += cats[c].nr;
       }
     }
-    if ((rc = ctx->survey ? survey_raw(ctx, hp, rawx) : sim_raw(ctx, hp, rawx))) return rc;
+    double dcx = 0.0;
+    for (int c = 0; c < 2; c++) dcx += cats[c].sw_d - cats[c].alpha * cats[c].sw_r;
+    if ((rc = ctx->survey ? survey_raw(ctx, hp, rawx, dcx) : sim_raw(ctx, hp, rawx))) return rc;
     const size_t nb = ctx->nbin;
     for (size_t q = 0; q < rawx.size(); q++) {
       const double d = rawx[q] - raw[0][q] - raw[1][q];
@@ -1457,11 +1635,19 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
     return set_err(ctx, PSH_ERR_CATA, "particle coordinate outside the box [0, BOX_SIZE)");
   if (flags & 2) return set_err(ctx, PSH_ERR_MESH, "inconsistent interlaced cell of a particle");
   const double vol = ctx->L[0] * ctx->L[1] * ctx->L[2];
+  const int nmu = ctx->bins.nmu;
+  std::vector<double> sums((size_t)ctx->conf.npole * ctx->nbin);
   for (int m = 0; m < nmock; m++) {
     fill_pk_common(ctx, &out[m]);
     const double shot = vol / weight_sum[m];
     const double norm = weight_sum[m] * weight_sum[m] / vol;
-    if (out[m].pl[0]) normalise_sim(ctx, raw.data() + (size_t)m * nacc, shot, norm, out[m].pl[0]);
+    const double *rm = raw.data() + (size_t)m * nacc;
+    for (size_t q = 0; q < sums.size(); q++) {   // |mu| cells summed per bin
+      double t = 0.0;
+      for (int c = 0; c < nmu; c++) t += rm[q * nmu + c];
+      sums[q] = t;
+    }
+    if (out[m].pl[0]) normalise_sim(ctx, sums.data(), shot, norm, out[m].pl[0]);
   }
   return PSH_OK;
 }
@@ -1517,6 +1703,8 @@ extern "C" int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out) {
       cd[c].alpha = cat->alpha[c];
       cd[c].shot = cat->shot[c];
       cd[c].norm = cat->norm[c];
+      cd[c].sw_d = weight_sum_host(cat->data[c], cat->ndata[c]);
+      cd[c].sw_r = weight_sum_host(cat->rand[c], cat->nrand[c]);
     } else {
       // shot/norm of simulation boxes (src/genr_mesh.c:904-910)
       const double w = cat->wdata[c];
@@ -1579,6 +1767,7 @@ extern "C" psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat) {
       return nullptr;
     }
     mesh->ndata[c] = cat->ndata[c];
+    mesh->sw_d[c] = weight_sum_host(cat->data[c], cat->ndata[c]);
     if (!conf->issim) {
       if (dev_alloc(ctx, &mesh->d_rand[c], cat->nrand[c] * 4) ||
           hipMemcpy(mesh->d_rand[c], cat->rand[c], sizeof(psh_data) * cat->nrand[c],
@@ -1588,6 +1777,7 @@ extern "C" psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat) {
         return nullptr;
       }
       mesh->nrand[c] = cat->nrand[c];
+      mesh->sw_r[c] = weight_sum_host(cat->rand[c], cat->nrand[c]);
     }
   }
   // shot noise and normalisation of simulation boxes (src/genr_mesh.c:904-910)
@@ -1660,6 +1850,8 @@ extern "C" psh_pk *psh_powspec(const psh_conf *conf, const psh_cata *cat, psh_me
     cd[c].alpha = cat->alpha[c];
     cd[c].shot = cat->shot[c];
     cd[c].norm = cat->norm[c];
+    cd[c].sw_d = mesh->sw_d[c];
+    cd[c].sw_r = mesh->sw_r[c];
   }
   if ((rc = run_catalogues(ctx, cd, mesh->num, pk))) {
     fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
@@ -1748,6 +1940,11 @@ extern "C" int psh_memcpy_d2h(void *dst, const void *src, size_t bytes) {
                                                                          : PSH_ERR_DEVICE;
 }
 
+extern "C" int psh_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? PSH_OK
+                                                                           : PSH_ERR_DEVICE;
+}
+
 extern "C" int psh_device_synchronize(void) {
   return hipDeviceSynchronize() == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
 }
@@ -1773,8 +1970,9 @@ extern "C" int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, s
   int rc;
   // box from the coordinate range of data and randoms (src/genr_mesh.c:427-578)
   double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  if ((rc = bounds_device(ctx, (const double *)d_data, nd, mn, mx)) ||
-      (rc = bounds_device(ctx, (const double *)d_rand, nr, mn, mx)) ||
+  double swd = 0, swr = 0;
+  if ((rc = bounds_device(ctx, (const double *)d_data, nd, mn, mx, &swd)) ||
+      (rc = bounds_device(ctx, (const double *)d_rand, nr, mn, mx, &swr)) ||
       (rc = def_box(ctx, mn, mx)) || (rc = ensure_geometry(ctx)))
     return rc;
   CatDev cd[2] = {};
@@ -1785,6 +1983,8 @@ extern "C" int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, s
   cd[0].alpha = alpha;
   cd[0].shot = shot;
   cd[0].norm = norm;
+  cd[0].sw_d = swd;
+  cd[0].sw_r = swr;
   return run_catalogues(ctx, cd, 1, out);
 }
 
@@ -1835,6 +2035,22 @@ int nmock_class(int nmock) {
 
 }  // namespace
 
+extern "C" int psh_ctx_set_fft_schedule(psh_ctx *ctx, int ysched) {
+  if (!ctx) return set_err(nullptr, PSH_ERR_ARG, "null context");
+  if (!psh_fft_y_sched_ok(ctx->Ng, ysched))
+    return set_err(ctx, PSH_ERR_ARG, "y-pass schedule %d does not apply at GRID_SIZE %d", ysched,
+                   ctx->Ng);
+  if (ysched != ctx->ysched) drop_graph(ctx);   // a captured graph holds the old launches
+  ctx->ysched = ysched;
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_get_fft_schedule(const psh_ctx *ctx, int *ysched) {
+  if (!ctx || !ysched) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  *ysched = ctx->ysched;
+  return PSH_OK;
+}
+
 extern "C" int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path) {
   if (!ctx || !cache_path) return set_err(ctx, PSH_ERR_ARG, "null argument");
   FILE *fp = fopen(cache_path, "r");
@@ -1844,10 +2060,22 @@ extern "C" int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path) {
   char line[512];
   int best[3] = {0, 0, 0};
   double best_ms = 1e300;
+  int ybest = -1;
+  double ybest_ms = 1e300;
   while (fgets(line, sizeof line, fp)) {
     char a[64];
     int Ng, scheme, il, nm, t[3];
     double npart, ms;
+    int ys;
+    // FFT schedules: "fft arch Ng intlace nmock_class y ysched ms"
+    if (sscanf(line, "fft %63s %d %d %d y %d %lf", a, &Ng, &il, &nm, &ys, &ms) == 6) {
+      if (arch == a && Ng == ctx->Ng && il == ctx->conf.intlace && nm == cls &&
+          psh_fft_y_sched_ok(ctx->Ng, ys) && ms < ybest_ms) {
+        ybest_ms = ms;
+        ybest = ys;
+      }
+      continue;
+    }
     if (line[0] == '#' ||
         sscanf(line, "%63s %d %d %d %d %lf %d %d %d %lf", a, &Ng, &scheme, &il, &nm, &npart,
                &t[0], &t[1], &t[2], &ms) != 10)
@@ -1861,8 +2089,81 @@ extern "C" int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path) {
     }
   }
   fclose(fp);
-  if (!best[0]) return set_err(ctx, PSH_ERR_ARG, "no tuning entry for this configuration");
-  return psh_ctx_set_tiles(ctx, best);
+  if (!best[0] && ybest < 0)
+    return set_err(ctx, PSH_ERR_ARG, "no tuning entry for this configuration");
+  int rc;
+  if (best[0] && (rc = psh_ctx_set_tiles(ctx, best))) return rc;
+  if (ybest >= 0 && (rc = psh_ctx_set_fft_schedule(ctx, ybest))) return rc;
+  return PSH_OK;
+}
+
+extern "C" int psh_autotune_fft(const psh_conf *conf, int nmock, const char *cache_path,
+                                psh_tune_fft *out) {
+  if (!conf || !out || nmock < 1) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  memset(out, 0, sizeof *out);
+  // the schedules are timed on the meshes of a box context of the same
+  // shape, zeroed (the transform's time does not depend on the values), with
+  // the y pass's live-column pruning of that binning
+  psh_conf c = *conf;
+  c.issim = 1;
+  c.ndata = 1;
+  c.iscross = 0;
+  if (!c.has_bsize || !(c.bsize[0] > 0)) {
+    c.has_bsize = 1;
+    c.bsize[0] = c.bsize[1] = c.bsize[2] = 1000.0;
+  }
+  psh_ctx *ctx = nullptr;
+  int rc = psh_ctx_create(&c, nmock, &ctx);
+  if (rc) return rc;
+  const int N = ctx->Ng;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<double> tms(PSH_FFT_Y_SCHEDS, -1.0);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      psh_fill_zero_u32((uint32_t *)ctx->meshA, ctx->mstride * nmock, ctx->stream) !=
+          hipSuccess)
+    rc = set_err(ctx, PSH_ERR_DEVICE, "FFT tuning set-up failed");
+  float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
+  double best = 1e300;
+  for (int ys = 0; ys < PSH_FFT_Y_SCHEDS && !rc; ys++) {
+    if (!psh_fft_y_sched_ok(N, ys)) continue;
+    double t[5];
+    for (int r = -1; r < 5 && !rc; r++) {
+      float ms = 0.f;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess ||
+          psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive, ys) !=
+              hipSuccess ||
+          hipEventRecord(e1, ctx->stream) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = set_err(ctx, PSH_ERR_DEVICE, "FFT tuning launch failed (schedule %d)", ys);
+        break;
+      }
+      if (r >= 0) t[r] = ms;   // the first launch warms up
+    }
+    if (rc) break;
+    std::sort(t, t + 5);
+    tms[ys] = t[2];
+    out->ncand++;
+    if (ys == 0) out->ms_default = t[2];
+    if (t[2] < best) {
+      best = t[2];
+      out->ysched = ys;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const std::string arch = arch_name();
+  psh_ctx_destroy(ctx);
+  if (rc) return set_err(nullptr, rc, "%s", g_last_error.c_str());
+  out->ms = best;
+  if (cache_path) {
+    FILE *fp = fopen(cache_path, "a");
+    if (!fp) return set_err(nullptr, PSH_ERR_FILE, "cannot open %s", cache_path);
+    fprintf(fp, "fft %s %d %d %d y %d %.5f\n", arch.c_str(), N, conf->intlace,
+            nmock_class(nmock), out->ysched, best);
+    if (fclose(fp)) return PSH_ERR_FILE;
+  }
+  return PSH_OK;
 }
 
 extern "C" int psh_autotune(const psh_conf *conf, const psh_data *d_data, const size_t *offsets,

@@ -32,6 +32,7 @@ struct PshAxisTables {
   const float *kvf[3];
   const float *ksqf[3];
   const float *alf[3];
+  float almax[3];   // largest alias factor per axis
 };
 
 // Binning geometry (src/multipole.c:306-421).
@@ -48,6 +49,9 @@ struct PshBins {
   // farther than an error bound from an integer, else recomputed in FP64
   float kedge0f, invdkf, k0dk;
   float losf[3];
+  // boxes: |mu| cells per bin of the fused x pass's sums (1: none); cell c
+  // holds the modes with |mu| in [c / nmu, (c + 1) / nmu)
+  int nmu;
 };
 
 // What the fused last FFT pass does with each mode.
@@ -147,8 +151,14 @@ hipError_t psh_fft_z_c2r(int Ng, float2 *inout, int nmesh,
 // ylive (forward y pass of fields that only the fused x pass reads, or
 // nullptr): per x-pass tile of k_z columns, the largest |k_y| index whose
 // tile holds binned modes; outputs past it are not stored.
+// sched: schedule of the column pass (0 default; see psh_fft_y_sched_ok).
 hipError_t psh_fft_y(int Ng, float2 *data, int nmesh, size_t mstride_floats,
-    const float2 *tw, int inverse, hipStream_t s, const int *ylive = nullptr);
+    const float2 *tw, int inverse, hipStream_t s, const int *ylive = nullptr,
+    int sched = 0);
+// Schedules of the column pass: 0..PSH_FFT_Y_SCHEDS-1; whether one is valid
+// at a grid size (its LDS fits and it differs from schedule 0).
+#define PSH_FFT_Y_SCHEDS 6
+int psh_fft_y_sched_ok(int Ng, int sched);
 
 // Complex FFT along x (axis 0) from `in` to `out` (may alias); no epilogue.
 hipError_t psh_fft_x(int Ng, const float2 *in, float2 *out, int nmesh,
@@ -173,13 +183,23 @@ struct PshXBinArgs {
   int grid;                // number of persistent workgroups per mock
   const int *ylive;        // per k_z tile, the largest |k_y| with binned modes
                            // (tiles past it are skipped), or nullptr: all tiles
+  int kb0, nbw;            // bins [kb0, kb0 + nbw) of this launch (nbw 0: all);
+                           // partials rows hold every bin, a launch writes its own
+  int has_dc;              // the k = 0 mode of the field is dc
+  double dc;               // (sum of its particles' weights, exact in FP64)
 };
 int psh_xbin_nacc(const PshBins &b, int epi);
 // k_z columns per tile of the fused x pass at a grid size.
 int psh_xbin_tile_cols(int Ng);
 // LDS bytes of the fused x pass for a grid size, interlacing and number of
 // accumulators (0 for an unsupported grid size); at most 160 KB run.
+// Grid sizes with compiled transforms: powers of two 4..2048, 3 * 2^k
+// (48..1536) and 5 * 2^k (80..1280).
+bool psh_grid_supported(int Ng);
 size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc);
+// Most bins one launch of the fused x pass holds in LDS with per_bin
+// accumulators per bin (more bins take several launches over bin windows).
+int psh_xbin_window_bins(int Ng, bool il, int per_bin);
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
     hipStream_t s);
 
@@ -188,12 +208,10 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
 // ---------------------------------------------------------------------------
 
 // cnt (uint64), km (sum of kmod*mult) and lcnt (sum of mult*L_l(mu), DC
-// skipped) per bin.  Writes partials then reduces into out_cnt, out_km,
-// out_lcnt[nl*nbin].
+// skipped) per bin into out_cnt, out_km, out_lcnt[nl*nbin]; one workgroup
+// per bin, sums in a fixed order (the same on every run).
 hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
-    unsigned long long *part_cnt, double *part_km, double *part_lcnt,
-    int grid, unsigned long long *out_cnt, double *out_km, double *out_lcnt,
-    hipStream_t s);
+    unsigned long long *out_cnt, double *out_km, double *out_lcnt, hipStream_t s);
 
 // Sum partials[nmock][grid][nacc] over grid in a fixed order into
 // out[nmock][nacc].
@@ -208,7 +226,26 @@ hipError_t psh_exclusive_scan_u32(const uint32_t *in, uint32_t *out, uint32_t *t
 // Fill a float buffer with zeros (graph-safe replacement of memset).
 hipError_t psh_fill_zero_u32(uint32_t *p, size_t n, hipStream_t s);
 
-// Min/max of the coordinates of n psh_data records (FP64, exact).
-// out: [blocks][6] partial (min x,y,z, max x,y,z).
+// Min/max of the coordinates of n psh_data records (FP64, exact) and the
+// sum of their weights (FP64, a fixed order per block).
+// out: [blocks][PSH_BOUNDS_STRIDE] partial (min x,y,z, max x,y,z, sum w).
+#define PSH_BOUNDS_STRIDE 7
 hipError_t psh_coord_bounds(const double *xyzw, size_t n, double *partial,
     int blocks, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Learned alias correction (k_mlp.hip)
+// ---------------------------------------------------------------------------
+
+// Feature rows X[nbin * nmu][32] of a plain box run from its raw monopole
+// cell sums cells[nbin][nmu] (device), mode counts cnt[nbin] (device), the
+// normalisation, shot noise, k_Ny and binning of the run, the scheme id.
+hipError_t psh_alias_features(const double *cells, const unsigned long long *cnt, int nbin,
+    int nmu, double norm, double shot, double kny, double kedge0, double dk, int logscale,
+    int scheme, float *X, hipStream_t s);
+// out[l][b] = sum_c exp(mean_n Y[n][b nmu + c]) cells[l][b][c] (device).
+hipError_t psh_alias_apply(const float *Y, int nnet, const float *X, const double *cells, int nl,
+    int nbin, int nmu, double *out, hipStream_t s);
+// Network outputs Y[B] of rows X[B][32] on stream s (no synchronisation).
+hipError_t psh_mlp_forward_async(const float *d_params, const float *d_X, int B, float *d_Y,
+    hipStream_t s);

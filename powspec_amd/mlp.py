@@ -1,36 +1,34 @@
-"""Learned alias correction of the plain (non-interlaced) estimator.
+"""Learned alias correction of the plain (non-interlaced) box estimator.
 
-A perceptron maps, per k bin of one catalogue, the features
-    [k / k_Ny, one-hot assignment scheme (4), 16-band log-power summary of
-     the plain spectrum in units of the shot noise over [k_Ny/4, k_Ny],
-     log10 of the bin's own plain power over the shot noise]
-to the log ratio of the interlaced to the plain (Jing-corrected) monopole,
-both un-subtracted (P_0 + shot), so that
-    P_corrected(k) = P_plain(k) exp(f(features)).
-Network: inputs padded to 32, three hidden layers of 64 with ReLU, one
-output (W4 padded to 16 columns); 9.7k live parameters.  Training and
-inference run on the GPU with bf16 matrix-core products and FP32
-accumulation (psh_mlp_*); `ref_forward` / `ref_grad` are the FP64 CPU
-reference of the same network used to pin the GPU gradients.  The
-correction applied is the mean of ENSEMBLE networks trained from different
-seeds (`train_ensemble`, `predict`).
+The library stage (psh_ctx_set_mu_cells, psh_ctx_set_alias_mlp) bins a plain
+box run in (k bin, |mu| cell) pairs and corrects every cell's raw sums by
+exp(f), f the mean output of a few perceptrons evaluated on the device on
+feature rows the library builds from the run itself (psh_ctx_alias_rows):
+    [k / k_Ny, |mu| cell centre, one-hot scheme (4), 16-band log10 summary
+     of the un-subtracted monopole over the shot noise on [k_Ny/4, k_Ny],
+     the bin's own log10(P_0 / shot), log10 of the cell's share of the bin]
+so that P_l of every pole is corrected, not only the monopole.  The target
+of a cell is log(S_il / S_plain), the ratio of its monopole sums in an
+interlaced and a plain run of the same catalogue.  Network: inputs padded
+to 32, three hidden layers of 64 with ReLU, one output (W4 padded to 16
+columns); 10.0k live parameters, bf16 matrix-core products with FP32
+accumulation for training (psh_mlp_train) and inference.
 
-The correction is a function of the measured plain spectrum, so its floor
-is the part of the alias power that the spectrum below k_Ny does not fix:
-boxes drawn from one power spectrum with different seeds scatter around
-their mean ratio by about 0.5 % at 0.8 k_Ny for P = 1e4 (k/0.1)^-1.5
-exp(-(k/0.5)^2) at N_g = 512, and by over 10 % for the highest-variance
-lognormal fields of the family (amplitude 2e4, index -1), whose small-scale
-power comes from a handful of cells.
+This module drives the training on the GPU: catalogues from the library's
+own generator, plain and interlaced runs, feature rows and targets from the
+library, and evaluation against the interlaced estimator, which stays the
+fallback and the judge of the error.
 """
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
 IN, H, OUT = 32, 64, 16
-NFEAT = 1 + 4 + 16 + 1
+NFEAT = 24
+NMU = 8
 _SHAPES = [("W1", (IN, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, H)),
            ("b3", (H,)), ("W4", (H, OUT)), ("b4", (OUT,))]
 NPAR = sum(int(np.prod(s)) for _, s in _SHAPES)
@@ -56,38 +54,6 @@ def init_params(seed=0):
     u["W3"][:] = rng.normal(0, np.sqrt(2.0 / H), (H, H))
     u["W4"][:, 0] = rng.normal(0, np.sqrt(1.0 / H), H)
     return p
-
-
-def ref_forward(p, X):
-    """FP64 forward: (output, cache)."""
-    u = {k: v.astype(np.float64) for k, v in unpack(np.asarray(p)).items()}
-    x = np.asarray(X, np.float64)
-    h1 = np.maximum(x @ u["W1"] + u["b1"], 0)
-    h2 = np.maximum(h1 @ u["W2"] + u["b2"], 0)
-    h3 = np.maximum(h2 @ u["W3"] + u["b3"], 0)
-    y = h3 @ u["W4"] + u["b4"]
-    return y[:, 0], (x, h1, h2, h3, u)
-
-
-def ref_loss(p, X, T):
-    y, _ = ref_forward(p, X)
-    return float(np.mean((y - T) ** 2))
-
-
-def ref_grad(p, X, T):
-    """FP64 analytic gradient of mean (y - t)^2, in the flat layout."""
-    y, (x, h1, h2, h3, u) = ref_forward(p, X)
-    B = len(T)
-    dy = np.zeros((B, OUT))
-    dy[:, 0] = 2 * (y - T) / B
-    g = {"W4": h3.T @ dy, "b4": dy.sum(0)}
-    d3 = (dy @ u["W4"].T) * (h3 > 0)
-    g["W3"], g["b3"] = h2.T @ d3, d3.sum(0)
-    d2 = (d3 @ u["W3"].T) * (h2 > 0)
-    g["W2"], g["b2"] = h1.T @ d2, d2.sum(0)
-    d1 = (d2 @ u["W2"].T) * (h1 > 0)
-    g["W1"], g["b1"] = x.T @ d1, d1.sum(0)
-    return np.concatenate([g[name].ravel() for name, _ in _SHAPES])
 
 
 # ---- GPU ------------------------------------------------------------------
@@ -137,61 +103,41 @@ def gpu_train(p, X, T, epochs=200, batch=256, lr=2e-3, seed=0):
     return d_p.to_numpy((NPAR,), np.float32), loss.value
 
 
-# ---- features -------------------------------------------------------------
+# ---- training data: plain vs interlaced runs of clustered boxes ------------
 
-SUMMARY_LO = 0.25   # the summary covers [SUMMARY_LO k_Ny, k_Ny]
-
-
-def summary16(k, p_plain, kf, kny, shot):
-    """log10 of the plain un-subtracted monopole in units of the shot noise,
-    averaged over 16 equal bands of [SUMMARY_LO k_Ny, k_Ny] (bins whose
-    centre falls in a band; an empty band takes the nearest populated bin):
-    the shape of the spectrum towards the Nyquist frequency, where it sets
-    how much power the window folds back, and its level against the shot
-    noise.  The bands hold thousands of modes each, so the summary carries
-    little sample variance."""
-    ok = np.isfinite(p_plain) & (p_plain > 0)
-    kk, pp = k[ok], p_plain[ok]
-    edges = np.linspace(SUMMARY_LO * kny, kny, 17)
-    out = np.empty(16)
-    for b in range(16):
-        sel = (kk >= edges[b]) & (kk < edges[b + 1])
-        if sel.any():
-            out[b] = pp[sel].mean()
-        else:
-            out[b] = pp[np.argmin(np.abs(kk - 0.5 * (edges[b] + edges[b + 1])))]
-    return np.log10(out / shot)
+def box_conf(P, Ng=512, L=1000.0, kbin=0.01, scheme=1, intlace=False):
+    return P.make_conf(gsize=Ng, assign=scheme, intlace=intlace, bsize=L, kbin=kbin,
+                       poles=[0, 2, 4])
 
 
-def features(k, p_plain_raw, scheme, kf, kny, shot):
-    """Rows X[nbin][IN] for one catalogue (bins with finite positive power)."""
-    s = summary16(k, p_plain_raw, kf, kny, shot)
-    X = np.zeros((len(k), IN), np.float32)
-    X[:, 0] = k / kny
-    X[:, 1 + scheme] = 1.0
-    X[:, 5:21] = s
-    with np.errstate(divide="ignore", invalid="ignore"):
-        X[:, 21] = np.log10(np.where(p_plain_raw > 0, p_plain_raw, shot) / shot)
-    return X
-
-
-# ---- training data: plain vs interlaced spectra of clustered boxes, N_g = 512 ---
-
-def mock_spectra(P, seed, A, nidx, Ng=512, ng_gen=512, L=1000.0, nbar=0.02, scheme=1,
-                 kbin=0.01):
-    """(k, raw plain P_0 + shot, raw interlaced P_0 + shot, shot) of one
-    clustered box generated on the GPU."""
+def mock_runs(P, seed, A, nidx, Ng=512, ng_gen=512, L=1000.0, nbar=0.02, scheme=1,
+              kbin=0.01, nmu=NMU):
+    """One clustered box generated on the GPU, run plain and interlaced with
+    |mu| cells: (feature rows, plain cells, interlaced cells, interlaced
+    P_l, plain P_l, shot), cells [3][nbin][nmu]."""
     dev, n = P.gen_clustered_box(ng=ng_gen, L=L, nbar=nbar, f=0.8, seed=seed, A=A, n=nidx)
-    out = []
+    out = {}
     for il in (False, True):
-        ctx = P.Context(P.make_conf(gsize=Ng, assign=scheme, intlace=il, bsize=L, kbin=kbin,
-                                    poles=[0]))
+        ctx = P.Context(box_conf(P, Ng, L, kbin, scheme, il))
+        ctx.set_mu_cells(nmu)
         r = ctx.run_device(dev, n, float(n))
-        out.append(r["pl"][0][0] + L ** 3 / n)
-        k = r["k"]
+        out[il] = (ctx.cells(), r["pl"][0])
+        if not il:
+            out["X"] = ctx.alias_rows()
+            out["k"] = r["k"]
         ctx.close()
     dev.free()
-    return k, out[0], out[1], L ** 3 / n
+    return out["X"], out[False][0], out[True][0], out[True][1], out[False][1], L ** 3 / n
+
+
+def targets(cells_plain, cells_il, X):
+    """log(S_il / S_plain) of the monopole cell sums; rows without features
+    or power are left out (mask)."""
+    sp, si = cells_plain[0].ravel(), cells_il[0].ravel()
+    ok = (X[:, 0] != 0) & (sp > 0) & (si > 0)
+    t = np.zeros(len(sp), np.float32)
+    t[ok] = np.log(si[ok] / sp[ok])
+    return t, ok
 
 
 def grid_design(m=8, amp=(5e3, 2e4), index=(-2.0, -1.0)):
@@ -210,23 +156,21 @@ def random_design(seeds, rng_seed=0, amp=(5e3, 2e4), index=(-2.0, -1.0)):
     return out
 
 
-def dataset(P, seeds, Ng=512, ng_gen=512, L=1000.0, nbar=0.02, scheme=1, kbin=0.01,
-            design=None):
-    """Feature rows, targets log(P_il / P_plain) and per-mock spectra for the
-    mocks `seeds`, mock i with power-law amplitude and index design[i]
-    (default: random_design(seeds))."""
+def dataset(P, seeds, design=None, **kw):
+    """Training rows, targets and per-mock evaluation records for the
+    mocks `seeds` (mock i with power-law amplitude and index design[i],
+    default random_design(seeds))."""
     seeds = list(seeds)
     design = random_design(seeds) if design is None else list(design)
-    kf, kny = 2 * np.pi / L, np.pi * Ng / L
-    Xs, Ts, spectra = [], [], []
+    Xs, Ts, recs = [], [], []
     for s, (A, nidx) in zip(seeds, design):
-        k, pp, pi, shot = mock_spectra(P, s, A, nidx, Ng, ng_gen, L, nbar, scheme, kbin)
-        ok = np.isfinite(pp) & (pp > 0) & (pi > 0)
-        X = features(k, pp, scheme, kf, kny, shot)[ok]
-        Xs.append(X)
-        Ts.append(np.log(pi[ok] / pp[ok]).astype(np.float32))
-        spectra.append((k[ok], pp[ok], pi[ok], X))
-    return np.concatenate(Xs), np.concatenate(Ts), spectra
+        X, cp, ci, pl_il, pl_plain, shot = mock_runs(P, s, A, nidx, **kw)
+        t, ok = targets(cp, ci, X)
+        Xs.append(X[ok])
+        Ts.append(t[ok])
+        recs.append({"seed": s, "A": A, "n": nidx, "pl_il": pl_il, "pl_plain": pl_plain,
+                     "shot": shot})
+    return np.concatenate(Xs), np.concatenate(Ts), recs
 
 
 SCHEDULE = ((300, 2e-3), (300, 5e-4), (200, 1e-4))
@@ -247,50 +191,81 @@ ENSEMBLE = 4
 
 def train_ensemble(X, T, n=ENSEMBLE, schedule=SCHEDULE, batch=256):
     """n networks trained from seeds 0..n-1: (params [n, NPAR], mean final
-    loss).  Training is reproducible (fixed-point gradient sums), but which
-    minimum a network reaches depends on its seed: on held-out boxes of the
-    reference spectrum single networks give worst-bin errors from 0.26 to
-    0.71 % by seed, means of four 0.46-0.47 %."""
+    loss).  Training is reproducible (fixed-point gradient sums); which
+    minimum a network reaches depends on its seed, and the mean of a few
+    averages that spread out."""
     ps, ls = zip(*(train(X, T, schedule, batch, seed=s) for s in range(n)))
     return np.stack(ps), float(np.mean(ls))
 
 
-def predict(params, X):
-    """Log correction f(features) per row: one network (params of shape
-    [NPAR]) or the mean over an ensemble ([n, NPAR])."""
-    P = np.asarray(params, np.float32)
-    if P.ndim == 1:
-        return gpu_forward(P, X).astype(np.float64)
-    return np.mean([gpu_forward(p, X).astype(np.float64) for p in P], axis=0)
+# ---- evaluation of the library stage against the interlaced estimator ------
+
+def corrected_runs(P, params, seeds, design, Ng=512, ng_gen=512, L=1000.0, nbar=0.02,
+                   scheme=1, kbin=0.01, nmu=NMU):
+    """Per held-out catalogue: (k, corrected P_l of the library stage,
+    interlaced P_l, plain Jing-corrected P_l, shot)."""
+    out = []
+    for s, (A, nidx) in zip(seeds, design):
+        dev, n = P.gen_clustered_box(ng=ng_gen, L=L, nbar=nbar, f=0.8, seed=s, A=A, n=nidx)
+        plain = P.Context(box_conf(P, Ng, L, kbin, scheme, False))
+        plain.set_mu_cells(nmu)
+        jing = plain.run_device(dev, n, float(n))
+        plain.set_alias_mlp(params)
+        corr = plain.run_device(dev, n, float(n))
+        il = P.Context(box_conf(P, Ng, L, kbin, scheme, True)).run_device(dev, n, float(n))
+        dev.free()
+        out.append((corr["k"], corr["pl"][0], il["pl"][0], jing["pl"][0], L ** 3 / n))
+    return out
 
 
-def correct(params, k, p_plain_raw, scheme, kf, kny, shot):
-    """Corrected un-subtracted monopole P_plain exp(f) for one catalogue
-    (bins with non-positive plain power are returned unchanged); params is
-    one network or an ensemble."""
-    X = features(k, p_plain_raw, scheme, kf, kny, shot)
-    y = predict(params, X)
-    ok = np.isfinite(p_plain_raw) & (p_plain_raw > 0)
-    return np.where(ok, p_plain_raw * np.exp(y), p_plain_raw)
-
-
-def errors(params, spectra, kny, kmax_frac=0.8):
-    """Per catalogue, the worst relative error over k <= kmax_frac k_Ny of
-    the corrected and of the Jing-corrected plain monopole against the
-    interlaced one: two arrays."""
-    e_mlp, e_jing = [], []
-    for k, pp, pi, X in spectra:
+def errors(runs, kny, kmax_frac=0.8):
+    """Worst error per catalogue over k <= kmax_frac k_Ny, for each pole,
+    relative to the bin's un-subtracted interlaced monopole S_b = P_0 + shot
+    (the parity scale): arrays [ncat][3] for the library stage and for the
+    Jing-corrected plain estimator."""
+    e_c, e_j = [], []
+    for k, pc, pi, pj, shot in runs:
         sel = k <= kmax_frac * kny
-        corr = pp * np.exp(predict(params, X))
-        e_mlp.append(float(np.max(np.abs(corr[sel] / pi[sel] - 1))))
-        e_jing.append(float(np.max(np.abs(pp[sel] / pi[sel] - 1))))
-    return np.array(e_mlp), np.array(e_jing)
+        S = pi[0][sel] + shot
+        e_c.append([float(np.max(np.abs(pc[l][sel] - pi[l][sel]) / S)) for l in range(3)])
+        e_j.append([float(np.max(np.abs(pj[l][sel] - pi[l][sel]) / S)) for l in range(3)])
+    return np.array(e_c), np.array(e_j)
 
 
-def evaluate(params, spectra, kny, kmax_frac=0.8):
-    """Worst errors over all catalogues: (corrected, Jing-corrected)."""
-    m, j = errors(params, spectra, kny, kmax_frac)
-    return float(m.max()), float(j.max())
+# ---- shipped networks ---------------------------------------------------------
+
+PARAMS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "alias_cic_ng512.npz")
+
+
+def save_params(params, path=PARAMS_FILE, **meta):
+    np.savez(path, params=np.asarray(params, np.float32),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_params(path=PARAMS_FILE):
+    """The shipped ensemble [n, NPAR] (trained by `python -m powspec_amd.mlp
+    train`), or None when absent."""
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as z:
+        return np.asarray(z["params"], np.float32)
+
+
+def train_and_save(P, path=PARAMS_FILE):
+    """The shipped networks: 64 clustered boxes (CIC, N_g = 512, seeds
+    5000..5063) on the 8 x 8 design of the power-law family, four networks."""
+    X, T, _ = dataset(P, range(5000, 5064), design=grid_design(8))
+    print(f"training rows: {len(T)}", flush=True)
+    ps, ls = [], []
+    for s in range(ENSEMBLE):
+        p, l = train(X, T, seed=s)
+        print(f"network {s}: final loss {l:.3e}", flush=True)
+        ps.append(p)
+        ls.append(l)
+    params, loss = np.stack(ps), float(np.mean(ls))
+    save_params(params, path, rows=len(T), loss=loss, nmu=NMU, scheme=1, ng=512,
+                seeds=np.arange(5000, 5064))
+    return params, loss
 
 
 def _bench():
@@ -319,5 +294,11 @@ if __name__ == "__main__":
     import sys
     if sys.argv[1:] == ["bench"]:
         _bench()
+    elif sys.argv[1:2] == ["train"]:
+        import powspec_amd as _P
+        _P.load()
+        out = sys.argv[2] if len(sys.argv) > 2 else PARAMS_FILE
+        _, loss = train_and_save(_P, out)
+        print(f"trained networks written to {out} (mean final loss {loss:.3e})")
     else:
-        print("usage: python -m powspec_amd.mlp bench")
+        print("usage: python -m powspec_amd.mlp bench | train [path]")

@@ -1,8 +1,11 @@
 """The bf16-MFMA perceptron against its FP64 reference, and the learned
-alias correction end to end on held-out clustered boxes."""
+alias correction as a library stage (|mu| cells, device features, device
+inference, corrected P_0, P_2, P_4) on held-out clustered boxes, judged by
+the interlaced estimator."""
 import numpy as np
 import pytest
 
+from oracle import mlp_ref as ref
 from powspec_amd import mlp
 
 pytestmark = pytest.mark.gpu
@@ -20,18 +23,18 @@ def test_forward_matches_reference(psh):
     X, _ = _data(1000, 1)
     p = mlp.init_params(5)
     got = mlp.gpu_forward(p, X)
-    ref, _ = mlp.ref_forward(p, X)
+    want, _ = ref.ref_forward(p, X)
     # bf16 operands (8-bit mantissa) through four layers
-    assert np.max(np.abs(got - ref)) <= 0.03 * (np.max(np.abs(ref)) + 1e-3)
-    assert np.corrcoef(got, ref)[0, 1] > 0.9995
+    assert np.max(np.abs(got - want)) <= 0.03 * (np.max(np.abs(want)) + 1e-3)
+    assert np.corrcoef(got, want)[0, 1] > 0.9995
 
 
 def test_gradient_matches_reference(psh):
     X, T = _data(2000, 2)
     p = mlp.init_params(6)
     g, loss = mlp.gpu_grad(p, X, T)
-    gr = mlp.ref_grad(p.astype(np.float64), X, T)
-    assert abs(loss - mlp.ref_loss(p, X, T)) <= 0.02 * mlp.ref_loss(p, X, T)
+    gr = ref.ref_grad(p.astype(np.float64), X, T)
+    assert abs(loss - ref.ref_loss(p, X, T)) <= 0.02 * ref.ref_loss(p, X, T)
     cos = g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr))
     assert cos > 0.999
     assert np.linalg.norm(g - gr) <= 0.04 * np.linalg.norm(gr)
@@ -42,11 +45,49 @@ def test_gradient_matches_reference(psh):
 def test_training_fits_smooth_target(psh):
     X, T = _data(4096, 3)
     p0 = mlp.init_params(7)
-    l0 = mlp.ref_loss(p0, X, T)
+    l0 = ref.ref_loss(p0, X, T)
     p, loss = mlp.gpu_train(p0, X, T, epochs=150, batch=256, lr=3e-3)
     assert loss < 0.05 * l0
     Xh, Th = _data(1024, 4)
-    assert mlp.ref_loss(p, Xh, Th) < 0.1 * l0   # generalises
+    assert ref.ref_loss(p, Xh, Th) < 0.1 * l0   # generalises
+
+
+def test_training_is_reproducible(psh):
+    """Fixed-point gradient sums: two runs from one seed give identical
+    parameters, whatever order the waves add their gradients in."""
+    X, T = _data(2048, 8)
+    p0 = mlp.init_params(9)
+    a, la = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
+    b, lb = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
+    assert np.array_equal(a, b)
+
+
+def test_stage_rejects_interlaced_and_keeps_cells(psh):
+    """The stage belongs to plain boxes with |mu| cells; the cells of a run
+    add up to its bins' raw sums, and the features come from the run."""
+    conf = psh.make_conf(gsize=64, assign=psh.CIC, intlace=True, bsize=500.0, kbin=0.02)
+    il = psh.Context(conf)
+    il.set_mu_cells(8)
+    with pytest.raises(psh.PshError):
+        il.set_alias_mlp(mlp.init_params(0))
+    d = np.column_stack([np.random.default_rng(1).uniform(0, 500.0, (50000, 3)),
+                         np.ones(50000)])
+    dev = psh.DeviceArray.from_numpy(d)
+    plain = psh.Context(psh.make_conf(gsize=64, assign=psh.CIC, bsize=500.0, kbin=0.02))
+    ref_run = plain.run_device(dev, len(d), float(len(d)))
+    plain.set_mu_cells(8)
+    cell_run = plain.run_device(dev, len(d), float(len(d)))
+    scale = np.abs(ref_run["pl"][0][0]) + 500.0 ** 3 / len(d)
+    assert np.all(np.abs(cell_run["pl"][0] - ref_run["pl"][0]) <= 1e-10 * scale)
+    X = plain.alias_rows()
+    assert X.shape == (plain.nbin() * 8, 32)
+    live = X[:, 0] != 0
+    assert live.sum() > 0.9 * len(X)
+    assert np.all(X[live, 2 + psh.CIC] == 1) and np.all(X[:, mlp.NFEAT:] == 0)
+    # a network with zero output leaves the plain estimate unchanged
+    plain.set_alias_mlp(np.zeros(mlp.NPAR, np.float32))
+    zero = plain.run_device(dev, len(d), float(len(d)))
+    assert np.all(np.abs(zero["pl"][0] - ref_run["pl"][0]) <= 1e-10 * scale)
 
 
 BOX512 = dict(Ng=512, ng_gen=512, L=1000.0, nbar=0.02, kbin=0.01)
@@ -60,40 +101,39 @@ def trained(psh):
     index [-2, -1]."""
     X, T, _ = mlp.dataset(psh, range(5000, 5064), design=mlp.grid_design(8), **BOX512)
     p, loss = mlp.train_ensemble(X, T)
+    print(f"training rows {len(T)}, mean final loss {loss:.3e}", flush=True)
     return p, loss
 
 
-def test_training_is_reproducible(psh):
-    """Fixed-point gradient sums: two runs from one seed give identical
-    parameters, whatever order the waves add their gradients in."""
-    X, T = _data(2048, 8)
-    p0 = mlp.init_params(9)
-    a, la = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
-    b, lb = mlp.gpu_train(p0, X, T, epochs=5, batch=256, lr=1e-3)
-    assert np.array_equal(a, b)
-
-
 def test_alias_correction_reference_spectrum(psh, trained):
-    """Held-out boxes of P = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2) (new seeds): the
-    corrected plain monopole is within 0.5 % of the interlaced one for
-    k <= 0.8 k_Ny, ten times closer than the Jing-corrected plain one."""
+    """Held-out boxes of P = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2) (new seeds), for
+    k <= 0.8 k_Ny and relative to the bin's un-subtracted interlaced
+    monopole: the library stage's P_0 is within 0.5 % of the interlaced
+    estimator's and ten times closer than the Jing-corrected plain one; its
+    P_2 and P_4, whose (2l+1) weights amplify the per-cell scatter of the
+    realisation's alias power, within 2 % and 4 %, three times closer than
+    Jing's (measured on MI355X: worst 1.6 % and 2.9 %, against 7.7 % and
+    11.4 % for Jing's)."""
     p, loss = trained
-    _, _, held = mlp.dataset(psh, range(6100, 6104), design=[(1e4, -1.5)] * 4, **BOX512)
-    e_mlp, e_jing = mlp.errors(p, held, KNY)
-    print(f"train loss {loss:.2e}; worst error per box: mlp {np.round(e_mlp, 4)}, "
-          f"jing {np.round(e_jing, 4)}")
-    assert e_mlp.max() <= 0.005
-    assert 10 * e_mlp.max() < e_jing.min()
+    runs = mlp.corrected_runs(psh, p, range(6100, 6104), [(1e4, -1.5)] * 4, **BOX512)
+    e_c, e_j = mlp.errors(runs, KNY)
+    print(f"worst error per box (l = 0, 2, 4): stage {np.round(e_c, 4).tolist()}, "
+          f"jing {np.round(e_j, 4).tolist()}")
+    assert e_c[:, 0].max() <= 0.005
+    assert 10 * e_c[:, 0].max() < e_j[:, 0].min()
+    assert e_c[:, 1].max() <= 0.02 and e_c[:, 2].max() <= 0.04
+    assert np.all(3 * e_c[:, 1:].max(axis=0) < e_j[:, 1:].min(axis=0))
 
 
 def test_alias_correction_family(psh, trained):
     """Held-out boxes with amplitude and index drawn from the training ranges:
-    the correction beats the Jing-corrected estimator on every box and by a
-    factor of ten in the median (its worst case is set by the realisation
-    scatter of the highest-variance lognormal fields)."""
+    the stage beats the Jing-corrected estimator on every box for every pole
+    and by a factor of ten in the median monopole error."""
     p, _ = trained
-    _, _, held = mlp.dataset(psh, range(6000, 6008), **BOX512)
-    e_mlp, e_jing = mlp.errors(p, held, KNY)
-    print(f"worst error per box: mlp {np.round(e_mlp, 4)}, jing {np.round(e_jing, 4)}")
-    assert np.all(e_mlp < e_jing)
-    assert 10 * np.median(e_mlp) < np.median(e_jing)
+    seeds = range(6000, 6008)
+    runs = mlp.corrected_runs(psh, p, seeds, mlp.random_design(seeds), **BOX512)
+    e_c, e_j = mlp.errors(runs, KNY)
+    print(f"worst error per box (l = 0, 2, 4): stage {np.round(e_c, 4).tolist()}, "
+          f"jing {np.round(e_j, 4).tolist()}")
+    assert np.all(e_c < e_j)
+    assert 10 * np.median(e_c[:, 0]) < np.median(e_j[:, 0])

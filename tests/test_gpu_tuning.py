@@ -38,19 +38,27 @@ def test_invalid_tiles_rejected(psh):
         assert e.value.code == psh.ERR_ARG
 
 
-def test_too_many_bins_rejected(psh):
-    """The fused x pass keeps a workgroup's bin sums in LDS: a binning that
-    does not fit is refused when the context is made, with the limit in the
-    message, and a coarser one at the same grid size runs."""
-    with pytest.raises(psh.PshError) as e:
-        psh.Context(psh.make_conf(gsize=256, bsize=1000.0, kbin=1e-4, intlace=True,
-                                  poles=[0, 2, 4]))
-    assert e.value.code == psh.ERR_UNSUPPORTED and "at most" in str(e.value)
-    ctx = psh.Context(psh.make_conf(gsize=256, bsize=1000.0, kbin=1e-3, intlace=True,
-                                    poles=[0, 2, 4]))
-    d = _uniform(20000, [1000.0] * 3, 54)
-    res = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
-    assert res["nbin"] == 804
+def test_many_bins_in_windows(psh, oracle):
+    """Binnings whose sums exceed the fused x pass's LDS are binned in
+    windows of bins, one launch each (the reference has no bin limit,
+    src/multipole.c:340-356): 8042 bins x 3 poles at N_g = 256 for a box and
+    over 10^4 bins of a survey at N_g = 128, against the oracle."""
+    from test_gpu_parity import _survey_cats
+    d = _uniform(200000, [1000.0] * 3, 54)
+    kw = dict(gsize=256, assign=2, bsize=1000.0, kbin=1e-4, intlace=True, poles=[0, 2, 4])
+    ctx = psh.Context(psh.make_conf(**kw))
+    assert ctx.nbin() == 8042
+    got = ctx.run_device(psh.DeviceArray.from_numpy(d), len(d), float(len(d)))
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    _check(got, ref, [0, 2, 4], ref["shot"][0])
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(100000, 55)
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    sd, sr = np.column_stack([dp, wd]), np.column_stack([rp, wr])
+    kw = dict(gsize=128, assign=1, intlace=True, issim=False, kbin=2e-5, poles=[0, 2])
+    got = psh.Context(psh.make_conf(**kw)).run_host([sd], [sr], [sums])
+    ref = oracle.run(oracle.make_conf(**kw), [sd], [sr], [sums])
+    assert ref["nbin"] > 10000
+    _check(got, ref, [0, 2], sums["shot"], sums["norm"])
 
 
 def test_autotune_and_cache(psh, tmp_path):
@@ -73,3 +81,38 @@ def test_autotune_and_cache(psh, tmp_path):
     other = psh.Context(psh.make_conf(gsize=32, assign=2, intlace=True, bsize=500.0, kbin=0.02))
     with pytest.raises(psh.PshError):
         other.load_tuning(str(cache))     # no entry for Ng = 32
+
+
+@pytest.mark.parametrize("Ng", [64, 256, 1024])
+def test_fft_schedules_keep_parity(psh, oracle, Ng):
+    """Every y-pass schedule that applies at a grid size (columns per tile,
+    values per thread, radix plan) gives the oracle's spectra; the tuner
+    times them, writes its choice to the cache and a context applies it."""
+    L = 1000.0
+    d = _uniform(150000 if Ng < 1024 else 400000, [L] * 3, 60 + Ng, weights=True)
+    dev = psh.DeviceArray.from_numpy(d)
+    kw = dict(gsize=Ng, assign=3, intlace=True, bsize=L, kbin=0.01 if Ng < 1024 else 0.02,
+              poles=[0, 2, 4])
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    ctx = psh.Context(psh.make_conf(**kw))
+    ran = []
+    for ys in range(6):
+        try:
+            ctx.set_fft_schedule(ys)
+        except psh.PshError as e:
+            assert e.code == psh.ERR_ARG
+            continue
+        assert ctx.fft_schedule() == ys
+        got = ctx.run_device(dev, len(d), float(d[:, 3].sum()))
+        _check(got, ref, [0, 2, 4], ref["shot"][0])
+        ran.append(ys)
+    assert 0 in ran and len(ran) >= 3
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as tmp:
+        cache = os.path.join(tmp, "tune.txt")
+        ys, ms, ncand, ms0 = psh.autotune_fft(psh.make_conf(**kw), 1, cache)
+        assert ncand == len(ran) and ys in ran and 0 < ms <= ms0
+        other = psh.Context(psh.make_conf(**kw))
+        other.load_tuning(cache)
+        assert other.fft_schedule() == ys
+    print(f"Ng {Ng}: schedules {ran}, tuned {ys} ({ms:.3f} ms vs {ms0:.3f} ms default)")

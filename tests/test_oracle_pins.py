@@ -349,8 +349,11 @@ def _numpy_sim_interlaced(data, Ng, L, dk, poles, scheme):
 
 
 @pytest.mark.parametrize("scheme", [0, 1, 2, 3])
-def test_run_sim_interlaced_matches_numpy(oracle, scheme):
-    Ng, L, dk, poles = 16, 400.0, 0.02, [0, 2, 4]
+@pytest.mark.parametrize("Ng", [16, 12, 20])
+def test_run_sim_interlaced_matches_numpy(oracle, scheme, Ng):
+    """Also on grids of 3 * 2^2 and 5 * 2^2 cells, whose transforms the
+    oracle takes as direct DFTs."""
+    L, dk, poles = 400.0, 0.02, [0, 2, 4]
     H = L / Ng
     d = _particles(800, [L] * 3, 70 + scheme, w=True)
     d[:20, 2] = L - 0.5 * H               # on the shifted grid's edge

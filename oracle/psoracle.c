@@ -12,7 +12,9 @@
  * contraction, like the reference (Makefile:1-25).
  *
  * The FFT is this file's own (FFTW3 is absent from the image): an exact-
- * twiddle radix-2 FFT for power-of-two lengths and a direct DFT otherwise,
+ * twiddle radix-2 FFT for power-of-two lengths, one decimation-in-time
+ * step by the odd factor for 2^a times an odd factor up to 25, and a
+ * direct DFT otherwise,
  * with FFTW's r2c/c2r layout and sign conventions (src/fftw_define.h:47-64).
  */
 #include "psoracle.h"
@@ -414,13 +416,16 @@ void pso_precomp(int Ng, const double L[3], int intlace, int assign,
 typedef struct {
   int n;
   int pow2;
+  int P, odd;   /* n = P * odd, P the largest power-of-two factor */
   double *tw;   /* n entries of exp(-2 pi i m / n) */
-  int *rev;     /* bit reversal (pow2 only) */
+  int *rev;     /* bit reversal of length P */
 } fftplan1;
 
 static void plan1_init(fftplan1 *p, int n) {
   p->n = n;
   p->pow2 = (n > 0) && ((n & (n - 1)) == 0);
+  p->P = n & -n;
+  p->odd = n / p->P;
   p->tw = malloc(sizeof(double) * 2 * (size_t)n);
   for (int m = 0; m < n; m++) {
     /* exact angle per entry, no recurrence */
@@ -428,16 +433,47 @@ static void plan1_init(fftplan1 *p, int n) {
     p->tw[2 * m] = cos(ang);
     p->tw[2 * m + 1] = sin(ang);
   }
-  p->rev = NULL;
-  if (p->pow2) {
-    int lg = 0;
-    while ((1 << lg) < n) lg++;
-    p->rev = malloc(sizeof(int) * (size_t)n);
-    for (int m = 0; m < n; m++) {
-      int r = 0;
-      for (int b = 0; b < lg; b++)
-        if (m & (1 << b)) r |= 1 << (lg - 1 - b);
-      p->rev[m] = r;
+  int lg = 0;
+  while ((1 << lg) < p->P) lg++;
+  p->rev = malloc(sizeof(int) * (size_t)p->P);
+  for (int m = 0; m < p->P; m++) {
+    int r = 0;
+    for (int b = 0; b < lg; b++)
+      if (m & (1 << b)) r |= 1 << (lg - 1 - b);
+    p->rev[m] = r;
+  }
+}
+
+/* Radix-2 transform of length P = p->P in place; its twiddles
+ * exp(-2 pi i q / P) are entries q * odd of the length-n table. */
+static void fft_pow2(const fftplan1 *p, double *a, int sign) {
+  const int n = p->P, ts = p->odd;
+  for (int m = 0; m < n; m++) {
+    int r = p->rev[m];
+    if (r > m) {
+      double t0 = a[2 * m], t1 = a[2 * m + 1];
+      a[2 * m] = a[2 * r];
+      a[2 * m + 1] = a[2 * r + 1];
+      a[2 * r] = t0;
+      a[2 * r + 1] = t1;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len >> 1;
+    int step = n / len;
+    for (int s = 0; s < n; s += len) {
+      for (int q = 0; q < half; q++) {
+        double wr = p->tw[2 * (q * step * ts)];
+        double wi = sign * p->tw[2 * (q * step * ts) + 1];
+        double *u = a + 2 * (s + q);
+        double *v = a + 2 * (s + q + half);
+        double tr = v[0] * wr - v[1] * wi;
+        double ti = v[0] * wi + v[1] * wr;
+        v[0] = u[0] - tr;
+        v[1] = u[1] - ti;
+        u[0] += tr;
+        u[1] += ti;
+      }
     }
   }
 }
@@ -453,33 +489,35 @@ static void fft1(const fftplan1 *p, double *a, int sign, double *work) {
   const int n = p->n;
   if (n == 1) return;
   if (p->pow2) {
-    for (int m = 0; m < n; m++) {
-      int r = p->rev[m];
-      if (r > m) {
-        double t0 = a[2 * m], t1 = a[2 * m + 1];
-        a[2 * m] = a[2 * r];
-        a[2 * m + 1] = a[2 * r + 1];
-        a[2 * r] = t0;
-        a[2 * r + 1] = t1;
+    fft_pow2(p, a, sign);
+    return;
+  }
+  if (p->P >= 2 && p->odd <= 25) {
+    /* n = P * odd, decimation in time by the odd factor: the odd
+     * subsequences y_r[j] = x[j odd + r] (r < odd) are transformed with
+     * the radix-2 code, then X[k] = sum_r exp(sign 2 pi i r k / n)
+     * Y_r[k mod P], O(n odd) */
+    const int P = p->P, m = p->odd;
+    for (int r = 0; r < m; r++) {
+      double *y = work + 2 * (size_t)r * P;
+      for (int j = 0; j < P; j++) {
+        y[2 * j] = a[2 * ((size_t)j * m + r)];
+        y[2 * j + 1] = a[2 * ((size_t)j * m + r) + 1];
       }
+      fft_pow2(p, y, sign);
     }
-    for (int len = 2; len <= n; len <<= 1) {
-      int half = len >> 1;
-      int step = n / len;
-      for (int s = 0; s < n; s += len) {
-        for (int q = 0; q < half; q++) {
-          double wr = p->tw[2 * (q * step)];
-          double wi = sign * p->tw[2 * (q * step) + 1];
-          double *u = a + 2 * (s + q);
-          double *v = a + 2 * (s + q + half);
-          double tr = v[0] * wr - v[1] * wi;
-          double ti = v[0] * wi + v[1] * wr;
-          v[0] = u[0] - tr;
-          v[1] = u[1] - ti;
-          u[0] += tr;
-          u[1] += ti;
-        }
+    for (int k = 0; k < n; k++) {
+      double sr = 0, si = 0;
+      const int kp = k % P;
+      for (int r = 0; r < m; r++) {
+        const size_t e = ((size_t)r * k) % n;
+        const double wr = p->tw[2 * e], wi = sign * p->tw[2 * e + 1];
+        const double *y = work + 2 * ((size_t)r * P + kp);
+        sr += y[0] * wr - y[1] * wi;
+        si += y[0] * wi + y[1] * wr;
       }
+      a[2 * k] = sr;
+      a[2 * k + 1] = si;
     }
     return;
   }

@@ -3,8 +3,9 @@ FP64 oracle, through the C-ABI.  The reference takes any GRID_SIZE
 (src/load_conf.c:1018-1030 checks only its range; FFTW plans every length);
 these grids run the mixed-radix transforms (a radix-3 or radix-5 last pass
 after the power-of-two passes) and deposit tiles no larger than the grid's
-power-of-two factor.  The oracle's transform of such a length is a direct
-DFT (pinned against numpy in test_oracle_math).
+power-of-two factor.  The oracle transforms such a length by one
+decimation-in-time step by the odd factor over radix-2 transforms (pinned
+against numpy in test_oracle_math).
 
 Bar as in test_gpu_parity: counts bit-exact, |dP_l| <= 1e-4 S_b per bin."""
 import numpy as np
@@ -28,6 +29,22 @@ def test_sim_mixed_radix_grids(psh, oracle, Ng, scheme, intlace):
     w = _check(got, ref, [0, 1, 2, 3, 4], ref["shot"][0])
     print(f"Ng={Ng} scheme={scheme} intlace={intlace}: {ref['nbin']} bins, worst {w:.2e}",
           flush=True)
+
+
+@pytest.mark.parametrize("Ng,scheme,intlace,kbin", [(768, 3, True, 0.005), (1536, 1, False, 0.01)])
+def test_sim_large_mixed_radix_grids(psh, oracle, Ng, scheme, intlace, kbin):
+    """Full-size grids of 3 * 2^8 and 3 * 2^9 cells (a radix-3 last pass on
+    N_g = 768 and 1536 with the streaming x, y and z kernels) against the
+    oracle."""
+    L, n = 1000.0, 2_000_000
+    d = _uniform(n, [L] * 3, Ng, weights=False)
+    kw = dict(gsize=Ng, assign=scheme, intlace=intlace, bsize=L, kbin=kbin, poles=[0, 2, 4])
+    got = psh.Context(psh.make_conf(**kw)).run_device(psh.DeviceArray.from_numpy(d), n,
+                                                      float(n))
+    print(f"Ng={Ng}: GPU done, {got['nbin']} bins", flush=True)
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    w = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"Ng={Ng} scheme={scheme} intlace={intlace}: worst {w:.2e}", flush=True)
 
 
 def test_sim_mixed_radix_rect_box(psh, oracle):

@@ -14,7 +14,9 @@ import scipy.special as sps
 
 # ---- FFT: against numpy (pocketfft) and a direct DFT -------------------------
 
-@pytest.mark.parametrize("Ng", [4, 6, 8, 9, 16, 32])
+# 6, 12, 20, 24, 40, 48, 50, 18: the odd-factor step (odd parts 3, 5, 9, 25);
+# 9, 14: the direct DFT
+@pytest.mark.parametrize("Ng", [4, 6, 8, 9, 12, 14, 16, 18, 20, 24, 32, 40, 48, 50])
 def test_fft_r2c_matches_numpy(oracle, Ng):
     rng = np.random.default_rng(Ng)
     x = rng.standard_normal((Ng, Ng, Ng))
@@ -59,8 +61,8 @@ def test_fft_parseval_and_c2r(oracle):
     assert np.allclose(oracle.fft_c2r(F, Ng) / Ng ** 3, x, atol=1e-13)
 
 
-def test_fft_c2r_matches_numpy_on_hermitian_input(oracle):
-    Ng = 8
+@pytest.mark.parametrize("Ng", [8, 12, 20])
+def test_fft_c2r_matches_numpy_on_hermitian_input(oracle, Ng):
     rng = np.random.default_rng(9)
     F = np.fft.rfftn(rng.standard_normal((Ng, Ng, Ng)))
     assert np.allclose(oracle.fft_c2r(F, Ng), np.fft.irfftn(F, s=(Ng,) * 3, axes=(0, 1, 2)) * Ng ** 3,

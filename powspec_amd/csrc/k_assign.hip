@@ -685,15 +685,18 @@ __device__ __forceinline__ void deposit_one(typename Acc<WIDE>::T *lds, const in
       }
     }
   }
-  // ZPAD: offset in a padded row of the window's first z cell, b - T0 + W in
-  // [0, T[2] + W] (cells up to T[2] + 2W - 1 < P2).  Windows wrapped by Ng
-  // (tile at z = 0, b in [Ng - W, Ng)) give b + W in [Ng, Ng + W): with
-  // T[2] + W < Ng subtracting Ng maps them to b - Ng + W and nothing else
-  // moves.
+  // ZPAD: offset in a padded row of the window's first z cell, b - T0 + W
+  // modulo Ng, in [0, T[2] + W] (cells up to T[2] + 2W - 1 < P2).  The
+  // difference itself lies in [W + T[2] - Ng, Ng + W): windows wrapped by Ng
+  // at the tile at z = 0 (b in [Ng - W, Ng)) give b + W in [Ng, Ng + W),
+  // and the shifted window of the last tile whose base wrapped to 0 gives
+  // T[2] + W - Ng < 0; with T[2] + W < Ng one addition or subtraction of Ng
+  // maps both into range and moves nothing else.
   int zoff = W;
   if (ZPAD) {
     zoff = b[2] - T0[2] + W;
     if (zoff >= Ng) zoff -= Ng;
+    else if (zoff < 0) zoff += Ng;
   }
   if (SCHEME == 3) w *= (1.0f / 216.0f);
   const float half = __builtin_copysignf(0.5f, w);   // window weights are >= 0

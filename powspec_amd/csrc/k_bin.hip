@@ -35,95 +35,118 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
   return kb;
 }
 
-// One workgroup per bin.  A mode (i, j, k) belongs to bin b when the
-// reference's FP64 expression puts it there (mode_bin above); the workgroup
-// of bin b visits every (i, j) row, bounds the k_z indices whose |k| can
-// fall in [kedge_b, kedge_b+1) -- widened by two indices against rounding --
-// and decides each candidate with mode_bin, so every mode is counted by
-// exactly one workgroup.  Threads take rows in a fixed stride and sum in
-// registers in row order; the block then adds the thread sums in a fixed
-// tree.  The results (cnt, km, lcnt) are therefore the same on every run.
+// Mode-parallel geometry with order-independent sums.  A work item is a run
+// of kGeoRun consecutive k_z of one (i, j) row; along a run |k| only grows,
+// so its modes fall in consecutive bins and a thread sums each bin's modes
+// in registers (FP64, in k_z order) and adds the run total to the
+// workgroup's LDS accumulators only when the bin changes.  Those additions
+// are integer: cnt exactly, km and lcnt as two-word fixed point (x 2^s split
+// into floor(x 2^s) and the remainder's 32 fractional bits, both summed as
+// 64-bit integers, so every sum is exact and independent of the order in
+// which threads arrive).  Each workgroup converts its sums to FP64 and
+// writes them as its row of the partials, which the fixed-order reduction
+// adds up: every output is the same on every run.  Bins beyond the LDS go
+// in windows of nbw bins, one launch each.
 constexpr int kGeoThreads = 256;
+constexpr int kGeoRun = 32;
+constexpr int kGeoGrid = 256;
+constexpr int kGeoMaxWindow = 1024;
+
+struct GeoArgs {
+  int N;
+  PshAxisTables ax;
+  PshBins b;
+  int kb0, nbw;
+  double skm, slc;   // fixed-point scales 2^s of km and lcnt
+  double kskip2;     // rows with k_x^2 + k_y^2 above this hold no binned mode
+};
+
+__device__ __forceinline__ void fixed_add(long long *hi, long long *lo, double x, double sc) {
+  const double y = x * sc;                        // exact (power of two)
+  const double h = floor(y);                      // exact
+  const double r = (y - h) * 4294967296.0;       // exact, in [0, 2^32)
+  atomicAdd(reinterpret_cast<unsigned long long *>(hi), (unsigned long long)(long long)h);
+  atomicAdd(reinterpret_cast<unsigned long long *>(lo), (unsigned long long)(long long)rint(r));
+}
+
+__device__ __forceinline__ double fixed_value(long long hi, long long lo, double sc) {
+  return (double)hi / sc + (double)lo / (sc * 4294967296.0);
+}
 
 __global__ void __launch_bounds__(kGeoThreads)
-geometry_kernel(int N, PshAxisTables ax, PshBins b, unsigned long long *__restrict__ out_cnt,
-                double *__restrict__ out_km, double *__restrict__ out_lcnt) {
+geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
+                double *__restrict__ part_km, double *__restrict__ part_lc) {
 #pragma clang fp contract(off)
-  __shared__ double red[kGeoThreads];
-  __shared__ unsigned long long redc[kGeoThreads];
-  const int bin = blockIdx.x;
-  const int M = N / 2;
-  // |k| range of the bin (log bins: the edges are 0.5 log10 |k|^2)
-  double klo = b.kedge0 + b.dk * bin, khi = b.kedge0 + b.dk * (bin + 1);
-  if (b.logscale) {
-    klo = pow(10.0, klo);
-    khi = pow(10.0, khi);
-  }
-  if (klo < 0) klo = 0;
-  const double klo2 = klo * klo, khi2 = khi * khi;
-  const double vz = sqrt(ax.ksq[2][1]);   // |k_z| of index 1
-  unsigned long long cnt = 0;
-  double km = 0.0;
-  double lc[PSH_MAX_NL];
-  for (int l = 0; l < PSH_MAX_NL; l++) lc[l] = 0.0;
-  const size_t rows = (size_t)N * N;
-  for (size_t row = threadIdx.x; row < rows; row += kGeoThreads) {
+  extern __shared__ long long gsh[];
+  const PshBins &b = g.b;
+  const int nbw = g.nbw, nl = b.nl;
+  unsigned long long *cnt = reinterpret_cast<unsigned long long *>(gsh);
+  long long *kmh = gsh + nbw, *kml = gsh + 2 * nbw;
+  long long *lch = gsh + 3 * nbw, *lcl = gsh + (3 + nl) * nbw;
+  for (int i = threadIdx.x; i < (3 + 2 * nl) * nbw; i += kGeoThreads) gsh[i] = 0;
+  __syncthreads();
+  const int N = g.N, M = N / 2;
+  const int nseg = (M + kGeoRun) / kGeoRun;   // runs per row: ceil((M + 1) / kGeoRun)
+  const size_t items = (size_t)N * N * nseg;
+  const size_t stride = (size_t)gridDim.x * kGeoThreads;
+  int cur = -1;
+  unsigned long long rc = 0;
+  double rkm = 0.0, rlc[PSH_MAX_NL];
+  auto flush = [&]() {
+    const int w = cur - g.kb0;
+    if (cur >= 0 && w >= 0 && w < nbw) {
+      atomicAdd(&cnt[w], rc);
+      fixed_add(&kmh[w], &kml[w], rkm, g.skm);
+      for (int l = 0; l < nl; l++) fixed_add(&lch[l * nbw + w], &lcl[l * nbw + w], rlc[l], g.slc);
+    }
+    rc = 0;
+    rkm = 0.0;
+    for (int l = 0; l < PSH_MAX_NL; l++) rlc[l] = 0.0;
+  };
+  for (int l = 0; l < PSH_MAX_NL; l++) rlc[l] = 0.0;
+  for (size_t it = blockIdx.x * (size_t)kGeoThreads + threadIdx.x; it < items; it += stride) {
+    const size_t row = it / nseg;
+    const int seg = (int)(it - row * nseg);
     const int i = (int)(row / N), j = (int)(row - (size_t)i * N);
-    const double kmij = ax.ksq[0][i] + ax.ksq[1][j];
-    if (kmij > khi2 * (1 + 1e-12) + 1e-300) continue;
-    // candidate k_z indices: |k_z| in [sqrt(klo^2 - kxy^2), sqrt(khi^2 - kxy^2))
-    const double zlo = klo2 > kmij ? sqrt(klo2 - kmij) / vz : 0.0;
-    const double zhi = sqrt(fmax(khi2 - kmij, 0.0)) / vz;
-    int k0 = (int)floor(zlo) - 2, k1 = (int)ceil(zhi) + 2;
-    if (k0 < 0) k0 = 0;
-    if (k1 > M) k1 = M;
+    const double kmij = g.ax.ksq[0][i] + g.ax.ksq[1][j];
+    if (kmij > g.kskip2) continue;
+    const int k0 = seg * kGeoRun, k1 = min(k0 + kGeoRun - 1, M);
+    cur = -1;
     for (int k = k0; k <= k1; k++) {
       double kmod2 = kmij;
-      if (k > 0) kmod2 = kmij + ax.ksq[2][k];
+      if (k > 0) kmod2 = kmij + g.ax.ksq[2][k];
       double kmod;
-      if (mode_bin(kmod2, b, kmod) != bin) continue;
+      const int kb = mode_bin(kmod2, b, kmod);
+      if (kb != cur) {
+        flush();
+        cur = kb;
+      }
+      if (kb < 0) continue;
       const bool single = (k == 0) || (k == M);
-      cnt += single ? 1ull : 2ull;
-      km += single ? kmod : 2 * kmod;
+      rc += single ? 1ull : 2ull;
+      rkm += single ? kmod : 2 * kmod;
       if (k == 0 && kmod2 == 0.0) continue;  // DC: no multipole weight
       // mu divides by |k| in both binnings (src/multipole.c:940, 966): with
       // log bins kmod is 0.5 log10 |k|^2, not |k|
       const double kabs = b.logscale ? sqrt(kmod2) : kmod;
-      const double mu = (ax.kv[0][i] * b.los[0] + ax.kv[1][j] * b.los[1] +
-                         ax.kv[2][k] * b.los[2]) / kabs;
+      const double mu = (g.ax.kv[0][i] * b.los[0] + g.ax.kv[1][j] * b.los[1] +
+                         g.ax.kv[2][k] * b.los[2]) / kabs;
       const double num = single ? 1.0 : 2.0;
-      for (int l = 0; l < b.nl; l++) {
+      for (int l = 0; l < nl; l++) {
         if (!single && (b.poles[l] & 1)) continue;
-        lc[l] += num * legendre(b.poles[l], mu);
+        rlc[l] += num * legendre(b.poles[l], mu);
       }
     }
+    flush();
   }
-  // fixed-order tree over the threads
-  auto tree = [&](double v) -> double {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int h = kGeoThreads / 2; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-      __syncthreads();
-    }
-    const double r = red[0];
-    __syncthreads();
-    return r;
-  };
-  redc[threadIdx.x] = cnt;
   __syncthreads();
-  for (int h = kGeoThreads / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) redc[threadIdx.x] += redc[threadIdx.x + h];
-    __syncthreads();
-  }
-  const double kms = tree(km);
-  if (threadIdx.x == 0) {
-    out_cnt[bin] = redc[0];
-    out_km[bin] = kms;
-  }
-  for (int l = 0; l < b.nl; l++) {
-    const double t = tree(lc[l]);
-    if (threadIdx.x == 0) out_lcnt[l * b.nbin + bin] = t;
+  const size_t o = (size_t)blockIdx.x * nbw;
+  for (int w = threadIdx.x; w < nbw; w += kGeoThreads) {
+    part_cnt[o + w] = cnt[w];
+    part_km[o + w] = fixed_value(kmh[w], kml[w], g.skm);
+    for (int l = 0; l < nl; l++)
+      part_lc[(size_t)l * gridDim.x * nbw + o + w] =
+          fixed_value(lch[l * nbw + w], lcl[l * nbw + w], g.slc);
   }
 }
 
@@ -208,13 +231,59 @@ bounds_kernel(const double *__restrict__ xyzw, size_t n,
 
 }  // namespace
 
+size_t psh_geometry_scratch_bytes(const PshBins &b) {
+  const int nbw = b.nbin < kGeoMaxWindow ? b.nbin : kGeoMaxWindow;
+  return sizeof(double) * (size_t)kGeoGrid * nbw * (2 + b.nl);
+}
+
 hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
                         unsigned long long *out_cnt, double *out_km, double *out_lcnt,
-                        hipStream_t s) {
+                        void *scratch, hipStream_t s) {
   if (b.nbin < 1 || b.nl > PSH_MAX_NL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(geometry_kernel, dim3(b.nbin), dim3(kGeoThreads), 0, s, Ng, ax, b,
-                     out_cnt, out_km, out_lcnt);
-  return hipGetLastError();
+  // window: LDS of (3 + 2 nl) 64-bit words per bin, at most kGeoMaxWindow bins
+  const size_t per_bin = sizeof(long long) * (3 + 2 * (size_t)b.nl);
+  int nbw = (int)((size_t)64 * 1024 / per_bin);
+  if (nbw > kGeoMaxWindow) nbw = kGeoMaxWindow;
+  if (nbw > b.nbin) nbw = b.nbin;
+  GeoArgs g{};
+  g.N = Ng;
+  g.ax = ax;
+  g.b = b;
+  // fixed-point scales: a workgroup takes at most `modes` modes, each adding
+  // at most 2 max|kmod| to km and 2 to each lcnt (|L_l| <= 1); 2^s times the
+  // largest possible sum stays below 2^61
+  const int M = Ng / 2;
+  const double items = (double)Ng * Ng * ((M + kGeoRun) / kGeoRun);
+  const double per_thread = ceil(items / ((double)kGeoGrid * kGeoThreads));
+  const double modes = per_thread * kGeoThreads * kGeoRun;
+  const double kbig = fmax(fmax(fabs(b.kedge0), fabs(b.kedgeN)), 1e-300);
+  int e;
+  (void)frexp(2.0 * kbig * modes, &e);
+  g.skm = ldexp(1.0, 61 - e);
+  (void)frexp(2.0 * modes, &e);
+  g.slc = ldexp(1.0, 61 - e);
+  // rows whose k_x^2 + k_y^2 alone reaches the last edge have no binned mode
+  const double top = b.logscale ? pow(10.0, 2.0 * b.kedgeN) : b.kedgeN * b.kedgeN;
+  g.kskip2 = top * (1.0 + 1e-9) + 1e-300;
+  unsigned long long *pc = static_cast<unsigned long long *>(scratch);
+  double *pk = reinterpret_cast<double *>(pc + (size_t)kGeoGrid * nbw);
+  double *pl = pk + (size_t)kGeoGrid * nbw;
+  hipError_t err;
+  for (int kb0 = 0; kb0 < b.nbin; kb0 += nbw) {
+    g.kb0 = kb0;
+    g.nbw = b.nbin - kb0 < nbw ? b.nbin - kb0 : nbw;
+    const size_t lds = per_bin * g.nbw;
+    hipLaunchKernelGGL(geometry_kernel, dim3(kGeoGrid), dim3(kGeoThreads), lds, s, g, pc, pk, pl);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if ((err = launch_reduce<unsigned long long>(pc, 1, kGeoGrid, g.nbw, out_cnt + kb0, s)))
+      return err;
+    if ((err = launch_reduce<double>(pk, 1, kGeoGrid, g.nbw, out_km + kb0, s))) return err;
+    for (int l = 0; l < b.nl; l++)
+      if ((err = launch_reduce<double>(pl + (size_t)l * kGeoGrid * g.nbw, 1, kGeoGrid, g.nbw,
+                                       out_lcnt + (size_t)l * b.nbin + kb0, s)))
+        return err;
+  }
+  return hipSuccess;
 }
 
 hipError_t psh_reduce_partials(const double *partials, int nmock, int grid, int nacc,

@@ -496,8 +496,14 @@ int ensure_geometry(psh_ctx *ctx) {
   if ((rc = dev_alloc(ctx, &ctx->d_gcnt, (size_t)nb))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_gkm, (size_t)nb))) return rc;
   if ((rc = dev_alloc(ctx, &ctx->d_glcnt, (size_t)nb * nl))) return rc;
-  HIPCHK(ctx, psh_geometry(ctx->Ng, ctx->ax, ctx->bins, ctx->d_gcnt, ctx->d_gkm,
-                           ctx->d_glcnt, ctx->stream));
+  // per-workgroup partials of the geometry pass, freed when it is done
+  char *gscratch = nullptr;
+  if ((rc = dev_alloc(ctx, &gscratch, psh_geometry_scratch_bytes(ctx->bins)))) return rc;
+  hipError_t gerr = psh_geometry(ctx->Ng, ctx->ax, ctx->bins, ctx->d_gcnt, ctx->d_gkm,
+                                 ctx->d_glcnt, gscratch, ctx->stream);
+  if (gerr == hipSuccess) gerr = hipStreamSynchronize(ctx->stream);
+  ctx_free(ctx, gscratch);
+  HIPCHK(ctx, gerr);
   ctx->cnt.assign(nb, 0);
   ctx->km.assign(nb, 0.0);
   ctx->lcnt.assign((size_t)nb * nl, 0.0);

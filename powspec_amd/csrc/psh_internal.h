@@ -208,10 +208,13 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
 // ---------------------------------------------------------------------------
 
 // cnt (uint64), km (sum of kmod*mult) and lcnt (sum of mult*L_l(mu), DC
-// skipped) per bin into out_cnt, out_km, out_lcnt[nl*nbin]; one workgroup
-// per bin, sums in a fixed order (the same on every run).
+// skipped) per bin into out_cnt, out_km, out_lcnt[nl*nbin]; the sums are
+// exact integer or fixed-point sums per workgroup added in a fixed order
+// (the same on every run).  scratch: psh_geometry_scratch_bytes(b) bytes.
+size_t psh_geometry_scratch_bytes(const PshBins &b);
 hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
-    unsigned long long *out_cnt, double *out_km, double *out_lcnt, hipStream_t s);
+    unsigned long long *out_cnt, double *out_km, double *out_lcnt, void *scratch,
+    hipStream_t s);
 
 // Sum partials[nmock][grid][nacc] over grid in a fixed order into
 // out[nmock][nacc].

@@ -211,19 +211,26 @@ def batched_ceiling(counts, Ng=256, nmesh=2):
     return len(counts) / (per.sum() / (HBM_PEAK_GBS * 1e9))
 
 
-def run_batched(P, nmock, seed=1000):
+def run_batched(P, nmock, seed=1000, fixed_tile=None):
     """Many small clustered mocks (TSC, N_g = 256, interlaced): one batched
-    launch sequence replayed as a hipGraph vs one plain run per mock."""
+    launch sequence replayed as a hipGraph vs one plain run per mock.
+    fixed_tile (profiling runs): deposit tiles of both instead of tuning
+    them, and y-pass schedule 0."""
     t = time.time()
     dev, off, sizes = gen_mocks(P, nmock, seed)
     log(f"batched mocks: {nmock} mocks, {int(off[-1])} particles generated in "
         f"{time.time() - t:.1f} s")
     kw = dict(gsize=256, assign=P.TSC, intlace=True, bsize=1000.0, kbin=0.01, poles=[0, 2, 4])
     t = time.time()
-    tile, tms, ncand = P.autotune(P.make_conf(**kw), nmock=nmock, data=dev, offsets=off)
-    log(f"batched tiles: {tile} ({tms:.1f} ms per batch, {ncand} candidates, "
-        f"{time.time() - t:.1f} s)")
-    ysched = P.autotune_fft(P.make_conf(**kw), nmock)[0]
+    if fixed_tile:
+        tile, ysched, ysched1 = tuple(fixed_tile), 0, 0
+        log(f"batched tiles: {tile} (given), y-pass schedule 0")
+    else:
+        tile, tms, ncand = P.autotune(P.make_conf(**kw), nmock=nmock, data=dev, offsets=off)
+        log(f"batched tiles: {tile} ({tms:.1f} ms per batch, {ncand} candidates, "
+            f"{time.time() - t:.1f} s)")
+        ysched = P.autotune_fft(P.make_conf(**kw), nmock)[0]
+        ysched1 = P.autotune_fft(P.make_conf(**kw), 1)[0]
     ctx = P.Context(P.make_conf(**kw), nmock_max=nmock)
     ctx.set_tiles(tile)
     ctx.set_fft_schedule(ysched)
@@ -240,7 +247,7 @@ def run_batched(P, nmock, seed=1000):
     # plain: one mock per launch sequence
     single = P.Context(P.make_conf(**kw))
     single.set_tiles(tile)
-    single.set_fft_schedule(P.autotune_fft(P.make_conf(**kw), 1)[0])
+    single.set_fft_schedule(ysched1)
     ptr0 = dev.ptr.value
 
     class View:  # device pointer into the concatenated buffer
@@ -282,7 +289,7 @@ def run_batched(P, nmock, seed=1000):
             "batched_particles_per_s": round(int(off[-1]) / t_batch, 1)}
 
 
-def run_alias(P, steps=10, warmup=2, params_path=None):
+def run_alias(P, steps=10, warmup=2, params_path=None, fixed_tile=None):
     """Learned alias correction: a held-out clustered box of the training
     family (P = 1e4 (k/0.1)^-1.5 exp(-(k/0.5)^2), 512^3 generator grid,
     nbar = 0.02, seed 6200), CIC, N_g = 512, l = 0,2,4, dk = 0.01; the plain
@@ -300,7 +307,8 @@ def run_alias(P, steps=10, warmup=2, params_path=None):
     out = {}
     for name, il in (("plain_mlp", False), ("interlaced", True)):
         conf = mlp.box_conf(P, 512, 1000.0, 0.01, P.CIC, il)
-        tile = P.autotune(conf, nmock=1, data=dev, offsets=[0, n])[0]
+        tile = (tuple(fixed_tile) if fixed_tile else
+                P.autotune(conf, nmock=1, data=dev, offsets=[0, n])[0])
         ctx = P.Context(conf)
         ctx.set_tiles(tile)
         if not il:
@@ -552,6 +560,9 @@ def main():
                     help="deposit tile of the N_g=1024 box 'tx,ty,tz' instead of autotuning (profiling runs)")
     ap.add_argument("--ysched", default=None,
                     help="y-pass schedule of the N_g=1024 box instead of tuning (profiling runs)")
+    ap.add_argument("--batch-tile", default=None,
+                    help="deposit tile of the batched workload 'tx,ty,tz' instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--only", default=None, choices=["batched", "survey", "alias"],
                     help="run one secondary workload alone, no JSON line (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -571,9 +582,12 @@ def main():
         sys.exit(1)
     arch = P.device_arch()
     if args.only:
-        r = {"batched": lambda: run_batched(P, args.nmock),
+        btile = [int(v) for v in args.batch_tile.split(",")] if args.batch_tile else None
+        r = {"batched": lambda: run_batched(P, args.nmock, fixed_tile=btile),
              "survey": lambda: run_survey(P),
-             "alias": lambda: run_alias(P, params_path=args.alias_params)}[args.only]()
+             "alias": lambda: run_alias(P, params_path=args.alias_params,
+                                        fixed_tile=[int(v) for v in args.tile.split(",")]
+                                        if args.tile else None)}[args.only]()
         log(json.dumps(r))
         return
     fixed = [int(v) for v in args.tile.split(",")] if args.tile else None

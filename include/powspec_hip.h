@@ -431,6 +431,15 @@ int psh_ctx_load_tuning(psh_ctx *ctx, const char *cache_path);
 int psh_ctx_set_fft_schedule(psh_ctx *ctx, int ysched);
 int psh_ctx_get_fft_schedule(const psh_ctx *ctx, int *ysched);
 
+/* The radix plan of a length-n transform (no counterpart in the
+ * reference, whose FFTW plans are opaque), n = 2^a, 3 * 2^a or 5 * 2^a
+ * (a >= 1) up to 4096 -- the lengths of the x and y passes (GRID_SIZE) and
+ * of the z pass's complex rows (GRID_SIZE / 2): plan 0, 1 or 2 as above;
+ * writes the radix of each pass in order into radices[0..max) and returns
+ * the number of passes, 0 for any other n.  Host-only (no device call);
+ * used to check the plans on the CPU. */
+int psh_fft_plan(int n, int plan, int *radices, int max);
+
 typedef struct {
   int ysched;      /* fastest y-pass schedule               */
   double ms;       /* its median time per launch            */

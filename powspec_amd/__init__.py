@@ -76,7 +76,7 @@ EXPORTS = [
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
     "psh_fault_inject", "psh_run_device_static", "psh_ctx_set_mu_cells", "psh_ctx_cells",
     "psh_ctx_alias_rows", "psh_ctx_set_alias_mlp", "psh_ctx_set_fft_schedule",
-    "psh_ctx_get_fft_schedule", "psh_autotune_fft",
+    "psh_ctx_get_fft_schedule", "psh_autotune_fft", "psh_fft_plan",
 ]
 
 
@@ -91,6 +91,15 @@ class TuneFft(C.Structure):
 
     _fields_ = [("ysched", C.c_int), ("ms", C.c_double), ("ncand", C.c_int),
                 ("ms_default", C.c_double)]
+
+
+def fft_plan(Ng, plan=0):
+    """Radices of the passes of a length-Ng transform under radix plan 0, 1
+    or 2 (host-only; [] when Ng is not a supported grid size)."""
+    L = load()
+    r = (C.c_int * 16)()
+    n = L.psh_fft_plan(int(Ng), int(plan), r, 16)
+    return [r[i] for i in range(min(n, 16))]
 
 
 def autotune_fft(conf, nmock=1, cache_path=None):
@@ -232,6 +241,7 @@ def load():
     L.psh_ctx_set_fft_schedule.argtypes = [vp, i]
     L.psh_ctx_get_fft_schedule.argtypes = [vp, C.POINTER(i)]
     L.psh_autotune_fft.argtypes = [C.POINTER(Conf), i, C.c_char_p, C.POINTER(TuneFft)]
+    L.psh_fft_plan.argtypes = [i, i, C.POINTER(C.c_int), i]
     _lib = L
     return L
 

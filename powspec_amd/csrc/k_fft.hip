@@ -1153,6 +1153,15 @@ struct XBinLds {
 };
 }  // namespace
 
+extern "C" int psh_fft_plan(int Ng, int plan, int *radices, int max) {
+  if (Ng < 2 || Ng > 4096 || plan < 0 || plan > 2 || !radices) return 0;
+  const int odd = odd_part(Ng);
+  if ((odd != 1 && odd != 3 && odd != 5) || (odd > 1 && pow2_part(Ng) < 2)) return 0;
+  const int n = npasses(Ng, plan);
+  for (int p = 0; p < n && p < max; p++) radices[p] = radix(Ng, p, plan);
+  return n;
+}
+
 bool psh_grid_supported(int Ng) {
   switch (Ng) {
     case 4: case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024:

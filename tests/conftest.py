@@ -37,3 +37,11 @@ def psh():
     if P.device_count() < 1:
         pytest.skip("no GPU visible")
     return P
+
+
+@pytest.fixture(scope="session")
+def psh_host():
+    """The HIP library for its host-only entry points (no GPU needed)."""
+    import powspec_amd as P
+    P.load()
+    return P

@@ -475,7 +475,7 @@ struct XBinDev {
   int nacc;        // accumulators of the launch's bin window (LDS)
   int nacc_full;   // accumulators per workgroup row of partials (all bins)
   int kb0, nbw;    // bin window [kb0, kb0 + nbw) of the launch
-  double xbound;   // (largest |contribution| / |v|) x N C, with margins
+  double xbound;   // (largest |pair contribution| / phase 1's max) x N C, with margins
   int losx0;       // the line of sight has no x component
   int has_dc;      // replace the k = 0 mode by dcf
   float dcf;
@@ -697,6 +697,14 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     // phase 1
     float2 pyz = make_float2(1.f, 0.f);
     if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
+    // the tile's largest |contribution| before the Legendre / Y_lm factor,
+    // |v| alias_x(i) w_yz (the simulation's DC mode, never binned, left
+    // out): it sets the tile's fixed-point scale, so the scale follows the
+    // modes that are summed and not the DC power or the alias factors of
+    // other columns
+    const float wcol = a.ax.alf[1][y] * a.ax.alf[2][kz] *
+                       ((kz == 0 || kz == M) ? 1.f : 2.f) *
+                       (INTLACE && EPI == PSH_EPI_SIM ? 0.25f : 1.f);
     float vmax = 0.f;
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
@@ -733,7 +741,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         }
         if (EPI == PSH_EPI_SURVEY0 && a.store) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
         lds[L::idx(c, j) + L::off(r * G::NS)] = v;   // i = j + r NS
-        vmax = fmaxf(vmax, fabsf(v));
+        if (EPI != PSH_EPI_SIM || y != 0 || kz != 0 || i != 0)
+          vmax = fmaxf(vmax, fabsf(v) * (xtab[i].x * wcol));
       }
     }
     // largest |v| of the tile (non-negative floats order as their bits)
@@ -744,9 +753,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
 
     // phase 2: column c, pairs (i, N - i) with i = threadIdx.x / C + k T / C
     // up to M.  The tile's integer sums take the scale 2^E with
-    // tmax xbound < 2^(E + 61): every contribution is below tmax xbound /
-    // (N C) and an accumulator takes at most N C of them, so no sum leaves
-    // 2^61 and each contribution stays below 2^51 (acc_fixed).
+    // tmax xbound < 2^(E + 61): every contribution (a pair's, w + w') is
+    // below tmax xbound / (N C) and an accumulator takes at most N C of
+    // them, so no sum leaves 2^61 and each contribution stays below 2^51
+    // (acc_fixed).
     int E = 0;
     {
       const float tmax = __uint_as_float(*tmaxp);
@@ -1227,15 +1237,13 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.nbw = a.nbw > 0 ? a.nbw : a.bins.nbin;
   if (d.kb0 < 0 || d.kb0 + d.nbw > a.bins.nbin) return hipErrorInvalidValue;
   d.nacc = per_bin * d.nbw;
-  // bound of |contribution| / |v| per mode pair, times N C (at least 1024):
-  // alias factors at their largest, multiplicity 2, the interlace 1/4 of
-  // the box epilogue, |L_l| or |Y_lm| (real form, l <= 6) below 1.01 and
-  // 2, two modes per pair, a 2 % margin for FP32 rounding
+  // bound of a pair's |contribution| over the tile's largest |v| alias_x
+  // w_yz (which phase 1 measures), times N C (at least 1024): |L_l| or
+  // |Y_lm| (real form, l <= 6) below 1.01 and 2, two modes per pair, a 2 %
+  // margin for FP32 rounding
   {
     const int C = psh_xbin_tile_cols(Ng);
-    double w = (double)a.ax.almax[0] * a.ax.almax[1] * a.ax.almax[2] * 2.0;
-    if (a.epi == PSH_EPI_SIM && a.B) w *= 0.25;
-    w *= a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01;
+    double w = a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01;
     w *= 2.0 * 1.02;
     const double nc = (double)Ng * C;
     d.xbound = w * (nc > 1024.0 ? nc : 1024.0);

@@ -108,6 +108,24 @@ def test_sim_config1_exact_setup(psh, oracle):
     print(f"config-1 worst={w:.2e}")
 
 
+@pytest.mark.parametrize("Ng", [512, 1024])
+def test_sim_shot_noise_large_grid(psh, oracle, Ng):
+    """A shot-noise-only catalogue (2e6 uniform particles) on a large PCS
+    interlaced grid: the lowest bins hold |F|^2 at the shot level, 1e6 times
+    below the (unbinned) DC power, with PCS alias factors up to 37 per axis
+    elsewhere on the grid.  Each x tile's fixed-point scale must follow the
+    contributions it sums; scaled by the DC power and the grid's largest
+    alias factors it quantised the low-k sums at 5e-4 of S_b."""
+    L, n = 1000.0, 2_000_000
+    d = _uniform(n, [L] * 3, 4242 + Ng)
+    kw = dict(gsize=Ng, assign=psh.PCS, intlace=True, bsize=L, kbin=0.005, poles=[0, 2, 4])
+    got = psh.Context(psh.make_conf(**kw)).run_device(psh.DeviceArray.from_numpy(d), n,
+                                                      float(n))
+    ref = oracle.run(oracle.make_conf(**kw), [d])
+    w = _check(got, ref, [0, 2, 4], ref["shot"][0])
+    print(f"shot noise Ng={Ng}: worst {w:.2e}")
+
+
 def test_sim_clustered_pcs_interlaced(psh, oracle):
     """Clustered particles (many per cell, strong large-scale power)."""
     Ng, L = 128, 1000.0

@@ -14,16 +14,6 @@
 
 namespace {
 
-__device__ __forceinline__ double legendre(int ell, double x) {
-  if (ell == 0) return 1.0;
-  double p0 = 1.0, p1 = x;
-  for (int n = 1; n < ell; n++) {
-    double p2 = ((2 * n + 1) * x * p1 - n * p0) / (n + 1);
-    p0 = p1;
-    p1 = p2;
-  }
-  return p1;
-}
 
 __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
                                         double &kmod) {
@@ -49,8 +39,24 @@ __device__ __forceinline__ int mode_bin(double kmod2, const PshBins &b,
 // in windows of nbw bins, one launch each.
 constexpr int kGeoThreads = 256;
 constexpr int kGeoRun = 32;
-constexpr int kGeoGrid = 256;
+constexpr int kGeoGrid = 1024;
 constexpr int kGeoMaxWindow = 1024;
+
+// L_l(mu) for l <= 6 in the reference's FP64 expressions
+// (math/legpoly.h:47-61; 3/14 is its 0x1.b6db6db6db6dbp-3)
+__device__ __forceinline__ double legendre_h(int ell, double x) {
+#pragma clang fp contract(off)
+  const double x2 = x * x;
+  switch (ell) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return 1.5 * x2 - 0.5;
+    case 3: return 2.5 * x * (x2 - 0.6);
+    case 4: return 4.375 * x2 * x2 - 3.75 * x2 + 0.375;
+    case 5: return 8.75 * x * (0.9 * x2 * x2 - x2 + 3.0 / 14.0);
+    default: return 14.4375 * x2 * x2 * x2 - 19.6875 * x2 * x2 + 6.5625 * x2 - 0.3125;
+  }
+}
 
 struct GeoArgs {
   int N;
@@ -86,9 +92,9 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
   for (int i = threadIdx.x; i < (3 + 2 * nl) * nbw; i += kGeoThreads) gsh[i] = 0;
   __syncthreads();
   const int N = g.N, M = N / 2;
-  const int nseg = (M + kGeoRun) / kGeoRun;   // runs per row: ceil((M + 1) / kGeoRun)
-  const size_t items = (size_t)N * N * nseg;
-  const size_t stride = (size_t)gridDim.x * kGeoThreads;
+  const unsigned nseg = (M + kGeoRun) / kGeoRun;   // runs per row: ceil((M + 1) / kGeoRun)
+  const unsigned items = (unsigned)N * N * nseg;   // < 2^28 for N <= 2048
+  const unsigned stride = gridDim.x * kGeoThreads;
   int cur = -1;
   unsigned long long rc = 0;
   double rkm = 0.0, rlc[PSH_MAX_NL];
@@ -104,10 +110,10 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
     for (int l = 0; l < PSH_MAX_NL; l++) rlc[l] = 0.0;
   };
   for (int l = 0; l < PSH_MAX_NL; l++) rlc[l] = 0.0;
-  for (size_t it = blockIdx.x * (size_t)kGeoThreads + threadIdx.x; it < items; it += stride) {
-    const size_t row = it / nseg;
+  for (unsigned it = blockIdx.x * kGeoThreads + threadIdx.x; it < items; it += stride) {
+    const unsigned row = it / nseg;
     const int seg = (int)(it - row * nseg);
-    const int i = (int)(row / N), j = (int)(row - (size_t)i * N);
+    const int i = (int)(row / (unsigned)N), j = (int)(row - (unsigned)i * N);
     const double kmij = g.ax.ksq[0][i] + g.ax.ksq[1][j];
     if (kmij > g.kskip2) continue;
     const int k0 = seg * kGeoRun, k1 = min(k0 + kGeoRun - 1, M);
@@ -134,7 +140,7 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
       const double num = single ? 1.0 : 2.0;
       for (int l = 0; l < nl; l++) {
         if (!single && (b.poles[l] & 1)) continue;
-        rlc[l] += num * legendre(b.poles[l], mu);
+        rlc[l] += num * legendre_h(b.poles[l], mu);
       }
     }
     flush();

@@ -697,14 +697,12 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     // phase 1
     float2 pyz = make_float2(1.f, 0.f);
     if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
-    // the tile's largest |contribution| before the Legendre / Y_lm factor,
-    // |v| alias_x(i) w_yz (the simulation's DC mode, never binned, left
-    // out): it sets the tile's fixed-point scale, so the scale follows the
-    // modes that are summed and not the DC power or the alias factors of
-    // other columns
-    const float wcol = a.ax.alf[1][y] * a.ax.alf[2][kz] *
-                       ((kz == 0 || kz == M) ? 1.f : 2.f) *
-                       (INTLACE && EPI == PSH_EPI_SIM ? 0.25f : 1.f);
+    // The tile's fixed-point scale follows the contributions it sums: the
+    // thread's largest |v| (the simulation's DC mode, never binned, left
+    // out) times its column's weight w_yz; the largest x alias factor is in
+    // xbound.  Scaled by the DC power and the grid's largest alias factors
+    // instead, the shot-level modes of a large grid lost 20 bits.
+    const bool dccol = EPI == PSH_EPI_SIM && y == 0 && kz == 0;
     float vmax = 0.f;
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
@@ -741,12 +739,15 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         }
         if (EPI == PSH_EPI_SURVEY0 && a.store) *row_elem<RS::BYTE32>(a.store + rbase, lo) = F;
         lds[L::idx(c, j) + L::off(r * G::NS)] = v;   // i = j + r NS
-        if (EPI != PSH_EPI_SIM || y != 0 || kz != 0 || i != 0)
-          vmax = fmaxf(vmax, fabsf(v) * (xtab[i].x * wcol));
+        vmax = fmaxf(vmax, (dccol && i == 0) ? 0.f : fabsf(v));
       }
     }
-    // largest |v| of the tile (non-negative floats order as their bits)
-    if (vmax > 0.f) atomicMax(tmaxp, __float_as_uint(vmax));
+    // largest |v| w_yz of the tile (non-negative floats order as their bits)
+    if (colok) {
+      vmax *= a.ax.alf[1][y] * a.ax.alf[2][kz] * ((kz == 0 || kz == M) ? 1.f : 2.f) *
+              (INTLACE && EPI == PSH_EPI_SIM ? 0.25f : 1.f);
+      if (vmax > 0.f) atomicMax(tmaxp, __float_as_uint(vmax));
+    }
     __syncthreads();
     const size_t next = next_live(tile + gridDim.x);
     if (next < ntiles) load_tile(next);
@@ -1237,13 +1238,13 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.nbw = a.nbw > 0 ? a.nbw : a.bins.nbin;
   if (d.kb0 < 0 || d.kb0 + d.nbw > a.bins.nbin) return hipErrorInvalidValue;
   d.nacc = per_bin * d.nbw;
-  // bound of a pair's |contribution| over the tile's largest |v| alias_x
-  // w_yz (which phase 1 measures), times N C (at least 1024): |L_l| or
-  // |Y_lm| (real form, l <= 6) below 1.01 and 2, two modes per pair, a 2 %
-  // margin for FP32 rounding
+  // bound of a pair's |contribution| over the tile's largest |v| w_yz
+  // (which phase 1 measures), times N C (at least 1024): the largest x
+  // alias factor, |L_l| or |Y_lm| (real form, l <= 6) below 1.01 and 2, two
+  // modes per pair, a 2 % margin for FP32 rounding
   {
     const int C = psh_xbin_tile_cols(Ng);
-    double w = a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01;
+    double w = (double)a.ax.almax[0] * (a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01);
     w *= 2.0 * 1.02;
     const double nc = (double)Ng * C;
     d.xbound = w * (nc > 1024.0 ? nc : 1024.0);

@@ -478,6 +478,7 @@ struct XBinDev {
   double xbound;   // (largest |pair contribution| / phase 1's max) x N C, with margins
   int losx0;       // the line of sight has no x component
   int has_dc;      // replace the k = 0 mode by dcf
+  int gsum;        // the workgroup's FP64 sums in its row of the partials, not in LDS
   float dcf;
   int ell, m;
   float ynorm;
@@ -530,18 +531,21 @@ struct XCfg {
   static_assert(T % C == 0, "a thread keeps one column");
 };
 
-// LDS of the x pass besides the bins: ksq_x (FP64), xtab, twiddles, exchange
-// tile.  The twiddles are read from LDS: read from global memory they are
-// invariant over the persistent tile loop and stay live in registers.
+// LDS of the x pass besides the bins: the tile maximum, xtab, twiddles,
+// exchange tile.  The twiddles are read from LDS: read from global memory
+// they are invariant over the persistent tile loop and stay live in
+// registers.
 template <int N, bool IL>
 constexpr size_t xbin_static_lds() {
-  return sizeof(double) * 2 + sizeof(double) * N + sizeof(float4) * N + sizeof(float2) * N +
+  return sizeof(double) * 2 + sizeof(float4) * N + sizeof(float2) * N +
          sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
 }
 
 // LDS per accumulator of the x pass: the workgroup's FP64 sum and the
-// current tile's 64-bit integer sum.
+// current tile's 64-bit integer sum (kXbinAccBytesG when the FP64 sums are
+// kept in the workgroup's row of the partials instead, see XBinDev::gsum).
 constexpr size_t kXbinAccBytes = sizeof(double) + sizeof(long long);
+constexpr size_t kXbinAccBytesG = sizeof(long long);
 constexpr size_t kXbinLdsMax = 160 * 1024;
 
 // Adds x * 2^-E, rounded to the nearest integer, to a 64-bit integer sum in
@@ -605,29 +609,37 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   using L = Lds<N, C, true>;
   using G = PassGeom<N, C, T, npasses(N) - 1, true>;
   extern __shared__ double dyn[];
-  double *bins = dyn;
   const int nacc_pad = (a.nacc + 1) & ~1;
-  long long *ibins = reinterpret_cast<long long *>(bins + nacc_pad);
-  unsigned *tmaxp = reinterpret_cast<unsigned *>(ibins + nacc_pad);
-  double *ksqx = bins + 2 * nacc_pad + 2;
-  float4 *xtab = reinterpret_cast<float4 *>(ksqx + N);
+  long long *ibins = reinterpret_cast<long long *>(dyn);
+  double *bins = a.gsum ? nullptr : dyn + nacc_pad;
+  unsigned *tmaxp = reinterpret_cast<unsigned *>(dyn + (a.gsum ? 1 : 2) * nacc_pad);
+  float4 *xtab = reinterpret_cast<float4 *>(dyn + (a.gsum ? 1 : 2) * nacc_pad + 2);
   float2 *twl = reinterpret_cast<float2 *>(xtab + N);
   float *lds = reinterpret_cast<float *>(twl + N);
+  const PshBins &b = a.bins;
+  const int mock = blockIdx.y;
+  // the workgroup's row of the partials; window accumulator i (pole slot l,
+  // bin kb0 + rest) is its entry (l nbin + kb0) nmu + rest
+  double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc_full;
+  const int nmu_o = EPI == PSH_EPI_SIM ? b.nmu : 1;
+  const int stride_o = a.nbw * nmu_o;
+  auto out_idx = [&](int i) -> size_t {
+    const int l = i / stride_o;
+    return (size_t)(l * b.nbin + a.kb0) * nmu_o + (i - l * stride_o);
+  };
   for (int i = threadIdx.x; i < a.nacc; i += T) {
-    bins[i] = 0.0;
     ibins[i] = 0;
+    if (a.gsum) out[out_idx(i)] = 0.0;
+    else bins[i] = 0.0;
   }
   if (threadIdx.x == 0) *tmaxp = 0u;
   for (int i = threadIdx.x; i < N; i += T) {
     const float2 ph = a.phase[i];
     xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
-    ksqx[i] = a.ax.ksq[0][i];
     twl[i] = tw[i];
   }
   __syncthreads();
 
-  const PshBins &b = a.bins;
-  const int mock = blockIdx.y;
   const size_t ES = (size_t)N * H;
   const uint32_t ES32 = (uint32_t)ES;
   using RS = RowSplit<N, C, T, true>;
@@ -795,7 +807,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         const float k2f = xt.y + ksqyz;
         const float rs = rsqrtf(k2f);
         int kb = bin_fast(b, k2f * rs);
-        if (kb == -2) kb = bin_exact(b, ksqx[i], ksqy, ksqz, kz);
+        if (kb == -2) kb = bin_exact(b, a.ax.ksq[0][i], ksqy, ksqz, kz);
         if (dcrow && i == 0) kb = -1;   // DC mode skipped
         kb -= a.kb0;
         if (kb < 0 || kb >= a.nbw) continue;
@@ -898,7 +910,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     for (int q2 = threadIdx.x; q2 < a.nacc; q2 += T) {
       const long long t = ibins[q2];
       if (t) {
-        bins[q2] += (double)t * sup;
+        if (a.gsum) out[out_idx(q2)] += (double)t * sup;
+        else bins[q2] += (double)t * sup;
         ibins[q2] = 0;
       }
     }
@@ -907,14 +920,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     tile = next;
   }
   // the window's accumulators into the workgroup's row of all bins
-  double *out = a.partials + ((size_t)mock * gridDim.x + blockIdx.x) * a.nacc_full;
-  const int nmu = EPI == PSH_EPI_SIM ? b.nmu : 1;
-  const int stride = a.nbw * nmu;
-  for (int i = threadIdx.x; i < a.nacc; i += T) {
-    const int l = i / stride;
-    const int rem = i - l * stride;
-    out[(size_t)(l * b.nbin + a.kb0) * nmu + rem] = bins[i];
-  }
+  if (!a.gsum)
+    for (int i = threadIdx.x; i < a.nacc; i += T) out[out_idx(i)] = bins[i];
 }
 
 // ---- dispatch ------------------------------------------------------------------
@@ -1051,9 +1058,22 @@ struct XBin {
     const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
     const int H = N / 2 + PSH_PAD;
     const int ntz = (N / 2 + 1 + C - 1) / C;
-    const size_t shmem = kXbinAccBytes * ((d.nacc + 1) & ~1) +
-                         (il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>());
-    const dim3 g(grid, nmock), b(il ? XCfg<N, true>::T : XCfg<N, false>::T);
+    const size_t stat = il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>();
+    const size_t np = (size_t)((d.nacc + 1) & ~1);
+    const int T = il ? XCfg<N, true>::T : XCfg<N, false>::T;
+    size_t shmem = kXbinAccBytes * np + stat;
+    // Two workgroups per CU (which the register budget allows up to 512
+    // threads) need at most half the LDS each: when only the 64-bit tile
+    // sums fit in that half, the FP64 sums live in the workgroup's row of
+    // the partials instead (measured on the learned correction's plain
+    // N_g = 512 pass with 8 |mu| cells: one workgroup per CU otherwise)
+    XBinDev dd = d;
+    dd.gsum = 0;
+    if (T <= 512 && shmem > kXbinLdsMax / 2 && kXbinAccBytesG * np + stat <= kXbinLdsMax / 2) {
+      dd.gsum = 1;
+      shmem = kXbinAccBytesG * np + stat;
+    }
+    const dim3 g(grid, nmock), b(T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
     auto go = [&](auto ntc) {
       constexpr bool NT = decltype(ntc)::value;
@@ -1064,7 +1084,7 @@ struct XBin {
       (void)hipFuncSetAttribute((const void *)k,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,      \
                                 (int)shmem);                                     \
-    hipLaunchKernelGGL(k, g, b, shmem, s, d, ntz, H, tw);                        \
+    hipLaunchKernelGGL(k, g, b, shmem, s, dd, ntz, H, tw);                       \
     return hipGetLastError();                                                    \
   }
       if (epi == PSH_EPI_SIM) {

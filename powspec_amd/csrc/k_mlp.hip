@@ -340,65 +340,100 @@ namespace {
 //   [23]     log10(nmu S_0[b][c] / sum_c S_0[b][c]), clipped to [-2, 2]: the
 //            cell's share of the bin (|mu| is uniform over directions)
 // Bins without modes or power get zero rows (and no correction).
+__device__ __forceinline__ double alias_kcen(int b, double kedge0, double dk, int logscale) {
+  const double c = kedge0 + dk * (b + 0.5);
+  return logscale ? pow(10.0, c) : c;
+}
+
+// Per bin: P[b] = the un-subtracted monopole over the shot noise (0 without
+// modes or power) and S[b] = the bin's monopole cell sum.
 __global__ void __launch_bounds__(256)
-alias_features_kernel(const double *__restrict__ cells, const unsigned long long *__restrict__ cnt,
-                      int nbin, int nmu, double norm, double shot, double kny, double kedge0,
-                      double dk, int logscale, int scheme, float *__restrict__ X) {
-  __shared__ float summ[16];
-  const int t = threadIdx.x;
-  auto kcen = [&](int b) {
-    const double c = kedge0 + dk * (b + 0.5);
-    return logscale ? pow(10.0, c) : c;
-  };
-  auto pbin = [&](int b) -> double {   // un-subtracted monopole / shot, or 0
-    if (!cnt[b]) return 0.0;
-    double s = 0.0;
-    for (int c = 0; c < nmu; c++) s += cells[(size_t)b * nmu + c];
-    const double p = s / (norm * (double)cnt[b]) / shot;
-    return p > 0 ? p : 0.0;
-  };
-  if (t < 16) {
-    const double e0 = kny * (0.25 + 0.75 * t / 16.0), e1 = kny * (0.25 + 0.75 * (t + 1) / 16.0);
-    double sum = 0.0;
-    int n = 0;
-    int best = -1;
-    double bestd = 1e300;
-    const double mid = 0.5 * (e0 + e1);
-    for (int b = 0; b < nbin; b++) {
-      const double p = pbin(b);
-      if (p <= 0) continue;
-      const double k = kcen(b);
-      if (k >= e0 && k < e1) {
-        sum += p;
-        n++;
-      }
-      const double d = fabs(k - mid);
-      if (d < bestd) {
-        bestd = d;
-        best = b;
+alias_bins_kernel(const double *__restrict__ cells, const unsigned long long *__restrict__ cnt,
+                  int nbin, int nmu, double norm, double shot, double *__restrict__ P,
+                  double *__restrict__ S) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nbin) return;
+  double s = 0.0;
+  for (int c = 0; c < nmu; c++) s += cells[(size_t)b * nmu + c];
+  S[b] = s;
+  double p = 0.0;
+  if (cnt[b]) {
+    p = s / (norm * (double)cnt[b]) / shot;
+    if (!(p > 0)) p = 0.0;
+  }
+  P[b] = p;
+}
+
+// The 16-band summary: 16 threads per band stride over the bins, each
+// keeping its band sum, count and the populated bin nearest the band centre
+// (the first in bin order on a tie); the threads' partials are combined in
+// a fixed order.
+__global__ void __launch_bounds__(256)
+alias_bands_kernel(const double *__restrict__ P, int nbin, double kny, double kedge0, double dk,
+                   int logscale, float *__restrict__ summ) {
+  __shared__ double ssum[256], sbd[256];
+  __shared__ int sn[256], sbest[256];
+  const int t = threadIdx.x, band = t >> 4, lane = t & 15;
+  const double e0 = kny * (0.25 + 0.75 * band / 16.0), e1 = kny * (0.25 + 0.75 * (band + 1) / 16.0);
+  const double mid = 0.5 * (e0 + e1);
+  double sum = 0.0, bestd = 1e300;
+  int n = 0, best = -1;
+  for (int b = lane; b < nbin; b += 16) {
+    const double p = P[b];
+    if (p <= 0) continue;
+    const double k = alias_kcen(b, kedge0, dk, logscale);
+    if (k >= e0 && k < e1) {
+      sum += p;
+      n++;
+    }
+    const double d = fabs(k - mid);
+    if (d < bestd) {
+      bestd = d;
+      best = b;
+    }
+  }
+  ssum[t] = sum;
+  sn[t] = n;
+  sbd[t] = bestd;
+  sbest[t] = best;
+  __syncthreads();
+  if (lane == 0) {
+    for (int q = 1; q < 16; q++) {
+      sum += ssum[t + q];
+      n += sn[t + q];
+      const int bq = sbest[t + q];
+      if (bq >= 0 && (sbd[t + q] < bestd || (sbd[t + q] == bestd && bq < best))) {
+        bestd = sbd[t + q];
+        best = bq;
       }
     }
-    const double v = n ? sum / n : (best >= 0 ? pbin(best) : 1.0);
-    summ[t] = (float)log10(v);
+    const double v = n ? sum / n : (best >= 0 ? P[best] : 1.0);
+    summ[band] = (float)log10(v);
   }
-  __syncthreads();
-  for (int r = t; r < nbin * nmu; r += 256) {
-    const int b = r / nmu, c = r - b * nmu;
-    float *x = X + (size_t)r * IN;
-    for (int q = 0; q < IN; q++) x[q] = 0.f;
-    const double p = pbin(b);
-    if (p <= 0) continue;
-    double sb = 0.0;
-    for (int q = 0; q < nmu; q++) sb += cells[(size_t)b * nmu + q];
-    const double sc = cells[(size_t)b * nmu + c];
-    x[0] = (float)(kcen(b) / kny);
-    x[1] = (float)((c + 0.5) / nmu);
-    x[2 + (scheme & 3)] = 1.f;
-    for (int q = 0; q < 16; q++) x[6 + q] = summ[q];
-    x[22] = (float)log10(p);
-    double share = (sb > 0 && sc > 0) ? log10(nmu * sc / sb) : -2.0;
-    x[23] = (float)fmin(2.0, fmax(-2.0, share));
-  }
+}
+
+// One thread per (bin, cell) row.
+__global__ void __launch_bounds__(256)
+alias_features_kernel(const double *__restrict__ cells, const double *__restrict__ P,
+                      const double *__restrict__ S, const float *__restrict__ summ, int nbin,
+                      int nmu, double kny, double kedge0, double dk, int logscale, int scheme,
+                      float *__restrict__ X) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= nbin * nmu) return;
+  const int b = r / nmu, c = r - b * nmu;
+  float *x = X + (size_t)r * IN;
+  for (int q = 0; q < IN; q++) x[q] = 0.f;
+  const double p = P[b];
+  if (p <= 0) return;
+  const double sb = S[b];
+  const double sc = cells[(size_t)b * nmu + c];
+  x[0] = (float)(alias_kcen(b, kedge0, dk, logscale) / kny);
+  x[1] = (float)((c + 0.5) / nmu);
+  x[2 + (scheme & 3)] = 1.f;
+  for (int q = 0; q < 16; q++) x[6 + q] = summ[q];
+  x[22] = (float)log10(p);
+  const double share = (sb > 0 && sc > 0) ? log10(nmu * sc / sb) : -2.0;
+  x[23] = (float)fmin(2.0, fmax(-2.0, share));
 }
 
 // Corrected raw sums: out[l][b] = sum_c exp(f[b][c]) S[l][b][c], f the mean
@@ -425,11 +460,22 @@ alias_apply_kernel(const float *__restrict__ Y, int nnet, int rows, const float 
 
 }  // namespace
 
+size_t psh_alias_scratch_doubles(int nbin) { return 2 * (size_t)nbin + 8; }
+
 hipError_t psh_alias_features(const double *cells, const unsigned long long *cnt, int nbin,
                               int nmu, double norm, double shot, double kny, double kedge0,
-                              double dk, int logscale, int scheme, float *X, hipStream_t s) {
-  hipLaunchKernelGGL(alias_features_kernel, dim3(1), dim3(256), 0, s, cells, cnt, nbin, nmu,
-                     norm, shot, kny, kedge0, dk, logscale, scheme, X);
+                              double dk, int logscale, int scheme, float *X, double *scratch,
+                              hipStream_t s) {
+  if (nbin < 1 || nmu < 1) return hipErrorInvalidValue;
+  double *P = scratch, *S = scratch + nbin;
+  float *summ = reinterpret_cast<float *>(scratch + 2 * (size_t)nbin);
+  hipLaunchKernelGGL(alias_bins_kernel, dim3((nbin + 255) / 256), dim3(256), 0, s, cells, cnt,
+                     nbin, nmu, norm, shot, P, S);
+  hipLaunchKernelGGL(alias_bands_kernel, dim3(1), dim3(256), 0, s, P, nbin, kny, kedge0, dk,
+                     logscale, summ);
+  const int rows = nbin * nmu;
+  hipLaunchKernelGGL(alias_features_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, cells, P,
+                     S, summ, nbin, nmu, kny, kedge0, dk, logscale, scheme, X);
   return hipGetLastError();
 }
 

@@ -143,6 +143,8 @@ struct psh_ctx {
   int mlp_nnet = 0;
   float *d_ax = nullptr, *d_ay = nullptr;   // feature rows, network outputs
   double *d_acorr = nullptr;          // corrected raw sums [nl][nbin]
+  double *d_afeat = nullptr;          // feature scratch (per-bin levels, band summary)
+  int afeat_nbin = 0;                 // bins d_afeat holds
   size_t arows_cap = 0;
 
   // binning
@@ -1172,7 +1174,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
                   ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
                   ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
                   ctx->d_glcnt, ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat,
-                  ctx->whist, ctx->d_mlp, ctx->d_ax, ctx->d_ay, ctx->d_acorr};
+                  ctx->whist, ctx->d_mlp, ctx->d_ax, ctx->d_ay, ctx->d_acorr, ctx->d_afeat};
   for (void *p : bufs) ctx_free(ctx, p);
   for (int i = 0; i < E_COUNT; i++)
     if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
@@ -1244,8 +1246,14 @@ int ensure_alias_rows(psh_ctx *ctx) {
     ctx->arows_cap = 0;
     if ((rc = dev_alloc(ctx, &ctx->d_ax, rows * 32))) return rc;
     if ((rc = dev_alloc(ctx, &ctx->d_ay, rows * (size_t)std::max(ctx->mlp_nnet, 1)))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->d_acorr, (size_t)ctx->conf.npole * ctx->nbin))) return rc;
     ctx->arows_cap = rows;
+  }
+  // per-bin buffers follow the bin count, whatever the number of cells
+  if (ctx->nbin > ctx->afeat_nbin) {
+    ctx->afeat_nbin = 0;
+    if ((rc = dev_alloc(ctx, &ctx->d_acorr, (size_t)ctx->conf.npole * ctx->nbin))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->d_afeat, psh_alias_scratch_doubles(ctx->nbin)))) return rc;
+    ctx->afeat_nbin = ctx->nbin;
   }
   return PSH_OK;
 }
@@ -1265,7 +1273,7 @@ extern "C" int psh_ctx_alias_rows(psh_ctx *ctx, float *X) {
                                  ctx->last_shot,
                                  M_PI * ctx->Ng / std::max(ctx->L[0], std::max(ctx->L[1], ctx->L[2])),
                                  ctx->bins.kedge0, ctx->bins.dk, ctx->bins.logscale,
-                                 ctx->conf.assign, ctx->d_ax, ctx->stream));
+                                 ctx->conf.assign, ctx->d_ax, ctx->d_afeat, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(X, ctx->d_ax, sizeof(float) * 32 * nb * nmu,
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1354,7 +1362,7 @@ int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double>
     HIPCHK(ctx, psh_alias_features(ctx->d_bins, ctx->d_gcnt, nb, nmu, norm, shot,
                                    M_PI * ctx->Ng / std::max(ctx->L[0], std::max(ctx->L[1], ctx->L[2])),
                                    ctx->bins.kedge0, ctx->bins.dk, ctx->bins.logscale,
-                                   ctx->conf.assign, ctx->d_ax, ctx->stream));
+                                   ctx->conf.assign, ctx->d_ax, ctx->d_afeat, ctx->stream));
     for (int n = 0; n < ctx->mlp_nnet; n++)
       HIPCHK(ctx, psh_mlp_forward_async(ctx->d_mlp + (size_t)n * P, ctx->d_ax, rows,
                                         ctx->d_ay + (size_t)n * rows, ctx->stream));

@@ -245,7 +245,9 @@ hipError_t psh_coord_bounds(const double *xyzw, size_t n, double *partial,
 // normalisation, shot noise, k_Ny and binning of the run, the scheme id.
 hipError_t psh_alias_features(const double *cells, const unsigned long long *cnt, int nbin,
     int nmu, double norm, double shot, double kny, double kedge0, double dk, int logscale,
-    int scheme, float *X, hipStream_t s);
+    int scheme, float *X, double *scratch, hipStream_t s);
+// Doubles of device scratch psh_alias_features needs for nbin bins.
+size_t psh_alias_scratch_doubles(int nbin);
 // out[l][b] = sum_c exp(mean_n Y[n][b nmu + c]) cells[l][b][c] (device).
 hipError_t psh_alias_apply(const float *Y, int nnet, const float *X, const double *cells, int nl,
     int nbin, int nmu, double *out, hipStream_t s);

@@ -478,7 +478,6 @@ struct XBinDev {
   double xbound;   // (largest |pair contribution| / phase 1's max) x N C, with margins
   int losx0;       // the line of sight has no x component
   int has_dc;      // replace the k = 0 mode by dcf
-  int gsum;        // the workgroup's FP64 sums in its row of the partials, not in LDS
   float dcf;
   int ell, m;
   float ynorm;
@@ -528,22 +527,27 @@ struct XCfg {
   // workgroups share a CU and hide each other's barriers; the small
   // interlaced kernel holds 2 x 16 values per thread and gets 2 (256 VGPRs)
   static constexpr int WAVES = SMALL ? 2 : 4;
+  // whether the next tile's rows are loaded during phase 2 (see x_bin_kernel):
+  // the interlaced kernel from N = 640 holds both fields' 16 values per
+  // thread and spills registers when the prefetch is in flight as well
+  // (measured on MI355X at N = 1024: 3.98 ms with it, 3.71 ms without)
+  static constexpr bool PREFETCH = !(IL && N >= 640);
   static_assert(T % C == 0, "a thread keeps one column");
 };
 
-// LDS of the x pass besides the bins: the tile maximum, xtab, twiddles,
-// exchange tile.  The twiddles are read from LDS: read from global memory
+// LDS of the x pass besides the bins: the tile maximum, ksq_x (FP64; read
+// from global memory by the GS layout), xtab, twiddles, exchange tile.  The twiddles are read from LDS: read from global memory
 // they are invariant over the persistent tile loop and stay live in
 // registers.
-template <int N, bool IL>
+template <int N, bool IL, bool GS = false>
 constexpr size_t xbin_static_lds() {
-  return sizeof(double) * 2 + sizeof(float4) * N + sizeof(float2) * N +
-         sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
+  return sizeof(double) * 2 + (GS ? 0 : sizeof(double) * N) + sizeof(float4) * N +
+         sizeof(float2) * N + sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
 }
 
 // LDS per accumulator of the x pass: the workgroup's FP64 sum and the
 // current tile's 64-bit integer sum (kXbinAccBytesG when the FP64 sums are
-// kept in the workgroup's row of the partials instead, see XBinDev::gsum).
+// kept in the workgroup's row of the partials instead, the GS layout).
 constexpr size_t kXbinAccBytes = sizeof(double) + sizeof(long long);
 constexpr size_t kXbinAccBytesG = sizeof(long long);
 constexpr size_t kXbinLdsMax = 160 * 1024;
@@ -601,7 +605,10 @@ __device__ __forceinline__ int bin_exact(const PshBins &b, double ksqx, double k
 //     accumulator's integer is converted and added to its FP64 sum by the
 //     thread that owns it.  The sums are therefore the same on every run.
 // xtab[i] = {alias_x, ksq_x, phase_x.re, phase_x.im} in LDS.
-template <int N, int EPI, bool INTLACE, bool NT>
+// GS: the workgroup's FP64 sums live in its row of the partials and ksq_x
+// is read from global memory, which halves the LDS (two workgroups per CU
+// for the plain pass with |mu| cells).
+template <int N, int EPI, bool INTLACE, bool NT, bool GS>
 __global__ void __launch_bounds__((XCfg<N, INTLACE>::T), (XCfg<N, INTLACE>::WAVES))
 x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   constexpr int C = XCfg<N, INTLACE>::C, T = XCfg<N, INTLACE>::T, M = N / 2;
@@ -611,9 +618,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   extern __shared__ double dyn[];
   const int nacc_pad = (a.nacc + 1) & ~1;
   long long *ibins = reinterpret_cast<long long *>(dyn);
-  double *bins = a.gsum ? nullptr : dyn + nacc_pad;
-  unsigned *tmaxp = reinterpret_cast<unsigned *>(dyn + (a.gsum ? 1 : 2) * nacc_pad);
-  float4 *xtab = reinterpret_cast<float4 *>(dyn + (a.gsum ? 1 : 2) * nacc_pad + 2);
+  double *bins = GS ? nullptr : dyn + nacc_pad;
+  unsigned *tmaxp = reinterpret_cast<unsigned *>(dyn + (GS ? 1 : 2) * nacc_pad);
+  double *ksqx = dyn + (GS ? 1 : 2) * nacc_pad + 2;   // unused by GS
+  float4 *xtab = reinterpret_cast<float4 *>(ksqx + (GS ? 0 : N));
   float2 *twl = reinterpret_cast<float2 *>(xtab + N);
   float *lds = reinterpret_cast<float *>(twl + N);
   const PshBins &b = a.bins;
@@ -629,13 +637,14 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   };
   for (int i = threadIdx.x; i < a.nacc; i += T) {
     ibins[i] = 0;
-    if (a.gsum) out[out_idx(i)] = 0.0;
+    if (GS) out[out_idx(i)] = 0.0;
     else bins[i] = 0.0;
   }
   if (threadIdx.x == 0) *tmaxp = 0u;
   for (int i = threadIdx.x; i < N; i += T) {
     const float2 ph = a.phase[i];
     xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
+    if (!GS) ksqx[i] = a.ax.ksq[0][i];
     twl[i] = tw[i];
   }
   __syncthreads();
@@ -664,7 +673,9 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   };
   // The next tile's rows are loaded into va/vb as soon as phase 1 has
   // consumed them, so their HBM latency hides behind phase 2 (one block per
-  // CU: nothing else would overlap it).
+  // CU: nothing else would overlap it) -- unless the registers cannot hold
+  // them (XCfg::PREFETCH false: each tile loads its rows when it starts).
+  constexpr bool PF = XCfg<N, INTLACE>::PREFETCH;
   float2 va[V];
   float2 vb[INTLACE ? V : 1];
   auto load_tile = [&](size_t t) {
@@ -693,10 +704,11 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     return t;
   };
   size_t tile = next_live(t0);
-  if (tile < ntiles) load_tile(tile);
+  if (PF && tile < ntiles) load_tile(tile);
   for (; tile < ntiles;) {
     int y, q;
     const size_t base = tile_base(tile, y, q);
+    if (!PF) load_tile(tile);
     const int ncv = min(C, M + 1 - q * C);
     if (INTLACE)
       Passes2<N, C, T, 0, true, false, 1>::run(va, vb, lds, twl);
@@ -762,7 +774,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     }
     __syncthreads();
     const size_t next = next_live(tile + gridDim.x);
-    if (next < ntiles) load_tile(next);
+    if (PF && next < ntiles) load_tile(next);
 
     // phase 2: column c, pairs (i, N - i) with i = threadIdx.x / C + k T / C
     // up to M.  The tile's integer sums take the scale 2^E with
@@ -807,7 +819,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         const float k2f = xt.y + ksqyz;
         const float rs = rsqrtf(k2f);
         int kb = bin_fast(b, k2f * rs);
-        if (kb == -2) kb = bin_exact(b, a.ax.ksq[0][i], ksqy, ksqz, kz);
+        if (kb == -2) kb = bin_exact(b, GS ? a.ax.ksq[0][i] : ksqx[i], ksqy, ksqz, kz);
         if (dcrow && i == 0) kb = -1;   // DC mode skipped
         kb -= a.kb0;
         if (kb < 0 || kb >= a.nbw) continue;
@@ -910,7 +922,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     for (int q2 = threadIdx.x; q2 < a.nacc; q2 += T) {
       const long long t = ibins[q2];
       if (t) {
-        if (a.gsum) out[out_idx(q2)] += (double)t * sup;
+        if (GS) out[out_idx(q2)] += (double)t * sup;
         else bins[q2] += (double)t * sup;
         ibins[q2] = 0;
       }
@@ -920,7 +932,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     tile = next;
   }
   // the window's accumulators into the workgroup's row of all bins
-  if (!a.gsum)
+  if (!GS)
     for (int i = threadIdx.x; i < a.nacc; i += T) out[out_idx(i)] = bins[i];
 }
 
@@ -1063,23 +1075,25 @@ struct XBin {
     const int T = il ? XCfg<N, true>::T : XCfg<N, false>::T;
     size_t shmem = kXbinAccBytes * np + stat;
     // Two workgroups per CU (which the register budget allows up to 512
-    // threads) need at most half the LDS each: when only the 64-bit tile
-    // sums fit in that half, the FP64 sums live in the workgroup's row of
-    // the partials instead (measured on the learned correction's plain
-    // N_g = 512 pass with 8 |mu| cells: one workgroup per CU otherwise)
-    XBinDev dd = d;
-    dd.gsum = 0;
-    if (T <= 512 && shmem > kXbinLdsMax / 2 && kXbinAccBytesG * np + stat <= kXbinLdsMax / 2) {
-      dd.gsum = 1;
-      shmem = kXbinAccBytesG * np + stat;
+    // threads, the plain pass up to N_g = 512) need at most half the LDS
+    // each: when only the GS layout fits in that half, it is used (measured
+    // on the learned correction's plain N_g = 512 pass with 8 |mu| cells:
+    // one workgroup per CU otherwise)
+    bool gs = false;
+    const size_t shmem_gs = kXbinAccBytesG * np + xbin_static_lds<N, false, true>();
+    if (!il && T <= 512 && shmem > kXbinLdsMax / 2 && shmem_gs <= kXbinLdsMax / 2) {
+      gs = true;
+      shmem = shmem_gs;
     }
+    const XBinDev &dd = d;
     const dim3 g(grid, nmock), b(T);
     if (shmem > 160 * 1024) return hipErrorInvalidValue;
     auto go = [&](auto ntc) {
       constexpr bool NT = decltype(ntc)::value;
 #define PSH_XB(E, I)                                                          \
   {                                                                              \
-    auto k = x_bin_kernel<N, E, I, NT>;                                          \
+    auto k = x_bin_kernel<N, E, I, NT, false>;                                   \
+    if (!(I) && gs) k = x_bin_kernel<N, E, false, NT, true>;                     \
     if (shmem > 64 * 1024)                                                       \
       (void)hipFuncSetAttribute((const void *)k,                                 \
                                 hipFuncAttributeMaxDynamicSharedMemorySize,      \

@@ -409,6 +409,14 @@ int psh_ctx_set_alias_mlp(psh_ctx *ctx, const float *params, int nnet);
  * PSH_ERR_ARG when invalid. */
 int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]);
 
+/* The tile of whole z rows with which the deposit also performs the z
+ * transform (no counterpart in the reference): for a simulation box of
+ * GRID_SIZE 128, 256, 512 or 1024, tile[0..2] with tile[2] = GRID_SIZE; a
+ * context given these tiles (psh_ctx_set_tiles, a tuning cache) writes
+ * half-complex rows from the deposit and skips the separate z pass.  Returns
+ * 1, or 0 (tile untouched) for other sizes.  Host-only. */
+int psh_fused_tile(int gsize, int intlace, int tile[3]);
+
 /* The deposit tiles in use (each axis clipped to GRID_SIZE): the shape set by
  * psh_ctx_set_tiles, a tuning cache, PSH_TILE or the default.  PSH_ERR_ARG
  * before the context has a box (surveys: before their first run). */

@@ -76,7 +76,7 @@ EXPORTS = [
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
     "psh_fault_inject", "psh_run_device_static", "psh_ctx_set_mu_cells", "psh_ctx_cells",
     "psh_ctx_alias_rows", "psh_ctx_set_alias_mlp", "psh_ctx_set_fft_schedule",
-    "psh_ctx_get_fft_schedule", "psh_autotune_fft", "psh_fft_plan",
+    "psh_ctx_get_fft_schedule", "psh_autotune_fft", "psh_fft_plan", "psh_fused_tile",
 ]
 
 
@@ -91,6 +91,13 @@ class TuneFft(C.Structure):
 
     _fields_ = [("ysched", C.c_int), ("ms", C.c_double), ("ncand", C.c_int),
                 ("ms_default", C.c_double)]
+
+
+def fused_tile(Ng, intlace):
+    """The deposit tile (whole z rows) that fuses the z transform into the
+    deposit for a box of Ng cells, or None (host-only)."""
+    t = (C.c_int * 3)()
+    return tuple(t) if load().psh_fused_tile(int(Ng), int(bool(intlace)), t) else None
 
 
 def fft_plan(Ng, plan=0):
@@ -242,6 +249,7 @@ def load():
     L.psh_ctx_get_fft_schedule.argtypes = [vp, C.POINTER(i)]
     L.psh_autotune_fft.argtypes = [C.POINTER(Conf), i, C.c_char_p, C.POINTER(TuneFft)]
     L.psh_fft_plan.argtypes = [i, i, C.POINTER(C.c_int), i]
+    L.psh_fused_tile.argtypes = [i, i, C.POINTER(C.c_int)]
     _lib = L
     return L
 

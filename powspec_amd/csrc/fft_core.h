@@ -75,6 +75,13 @@ constexpr int ns_before(int N, int p, int PL = 0) {
   return p == 0 ? 1 : ns_before(N, p - 1, PL) * radix(N, p - 1, PL);
 }
 
+// M - k modulo M (the partner of element k of an M-point line).
+template <int M>
+__device__ __forceinline__ int mirror(int k) {
+  if constexpr (is_pow2(M)) return (M - k) & (M - 1);
+  else return k == 0 ? 0 : M - k;
+}
+
 // exp(-2 pi i m / 16) for a compile-time m (after unrolling)
 __device__ __forceinline__ float2 w16(int m) {
   constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f;
@@ -303,6 +310,19 @@ struct Plan {
   static constexpr int P = npasses(N);
   static constexpr int V = ((C * N + T - 1) / T) > 0 ? (C * N + T - 1) / T : 1;
 };
+
+// Largest per-thread value count over the passes of a plan.
+template <int N, int C, int T, bool COLMAP, int P = 0, int PL = 0>
+constexpr int max_values() {
+  if constexpr (P >= npasses(N, PL)) {
+    return 1;
+  } else {
+    constexpr int here =
+        PassGeom<N, C, T, P, COLMAP, PL>::NB * PassGeom<N, C, T, P, COLMAP, PL>::R;
+    constexpr int rest = max_values<N, C, T, COLMAP, P + 1, PL>();
+    return here > rest ? here : rest;
+  }
+}
 
 // Load the inputs of pass P through `ld(c, n) -> float2`.
 template <int N, int C, int T, int P, bool COLMAP, class LD, int PL = 0>

@@ -20,6 +20,9 @@
 // -alpha*w, which fuses the data-minus-random subtraction
 // (src/genr_mesh.c:802-806, 841-845).
 #include "psh_internal.h"
+#include "fft_core.h"
+
+#include <type_traits>
 
 namespace {
 
@@ -749,7 +752,21 @@ __device__ __forceinline__ void store4(float *p, float4 v) {
   else *reinterpret_cast<v4f_t *>(p) = x;
 }
 
-template <int SCHEME, bool INTLACE, bool WIDE, bool NT>
+// Deposit with the z transform fused in (boxes, 32-bit cells): a tile
+// holds TX x TY whole z rows per grid, C = 32768 / N lines over both grids,
+// so that 1024 threads transform them with 16 values each, like the z pass.
+// Each grid's rows (N + 2W cells) take about 130 KB of LDS at every size.
+template <int N, bool IL>
+struct ZfCfg {
+  static constexpr int C = 32768 / N;            // lines per tile, both grids
+  static constexpr int R = IL ? C / 2 : C;       // rows per grid
+  static constexpr int TX = N >= 1024 ? (IL ? 4 : 8) : (N >= 512 ? 8 : 16);
+  static constexpr int TY = R / TX;
+  static constexpr int T = 1024;
+  static_assert(TX * TY == R && TY >= 4, "tile rows");
+};
+
+template <int SCHEME, bool INTLACE, bool WIDE, bool NT, int NZ>
 __global__ void __launch_bounds__(1024)
 assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
                const Rec *__restrict__ recs, size_t rec_cap,
@@ -862,6 +879,66 @@ assign_deposit(PshAssignGeom g, const uint32_t *__restrict__ starts,
   }
   __syncthreads();
 
+  if constexpr (NZ > 0) {
+    // The tile's rows, both grids, as NZ / 2-point complex lines (cells
+    // 2n, 2n + 1 of a row), transformed like the z pass (z_r2c_kernel) and
+    // split into the half-complex row X[0..NZ/2] (k_z <= kzlim stored):
+    // X[k] = E + W^k O, X[M - k] = conj(E - W^k O).
+    using namespace pshfft;
+    using Z = ZfCfg<NZ, INTLACE>;
+    constexpr int M = NZ / 2, C = Z::C, TT = Z::T, W = SCHEME + 1;
+    using LZ = Lds<M, C, false>;
+    constexpr int V = max_values<M, C, TT, false>();
+    float2 v[V];
+    pass_load<M, C, TT, 0, false>(v, [&](int c, int n) -> float2 {
+      const AT *row = (c < Z::R ? ldsA + c * P2 : ldsB + (c - Z::R) * P2) + W;
+      return make_float2((float)row[2 * n] * inv, (float)row[2 * n + 1] * inv);
+    });
+    __syncthreads();   // every cell read before the exchange tile overwrites them
+    float *xl = reinterpret_cast<float *>(ldsq);
+    Passes<M, C, TT, 0, false, false, 2>::run(v, xl, g.tw);
+    constexpr int NP = M / 2 + 1;
+    constexpr int NI = (C * NP + TT - 1) / TT;
+    float rk[NI], rmk[NI];
+    store_last<M, C, TT, false>(v, [&](int c, int n, float2 x) { xl[LZ::idx(c, n)] = x.x; });
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+      const int item = threadIdx.x + it * TT;
+      const int c = item / NP, k = item - c * NP;
+      if (item < C * NP) {
+        rk[it] = xl[LZ::idx(c, k)];
+        rmk[it] = xl[LZ::idx(c, mirror<M>(k))];
+      }
+    }
+    __syncthreads();
+    store_last<M, C, TT, false>(v, [&](int c, int n, float2 x) { xl[LZ::idx(c, n)] = x.y; });
+    __syncthreads();
+    float2 *cA = reinterpret_cast<float2 *>(meshA + (size_t)mock * mstride);
+    float2 *cB = INTLACE ? reinterpret_cast<float2 *>(meshB + (size_t)mock * mstride) : nullptr;
+    const int H = g.row_pitch / 2;
+#pragma unroll
+    for (int it = 0; it < NI; it++) {
+      const int item = threadIdx.x + it * TT;
+      const int c = item / NP, k = item - c * NP;
+      if (item >= C * NP) continue;
+      const float ik = xl[LZ::idx(c, k)];
+      const float imk = xl[LZ::idx(c, mirror<M>(k))];
+      const float2 E = make_float2(0.5f * (rk[it] + rmk[it]), 0.5f * (ik - imk));
+      const float2 O = make_float2(0.5f * (ik + imk), -0.5f * (rk[it] - rmk[it]));
+      const float2 WO = cmul(g.tw[k], O);
+      const int grid = c / Z::R, row = c - grid * Z::R;
+      const int lx = row / Z::TY, ly = row - lx * Z::TY;
+      float2 *orow = (grid ? cB : cA) + ((size_t)(T0[0] + lx) * g.Ng + (T0[1] + ly)) * H;
+      if (k <= g.kzlim) mesh_store<NT>(orow + k, cadd(E, WO));
+      if (M - k != k && M - k <= g.kzlim) {
+        const float2 d = csub(E, WO);
+        mesh_store<NT>(orow + (M - k), make_float2(d.x, -d.y));
+      }
+    }
+    return;
+  }
+
   // write the tile: rows of T[2] floats along z, 4 cells per thread
   float *outA = meshA + (size_t)mock * mstride;
   float *outB = INTLACE ? meshB + (size_t)mock * mstride : nullptr;
@@ -907,14 +984,36 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
                           (g.intlace ? 2 : 1) >
                       ((size_t)256 << 20);
   auto k = stream
-               ? (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, true>
-                                      : assign_deposit<SCHEME, false, true, true>)
-                         : (g.intlace ? assign_deposit<SCHEME, true, false, true>
-                                      : assign_deposit<SCHEME, false, false, true>))
-               : (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, false>
-                                      : assign_deposit<SCHEME, false, true, false>)
-                         : (g.intlace ? assign_deposit<SCHEME, true, false, false>
-                                      : assign_deposit<SCHEME, false, false, false>));
+               ? (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, true, 0>
+                                      : assign_deposit<SCHEME, false, true, true, 0>)
+                         : (g.intlace ? assign_deposit<SCHEME, true, false, true, 0>
+                                      : assign_deposit<SCHEME, false, false, true, 0>))
+               : (g.wide ? (g.intlace ? assign_deposit<SCHEME, true, true, false, 0>
+                                      : assign_deposit<SCHEME, false, true, false, 0>)
+                         : (g.intlace ? assign_deposit<SCHEME, true, false, false, 0>
+                                      : assign_deposit<SCHEME, false, false, false, 0>));
+  if (g.zfuse) {
+    if (g.wide) return hipErrorInvalidValue;
+    // the fused z transform writes half-complex rows, 8 B per stored mode
+    auto pick = [&](auto nz) {
+      constexpr int NZ = decltype(nz)::value;
+      if (g.tile[0] != ZfCfg<NZ, true>::TX && g.intlace) k = nullptr;
+      else if (!g.intlace && g.tile[0] != ZfCfg<NZ, false>::TX) k = nullptr;
+      else
+        k = stream ? (g.intlace ? assign_deposit<SCHEME, true, false, true, NZ>
+                                : assign_deposit<SCHEME, false, false, true, NZ>)
+                   : (g.intlace ? assign_deposit<SCHEME, true, false, false, NZ>
+                                : assign_deposit<SCHEME, false, false, false, NZ>);
+    };
+    switch (g.Ng) {
+      case 128: pick(std::integral_constant<int, 128>{}); break;
+      case 256: pick(std::integral_constant<int, 256>{}); break;
+      case 512: pick(std::integral_constant<int, 512>{}); break;
+      case 1024: pick(std::integral_constant<int, 1024>{}); break;
+      default: k = nullptr;
+    }
+    if (!k) return hipErrorInvalidValue;
+  }
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
@@ -923,6 +1022,7 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
   const size_t tcells = (size_t)g.tile[0] * g.tile[1] * g.tile[2];
   unsigned threads = 256;
   while (threads < 1024 && threads * 16 < tcells) threads *= 2;
+  if (g.zfuse) threads = 1024;   // the fused transform's block shape
   hipLaunchKernelGGL(k, dim3((unsigned)ntiles), dim3(threads), lds, s, g, starts, recs, rec_cap,
                      meshA, meshB, mstride, flags, ntotal);
   return hipGetLastError();
@@ -931,6 +1031,23 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
 }  // namespace
 
 size_t psh_assign_record_bytes() { return sizeof(Rec); }
+
+bool psh_zfuse_tile(int Ng, int intlace, int tile[3]) {
+  auto set = [&](auto nz) {
+    constexpr int NZ = decltype(nz)::value;
+    tile[0] = intlace ? ZfCfg<NZ, true>::TX : ZfCfg<NZ, false>::TX;
+    tile[1] = intlace ? ZfCfg<NZ, true>::TY : ZfCfg<NZ, false>::TY;
+    tile[2] = NZ;
+    return true;
+  };
+  switch (Ng) {
+    case 128: return set(std::integral_constant<int, 128>{});
+    case 256: return set(std::integral_constant<int, 256>{});
+    case 512: return set(std::integral_constant<int, 512>{});
+    case 1024: return set(std::integral_constant<int, 1024>{});
+    default: return false;
+  }
+}
 
 size_t psh_assign_whist_words(const PshAssignGeom &g, size_t ntotal) {
   const size_t tpm = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];

@@ -55,19 +55,6 @@ struct ZCfg {
   static constexpr int T = 256;
 };
 
-// Largest per-thread value count over the passes of a plan.
-template <int N, int C, int T, bool COLMAP, int P = 0, int PL = 0>
-constexpr int max_values() {
-  if constexpr (P >= npasses(N, PL)) {
-    return 1;
-  } else {
-    constexpr int here =
-        PassGeom<N, C, T, P, COLMAP, PL>::NB * PassGeom<N, C, T, P, COLMAP, PL>::R;
-    constexpr int rest = max_values<N, C, T, COLMAP, P + 1, PL>();
-    return here > rest ? here : rest;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Real-space Y_lm weights of the survey estimator (src/mp_template.c:69-95),
 // evaluated in FP32 from per-axis position tables.
@@ -115,13 +102,6 @@ __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
   float ir22 = rsqrtf(r2);
   float r22 = r2 * ir22;
   return ylm_eval(y.ell, y.m, y.norm, rz * ir3, r22 * ir3, rx * ir22, ry * ir22);
-}
-
-// M - k modulo M (the partner of element k of an M-point line).
-template <int M>
-__device__ __forceinline__ int mirror(int k) {
-  if constexpr (is_pow2(M)) return (M - k) & (M - 1);
-  else return k == 0 ? 0 : M - k;
 }
 
 // ---------------------------------------------------------------------------

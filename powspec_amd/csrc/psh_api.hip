@@ -558,6 +558,13 @@ void set_assign_geom(psh_ctx *ctx) {
     g.smin[a] = ctx->bmin[a] - 0.5 * ctx->L[a] / ctx->Ng;   // src/genr_mesh.c:817-818
     g.L[a] = ctx->L[a];
   }
+  // tiles of whole z rows in the fused shape: the deposit transforms its rows
+  // (boxes; surveys keep the real mesh for their Y_lm-weighted passes)
+  int zt[3];
+  g.zfuse = !ctx->survey && psh_zfuse_tile(ctx->Ng, ctx->conf.intlace, zt) &&
+            g.tile[0] == zt[0] && g.tile[1] == zt[1] && g.tile[2] == zt[2];
+  g.kzlim = ctx->kzlim;
+  g.tw = ctx->tw;
   ctx->ntiles_mock = (size_t)g.ntile[0] * g.ntile[1] * g.ntile[2];
 }
 
@@ -804,11 +811,13 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
   PshYlmR noy{};
   float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
   float2 *B = ctx->conf.intlace ? reinterpret_cast<float2 *>(ctx->meshB) : nullptr;
-  HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy, ctx->stream,
-                            ctx->kzlim));
-  if (B)
-    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy, ctx->stream,
-                              ctx->kzlim));
+  if (!ctx->ag.zfuse) {   // else the deposit wrote half-complex rows already
+    HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshA, A, nmock, ctx->mstride, ctx->tw, noy,
+                              ctx->stream, ctx->kzlim));
+    if (B)
+      HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, B, nmock, ctx->mstride, ctx->tw, noy,
+                                ctx->stream, ctx->kzlim));
+  }
   ev_record(ctx, E_ZDONE);
   HIPCHK(ctx, psh_fft_y(N, A, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
                         ctx->ysched));
@@ -2014,6 +2023,14 @@ extern "C" int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]) {
   return PSH_OK;
 }
 
+extern "C" int psh_fused_tile(int gsize, int intlace, int tile[3]) {
+  if (!tile) return 0;
+  int t[3];
+  if (!psh_zfuse_tile(gsize, intlace, t)) return 0;
+  for (int a = 0; a < 3; a++) tile[a] = t[a];
+  return 1;
+}
+
 extern "C" int psh_ctx_set_tiles(psh_ctx *ctx, const int tile[3]) {
   if (!ctx || !tile) return set_err(ctx, PSH_ERR_ARG, "null argument");
   if (!tiles_ok(tile, ctx->conf.assign, ctx->conf.intlace, ctx->survey ? 8 : 4))
@@ -2230,17 +2247,27 @@ extern "C" int psh_autotune(const psh_conf *conf, const psh_data *d_data, const 
     pk[m].lcnt = lc.data();
     pk[m].pl[0] = pl.data();
   }
-  // candidate shapes: 2^11..2^14 cells, z rows of 16..128 cells
+  // candidate shapes: 2^11..2^14 cells, z rows of 16..128 cells, and for
+  // boxes of the grid sizes that have it the fused shape (whole z rows, the
+  // z transform in the deposit; its time includes that transform)
   const int opts[4] = {4, 8, 16, 32}, zopts[4] = {16, 32, 64, 128};
+  int zt[3];
+  const bool zf = conf->issim && psh_zfuse_tile(c.gsize, c.intlace, zt);
   double best_ms = 1e300;
   for (int a = 0; a < 4 && !rc; a++)
     for (int b = 0; b < 4 && !rc; b++)
-      for (int z = 0; z < 4 && !rc; z++) {
-        const int t[3] = {opts[a], opts[b], zopts[z]};
-        const size_t cells = (size_t)t[0] * t[1] * t[2];
-        if (cells < 2048 || cells > 16384 || t[0] > t[1] || t[0] > c.gsize ||
-            t[1] > c.gsize || t[2] > c.gsize || !tiles_ok(t, c.assign, c.intlace, conf->issim ? 4 : 8))
-          continue;
+      for (int z = 0; z < 5 && !rc; z++) {
+        int t[3] = {opts[a], opts[b], z < 4 ? zopts[z] : 0};
+        if (z == 4) {   // the fused shape, once
+          if (!zf || a || b) continue;
+          for (int q = 0; q < 3; q++) t[q] = zt[q];
+        } else {
+          const size_t cells = (size_t)t[0] * t[1] * t[2];
+          if (cells < 2048 || cells > 16384 || t[0] > t[1] || t[0] > c.gsize ||
+              t[1] > c.gsize || t[2] > c.gsize ||
+              !tiles_ok(t, c.assign, c.intlace, conf->issim ? 4 : 8))
+            continue;
+        }
         if ((rc = psh_ctx_set_tiles(ctx, t))) break;
         double times[4];
         for (int r = 0; r < 4 && !rc; r++) {

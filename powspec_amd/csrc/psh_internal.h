@@ -78,7 +78,17 @@ struct PshAssignGeom {
   double bmin[3];       // box corner of the plain grid
   double smin[3];       // box corner of the shifted grid
   double L[3];          // box size
+  // z transform fused into the deposit (tiles of whole z rows, boxes): the
+  // deposit writes half-complex rows (k_z <= kzlim) instead of real ones
+  int zfuse;
+  int kzlim;
+  const float2 *tw;     // exp(-2 pi i m / Ng), m < Ng
 };
+
+// Tile of the deposit with the fused z transform for a grid size: whole z
+// rows, tx x ty rows per grid (32768 / Ng lines per tile over both grids);
+// false when this build has no fused deposit for Ng.
+bool psh_zfuse_tile(int Ng, int intlace, int tile[3]);
 
 // Deposit tile in LDS, per grid: rows along z padded by the window width
 // W = scheme + 1 on each side (window cells past the tile's z range land

@@ -752,15 +752,20 @@ __device__ __forceinline__ void store4(float *p, float4 v) {
   else *reinterpret_cast<v4f_t *>(p) = x;
 }
 
-// Deposit with the z transform fused in (boxes, 32-bit cells): a tile
-// holds TX x TY whole z rows per grid, C = 32768 / N lines over both grids,
-// so that 1024 threads transform them with 16 values each, like the z pass.
-// Each grid's rows (N + 2W cells) take about 130 KB of LDS at every size.
+// Deposit with the z transform fused in (boxes): a tile holds TX x TY whole
+// z rows per grid, C lines over both grids, transformed by 1024 threads
+// like the z pass; the rows (N + 2W cells) take about 130 KB of LDS at
+// every size.  From N = 512, C = 32768 / N lines of 32-bit cells (16 values
+// per thread).  Smaller grids put many more records into a tile, and a
+// 32-bit cell's fixed-point step grows with the tile's record count (at
+// N = 128 the quantisation reached 1e-5 of P(k) in sparsely populated
+// bins): they take 64-bit cells, C = 16384 / N lines, 8 values per thread.
 template <int N, bool IL>
 struct ZfCfg {
-  static constexpr int C = 32768 / N;            // lines per tile, both grids
-  static constexpr int R = IL ? C / 2 : C;       // rows per grid
-  static constexpr int TX = N >= 1024 ? (IL ? 4 : 8) : (N >= 512 ? 8 : 16);
+  static constexpr bool WIDE = N <= 256;
+  static constexpr int C = (WIDE ? 16384 : 32768) / N;   // lines per tile, both grids
+  static constexpr int R = IL ? C / 2 : C;               // rows per grid
+  static constexpr int TX = R >= 128 ? 16 : (R >= 32 ? 8 : 4);
   static constexpr int TY = R / TX;
   static constexpr int T = 1024;
   static_assert(TX * TY == R && TY >= 4, "tile rows");
@@ -993,17 +998,20 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
                          : (g.intlace ? assign_deposit<SCHEME, true, false, false, 0>
                                       : assign_deposit<SCHEME, false, false, false, 0>));
   if (g.zfuse) {
-    if (g.wide) return hipErrorInvalidValue;
     // the fused z transform writes half-complex rows, 8 B per stored mode
     auto pick = [&](auto nz) {
       constexpr int NZ = decltype(nz)::value;
-      if (g.tile[0] != ZfCfg<NZ, true>::TX && g.intlace) k = nullptr;
-      else if (!g.intlace && g.tile[0] != ZfCfg<NZ, false>::TX) k = nullptr;
-      else
-        k = stream ? (g.intlace ? assign_deposit<SCHEME, true, false, true, NZ>
-                                : assign_deposit<SCHEME, false, false, true, NZ>)
-                   : (g.intlace ? assign_deposit<SCHEME, true, false, false, NZ>
-                                : assign_deposit<SCHEME, false, false, false, NZ>);
+      constexpr bool W = ZfCfg<NZ, true>::WIDE;
+      const int tx = g.intlace ? ZfCfg<NZ, true>::TX : ZfCfg<NZ, false>::TX;
+      const int ty = g.intlace ? ZfCfg<NZ, true>::TY : ZfCfg<NZ, false>::TY;
+      if (g.tile[0] != tx || g.tile[1] != ty || g.tile[2] != NZ || (g.wide != 0) != W) {
+        k = nullptr;
+        return;
+      }
+      k = stream ? (g.intlace ? assign_deposit<SCHEME, true, W, true, NZ>
+                              : assign_deposit<SCHEME, false, W, true, NZ>)
+                 : (g.intlace ? assign_deposit<SCHEME, true, W, false, NZ>
+                              : assign_deposit<SCHEME, false, W, false, NZ>);
     };
     switch (g.Ng) {
       case 128: pick(std::integral_constant<int, 128>{}); break;
@@ -1031,6 +1039,8 @@ hipError_t launch_deposit(const PshAssignGeom &g, size_t ntiles,
 }  // namespace
 
 size_t psh_assign_record_bytes() { return sizeof(Rec); }
+
+bool psh_zfuse_wide(int Ng) { return Ng <= 256; }
 
 bool psh_zfuse_tile(int Ng, int intlace, int tile[3]) {
   auto set = [&](auto nz) {

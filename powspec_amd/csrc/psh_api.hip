@@ -768,7 +768,7 @@ void ev_collect(psh_ctx *ctx) {
 int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
                float *meshA, float *meshB) {
   size_t ntotal = 0;
-  ctx->ag.wide = 0;
+  ctx->ag.wide = ctx->ag.zfuse && psh_zfuse_wide(ctx->Ng);   // the fused shape's cells
   for (auto &p : hp) {
     ntotal += p.n;
     if (p.wscale < 0.f) ctx->ag.wide = 1;   // data - alpha * randoms

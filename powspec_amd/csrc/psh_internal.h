@@ -86,9 +86,10 @@ struct PshAssignGeom {
 };
 
 // Tile of the deposit with the fused z transform for a grid size: whole z
-// rows, tx x ty rows per grid (32768 / Ng lines per tile over both grids);
-// false when this build has no fused deposit for Ng.
+// rows, tx x ty rows per grid; false when this build has no fused deposit
+// for Ng.  psh_zfuse_wide: whether it accumulates 64-bit cells.
 bool psh_zfuse_tile(int Ng, int intlace, int tile[3]);
+bool psh_zfuse_wide(int Ng);
 
 // Deposit tile in LDS, per grid: rows along z padded by the window width
 // W = scheme + 1 on each side (window cells past the tile's z range land

@@ -4,8 +4,8 @@ in batched graph replays, with k_z pruning (k_max below Nyquist), and
 bit for bit from run to run.
 
 Bar as in test_gpu_parity: counts bit-exact, |dP_l| <= 1e-4 S_b per bin; the
-fused and separate paths differ by FP32 rounding only (the same FFT passes,
-one fixed-point scale per tile of different shape)."""
+fused path's worst error is at most a few times the separate path's (the
+same FFT passes; the tiles differ in shape, and so in fixed-point step)."""
 import numpy as np
 import pytest
 
@@ -18,9 +18,11 @@ def test_fused_tile_shapes(psh):
     for Ng in (128, 256, 512, 1024):
         for il in (False, True):
             t = psh.fused_tile(Ng, il)
-            assert t is not None and t[2] == Ng
-            # 32768 / Ng lines of Ng / 2 points over both grids
-            assert t[0] * t[1] * (2 if il else 1) * Ng == 32768
+            assert t is not None and t[2] == Ng and min(t[0], t[1]) >= 4
+            # 32768 / Ng lines of Ng / 2 points over both grids (16384 / Ng
+            # with the 64-bit cells of N_g <= 256)
+            lines = (16384 if Ng <= 256 else 32768) // Ng
+            assert t[0] * t[1] * (2 if il else 1) == lines
     assert psh.fused_tile(64, True) is None and psh.fused_tile(768, True) is None
 
 
@@ -45,9 +47,12 @@ def test_fused_vs_oracle(psh, oracle, Ng, scheme, intlace):
     ref = oracle.run(oracle.make_conf(**kw), [d])
     w = _check(got, ref, [0, 2, 4], ref["shot"][0])
     _, sep = _run(psh, kw, d, (8, 16, 64))
-    scale = np.abs(ref["pl"][0][0]) + ref["shot"][0]
-    assert np.all(np.abs(got["pl"][0] - sep["pl"][0]) <= 1e-5 * scale)
-    print(f"Ng={Ng} scheme={scheme} intlace={intlace}: fused vs oracle worst {w:.2e}")
+    ws = _check(sep, ref, [0, 2, 4], ref["shot"][0])
+    # no worse than the separate path beyond the fixed-point step of its
+    # larger tiles (both are far inside the bar)
+    assert w <= 3 * ws + 1e-6, (w, ws)
+    print(f"Ng={Ng} scheme={scheme} intlace={intlace}: worst vs oracle fused {w:.2e}, "
+          f"separate {ws:.2e}")
 
 
 def test_fused_1024_clustered_vs_oracle(psh, oracle):

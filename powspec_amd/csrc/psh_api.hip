@@ -1582,6 +1582,9 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, nmock, ntotal))) return rc;
+  // boxes: weights of one sign, 64-bit cells only for the fused shape's
+  // small grids (run_assign sets the same for the eager path)
+  ctx->ag.wide = ctx->ag.zfuse && psh_zfuse_wide(ctx->Ng);
   float *mB = ctx->conf.intlace ? ctx->meshB : nullptr;
   std::vector<double> raw((size_t)nmock * nacc);
   int flags = 0;

@@ -59,49 +59,87 @@ struct ZCfg {
 // Real-space Y_lm weights of the survey estimator (src/mp_template.c:69-95),
 // evaluated in FP32 from per-axis position tables.
 // ---------------------------------------------------------------------------
+// A real-form Y_lm as norm * sin^|m|(theta) * Q(cos theta) * trig(|m| phi),
+// Q the polynomial of degree l - |m| that the associated Legendre recursion
+// below produces divided by sin^|m| (coefficients built on the host, norm
+// folded in), trig = cos for m > 0, sin for m < 0, 1 for m = 0.  Horner in
+// cos theta replaces the recursion's per-point divisions.
+struct YlmPoly {
+  int am;        // |m|
+  int deg;       // l - |m|
+  int sinpart;   // m < 0
+  float q[8];    // coefficients of Q, ascending powers, times norm
+  float psgn;    // Y_lm at (-k_x, k_y, k_z) over Y_lm at (k_x, k_y, k_z)
+};
+
 struct YlmDev {
   int ell, m;
   float norm;            // sqrt(2 (2l+1)/(4pi) (l-|m|)!/(l+|m|)!), no sqrt2 at m=0
   const float *rpos[3];  // (i + smin_g) * L per axis (r-space)
+  YlmPoly poly;
 };
 
-// Real-form spherical harmonic from the associated Legendre recursion (no
-// Condon-Shortley phase; m>0 cos(m phi), m<0 sin(|m| phi)).
-__device__ __forceinline__ float ylm_eval(int ell, int m, float norm, float ct,
-                                          float st, float cp, float sp) {
-  int am = m < 0 ? -m : m;
-  float pmm = 1.f;
-  for (int i = 1; i <= am; i++) pmm *= (float)(2 * i - 1) * st;
-  float plm = pmm;
+// Host: Q from the associated Legendre recursion without the Condon-Shortley
+// phase (P_m^m = (2m-1)!! sin^m, P_{m+1}^m = (2m+1) cos P_m^m, P_l^m =
+// ((2l-1) cos P_{l-1}^m - (l+m-1) P_{l-2}^m) / (l-m)), run on polynomials in
+// cos theta in FP64 with the common sin^m factored out.
+inline YlmPoly make_ylm_poly(int ell, int m, float norm) {
+  YlmPoly p{};
+  const int am = m < 0 ? -m : m;
+  p.am = am;
+  p.deg = ell - am;
+  p.sinpart = m < 0;
+  double pmm[8] = {0}, pm1[8] = {0}, pl[8];
+  double df = 1.0;
+  for (int i = 1; i <= am; i++) df *= 2 * i - 1;
+  pmm[0] = df;
+  const double *res = pmm;
   if (ell > am) {
-    float pm1 = ct * (float)(2 * am + 1) * pmm;
+    for (int k = 0; k < 7; k++) pm1[k + 1] = (2 * am + 1) * pmm[k];
     for (int l = am + 2; l <= ell; l++) {
-      float pl = ((float)(2 * l - 1) * ct * pm1 - (float)(l + am - 1) * pmm) /
-                 (float)(l - am);
-      pmm = pm1;
-      pm1 = pl;
+      for (int k = 0; k < 8; k++) pl[k] = -(double)(l + am - 1) * pmm[k];
+      for (int k = 0; k < 7; k++) pl[k + 1] += (2 * l - 1) * pm1[k];
+      for (int k = 0; k < 8; k++) {
+        pl[k] /= (double)(l - am);
+        pmm[k] = pm1[k];
+        pm1[k] = pl[k];
+      }
     }
-    plm = pm1;
+    res = pm1;
   }
-  if (m == 0) return norm * plm;
+  for (int k = 0; k < 8; k++) p.q[k] = (float)(res[k] * norm);
+  // k_x -> -k_x: phi -> pi - phi, cos(|m| phi) -> (-1)^|m| cos, sin -> -(-1)^|m| sin
+  const float par = (am & 1) ? -1.f : 1.f;
+  p.psgn = am == 0 ? 1.f : (m > 0 ? par : -par);
+  return p;
+}
+
+// Y_lm at the unit vector (ux, uy, uz): sin^|m| theta e^{i |m| phi} is
+// (ux + i uy)^|m|, so neither sin theta nor the azimuth is formed.
+__device__ __forceinline__ float ylm_poly_eval(const YlmPoly &p, float uz, float ux,
+                                               float uy) {
+  float q = 0.f;
+#pragma unroll
+  for (int k = 6; k >= 0; k--)
+    if (k <= p.deg) q = __builtin_fmaf(q, uz, p.q[k]);
+  if (p.am == 0) return q;
   float c = 1.f, s = 0.f;
-  for (int i = 0; i < am; i++) {
-    float c2 = c * cp - s * sp;
-    s = s * cp + c * sp;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    if (i >= p.am) break;
+    const float c2 = c * ux - s * uy;
+    s = s * ux + c * uy;
     c = c2;
   }
-  return norm * plm * (m > 0 ? c : s);
+  return q * (p.sinpart ? s : c);
 }
 
 __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
                                               int k) {
   if (i == 0 && j == 0) return 1.f;  // first row unweighted (mp_template.c:75)
   float rx = y.rpos[0][i], ry = y.rpos[1][j], rz = y.rpos[2][k];
-  float r2 = rx * rx + ry * ry;
-  float ir3 = rsqrtf(r2 + rz * rz);
-  float ir22 = rsqrtf(r2);
-  float r22 = r2 * ir22;
-  return ylm_eval(y.ell, y.m, y.norm, rz * ir3, r22 * ir3, rx * ir22, ry * ir22);
+  const float ir = rsqrtf(rx * rx + ry * ry + rz * rz);
+  return ylm_poly_eval(y.poly, rz * ir, rx * ir, ry * ir);
 }
 
 // ---------------------------------------------------------------------------
@@ -461,6 +499,7 @@ struct XBinDev {
   float dcf;
   int ell, m;
   float ynorm;
+  YlmPoly ypoly;
   float vec0f;   // 2 pi / L_x
   // SIM: L_ell(mu) = mu^(ell & 1) * (((c3 x + c2) x + c1) x + c0), x = mu^2,
   // for the requested pole l; bit l of oddmask set when ell is odd
@@ -883,13 +922,11 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           const float kdx = (float)a.ax.kd[0][i];
           const float k2 = kdx * kdx + kdy * kdy;
           if (k2 != 0.f) {
-            const float ik3 = rsqrtf(k2 + kdz * kdz);
-            const float ik22 = rsqrtf(k2);
-            yw = ylm_eval(a.ell, a.m, a.ynorm, kdz * ik3, k2 * ik22 * ik3, kdx * ik22,
-                          kdy * ik22);
-            if (pair)
-              ywp = ylm_eval(a.ell, a.m, a.ynorm, kdz * ik3, k2 * ik22 * ik3, -kdx * ik22,
-                             kdy * ik22);
+            const float ik = rsqrtf(k2 + kdz * kdz);
+            yw = ylm_poly_eval(a.ypoly, kdz * ik, kdx * ik, kdy * ik);
+            // the partner -k_x: the same polynomial part, the azimuthal
+            // factor's parity (YlmPoly::psgn)
+            if (pair) ywp = yw * a.ypoly.psgn;
           }
           acc_fixed(&ibins[kb], w * yw + wp * ywp, sdown);
         }
@@ -1122,6 +1159,10 @@ hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
   y.m = ylm.m;
   y.norm = ylm.norm;
   for (int a = 0; a < 3; a++) y.rpos[a] = ylm.rpos[a];
+  if (ylm.ell > 0) {
+    if (ylm.ell > 6 || ylm.m < -ylm.ell || ylm.m > ylm.ell) return hipErrorInvalidValue;
+    y.poly = make_ylm_poly(ylm.ell, ylm.m, ylm.norm);
+  }
   return dispatch_n<ZR2C>(Ng, in, out, nmesh, mstride_floats, tw, y, s, kzlim);
 }
 
@@ -1269,6 +1310,10 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.ell = a.ell;
   d.m = a.m;
   d.ynorm = a.ynorm;
+  if (a.epi == PSH_EPI_SURVEYL) {
+    if (a.ell < 0 || a.ell > 6 || a.m < -a.ell || a.m > a.ell) return hipErrorInvalidValue;
+    d.ypoly = make_ylm_poly(a.ell, a.m, a.ynorm);
+  }
   d.vec0f = a.vec0f;
   d.ylive = a.ylive;
   // Legendre polynomials L_0..L_6 (math/legpoly.h:47-61 of the reference) as

@@ -546,11 +546,13 @@ struct XCfg {
   // workgroups share a CU and hide each other's barriers; the small
   // interlaced kernel holds 2 x 16 values per thread and gets 2 (256 VGPRs)
   static constexpr int WAVES = SMALL ? 2 : 4;
-  // whether the next tile's rows are loaded during phase 2 (see x_bin_kernel):
-  // the interlaced kernel from N = 640 holds both fields' 16 values per
-  // thread and spills registers when the prefetch is in flight as well
-  // (measured on MI355X at N = 1024: 3.98 ms with it, 3.71 ms without)
-  static constexpr bool PREFETCH = !(IL && N >= 640);
+  // the next tile's rows loaded during phase 2 (see x_bin_kernel): 2 every
+  // field, 1 the first field only (the second loads when the tile starts),
+  // 0 none.  The interlaced kernel from N = 640 holds both fields' 16 values
+  // per thread and spills registers when both prefetches are in flight as
+  // well (measured on MI355X at N = 1024: 4.31 ms with both, 4.00 ms with
+  // the first field's, 3.46 ms without)
+  static constexpr int PREFETCH = (IL && N >= 640) ? 0 : 2;
   static_assert(T % C == 0, "a thread keeps one column");
 };
 
@@ -692,17 +694,21 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   };
   // The next tile's rows are loaded into va/vb as soon as phase 1 has
   // consumed them, so their HBM latency hides behind phase 2 (one block per
-  // CU: nothing else would overlap it) -- unless the registers cannot hold
-  // them (XCfg::PREFETCH false: each tile loads its rows when it starts).
-  constexpr bool PF = XCfg<N, INTLACE>::PREFETCH;
+  // CU: nothing else would overlap it) -- as far as the registers hold them
+  // (XCfg::PREFETCH: the fields not prefetched load when the tile starts).
+  constexpr int PF = XCfg<N, INTLACE>::PREFETCH;
   float2 va[V];
   float2 vb[INTLACE ? V : 1];
-  auto load_tile = [&](size_t t) {
+  auto load_field = [&](size_t t, bool second) {
     int ty, tq;
     const size_t tb = tile_base(t, ty, tq);
     const int tn = min(C, M + 1 - tq * C);
-    pass_load_rows<N, C, T, true>(va, ld_field(a.A, tb, tn));
-    if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
+    if (!second) pass_load_rows<N, C, T, true>(va, ld_field(a.A, tb, tn));
+    else if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
+  };
+  auto prefetch = [&](size_t t) {
+    if (PF >= 1) load_field(t, false);
+    if (PF >= 2) load_field(t, true);
   };
   // A tile whose nearest mode (k_x = 0, lowest k_z) already lies at or past
   // the last bin edge has no mode in the bins: its x transform is skipped.
@@ -723,11 +729,12 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     return t;
   };
   size_t tile = next_live(t0);
-  if (PF && tile < ntiles) load_tile(tile);
+  if (tile < ntiles) prefetch(tile);
   for (; tile < ntiles;) {
     int y, q;
     const size_t base = tile_base(tile, y, q);
-    if (!PF) load_tile(tile);
+    if (PF < 1) load_field(tile, false);
+    if (PF < 2) load_field(tile, true);
     const int ncv = min(C, M + 1 - q * C);
     if (INTLACE)
       Passes2<N, C, T, 0, true, false, 1>::run(va, vb, lds, twl);
@@ -793,7 +800,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     }
     __syncthreads();
     const size_t next = next_live(tile + gridDim.x);
-    if (PF && next < ntiles) load_tile(next);
+    if (next < ntiles) prefetch(next);
 
     // phase 2: column c, pairs (i, N - i) with i = threadIdx.x / C + k T / C
     // up to M.  The tile's integer sums take the scale 2^E with
@@ -1082,55 +1089,76 @@ struct Col {
 
 template <int N>
 struct XBin {
-  static hipError_t run(const XBinDev &d, int epi, bool il, int grid, int nmock,
-                        const float2 *tw, hipStream_t s) {
-    const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
-    const int H = N / 2 + PSH_PAD;
-    const int ntz = (N / 2 + 1 + C - 1) / C;
+  using KFn = void (*)(XBinDev, int, int, const float2 *);
+  // The kernel of a window of nacc accumulators, its dynamic LDS and
+  // threads.  Two workgroups per CU (which the register budget allows up to
+  // 512 threads, the plain pass up to N_g = 512) need at most half the LDS
+  // each: when only the GS layout fits in that half, it is used (measured on
+  // the learned correction's plain N_g = 512 pass with 8 |mu| cells: one
+  // workgroup per CU otherwise).
+  template <bool NT>
+  static KFn pick(int epi, bool il, int nacc, size_t &shmem, int &T) {
     const size_t stat = il ? xbin_static_lds<N, true>() : xbin_static_lds<N, false>();
-    const size_t np = (size_t)((d.nacc + 1) & ~1);
-    const int T = il ? XCfg<N, true>::T : XCfg<N, false>::T;
-    size_t shmem = kXbinAccBytes * np + stat;
-    // Two workgroups per CU (which the register budget allows up to 512
-    // threads, the plain pass up to N_g = 512) need at most half the LDS
-    // each: when only the GS layout fits in that half, it is used (measured
-    // on the learned correction's plain N_g = 512 pass with 8 |mu| cells:
-    // one workgroup per CU otherwise)
+    const size_t np = (size_t)((nacc + 1) & ~1);
+    T = il ? XCfg<N, true>::T : XCfg<N, false>::T;
+    shmem = kXbinAccBytes * np + stat;
     bool gs = false;
     const size_t shmem_gs = kXbinAccBytesG * np + xbin_static_lds<N, false, true>();
     if (!il && T <= 512 && shmem > kXbinLdsMax / 2 && shmem_gs <= kXbinLdsMax / 2) {
       gs = true;
       shmem = shmem_gs;
     }
+    KFn k = nullptr;
+#define PSH_XB(E, I)                                                 \
+  {                                                                     \
+    k = x_bin_kernel<N, E, I, NT, false>;                               \
+    if (!(I) && gs) k = x_bin_kernel<N, E, false, NT, true>;            \
+  }
+    if (epi == PSH_EPI_SIM) {
+      if (il) PSH_XB(PSH_EPI_SIM, true) else PSH_XB(PSH_EPI_SIM, false)
+    } else if (epi == PSH_EPI_SURVEY0) {
+      if (il) PSH_XB(PSH_EPI_SURVEY0, true) else PSH_XB(PSH_EPI_SURVEY0, false)
+    } else {
+      PSH_XB(PSH_EPI_SURVEYL, false)
+    }
+#undef PSH_XB
+    if (shmem > 64 * 1024)
+      (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem);
+    return k;
+  }
+
+  static hipError_t run(const XBinDev &d, int epi, bool il, int grid, int nmock,
+                        const float2 *tw, hipStream_t s) {
+    const int C = il ? XCfg<N, true>::C : XCfg<N, false>::C;
+    const int H = N / 2 + PSH_PAD;
+    const int ntz = (N / 2 + 1 + C - 1) / C;
     const XBinDev &dd = d;
-    const dim3 g(grid, nmock), b(T);
-    if (shmem > 160 * 1024) return hipErrorInvalidValue;
     auto go = [&](auto ntc) {
       constexpr bool NT = decltype(ntc)::value;
-#define PSH_XB(E, I)                                                          \
-  {                                                                              \
-    auto k = x_bin_kernel<N, E, I, NT, false>;                                   \
-    if (!(I) && gs) k = x_bin_kernel<N, E, false, NT, true>;                     \
-    if (shmem > 64 * 1024)                                                       \
-      (void)hipFuncSetAttribute((const void *)k,                                 \
-                                hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                                (int)shmem);                                     \
-    hipLaunchKernelGGL(k, g, b, shmem, s, dd, ntz, H, tw);                       \
-    return hipGetLastError();                                                    \
-  }
-      if (epi == PSH_EPI_SIM) {
-        if (il) PSH_XB(PSH_EPI_SIM, true) else PSH_XB(PSH_EPI_SIM, false)
-      } else if (epi == PSH_EPI_SURVEY0) {
-        if (il) PSH_XB(PSH_EPI_SURVEY0, true) else PSH_XB(PSH_EPI_SURVEY0, false)
-      } else {
-        PSH_XB(PSH_EPI_SURVEYL, false)
-      }
-#undef PSH_XB
+      size_t shmem;
+      int T;
+      KFn k = pick<NT>(epi, il, d.nacc, shmem, T);
+      if (shmem > kXbinLdsMax) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k, dim3(grid, nmock), dim3(T), shmem, s, dd, ntz, H, tw);
+      return hipGetLastError();
     };
     // the x pass reads nmock fields (two when interlaced) of mstride_c values
     const size_t bytes = sizeof(float2) * d.mstride_c * (size_t)nmock * (il ? 2 : 1);
     if constexpr (stream_mesh<N>()) return go(std::true_type{});
     else return stream_bytes(bytes) ? go(std::true_type{}) : go(std::false_type{});
+  }
+};
+
+// Workgroups per CU of the x pass kernel for a window of nacc accumulators.
+template <int N>
+struct XBinOcc {
+  static hipError_t run(int epi, bool il, int nacc, int *per_cu) {
+    size_t shmem;
+    int T;
+    auto k = XBin<N>::template pick<true>(epi, il, nacc, shmem, T);
+    if (shmem > kXbinLdsMax) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void *)k, T, shmem);
   }
 };
 
@@ -1273,6 +1301,12 @@ int psh_xbin_window_bins(int Ng, bool il, int per_bin) {
   long nb = (long)((kXbinLdsMax - fixed) / kXbinAccBytes) / per_bin;
   while (nb > 0 && psh_xbin_lds_bytes(Ng, il, (int)(nb * per_bin)) > kXbinLdsMax) nb--;
   return nb > (1 << 30) ? (1 << 30) : (int)nb;
+}
+
+int psh_xbin_per_cu(int Ng, int epi, bool il, int nacc) {
+  int nb = 0;
+  if (dispatch_n<XBinOcc>(Ng, epi, il, nacc, &nb) != hipSuccess) return 0;
+  return nb;
 }
 
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,

@@ -134,6 +134,10 @@ struct psh_ctx {
   double *d_gkm = nullptr, *d_glcnt = nullptr;
   // bins per launch of the fused x pass (its LDS holds a window of the bins)
   int xwin_sim = 0, xwin_survey = 0;
+  // CUs of the device and workgroups per CU of the x pass kernels (boxes;
+  // survey l = 0 and l > 0 fields) at their first, largest window
+  int ncu = 256;
+  int xres_sim = 1, xres_s0 = 1, xres_sl = 1;
 
   // |mu| cells of box runs and the learned alias correction of plain boxes
   int nmu = 1;
@@ -392,6 +396,22 @@ int make_bins(psh_ctx *ctx) {
   if ((c.issim ? ctx->xwin_sim : ctx->xwin_survey) < 1)
     return set_err(ctx, PSH_ERR_UNSUPPORTED,
                    "the fused x pass has no LDS left for bin sums at GRID_SIZE %d", ctx->Ng);
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      ncu > 0)
+    ctx->ncu = ncu;
+  auto res = [&](int epi, bool il, int nacc) {
+    const int r = psh_xbin_per_cu(ctx->Ng, epi, il, nacc);
+    return r > 0 ? r : 1;
+  };
+  if (c.issim) {
+    ctx->xres_sim = res(PSH_EPI_SIM, c.intlace != 0,
+                        std::min(ctx->xwin_sim, b.nbin) * c.npole * b.nmu);
+  } else {
+    ctx->xres_s0 = res(PSH_EPI_SURVEY0, c.intlace != 0, std::min(ctx->xwin_survey, b.nbin));
+    ctx->xres_sl = res(PSH_EPI_SURVEYL, false, std::min(ctx->xwin_survey, b.nbin));
+  }
   return PSH_OK;
 }
 
@@ -729,13 +749,19 @@ int ensure_partials(psh_ctx *ctx, int nmock, int grid, int nacc, int slots) {
   return PSH_OK;
 }
 
-// Persistent workgroups per mock of the fused x pass: about 2048 in total, a
-// multiple of 8 (one share per XCD), at most one per tile.
-int xgrid_for(psh_ctx *ctx, int nmock) {
+// Persistent workgroups per mock of the fused x pass: as many as the device
+// holds at once (CUs x per_cu, the kernel's workgroups per CU), shared among
+// the mocks, a multiple of 8 (one share per XCD), at most one per tile.
+// More workgroups than that only add rounds of set-up, partial-sum rows and
+// a ragged last round: measured on MI355X, 256 workgroups instead of 2048
+// take the interlaced N_g = 1024 PCS box pass from 3.70 to 3.48 ms and 512
+// the plain N_g = 512 pass with 8 |mu| cells from 0.43 to 0.36 ms; 1.5
+// rounds (384 at one per CU) cost 20 %.
+int xgrid_for(psh_ctx *ctx, int nmock, int per_cu) {
   const int C = 16;
   const int ntz = (ctx->M + 1 + C - 1) / C;
   const long ntiles = (long)ctx->Ng * ntz;
-  long g = (2048 / nmock) & ~7L;
+  long g = ((long)ctx->ncu * per_cu / nmock) & ~7L;
   if (g < 8) g = 8;
   if (g > ntiles) g = ntiles;
   return (int)g;
@@ -825,7 +851,7 @@ int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/)
     HIPCHK(ctx, psh_fft_y(N, B, nmock, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
                           ctx->ysched));
   ev_record(ctx, E_YDONE);
-  const int grid = xgrid_for(ctx, nmock);
+  const int grid = xgrid_for(ctx, nmock, ctx->xres_sim);
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   int rc;
   if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
@@ -932,9 +958,11 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
                           ctx->ysched));
   }
   ev_record(ctx, E_YDONE);
-  const int grid = xgrid_for(ctx, 1);
+  // the l = 0 and l > 0 passes each at their own grid, over one partials buffer
+  const int grid = xgrid_for(ctx, 1, ctx->xres_s0);
+  const int gridl = xgrid_for(ctx, 1, ctx->xres_sl);
   const int nslots = survey_slots(ctx);
-  if ((rc = ensure_partials(ctx, 1, grid, nb, nslots))) return rc;
+  if ((rc = ensure_partials(ctx, 1, std::max(grid, gridl), nb, nslots))) return rc;
   PshXBinArgs a{};
   a.epi = PSH_EPI_SURVEY0;
   a.A = A;
@@ -988,6 +1016,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
                             ctx->ysched));
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;
+      b.grid = gridl;
       b.has_dc = 0;
       b.ylive = ctx->d_ylive;   // S is read by this pass only
       b.A = S;
@@ -1002,7 +1031,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
         b.nbw = std::min(ctx->xwin_survey, nb - kb0);
         HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
       }
-      HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb,
+      HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, gridl, nb,
                                       ctx->d_bins + (size_t)slot * nb, ctx->stream));
     }
   }
@@ -1358,7 +1387,7 @@ int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double>
   if (rc == PSH_ERR_CATA) return sim_box_error(ctx, hp);
   if (rc) return rc;
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
-  if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1), nacc, 1))) return rc;
+  if ((rc = ensure_partials(ctx, 1, xgrid_for(ctx, 1, ctx->xres_sim), nacc, 1))) return rc;
   if ((rc = run_sim_fft_bin(ctx, 1, ctx->d_bins))) return rc;
   const int nb = ctx->nbin, nl = ctx->conf.npole, nmu = ctx->bins.nmu;
   const bool mlp = correct && ctx->mlp_nnet > 0 && !ctx->conf.intlace && nmu > 1;
@@ -1578,7 +1607,7 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
                        offsets[m], 1.0f, m};
   const size_t ntotal = offsets[nmock] - offsets[0];
   for (int m = 0; m < nmock; m++) hp[m].start = offsets[m] - offsets[0];
-  const int grid = xgrid_for(ctx, nmock);
+  const int grid = xgrid_for(ctx, nmock, ctx->xres_sim);
   const int nacc = psh_xbin_nacc(ctx->bins, PSH_EPI_SIM);
   if ((rc = ensure_partials(ctx, nmock, grid, nacc, 1))) return rc;
   if ((rc = ensure_assign_scratch(ctx, nmock, nmock, ntotal))) return rc;

@@ -211,6 +211,9 @@ size_t psh_xbin_lds_bytes(int Ng, bool il, int nacc);
 // Most bins one launch of the fused x pass holds in LDS with per_bin
 // accumulators per bin (more bins take several launches over bin windows).
 int psh_xbin_window_bins(int Ng, bool il, int per_bin);
+// Workgroups of the fused x pass one CU holds at once for a window of nacc
+// accumulators (0 when the occupancy query fails).
+int psh_xbin_per_cu(int Ng, int epi, bool il, int nacc);
 hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
     hipStream_t s);
 

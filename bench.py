@@ -45,13 +45,13 @@ def box_conf(P):
 def x_live_fraction(Ng, L, kedge_max, C=16):
     """Share of the stored modes the fused x pass reads (and the y pass
     writes): the (k_y, k_z-tile) columns of C k_z values whose nearest mode
-    lies before the last bin edge (same test as the library, for a cubic box
-    of side L)."""
-    kf = 2 * np.pi / L
+    lies before the last bin edge (same test as the library; L the side of a
+    cubic box or the three sides)."""
+    Ls = np.broadcast_to(np.asarray(L, float), (3,))
     M = Ng // 2
-    ky = np.fft.fftfreq(Ng, 1.0 / Ng) * kf
+    ky = np.fft.fftfreq(Ng, 1.0 / Ng) * (2 * np.pi / Ls[1])
     q = np.arange((M + 1 + C - 1) // C)
-    kz0 = q * C * kf
+    kz0 = q * C * (2 * np.pi / Ls[2])
     live = np.sqrt(ky[:, None] ** 2 + kz0[None, :] ** 2) < kedge_max
     cols = np.minimum(C, M + 1 - q * C)
     return float((live * cols[None, :]).sum() / (Ng * (M + 1)))
@@ -383,12 +383,38 @@ def run_survey(P, steps=3):
     ms = (time.perf_counter() - t0) / steps * 1e3
     log(f"survey: {ms:.2f} ms per P_l evaluation (generation + conversion {t_gen:.1f} s, "
         f"Gauss-Legendre order {order}), nbin {res['nbin']}")
+    # the dominant kernel (the l > 0 fields' fused x pass, x_bin_kernel<512,
+    # SURVEYL>): HIP events around its launches in one more run, one launch
+    # per (l, m) field and bin window; each launch reads the field and F0 on
+    # the (k_y, k_z) columns with binned modes
+    ctx.set_profiling(True)
+    ctx.run_survey_device(dd, nd, rr, nr, alpha, shot, norm)
+    tm = ctx.timings()
+    ctx.set_profiling(False)
+    side, _ = ctx.box()
+    nlm = 5 + 9
+    Ng = 512
+    live = x_live_fraction(Ng, side, float(res["kedge"][-1]))
+    field_bytes = 2 * 8 * Ng * Ng * (Ng // 2 + 1) * live
+    dom = None
+    if tm[5] > 0:
+        rate = field_bytes * nlm / (tm[5] * 1e-3) / 1e9
+        dom = {"kernel": "x_bin_kernel<512, SURVEYL> (l > 0 fused x pass)",
+               "fields_per_evaluation": nlm, "ms_per_evaluation": round(tm[5], 3),
+               "us_per_field": round(tm[5] / nlm * 1e3, 1),
+               "bytes_per_field": round(field_bytes), "live_column_fraction": round(live, 4),
+               "achieved_GBps": round(rate, 1), "peak_GBps": 8000.0,
+               "frac": round(rate / 8000.0, 4),
+               "z_y_passes_ms_per_evaluation": round(tm[6], 3)}
+        log(f"survey: l > 0 fused x passes {tm[5]:.2f} ms per evaluation ({nlm} fields, "
+            f"{field_bytes / 1e9:.3f} GB each): {rate:.0f} GB/s, {rate / 8000:.3f} of HBM peak; "
+            f"their z + y passes {tm[6]:.2f} ms")
     return {"workload": "survey sky cap 60x60 deg, z in [0.4,0.7] Gaussian n(z), 5e6 data + "
                         "1e8 randoms, GPU coordinate conversion, CIC N_g=512 interlaced, "
                         "l=0,2,4, dk=0.01",
             "ms_per_evaluation": round(ms, 3), "evaluations_per_s": round(1e3 / ms, 3),
             "particles_per_s": round((nd + nr) / (ms * 1e-3), 1), "nbin": res["nbin"],
-            "fft_transforms": 2 + (5 + 9)}
+            "fft_transforms": 2 + (5 + 9), "dominant_kernel": dom}
 
 
 def run_ingest(P, data, tile, reps=3):

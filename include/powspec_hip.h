@@ -288,7 +288,11 @@ int psh_ctx_run_survey_device(psh_ctx *ctx, const psh_data *d_data, size_t nd,
  * events on the context's stream (profiling on): [0] assignment incl. the
  * coordinate-bounds pass, [1] z and y FFT passes, [2] x FFT pass fused with
  * interlacing and binning, [3] total, [4] deposit kernel, [5] z passes,
- * [6] y passes, [7] bucketing kernels (count, scan, fill). */
+ * [6] y passes, [7] bucketing kernels (count, scan, fill).  Survey runs:
+ * [1] and [2] are the l = 0 field's passes, [5] the sum of the l > 0
+ * fields' fused x passes (one per (l, m) and bin window), [6] the sum of
+ * their z and y passes; [3] the survey estimator (the last run when there
+ * are several catalogues). */
 int psh_ctx_timings(const psh_ctx *ctx, double ms[8]);
 
 /* Enable/disable per-stage event timing (adds event records between
@@ -421,6 +425,11 @@ int psh_fused_tile(int gsize, int intlace, int tile[3]);
  * psh_ctx_set_tiles, a tuning cache, PSH_TILE or the default.  PSH_ERR_ARG
  * before the context has a box (surveys: before their first run). */
 int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]);
+
+/* The box of the context's grid: side lengths and lower corner (surveys: the
+ * padded box of the last run's catalogues, src/genr_mesh.c:427-578, which the
+ * reference reports in its log).  PSH_ERR_ARG before the context has a box. */
+int psh_ctx_get_box(const psh_ctx *ctx, double bsize[3], double bmin[3]);
 
 /* Applies the fastest entries of a tuning cache for the context's device
  * architecture, grid size, scheme, interlacing and batch size (rounded down

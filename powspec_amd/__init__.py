@@ -70,6 +70,7 @@ EXPORTS = [
     "psh_save_res", "psh_cnvt_coord", "psh_cnvt_device", "psh_gen_philox",
     "psh_gen_uniform_box", "psh_gen_clustered_box", "psh_gen_sky", "psh_ctx_set_tiles",
     "psh_ctx_get_tiles",
+    "psh_ctx_get_box",
     "psh_ctx_load_tuning", "psh_autotune", "psh_ctx_run_survey_device", "psh_mlp_param_count",
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
@@ -207,6 +208,7 @@ def load():
     L.psh_ctx_set_box.argtypes = [vp, C.POINTER(d), C.POINTER(d)]
     L.psh_ctx_nbin.argtypes = [vp]
     L.psh_ctx_get_tiles.argtypes = [vp, C.POINTER(C.c_int)]
+    L.psh_ctx_get_box.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.psh_ctx_run_host.argtypes = [vp, C.POINTER(Cata), C.POINTER(Pk)]
     L.psh_ctx_define_box.argtypes = [vp, C.POINTER(Cata)]
     L.psh_ctx_run_device.argtypes = [vp, vp, sz, d, C.POINTER(Pk)]
@@ -444,6 +446,12 @@ class Context:
         t = (C.c_int * 3)()
         self._check(self.L.psh_ctx_get_tiles(self.h, t))
         return tuple(t)
+
+    def box(self):
+        """(side lengths, lower corner) of the grid's box."""
+        b, m = (C.c_double * 3)(), (C.c_double * 3)()
+        self._check(self.L.psh_ctx_get_box(self.h, b, m))
+        return tuple(b), tuple(m)
 
     def set_mu_cells(self, nmu):
         self._check(self.L.psh_ctx_set_mu_cells(self.h, int(nmu)))

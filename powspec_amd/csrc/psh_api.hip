@@ -76,6 +76,8 @@ enum {
 
 struct Timer {
   hipEvent_t ev[E_COUNT] = {};
+  // survey fields l > 0, three per (l, m): z pass start, y pass end, x pass end
+  std::vector<hipEvent_t> lev;
 };
 
 }  // namespace
@@ -771,9 +773,13 @@ void ev_record(psh_ctx *ctx, int i) {
   if (ctx->profiling && ctx->tm.ev[i]) (void)hipEventRecord(ctx->tm.ev[i], ctx->stream);
 }
 
+// -1 for a pair of events the run did not record; the failed query's error
+// is cleared, or the next launch check would report it
 double ev_ms(psh_ctx *ctx, int a, int b) {
   float t;
-  return hipEventElapsedTime(&t, ctx->tm.ev[a], ctx->tm.ev[b]) == hipSuccess ? t : -1.0;
+  if (hipEventElapsedTime(&t, ctx->tm.ev[a], ctx->tm.ev[b]) == hipSuccess) return t;
+  (void)hipGetLastError();
+  return -1.0;
 }
 
 // ms: [0] assignment (incl. bounds), [1] z+y passes, [2] fused x pass,
@@ -989,6 +995,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
     if (wi > 0) w.store = nullptr;
     HIPCHK(ctx, psh_fft_x_bin(N, w, ctx->tw, ctx->stream));
   }
+  ev_record(ctx, E_XDONE);
   HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, grid, nb, ctx->d_bins, ctx->stream));
   if (il) {
     // back to configuration space: B = c2r(F0) / Ntot (src/multipole.c:489-500)
@@ -1002,9 +1009,21 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
   // the first pole is skipped when it is not 0, as in the reference
   int slot = 1;
   float2 *S = reinterpret_cast<float2 *>(ctx->meshS);
+  const bool lprof = ctx->profiling && !ctx->arena;
+  if (lprof)
+    while (ctx->tm.lev.size() < (size_t)3 * (nslots - 1)) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) break;
+      ctx->tm.lev.push_back(e);
+    }
+  auto lrec = [&](int q) {
+    if (lprof && (size_t)q < ctx->tm.lev.size())
+      (void)hipEventRecord(ctx->tm.lev[q], ctx->stream);
+  };
   for (int n = 1; n < nl; n++) {
     const int ell = ctx->conf.poles[n];
     for (int m = -ell; m <= ell; m++, slot++) {
+      lrec(3 * (slot - 1));
       PshYlmR y{};
       y.ell = ell;
       y.m = m;
@@ -1014,6 +1033,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
                                 ctx->kzlim));
       HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
                             ctx->ysched));
+      lrec(3 * (slot - 1) + 1);
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;
       b.grid = gridl;
@@ -1031,6 +1051,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
         b.nbw = std::min(ctx->xwin_survey, nb - kb0);
         HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
       }
+      lrec(3 * (slot - 1) + 2);
       HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, gridl, nb,
                                       ctx->d_bins + (size_t)slot * nb, ctx->stream));
     }
@@ -1041,6 +1062,22 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
                              hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ev_collect(ctx);
+  if (ctx->profiling) {
+    // surveys: [5] the l > 0 fields' fused x passes, [6] their z and y passes
+    double tx = 0.0, tzy = 0.0;
+    const size_t nev = ctx->tm.lev.size();
+    bool ok = lprof && nev >= (size_t)3 * (nslots - 1);
+    for (int q = 0; ok && q < nslots - 1; q++) {
+      float a0, a1;
+      ok = hipEventElapsedTime(&a0, ctx->tm.lev[3 * q], ctx->tm.lev[3 * q + 1]) == hipSuccess &&
+           hipEventElapsedTime(&a1, ctx->tm.lev[3 * q + 1], ctx->tm.lev[3 * q + 2]) == hipSuccess;
+      if (!ok) (void)hipGetLastError();
+      tzy += a0;
+      tx += a1;
+    }
+    ctx->ms[5] = ok ? tx : -1.0;
+    ctx->ms[6] = ok ? tzy : -1.0;
+  }
   return PSH_OK;
 }
 
@@ -1216,6 +1253,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
   for (void *p : bufs) ctx_free(ctx, p);
   for (int i = 0; i < E_COUNT; i++)
     if (ctx->tm.ev[i]) (void)hipEventDestroy(ctx->tm.ev[i]);
+  for (hipEvent_t e : ctx->tm.lev) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PSH_OK;
@@ -1679,8 +1717,7 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
   }
   if (graph) {
     if (ctx->profiling) {
-      float t;
-      ctx->ms[3] = hipEventElapsedTime(&t, ctx->tm.ev[E_START], ctx->tm.ev[E_END]) == hipSuccess ? t : -1;
+      ctx->ms[3] = ev_ms(ctx, E_START, E_END);
       ctx->ms[0] = ctx->ms[1] = ctx->ms[2] = -1;
     }
   } else {
@@ -2052,6 +2089,17 @@ extern "C" int psh_ctx_get_tiles(const psh_ctx *ctx, int tile[3]) {
   if (!ctx->box_set)
     return set_err(const_cast<psh_ctx *>(ctx), PSH_ERR_ARG, "the context has no box yet");
   for (int a = 0; a < 3; a++) tile[a] = ctx->ag.tile[a];
+  return PSH_OK;
+}
+
+extern "C" int psh_ctx_get_box(const psh_ctx *ctx, double bsize[3], double bmin[3]) {
+  if (!ctx || !bsize || !bmin) return set_err(nullptr, PSH_ERR_ARG, "null argument");
+  if (!ctx->box_set)
+    return set_err(const_cast<psh_ctx *>(ctx), PSH_ERR_ARG, "the context has no box yet");
+  for (int a = 0; a < 3; a++) {
+    bsize[a] = ctx->L[a];
+    bmin[a] = ctx->bmin[a];
+  }
   return PSH_OK;
 }
 

@@ -553,11 +553,20 @@ def test_survey_device_entry(psh, oracle):
     r = np.column_stack([rp, wr])
     kw = dict(gsize=32, assign=2, intlace=True, issim=False, kbin=0.01, poles=[0, 2, 4])
     ctx = psh.Context(psh.make_conf(**kw))
+    ctx.set_profiling(True)
     got = ctx.run_survey_device(psh.DeviceArray.from_numpy(d), len(d),
                                 psh.DeviceArray.from_numpy(r), len(r),
                                 sums["alpha"], sums["shot"], sums["norm"])
     ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
     _check(got, ref, [0, 2, 4], sums["shot"], sums["norm"])
+    # the padded box of the catalogues (src/genr_mesh.c:427-578)
+    side, lo = ctx.box()
+    assert np.allclose(side, ref["bsize"], rtol=1e-12) and np.allclose(lo, ref["bmin"], rtol=1e-12)
+    # stage times of a survey run: the l = 0 x pass, the l > 0 fields' x
+    # passes and z + y passes, all inside the total
+    ms = ctx.timings()
+    assert all(v > 0 for v in (ms[2], ms[3], ms[5], ms[6]))
+    assert ms[2] + ms[5] + ms[6] < ms[3]
 
 
 @pytest.mark.parametrize("intlace", [False, True])

@@ -75,7 +75,7 @@ struct YlmPoly {
 struct YlmDev {
   int ell, m;
   float norm;            // sqrt(2 (2l+1)/(4pi) (l-|m|)!/(l+|m|)!), no sqrt2 at m=0
-  const float *rpos[3];  // (i + smin_g) * L per axis (r-space)
+  float org[3], step[3];  // r-space position of cell i: org + i step per axis
   YlmPoly poly;
 };
 
@@ -115,18 +115,18 @@ inline YlmPoly make_ylm_poly(int ell, int m, float norm) {
 }
 
 // Y_lm at the unit vector (ux, uy, uz): sin^|m| theta e^{i |m| phi} is
-// (ux + i uy)^|m|, so neither sin theta nor the azimuth is formed.
+// (ux + i uy)^|m|, so neither sin theta nor the azimuth is formed.  Horner
+// runs over all seven coefficients (those above the degree are 0) and the
+// power is a loop of |m| steps, uniform over the launch: selects over the
+// degree and |m| per point cost more than the arithmetic they skip.
 __device__ __forceinline__ float ylm_poly_eval(const YlmPoly &p, float uz, float ux,
                                                float uy) {
-  float q = 0.f;
+  float q = p.q[6];
 #pragma unroll
-  for (int k = 6; k >= 0; k--)
-    if (k <= p.deg) q = __builtin_fmaf(q, uz, p.q[k]);
-  if (p.am == 0) return q;
+  for (int k = 5; k >= 0; k--) q = __builtin_fmaf(q, uz, p.q[k]);
   float c = 1.f, s = 0.f;
-#pragma unroll
-  for (int i = 0; i < 6; i++) {
-    if (i >= p.am) break;
+#pragma unroll 1
+  for (int i = 0; i < p.am; i++) {
     const float c2 = c * ux - s * uy;
     s = s * ux + c * uy;
     c = c2;
@@ -137,8 +137,13 @@ __device__ __forceinline__ float ylm_poly_eval(const YlmPoly &p, float uz, float
 __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
                                               int k) {
   if (i == 0 && j == 0) return 1.f;  // first row unweighted (mp_template.c:75)
-  float rx = y.rpos[0][i], ry = y.rpos[1][j], rz = y.rpos[2][k];
-  const float ir = rsqrtf(rx * rx + ry * ry + rz * rz);
+  // positions from their affine form, not from tables: no load (and no
+  // wait for one) per point
+  const float rx = __builtin_fmaf((float)i, y.step[0], y.org[0]);
+  const float ry = __builtin_fmaf((float)j, y.step[1], y.org[1]);
+  const float rz = __builtin_fmaf((float)k, y.step[2], y.org[2]);
+  // r^2 is far from the denormal range: the bare reciprocal square root
+  const float ir = __builtin_amdgcn_rsqf(rx * rx + ry * ry + rz * rz);
   return ylm_poly_eval(y.poly, rz * ir, rx * ir, ry * ir);
 }
 
@@ -168,20 +173,37 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
     rowi = line - mesh * lpm;
     return reinterpret_cast<const float2 *>(in + mesh * mstride_f + rowi * (size_t)(2 * H));
   };
-  transform<M, C, T, false, false, 2>(v, lds, tw, [&](int c, int n) -> float2 {
+  // every value of the thread is loaded before any is weighted: a weight
+  // computed between a load and its use keeps the compiler from issuing
+  // the next loads, and each load's latency is then waited for alone
+  pass_load<M, C, T, 0, false>(v, [&](int c, int n) -> float2 {
     const int cu = UNIFORM ? __builtin_amdgcn_readfirstlane(c) : c;
     const size_t line = line0 + cu;
     if (line >= nlines) return make_float2(0.f, 0.f);
     size_t rowi;
     const float2 *row = row_of(line, rowi);
-    float2 z = UNIFORM ? mesh_load<NT>(row_elem<true>(row, (uint32_t)n)) : mesh_load<NT>(row + n);
-    if (YLM) {
-      int i = (int)(rowi / N), j = (int)(rowi - (size_t)i * N);
-      z.x *= ylm_r_weight(ylm, i, j, 2 * n);
-      z.y *= ylm_r_weight(ylm, i, j, 2 * n + 1);
-    }
-    return z;
+    return UNIFORM ? mesh_load<NT>(row_elem<true>(row, (uint32_t)n)) : mesh_load<NT>(row + n);
   });
+  if (YLM) {
+    using G0 = PassGeom<M, C, T, 0, false>;
+#pragma unroll
+    for (int q = 0; q < G0::NB; q++) {
+      int c, j0;
+      const bool ok = G0::map(q, c, j0);
+      const size_t line = line0 + c;
+      const size_t rowi = line % lpm;
+      const int i = (int)(rowi / N), j = (int)(rowi - (size_t)i * N);
+#pragma unroll
+      for (int r = 0; r < G0::R; r++) {
+        const int n = j0 + r * G0::NBF;
+        if (ok && line < nlines) {
+          v[q * G0::R + r].x *= ylm_r_weight(ylm, i, j, 2 * n);
+          v[q * G0::R + r].y *= ylm_r_weight(ylm, i, j, 2 * n + 1);
+        }
+      }
+    }
+  }
+  Passes<M, C, T, 0, false, false, 2>::run(v, lds, tw);
 
   // post-process: X[k] = E + W^k O, X[M-k] = conj(E - W^k O)
   constexpr int NP = M / 2 + 1;
@@ -501,6 +523,7 @@ struct XBinDev {
   float ynorm;
   YlmPoly ypoly;
   float vec0f;   // 2 pi / L_x
+  float kd0f;    // 1 / L_x (Y_lm(k) direction; k_y, k_z from the FP64 tables)
   // SIM: L_ell(mu) = mu^(ell & 1) * (((c3 x + c2) x + c1) x + c0), x = mu^2,
   // for the requested pole l; bit l of oddmask set when ell is odd
   float legc[PSH_MAX_NL][4];
@@ -926,10 +949,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           acc_fixed(&ibins[kb], w + wp, sdown);
         } else {  // Y_lm(k) weight; kx = ky = 0 is added unweighted (mp_template.c:121)
           float yw = 1.f, ywp = 0.f;
-          const float kdx = (float)a.ax.kd[0][i];
+          const float kdx = (float)i * a.kd0f;   // i <= M: k_x = i / L_x
           const float k2 = kdx * kdx + kdy * kdy;
           if (k2 != 0.f) {
-            const float ik = rsqrtf(k2 + kdz * kdz);
+            const float ik = __builtin_amdgcn_rsqf(k2 + kdz * kdz);
             yw = ylm_poly_eval(a.ypoly, kdz * ik, kdx * ik, kdy * ik);
             // the partner -k_x: the same polynomial part, the azimuthal
             // factor's parity (YlmPoly::psgn)
@@ -1186,7 +1209,10 @@ hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
   y.ell = ylm.ell;
   y.m = ylm.m;
   y.norm = ylm.norm;
-  for (int a = 0; a < 3; a++) y.rpos[a] = ylm.rpos[a];
+  for (int a = 0; a < 3; a++) {
+    y.org[a] = ylm.org[a];
+    y.step[a] = ylm.step[a];
+  }
   if (ylm.ell > 0) {
     if (ylm.ell > 6 || ylm.m < -ylm.ell || ylm.m > ylm.ell) return hipErrorInvalidValue;
     y.poly = make_ylm_poly(ylm.ell, ylm.m, ylm.norm);
@@ -1344,6 +1370,7 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.ell = a.ell;
   d.m = a.m;
   d.ynorm = a.ynorm;
+  d.kd0f = (float)((double)a.vec0f / (2 * M_PI));
   if (a.epi == PSH_EPI_SURVEYL) {
     if (a.ell < 0 || a.ell > 6 || a.m < -a.ell || a.m > a.ell) return hipErrorInvalidValue;
     d.ypoly = make_ylm_poly(a.ell, a.m, a.ynorm);

@@ -125,7 +125,6 @@ struct psh_ctx {
   std::vector<double> kedge, kcen;
   PshBins bins{};
   double *d_tab = nullptr;
-  float *d_rpos = nullptr;
   int *d_ylive = nullptr;   // per x-pass k_z tile, the largest live |k_y| (nullptr: all)
   int kzlim = 1 << 30;      // last k_z column of a live x-pass tile (z pass stores up to it)
   PshAxisTables ax{};
@@ -496,18 +495,6 @@ int make_tables(psh_ctx *ctx) {
       if (yl[q] >= 0) ctx->kzlim = std::min(N / 2, (q + 1) * CX - 1);
     if ((rc = dev_alloc(ctx, &ctx->d_ylive, yl.size()))) return rc;
     HIPCHK(ctx, hipMemcpy(ctx->d_ylive, yl.data(), sizeof(int) * yl.size(),
-                          hipMemcpyHostToDevice));
-  }
-  if (ctx->survey) {
-    // r-space positions (i + smin_g) * L with smin_g = bmin * Ng / L
-    // (src/genr_mesh.c:913-914, src/mp_template.c:71-83)
-    std::vector<float> r((size_t)3 * N);
-    for (int a = 0; a < 3; a++) {
-      const double sg = ctx->bmin[a] * N / ctx->L[a];
-      for (int i = 0; i < N; i++) r[(size_t)a * N + i] = (float)((i + sg) * ctx->L[a]);
-    }
-    if ((rc = dev_alloc(ctx, &ctx->d_rpos, r.size()))) return rc;
-    HIPCHK(ctx, hipMemcpy(ctx->d_rpos, r.data(), sizeof(float) * r.size(),
                           hipMemcpyHostToDevice));
   }
   return PSH_OK;
@@ -1028,7 +1015,12 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
       y.ell = ell;
       y.m = m;
       y.norm = ylm_norm(ell, m);
-      for (int ax = 0; ax < 3; ax++) y.rpos[ax] = ctx->d_rpos + (size_t)ax * N;
+      // r-space position of cell i: (i + smin_g) L with smin_g = bmin Ng / L
+      // (src/genr_mesh.c:913-914, src/mp_template.c:71-83), i L + bmin Ng
+      for (int ax = 0; ax < 3; ax++) {
+        y.org[ax] = (float)(ctx->bmin[ax] * N);
+        y.step[ax] = (float)ctx->L[ax];
+      }
       HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream,
                                 ctx->kzlim));
       HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
@@ -1247,7 +1239,7 @@ extern "C" int psh_ctx_destroy(psh_ctx *ctx) {
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   void *bufs[] = {ctx->meshA, ctx->meshB, ctx->meshS, ctx->counts, ctx->starts, ctx->cursors,
                   ctx->scan_tmp, ctx->d_parts, ctx->records, ctx->d_flags, ctx->tw,
-                  ctx->phase, ctx->d_tab, ctx->d_rpos, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
+                  ctx->phase, ctx->d_tab, ctx->d_ylive, ctx->d_gcnt, ctx->d_gkm,
                   ctx->d_glcnt, ctx->partials, ctx->d_bins, ctx->d_bounds, ctx->d_cat,
                   ctx->whist, ctx->d_mlp, ctx->d_ax, ctx->d_ay, ctx->d_acorr, ctx->d_afeat};
   for (void *p : bufs) ctx_free(ctx, p);

@@ -145,7 +145,9 @@ hipError_t psh_fft_twiddles_fill(int N, float2 *d_tw);
 struct PshYlmR {
   int ell, m;            // ell = 0: no weighting
   float norm;            // normalisation of the real-form Y_lm
-  const float *rpos[3];  // per-axis (i + smin_g) * L, device arrays of Ng
+  // r-space position of cell i on axis a: org[a] + i step[a], the cell's
+  // (i + smin_g) L (a common factor of all three axes leaves Y_lm unchanged)
+  float org[3], step[3];
 };
 // tw: twiddle table of N = Ng entries.
 // kzlim: outputs past k_z = kzlim are not stored (fields that only the

@@ -777,6 +777,24 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     // instead, the shot-level modes of a large grid lost 20 bits.
     const bool dccol = EPI == PSH_EPI_SIM && y == 0 && kz == 0;
     float vmax = 0.f;
+    // l > 0 survey fields: every F0 value of the thread is loaded before the
+    // first is used, so the loads' latencies overlap
+    float2 f0v[EPI == PSH_EPI_SURVEYL ? V : 1];
+    if (EPI == PSH_EPI_SURVEYL) {
+#pragma unroll
+      for (int qq = 0; qq < G::NB; qq++) {
+        const int bt = opaque_tid<T>() + qq * T;
+        const int j = bt / C;
+        const bool ok = colok && bt < G::NBT;
+#pragma unroll
+        for (int r = 0; r < G::R; r++) {
+          const size_t rbase = base + (size_t)(r * RS::STORE) * ES;
+          const uint32_t lo = (uint32_t)j * ES32 + c;
+          f0v[qq * G::R + r] =
+              ok ? *row_elem<RS::BYTE32>(a.F0 + rbase, lo) : make_float2(0.f, 0.f);
+        }
+      }
+    }
 #pragma unroll
     for (int qq = 0; qq < G::NB; qq++) {
       const int bt = opaque_tid<T>() + qq * T;
@@ -805,7 +823,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
         const uint32_t lo = (uint32_t)j * ES32 + c;
         float v;
         if (EPI == PSH_EPI_SURVEYL) {
-          const float2 f0 = *row_elem<RS::BYTE32>(a.F0 + rbase, lo);
+          const float2 f0 = f0v[qq * G::R + r];
           v = f0.x * F.x + f0.y * F.y;
         } else {
           v = F.x * F.x + F.y * F.y;

@@ -586,7 +586,8 @@ struct XCfg {
 template <int N, bool IL, bool GS = false>
 constexpr size_t xbin_static_lds() {
   return sizeof(double) * 2 + (GS ? 0 : sizeof(double) * N) + sizeof(float4) * N +
-         sizeof(float2) * N + sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
+         sizeof(float2) * N + sizeof(float) * 4 * PSH_MAX_NL +
+         sizeof(float) * Lds<N, XCfg<N, IL>::C, true>::SIZE;
 }
 
 // LDS per accumulator of the x pass: the workgroup's FP64 sum and the
@@ -667,7 +668,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
   double *ksqx = dyn + (GS ? 1 : 2) * nacc_pad + 2;   // unused by GS
   float4 *xtab = reinterpret_cast<float4 *>(ksqx + (GS ? 0 : N));
   float2 *twl = reinterpret_cast<float2 *>(xtab + N);
-  float *lds = reinterpret_cast<float *>(twl + N);
+  // the Legendre coefficients of the general pole loop, read from LDS: held
+  // in scalar registers they spill most of the kernel's other scalars
+  float4 *legl = reinterpret_cast<float4 *>(twl + N);
+  float *lds = reinterpret_cast<float *>(legl + PSH_MAX_NL);
   const PshBins &b = a.bins;
   const int mock = blockIdx.y;
   // the workgroup's row of the partials; window accumulator i (pole slot l,
@@ -690,6 +694,10 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     xtab[i] = make_float4(a.ax.alf[0][i], a.ax.ksqf[0][i], ph.x, ph.y);
     if (!GS) ksqx[i] = a.ax.ksq[0][i];
     twl[i] = tw[i];
+  }
+  if (EPI == PSH_EPI_SIM && threadIdx.x < PSH_MAX_NL) {
+    const int l = threadIdx.x;
+    legl[l] = make_float4(a.legc[l][0], a.legc[l][1], a.legc[l][2], a.legc[l][3]);
   }
   __syncthreads();
 
@@ -945,13 +953,15 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           }
           const float x2 = mu * mu, y2 = mup * mup;
           const int basep = kb * nmu + cp;
-#pragma unroll
+          // the general pole set (rare): a rolled loop keeps its registers
+          // out of the common path's
+#pragma unroll 1
           for (int l = 0; l < PSH_MAX_NL; l++) {
             if (l >= b.nl) break;
             if (!((lmask >> l) & 1)) continue;
-            const float *cf = a.legc[l];
-            float pl = ((cf[3] * x2 + cf[2]) * x2 + cf[1]) * x2 + cf[0];
-            float ql = ((cf[3] * y2 + cf[2]) * y2 + cf[1]) * y2 + cf[0];
+            const float4 cf = legl[l];
+            float pl = ((cf.w * x2 + cf.z) * x2 + cf.y) * x2 + cf.x;
+            float ql = ((cf.w * y2 + cf.z) * y2 + cf.y) * y2 + cf.x;
             if ((a.oddmask >> l) & 1) {
               pl *= mu;
               ql *= mup;

@@ -307,8 +307,11 @@ def run_alias(P, steps=10, warmup=2, params_path=None, fixed_tile=None):
     out = {}
     for name, il in (("plain_mlp", False), ("interlaced", True)):
         conf = mlp.box_conf(P, 512, 1000.0, 0.01, P.CIC, il)
-        tile = (tuple(fixed_tile) if fixed_tile else
-                P.autotune(conf, nmock=1, data=dev, offsets=[0, n])[0])
+        # fixed_tile: one tile for both paths, or (plain tile, interlaced tile)
+        ft = fixed_tile
+        if ft and isinstance(ft[0], (list, tuple)):
+            ft = ft[1] if il else ft[0]
+        tile = (tuple(ft) if ft else P.autotune(conf, nmock=1, data=dev, offsets=[0, n])[0])
         ctx = P.Context(conf)
         ctx.set_tiles(tile)
         if not il:
@@ -569,6 +572,14 @@ def cpu_baseline(box_data, gpu_res=None):
             "particles_per_s": round(len(box_data) / wall, 1)}
 
 
+def _tiles(spec):
+    """'tx,ty,tz' -> one tile; 'a,b,c/d,e,f' -> (plain, interlaced) tiles."""
+    if not spec:
+        return None
+    t = [[int(v) for v in s.split(",")] for s in spec.split("/")]
+    return t[0] if len(t) == 1 else t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -583,7 +594,9 @@ def main():
     ap.add_argument("--alias-params", default=None,
                     help="networks of the learned correction (default: the shipped ones)")
     ap.add_argument("--tile", default=None,
-                    help="deposit tile of the N_g=1024 box 'tx,ty,tz' instead of autotuning (profiling runs)")
+                    help="deposit tile of the N_g=1024 box 'tx,ty,tz' instead of autotuning "
+                         "(profiling runs); with --only alias 'plain/interlaced' tiles, "
+                         "e.g. 8,8,512/8,4,512")
     ap.add_argument("--ysched", default=None,
                     help="y-pass schedule of the N_g=1024 box instead of tuning (profiling runs)")
     ap.add_argument("--batch-tile", default=None,
@@ -612,8 +625,7 @@ def main():
         r = {"batched": lambda: run_batched(P, args.nmock, fixed_tile=btile),
              "survey": lambda: run_survey(P),
              "alias": lambda: run_alias(P, params_path=args.alias_params,
-                                        fixed_tile=[int(v) for v in args.tile.split(",")]
-                                        if args.tile else None)}[args.only]()
+                                        fixed_tile=_tiles(args.tile))}[args.only]()
         log(json.dumps(r))
         return
     fixed = [int(v) for v in args.tile.split(",")] if args.tile else None

@@ -885,25 +885,21 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       constexpr int S = T / C;   // rows between a thread's modes
       const int i0 = opaque_tid<T>() / C;
       const float *vrow = lds + L::idx(c, i0);
-#pragma unroll 2
-      for (int k = 0; k <= M / S; k++) {
-        const int i = i0 + k * S;
-        if (i > M) break;
-        const float4 xt = xtab[i];
-        const float v = vrow[L::off(k * S)];
+      // One mode pair: its bin, weights and accumulations.  xt, v, vp, ap
+      // come from LDS, read one pair ahead by the loop below.
+      auto mode_pair = [&](int i, float4 xt, float v, float vp, float ap) {
         const float k2f = xt.y + ksqyz;
         const float rs = rsqrtf(k2f);
         int kb = bin_fast(b, k2f * rs);
         if (kb == -2) kb = bin_exact(b, GS ? a.ax.ksq[0][i] : ksqx[i], ksqy, ksqz, kz);
         if (dcrow && i == 0) kb = -1;   // DC mode skipped
         kb -= a.kb0;
-        if (kb < 0 || kb >= a.nbw) continue;
+        if (kb < 0 || kb >= a.nbw) return;
         // the partner k_x = -i (none for i = 0 and the Nyquist index M)
         const bool pair = i != 0 && i != M;
-        float vp = 0.f, ap = 0.f;
-        if (pair) {
-          vp = lds[L::idx(c, N - i)];
-          ap = xtab[N - i].x;
+        if (!pair) {
+          vp = 0.f;
+          ap = 0.f;
         }
         const float w = v * (xt.x * wyz);
         const float wp = vp * (ap * wyz);
@@ -931,7 +927,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
               acc_fixed(&ibins[base], s0, sdown);
               acc_fixed(&ibins[stride + base], s0 * p2, sdown);
               acc_fixed(&ibins[2 * stride + base], s0 * p4, sdown);
-              continue;
+              return;
             }
             const float y2 = mup * mup;
             const float q2 = __builtin_fmaf(1.5f, y2, -0.5f);
@@ -949,7 +945,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
               acc_fixed(&ibins[stride + basep], wp * q2, sdown);
               acc_fixed(&ibins[2 * stride + basep], wp * q4, sdown);
             }
-            continue;
+            return;
           }
           const float x2 = mu * mu, y2 = mup * mup;
           const int basep = kb * nmu + cp;
@@ -988,6 +984,34 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
           }
           acc_fixed(&ibins[kb], w * yw + wp * ywp, sdown);
         }
+      };
+      // The next pair's operands are read before the current pair's
+      // accumulations: the compiler cannot tell the accumulators from the
+      // tile and would otherwise wait for each pair's atomics before
+      // reading the next pair.  Partner indices outside the pairs (i = 0,
+      // M) read a valid entry that mode_pair discards.
+      auto fetch = [&](int k, float4 &xt, float &v, float &vp, float &ap) {
+        const int i = i0 + k * S;
+        const int ip = (i != 0 && i != M) ? N - i : i;
+        xt = xtab[i];
+        v = vrow[L::off(k * S)];
+        vp = lds[L::idx(c, ip)];
+        ap = xtab[ip].x;
+      };
+      const int kend = i0 <= M ? (M - i0) / S : -1;   // last k with i <= M
+      float4 xt = make_float4(0.f, 0.f, 0.f, 0.f);
+      float v = 0.f, vp = 0.f, ap = 0.f;
+      if (kend >= 0) fetch(0, xt, v, vp, ap);
+#pragma unroll 1
+      for (int k = 0; k <= kend; k++) {
+        float4 xtn = xt;
+        float vn = 0.f, vpn = 0.f, apn = 0.f;
+        if (k < kend) fetch(k + 1, xtn, vn, vpn, apn);
+        mode_pair(i0 + k * S, xt, v, vp, ap);
+        xt = xtn;
+        v = vn;
+        vp = vpn;
+        ap = apn;
       }
     }
     __syncthreads();

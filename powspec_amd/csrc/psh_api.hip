@@ -16,6 +16,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -756,6 +758,16 @@ int xgrid_for(psh_ctx *ctx, int nmock, int per_cu) {
   return (int)g;
 }
 
+// A named host range around a stage's launches (roctx; shown by
+// rocprofv3 --marker-trace next to the kernels it enqueued).  The calls cost
+// a few nanoseconds when no profiler is attached.
+struct Mark {
+  explicit Mark(const char *name) { roctxRangePushA(name); }
+  ~Mark() { roctxRangePop(); }
+  Mark(const Mark &) = delete;
+  Mark &operator=(const Mark &) = delete;
+};
+
 void ev_record(psh_ctx *ctx, int i) {
   if (ctx->profiling && ctx->tm.ev[i]) (void)hipEventRecord(ctx->tm.ev[i], ctx->stream);
 }
@@ -786,6 +798,7 @@ void ev_collect(psh_ctx *ctx) {
 // Assign parts (device table already built on host) into meshes.
 int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
                float *meshA, float *meshB) {
+  const Mark mark("psh:assign");
   size_t ntotal = 0;
   ctx->ag.wide = ctx->ag.zfuse && psh_zfuse_wide(ctx->Ng);   // the fused shape's cells
   for (auto &p : hp) {
@@ -826,6 +839,7 @@ int run_assign(psh_ctx *ctx, const std::vector<PshDevPart> &hp, int nmock,
 
 // FFTs of the plain (and shifted) meshes of nmock mocks, fused binning.
 int run_sim_fft_bin(psh_ctx *ctx, int nmock, double *out_bins /*[nmock][nacc]*/) {
+  const Mark mark("psh:fft_bin");
   const int N = ctx->Ng;
   PshYlmR noy{};
   float2 *A = reinterpret_cast<float2 *>(ctx->meshA);
@@ -925,6 +939,7 @@ int survey_slots(const psh_ctx *ctx) {
 // of Y_lm(r) times the field (src/mp_template.c:48-149).
 int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double> &raw,
                double dc) {
+  const Mark mark("psh:survey");
   const int N = ctx->Ng;
   const bool il = ctx->conf.intlace;
   const int nb = ctx->nbin, nl = ctx->conf.npole;
@@ -1010,6 +1025,7 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
   for (int n = 1; n < nl; n++) {
     const int ell = ctx->conf.poles[n];
     for (int m = -ell; m <= ell; m++, slot++) {
+      const Mark lm("psh:survey_lm");
       lrec(3 * (slot - 1));
       PshYlmR y{};
       y.ell = ell;
@@ -1619,6 +1635,7 @@ int run_catalogues(psh_ctx *ctx, const CatDev *cats, int num, psh_pk *out) {
 extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
                                  const size_t *offsets, const double *weight_sum,
                                  int nmock, int graph, psh_pk *out) {
+  const Mark mark("psh:batch");
   if (!ctx || !d_data || !offsets || !weight_sum || !out)
     return set_err(ctx, PSH_ERR_ARG, "null argument");
   if (ctx->survey) return set_err(ctx, PSH_ERR_ARG, "batched runs are for simulation boxes");

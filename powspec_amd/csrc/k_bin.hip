@@ -93,7 +93,11 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
   __syncthreads();
   const int N = g.N, M = N / 2;
   const unsigned nseg = (M + kGeoRun) / kGeoRun;   // runs per row: ceil((M + 1) / kGeoRun)
-  const unsigned items = (unsigned)N * N * nseg;   // < 2^28 for N <= 2048
+  // rows (i, j) with 0 <= i, j <= M; each stands for its mirrors N - i and
+  // N - j as well (0 < i, j < M), which share |k|^2 exactly (ksq of the
+  // signed index), so the bin, |k| and the run structure are computed once
+  const unsigned nq = (unsigned)(M + 1);
+  const unsigned items = nq * nq * nseg;
   const unsigned stride = gridDim.x * kGeoThreads;
   int cur = -1;
   unsigned long long rc = 0;
@@ -113,9 +117,20 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
   for (unsigned it = blockIdx.x * kGeoThreads + threadIdx.x; it < items; it += stride) {
     const unsigned row = it / nseg;
     const int seg = (int)(it - row * nseg);
-    const int i = (int)(row / (unsigned)N), j = (int)(row - (unsigned)i * N);
+    const int i = (int)(row / nq), j = (int)(row - (unsigned)i * nq);
     const double kmij = g.ax.ksq[0][i] + g.ax.ksq[1][j];
     if (kmij > g.kskip2) continue;
+    // mirrors: x -> -x when 0 < i < M, y -> -y when 0 < j < M
+    const int mx = (i > 0 && i < M) ? 2 : 1, my = (j > 0 && j < M) ? 2 : 1;
+    const unsigned nmir = (unsigned)(mx * my);
+    // the reference's mu numerator, ((k_x l_x + k_y l_y) + k_z l_z), of a
+    // mirrored mode is the same expression with the mirrored products, and
+    // the negation of a product is exact
+    const double px = g.ax.kv[0][i] * b.los[0], py = g.ax.kv[1][j] * b.los[1];
+    // mirrors along an axis the line of sight does not follow share mu:
+    // one Legendre evaluation stands for them (weight 2, exact)
+    const int ux = b.los[0] == 0.0 ? 1 : mx, uy = b.los[1] == 0.0 ? 1 : my;
+    const double wmir = (double)((mx / ux) * (my / uy));
     const int k0 = seg * kGeoRun, k1 = min(k0 + kGeoRun - 1, M);
     cur = -1;
     for (int k = k0; k <= k1; k++) {
@@ -129,19 +144,22 @@ geometry_kernel(GeoArgs g, unsigned long long *__restrict__ part_cnt,
       }
       if (kb < 0) continue;
       const bool single = (k == 0) || (k == M);
-      rc += single ? 1ull : 2ull;
-      rkm += single ? kmod : 2 * kmod;
+      rc += (single ? 1ull : 2ull) * nmir;
+      rkm += (single ? kmod : 2 * kmod) * (double)nmir;   // nmir = 1, 2, 4: exact
       if (k == 0 && kmod2 == 0.0) continue;  // DC: no multipole weight
       // mu divides by |k| in both binnings (src/multipole.c:940, 966): with
       // log bins kmod is 0.5 log10 |k|^2, not |k|
       const double kabs = b.logscale ? sqrt(kmod2) : kmod;
-      const double mu = (g.ax.kv[0][i] * b.los[0] + g.ax.kv[1][j] * b.los[1] +
-                         g.ax.kv[2][k] * b.los[2]) / kabs;
-      const double num = single ? 1.0 : 2.0;
-      for (int l = 0; l < nl; l++) {
-        if (!single && (b.poles[l] & 1)) continue;
-        rlc[l] += num * legendre_h(b.poles[l], mu);
-      }
+      const double pz = g.ax.kv[2][k] * b.los[2];
+      const double num = (single ? 1.0 : 2.0) * wmir;
+      for (int sx = 0; sx < ux; sx++)
+        for (int sy = 0; sy < uy; sy++) {
+          const double mu = ((sx ? -px : px) + (sy ? -py : py) + pz) / kabs;
+          for (int l = 0; l < nl; l++) {
+            if (!single && (b.poles[l] & 1)) continue;
+            rlc[l] += num * legendre_h(b.poles[l], mu);
+          }
+        }
     }
     flush();
   }
@@ -259,9 +277,9 @@ hipError_t psh_geometry(int Ng, const PshAxisTables &ax, const PshBins &b,
   // at most 2 max|kmod| to km and 2 to each lcnt (|L_l| <= 1); 2^s times the
   // largest possible sum stays below 2^61
   const int M = Ng / 2;
-  const double items = (double)Ng * Ng * ((M + kGeoRun) / kGeoRun);
+  const double items = (double)(M + 1) * (M + 1) * ((M + kGeoRun) / kGeoRun);
   const double per_thread = ceil(items / ((double)kGeoGrid * kGeoThreads));
-  const double modes = per_thread * kGeoThreads * kGeoRun;
+  const double modes = per_thread * kGeoThreads * kGeoRun * 4;   // up to 4 mirrors per row
   const double kbig = fmax(fmax(fabs(b.kedge0), fabs(b.kedgeN)), 1e-300);
   int e;
   (void)frexp(2.0 * kbig * modes, &e);

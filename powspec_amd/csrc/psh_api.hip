@@ -23,6 +23,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #define PSH_BOX_CEIL 10   // src/define.h:95
@@ -1158,17 +1159,156 @@ int bounds_device(psh_ctx *ctx, const double *d, size_t n, double mn[3], double 
 }
 
 // Sum of the weights of n host records, in record order.
-double weight_sum_host(const psh_data *d, size_t n) {
-  double s = 0.0;
-  for (size_t p = 0; p < n; p++) s += d[p].w;
-  return s;
+// One pass over host records: coordinate bounds and the sum of the
+// weights.  The records are cut into kScanPieces fixed pieces scanned by up
+// to 16 host threads (one 0.64 GB catalogue: about 10 ms instead of the
+// 100 ms of two single-threaded passes); the pieces' sums are added in piece
+// order, so the result does not depend on the thread count.
+struct HostScan {
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+  double mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double wsum = 0.0;
+};
+
+constexpr int kScanPieces = 64;
+
+HostScan scan_host(const psh_data *d, size_t n) {
+  HostScan piece[kScanPieces];
+  auto run = [&](int q) {
+    const size_t lo = n * q / kScanPieces, hi = n * (q + 1) / kScanPieces;
+    HostScan &s = piece[q];
+    for (size_t p = lo; p < hi; p++) {
+      for (int a = 0; a < 3; a++) {
+        const double v = d[p].x[a];
+        if (s.mn[a] > v) s.mn[a] = v;
+        if (s.mx[a] < v) s.mx[a] = v;
+      }
+      s.wsum += d[p].w;
+    }
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = n < ((size_t)1 << 20) ? 1 : (int)std::min(16u, std::max(1u, hw));
+  if (nt == 1) {
+    for (int q = 0; q < kScanPieces; q++) run(q);
+  } else {
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; t++)
+      th.emplace_back([&, t]() {
+        for (int q = t; q < kScanPieces; q += nt) run(q);
+      });
+    for (auto &x : th) x.join();
+  }
+  HostScan r;
+  for (int q = 0; q < kScanPieces; q++) {
+    for (int a = 0; a < 3; a++) {
+      if (r.mn[a] > piece[q].mn[a]) r.mn[a] = piece[q].mn[a];
+      if (r.mx[a] < piece[q].mx[a]) r.mx[a] = piece[q].mx[a];
+    }
+    r.wsum += piece[q].wsum;
+  }
+  return r;
+}
+
+// Bounds of a catalogue set (data, and randoms of surveys) from the scans of
+// its arrays; sd[c] / sr[c] keep each array's weight sum.
+void scan_cat(const psh_ctx *ctx, const psh_cata *cat, double mn[3], double mx[3],
+              double sd[2], double sr[2]) {
+  for (int a = 0; a < 3; a++) {
+    mn[a] = DBL_MAX;
+    mx[a] = -DBL_MAX;
+  }
+  auto merge = [&](const HostScan &s) {
+    for (int a = 0; a < 3; a++) {
+      if (mn[a] > s.mn[a]) mn[a] = s.mn[a];
+      if (mx[a] < s.mx[a]) mx[a] = s.mx[a];
+    }
+  };
+  for (int c = 0; c < cat->num && c < 2; c++) {
+    const HostScan s = scan_host(cat->data[c], cat->ndata[c]);
+    merge(s);
+    sd[c] = s.wsum;
+    sr[c] = 0.0;
+    if (!ctx->conf.issim) {
+      const HostScan r = scan_host(cat->rand[c], cat->nrand[c]);
+      merge(r);
+      sr[c] = r.wsum;
+    }
+  }
+}
+
+// Host records to device memory through two pinned chunks, allocated on the
+// first upload of the process and kept (pinned host memory serves every
+// device).  Host threads fill one chunk while the other's DMA runs on the
+// context's stream; the call returns when the copy is complete, so the
+// caller may free the source.  The runtime's pageable path ran at 7 to 27 GB/s
+// on MI355X depending on the source buffer's history (a fresh mapping at an
+// address an earlier, freed catalogue had used was the slow case).
+constexpr size_t kUpChunk = (size_t)32 << 20;
+std::mutex g_up_mu;
+void *g_up_buf[2] = {nullptr, nullptr};
+
+int upload_host(psh_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return PSH_OK;
+  std::lock_guard<std::mutex> lk(g_up_mu);
+  for (int b = 0; b < 2; b++)
+    if (!g_up_buf[b] && hipHostMalloc(&g_up_buf[b], kUpChunk, hipHostMallocDefault) != hipSuccess) {
+      g_up_buf[b] = nullptr;
+      (void)hipGetLastError();
+      // no pinned staging: the runtime's own pageable copy
+      HIPCHK(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+      return PSH_OK;
+    }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = PSH_OK;
+  for (int b = 0; b < 2 && !rc; b++)
+    if (hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess)
+      rc = set_err(ctx, PSH_ERR_DEVICE, "event creation failed");
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::min(8u, std::max(1u, hw));
+  bool used[2] = {false, false};
+  for (size_t off = 0, i = 0; off < bytes && !rc; off += kUpChunk, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(kUpChunk, bytes - off);
+    // the chunk's previous DMA must be done before it is refilled
+    if (used[b] && hipEventSynchronize(ev[b]) != hipSuccess) {
+      rc = set_err(ctx, PSH_ERR_DEVICE, "upload wait failed");
+      break;
+    }
+    char *buf = static_cast<char *>(g_up_buf[b]);
+    const char *s = static_cast<const char *>(src) + off;
+    if (nt == 1 || len < ((size_t)1 << 20)) {
+      memcpy(buf, s, len);
+    } else {
+      std::vector<std::thread> th;
+      th.reserve(nt);
+      for (int t = 0; t < nt; t++)
+        th.emplace_back([=]() {
+          const size_t lo = len * t / nt, hi = len * (t + 1) / nt;
+          memcpy(buf + lo, s + lo, hi - lo);
+        });
+      for (auto &x : th) x.join();
+    }
+    if (hipMemcpyAsync(static_cast<char *>(dst) + off, buf, len, hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipEventRecord(ev[b], ctx->stream) != hipSuccess) {
+      rc = set_err(ctx, PSH_ERR_DEVICE, "upload copy failed");
+      break;
+    }
+    used[b] = true;
+  }
+  // the staging chunks are shared: every DMA from them ends before the lock
+  // is released, also after an error
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc)
+    rc = set_err(ctx, PSH_ERR_DEVICE, "upload failed");
+  for (int b = 0; b < 2; b++)
+    if (ev[b]) (void)hipEventDestroy(ev[b]);
+  return rc;
 }
 
 int stage_host(psh_ctx *ctx, const psh_data *h, size_t n, size_t offset_records) {
   if (n == 0) return PSH_OK;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->d_cat + 4 * offset_records, h, sizeof(psh_data) * n,
-                             hipMemcpyHostToDevice, ctx->stream));
-  return PSH_OK;
+  return upload_host(ctx, ctx->d_cat + 4 * offset_records, h, sizeof(psh_data) * n);
 }
 
 }  // namespace
@@ -1755,23 +1895,10 @@ extern "C" int psh_ctx_run_batch(psh_ctx *ctx, const psh_data *d_data,
 
 extern "C" int psh_ctx_define_box(psh_ctx *ctx, const psh_cata *cat) {
   if (!ctx || !cat) return set_err(ctx, PSH_ERR_ARG, "null argument");
+  if (cat->num < 1 || cat->num > 2) return set_err(ctx, PSH_ERR_ARG, "invalid catalogue count");
   // bounds on the host (the catalogues are host arrays here)
-  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (int c = 0; c < cat->num; c++) {
-    for (size_t p = 0; p < cat->ndata[c]; p++)
-      for (int a = 0; a < 3; a++) {
-        double v = cat->data[c][p].x[a];
-        if (mn[a] > v) mn[a] = v;
-        if (mx[a] < v) mx[a] = v;
-      }
-    if (!ctx->conf.issim)
-      for (size_t p = 0; p < cat->nrand[c]; p++)
-        for (int a = 0; a < 3; a++) {
-          double v = cat->rand[c][p].x[a];
-          if (mn[a] > v) mn[a] = v;
-          if (mx[a] < v) mx[a] = v;
-        }
-  }
+  double mn[3], mx[3], sd[2], sr[2];
+  scan_cat(ctx, cat, mn, mx, sd, sr);
   return def_box(ctx, mn, mx);
 }
 
@@ -1780,7 +1907,9 @@ extern "C" int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out) {
   if (cat->num != ctx->conf.ndata)
     return set_err(ctx, PSH_ERR_ARG, "catalogue count %d != ndata %d", cat->num, ctx->conf.ndata);
   int rc;
-  if ((rc = psh_ctx_define_box(ctx, cat))) return rc;
+  double mn[3], mx[3], swd[2] = {0, 0}, swr[2] = {0, 0};
+  scan_cat(ctx, cat, mn, mx, swd, swr);
+  if ((rc = def_box(ctx, mn, mx))) return rc;
   if ((rc = ensure_geometry(ctx))) return rc;
   size_t need = 0;
   for (int c = 0; c < cat->num; c++) need += cat->ndata[c] + (ctx->survey ? cat->nrand[c] : 0);
@@ -1804,8 +1933,8 @@ extern "C" int psh_ctx_run_host(psh_ctx *ctx, psh_cata *cat, psh_pk *out) {
       cd[c].alpha = cat->alpha[c];
       cd[c].shot = cat->shot[c];
       cd[c].norm = cat->norm[c];
-      cd[c].sw_d = weight_sum_host(cat->data[c], cat->ndata[c]);
-      cd[c].sw_r = weight_sum_host(cat->rand[c], cat->nrand[c]);
+      cd[c].sw_d = swd[c];
+      cd[c].sw_r = swr[c];
     } else {
       // shot/norm of simulation boxes (src/genr_mesh.c:904-910)
       const double w = cat->wdata[c];
@@ -1844,7 +1973,14 @@ extern "C" psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat) {
     return nullptr;
   }
   psh_ctx *ctx = mesh->ctx;
-  if ((rc = psh_ctx_define_box(ctx, cat))) {
+  double mn[3], mx[3], swd[2] = {0, 0}, swr[2] = {0, 0};
+  if (cat->num < 1 || cat->num > 2) {
+    fprintf(stderr, "\nError: invalid number of catalogues: %d\n", cat->num);
+    psh_mesh_destroy(mesh);
+    return nullptr;
+  }
+  scan_cat(ctx, cat, mn, mx, swd, swr);
+  if ((rc = def_box(ctx, mn, mx))) {
     fprintf(stderr, "\nError: %s\n", ctx->err.c_str());
     psh_mesh_destroy(mesh);
     return nullptr;
@@ -1861,24 +1997,22 @@ extern "C" psh_mesh *psh_genr_mesh(const psh_conf *conf, psh_cata *cat) {
   // that the device holds one catalogue's meshes at a time
   for (int c = 0; c < cat->num; c++) {
     if (dev_alloc(ctx, &mesh->d_data[c], cat->ndata[c] * 4) ||
-        hipMemcpy(mesh->d_data[c], cat->data[c], sizeof(psh_data) * cat->ndata[c],
-                  hipMemcpyHostToDevice) != hipSuccess) {
+        upload_host(ctx, mesh->d_data[c], cat->data[c], sizeof(psh_data) * cat->ndata[c])) {
       fprintf(stderr, "\nError: failed to copy the catalogue to the device\n");
       psh_mesh_destroy(mesh);
       return nullptr;
     }
     mesh->ndata[c] = cat->ndata[c];
-    mesh->sw_d[c] = weight_sum_host(cat->data[c], cat->ndata[c]);
+    mesh->sw_d[c] = swd[c];
     if (!conf->issim) {
       if (dev_alloc(ctx, &mesh->d_rand[c], cat->nrand[c] * 4) ||
-          hipMemcpy(mesh->d_rand[c], cat->rand[c], sizeof(psh_data) * cat->nrand[c],
-                    hipMemcpyHostToDevice) != hipSuccess) {
+          upload_host(ctx, mesh->d_rand[c], cat->rand[c], sizeof(psh_data) * cat->nrand[c])) {
         fprintf(stderr, "\nError: failed to copy the catalogue to the device\n");
         psh_mesh_destroy(mesh);
         return nullptr;
       }
       mesh->nrand[c] = cat->nrand[c];
-      mesh->sw_r[c] = weight_sum_host(cat->rand[c], cat->nrand[c]);
+      mesh->sw_r[c] = swr[c];
     }
   }
   // shot noise and normalisation of simulation boxes (src/genr_mesh.c:904-910)

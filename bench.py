@@ -420,6 +420,76 @@ def run_survey(P, steps=3):
             "fft_transforms": 2 + (5 + 9), "dominant_kernel": dom}
 
 
+def run_dropin(P, data, reps=3):
+    """One-shot drop-in call on the clustered-box catalogue as the reference's
+    host would make it: psh_genr_mesh (new context, box, mode geometry, copy
+    of the malloc'd catalogue to the device, which the library then frees)
+    and psh_powspec (assignment, FFTs, binning), wall time per call."""
+    import ctypes as C
+    L = P.load()
+    libc = C.CDLL("libc.so.6")
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    L.psh_genr_mesh.restype = C.c_void_p
+    L.psh_genr_mesh.argtypes = [C.c_void_p, C.c_void_p]
+    L.psh_powspec.restype = C.POINTER(P.Pk)
+    L.psh_powspec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.psh_mesh_destroy.argtypes = [C.c_void_p]
+    L.psh_powspec_destroy.argtypes = [C.c_void_p]
+    n = len(data)
+    data = np.ascontiguousarray(data)
+    conf = P.make_conf(**box_conf(P))
+    times = []
+    for _ in range(reps):
+        psz = C.sizeof(C.c_void_p)
+        dptr_p, rptr_p = libc.malloc(psz), libc.malloc(psz)
+        C.memset(rptr_p, 0, psz)
+        p = libc.malloc(data.nbytes)
+        C.memmove(p, data.ctypes.data, data.nbytes)
+        (C.c_void_p * 1).from_address(dptr_p)[0] = p
+        nd = (C.c_size_t * 1)(n)
+        vals = {k: (C.c_double * 1)() for k in ("wdata", "wrand", "alpha", "shot", "norm")}
+        vals["wdata"][0] = float(n)
+        cat = P.Cata()
+        cat.num = 1
+        cat.data, cat.rand = dptr_p, rptr_p
+        cat.ndata = C.cast(nd, C.c_void_p)
+        cat.nrand = C.cast((C.c_size_t * 1)(), C.c_void_p)
+        for k, v in vals.items():
+            setattr(cat, k, C.cast(v, C.c_void_p))
+        # the drop-in layer prints the reference's progress lines on stdout:
+        # send them to stderr, so that stdout keeps the one JSON line
+        sys.stdout.flush()
+        libc.fflush(None)
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            t0 = time.perf_counter()
+            mesh = L.psh_genr_mesh(C.byref(conf), C.byref(cat))
+            t1 = time.perf_counter()
+            pk = L.psh_powspec(C.byref(conf), C.byref(cat), mesh) if mesh else None
+            t2 = time.perf_counter()
+        finally:
+            libc.fflush(None)
+            os.dup2(saved, 1)
+            os.close(saved)
+        if not mesh or not pk:
+            log("drop-in: call failed")
+            return None
+        L.psh_powspec_destroy(pk)
+        L.psh_mesh_destroy(C.c_void_p(mesh))
+        times.append((1e3 * (t1 - t0), 1e3 * (t2 - t1)))
+    g = [t[0] for t in times]
+    s = [t[1] for t in times]
+    log(f"drop-in one-shot: genr_mesh {np.round(g, 1).tolist()} ms, powspec "
+        f"{np.round(s, 1).tolist()} ms")
+    return {"workload": "psh_genr_mesh + psh_powspec on the clustered-box catalogue "
+                        f"({n} particles in host memory, PCS N_g=1024 interlaced), new "
+                        "context per call (box, mode geometry, upload, path)",
+            "genr_mesh_ms": [round(v, 2) for v in g], "powspec_ms": [round(v, 2) for v in s],
+            "total_ms_median": round(float(np.median(np.add(g, s))), 2)}
+
+
 def run_ingest(P, data, tile, reps=3):
     """Catalogue ingest of the clustered box catalogue (x, y, z float64 columns, unit
     weights): binary file -> HBM and host columns -> HBM through the pinned
@@ -638,6 +708,7 @@ def main():
     survey = None if args.no_survey else run_survey(P)
     alias = None if args.no_alias else run_alias(P, params_path=args.alias_params)
     ingest = None if args.no_ingest else run_ingest(P, data, tile)
+    dropin = None if args.no_ingest else run_dropin(P, data)
     cpu = None if args.no_cpu else cpu_baseline(data, res)
     line = {
         "metric": METRIC,
@@ -665,6 +736,7 @@ def main():
         "survey": survey,
         "alias_correction": alias,
         "ingest": ingest,
+        "dropin_one_shot": dropin,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

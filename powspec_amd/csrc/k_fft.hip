@@ -69,14 +69,12 @@ struct YlmPoly {
   int deg;       // l - |m|
   int sinpart;   // m < 0
   float q[8];    // coefficients of Q, ascending powers, times norm
-  float psgn;    // Y_lm at (-k_x, k_y, k_z) over Y_lm at (k_x, k_y, k_z)
 };
 
 struct YlmDev {
-  int ell, m;
-  float norm;            // sqrt(2 (2l+1)/(4pi) (l-|m|)!/(l+|m|)!), no sqrt2 at m=0
+  int ell;                // 0: no weighting
   float org[3], step[3];  // r-space position of cell i: org + i step per axis
-  YlmPoly poly;
+  YlmPoly poly[1];        // the (l, m) weight, norm folded in
 };
 
 // Host: Q from the associated Legendre recursion without the Condon-Shortley
@@ -108,9 +106,6 @@ inline YlmPoly make_ylm_poly(int ell, int m, float norm) {
     res = pm1;
   }
   for (int k = 0; k < 8; k++) p.q[k] = (float)(res[k] * norm);
-  // k_x -> -k_x: phi -> pi - phi, cos(|m| phi) -> (-1)^|m| cos, sin -> -(-1)^|m| sin
-  const float par = (am & 1) ? -1.f : 1.f;
-  p.psgn = am == 0 ? 1.f : (m > 0 ? par : -par);
   return p;
 }
 
@@ -134,8 +129,8 @@ __device__ __forceinline__ float ylm_poly_eval(const YlmPoly &p, float uz, float
   return q * (p.sinpart ? s : c);
 }
 
-__device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
-                                              int k) {
+__device__ __forceinline__ float ylm_r_weight(const YlmDev &y, const YlmPoly &p, int i,
+                                              int j, int k) {
   if (i == 0 && j == 0) return 1.f;  // first row unweighted (mp_template.c:75)
   // positions from their affine form, not from tables: no load (and no
   // wait for one) per point
@@ -144,7 +139,7 @@ __device__ __forceinline__ float ylm_r_weight(const YlmDev &y, int i, int j,
   const float rz = __builtin_fmaf((float)k, y.step[2], y.org[2]);
   // r^2 is far from the denormal range: the bare reciprocal square root
   const float ir = __builtin_amdgcn_rsqf(rx * rx + ry * ry + rz * rz);
-  return ylm_poly_eval(y.poly, rz * ir, rx * ir, ry * ir);
+  return ylm_poly_eval(p, rz * ir, rx * ir, ry * ir);
 }
 
 // ---------------------------------------------------------------------------
@@ -197,8 +192,8 @@ z_r2c_kernel(const float *__restrict__ in, float2 *__restrict__ out,
       for (int r = 0; r < G0::R; r++) {
         const int n = j0 + r * G0::NBF;
         if (ok && line < nlines) {
-          v[q * G0::R + r].x *= ylm_r_weight(ylm, i, j, 2 * n);
-          v[q * G0::R + r].y *= ylm_r_weight(ylm, i, j, 2 * n + 1);
+          v[q * G0::R + r].x *= ylm_r_weight(ylm, ylm.poly[0], i, j, 2 * n);
+          v[q * G0::R + r].y *= ylm_r_weight(ylm, ylm.poly[0], i, j, 2 * n + 1);
         }
       }
     }
@@ -519,9 +514,11 @@ struct XBinDev {
   int losx0;       // the line of sight has no x component
   int has_dc;      // replace the k = 0 mode by dcf
   float dcf;
-  int ell, m;
-  float ynorm;
-  YlmPoly ypoly;
+  // SURVEYL: nf fields of one pole l (at A + f fstride_c), field f weighted
+  // by the real-form Y_lm(k) of yp[f]
+  int nf;
+  size_t fstride_c;
+  YlmPoly yp[PSH_MAX_YLM_FIELDS];
   float vec0f;   // 2 pi / L_x
   float kd0f;    // 1 / L_x (Y_lm(k) direction; k_y, k_z from the FP64 tables)
   // SIM: L_ell(mu) = mu^(ell & 1) * (((c3 x + c2) x + c1) x + c0), x = mu^2,
@@ -737,6 +734,13 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     if (!second) pass_load_rows<N, C, T, true>(va, ld_field(a.A, tb, tn));
     else if (INTLACE) pass_load_rows<N, C, T, true>(vb, ld_field(a.B, tb, tn));
   };
+  // field f of a SURVEYL group (f = 0 is A)
+  auto load_group_field = [&](size_t t, int f) {
+    int ty, tq;
+    const size_t tb = tile_base(t, ty, tq);
+    const int tn = min(C, M + 1 - tq * C);
+    pass_load_rows<N, C, T, true>(va, ld_field(a.A + (size_t)f * a.fstride_c, tb, tn));
+  };
   auto prefetch = [&](size_t t) {
     if (PF >= 1) load_field(t, false);
     if (PF >= 2) load_field(t, true);
@@ -775,6 +779,47 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
     const bool colok = c < ncv;
     const int kz = min(q * C + c, M);
 
+    // l > 0 survey fields: G = sum over the group's fields f of
+    // Y_lm_f(k) F_f, one x transform per field, each weighted as it leaves
+    // the transform; the mode k_x = k_y = 0 is added unweighted
+    // (src/mp_template.c:121).  Binning Re(F0 conj G) once per group
+    // replaces a binning per field.
+    float2 gsum[EPI == PSH_EPI_SURVEYL ? V : 1];
+    if (EPI == PSH_EPI_SURVEYL) {
+      const float kdy = (float)a.ax.kd[1][y], kdz = (float)a.ax.kd[2][kz];
+#pragma unroll 1
+      for (int f = 0; f < a.nf; f++) {
+        if (f > 0) {
+          // every thread has read the previous transform's last exchange
+          __syncthreads();
+          load_group_field(tile, f);
+          Passes<N, C, T, 0, true, false, 1>::run(va, lds, twl);
+        }
+        const YlmPoly &yp = a.yp[f];
+#pragma unroll
+        for (int qq = 0; qq < G::NB; qq++) {
+          const int bt = opaque_tid<T>() + qq * T;
+          const int j = bt / C;
+#pragma unroll
+          for (int r = 0; r < G::R; r++) {
+            const int i = j + r * G::NS;
+            const int si = i <= M ? i : i - N;
+            const float kdx = (float)si * a.kd0f;
+            const float k2 = kdx * kdx + kdy * kdy;
+            float yw = 1.f;
+            if (k2 != 0.f) {
+              const float ik = __builtin_amdgcn_rsqf(k2 + kdz * kdz);
+              yw = ylm_poly_eval(yp, kdz * ik, kdx * ik, kdy * ik);
+            }
+            const float2 F = va[qq * G::R + r];
+            float2 &g = gsum[qq * G::R + r];
+            if (f == 0) g = make_float2(0.f, 0.f);
+            g = make_float2(__builtin_fmaf(yw, F.x, g.x), __builtin_fmaf(yw, F.y, g.y));
+          }
+        }
+      }
+    }
+
     // phase 1
     float2 pyz = make_float2(1.f, 0.f);
     if (INTLACE) pyz = cmul(a.phase[y], a.phase[kz]);
@@ -812,7 +857,7 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       for (int r = 0; r < G::R; r++) {
         if (!ok) continue;
         const int i = j + r * G::NS;
-        float2 F = va[qq * G::R + r];
+        float2 F = EPI == PSH_EPI_SURVEYL ? gsum[qq * G::R + r] : va[qq * G::R + r];
         if (INTLACE) {
           const float4 xt = xtab[i];
           const float2 t = cmul(cmul(make_float2(xt.z, xt.w), pyz), vb[qq * G::R + r]);
@@ -877,11 +922,6 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
       const bool dcrow = (EPI == PSH_EPI_SIM) && y == 0 && kz == 0;
       // the two half-spaces cancel odd poles off the kz = 0, M planes
       const int lmask = single ? ~0 : ~a.oddmask;
-      float kdy = 0.f, kdz = 0.f;
-      if (EPI == PSH_EPI_SURVEYL) {
-        kdy = (float)a.ax.kd[1][y];
-        kdz = (float)a.ax.kd[2][kz];
-      }
       constexpr int S = T / C;   // rows between a thread's modes
       const int i0 = opaque_tid<T>() / C;
       const float *vrow = lds + L::idx(c, i0);
@@ -969,20 +1009,8 @@ x_bin_kernel(XBinDev a, int ntz, int H, const float2 *__restrict__ tw) {
               acc_fixed(&ibins[l * stride + basep], wp * ql, sdown);
             }
           }
-        } else if (EPI == PSH_EPI_SURVEY0) {
+        } else {   // SURVEY0, and SURVEYL, whose Y_lm weights are in v already
           acc_fixed(&ibins[kb], w + wp, sdown);
-        } else {  // Y_lm(k) weight; kx = ky = 0 is added unweighted (mp_template.c:121)
-          float yw = 1.f, ywp = 0.f;
-          const float kdx = (float)i * a.kd0f;   // i <= M: k_x = i / L_x
-          const float k2 = kdx * kdx + kdy * kdy;
-          if (k2 != 0.f) {
-            const float ik = __builtin_amdgcn_rsqf(k2 + kdz * kdz);
-            yw = ylm_poly_eval(a.ypoly, kdz * ik, kdx * ik, kdy * ik);
-            // the partner -k_x: the same polynomial part, the azimuthal
-            // factor's parity (YlmPoly::psgn)
-            if (pair) ywp = yw * a.ypoly.psgn;
-          }
-          acc_fixed(&ibins[kb], w * yw + wp * ywp, sdown);
         }
       };
       // The next pair's operands are read before the current pair's
@@ -1259,15 +1287,13 @@ hipError_t psh_fft_z_r2c(int Ng, const float *in, float2 *out, int nmesh,
                          const PshYlmR &ylm, hipStream_t s, int kzlim) {
   YlmDev y{};
   y.ell = ylm.ell;
-  y.m = ylm.m;
-  y.norm = ylm.norm;
   for (int a = 0; a < 3; a++) {
     y.org[a] = ylm.org[a];
     y.step[a] = ylm.step[a];
   }
   if (ylm.ell > 0) {
     if (ylm.ell > 6 || ylm.m < -ylm.ell || ylm.m > ylm.ell) return hipErrorInvalidValue;
-    y.poly = make_ylm_poly(ylm.ell, ylm.m, ylm.norm);
+    y.poly[0] = make_ylm_poly(ylm.ell, ylm.m, ylm.norm);
   }
   return dispatch_n<ZR2C>(Ng, in, out, nmesh, mstride_floats, tw, y, s, kzlim);
 }
@@ -1406,12 +1432,12 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   if (d.kb0 < 0 || d.kb0 + d.nbw > a.bins.nbin) return hipErrorInvalidValue;
   d.nacc = per_bin * d.nbw;
   // bound of a pair's |contribution| over the tile's largest |v| w_yz
-  // (which phase 1 measures), times N C (at least 1024): the largest x
-  // alias factor, |L_l| or |Y_lm| (real form, l <= 6) below 1.01 and 2, two
-  // modes per pair, a 2 % margin for FP32 rounding
+  // (which phase 1 measures; for l > 0 survey fields v carries the Y_lm
+  // weights), times N C (at least 1024): the largest x alias factor and
+  // |L_l| below 1.01, two modes per pair, a 2 % margin for FP32 rounding
   {
     const int C = psh_xbin_tile_cols(Ng);
-    double w = (double)a.ax.almax[0] * (a.epi == PSH_EPI_SURVEYL ? 2.0 : 1.01);
+    double w = (double)a.ax.almax[0] * 1.01;
     w *= 2.0 * 1.02;
     const double nc = (double)Ng * C;
     d.xbound = w * (nc > 1024.0 ? nc : 1024.0);
@@ -1419,13 +1445,17 @@ hipError_t psh_fft_x_bin(int Ng, const PshXBinArgs &a, const float2 *tw,
   d.losx0 = a.bins.losf[0] == 0.f;
   d.has_dc = a.has_dc;
   d.dcf = (float)a.dc;
-  d.ell = a.ell;
-  d.m = a.m;
-  d.ynorm = a.ynorm;
   d.kd0f = (float)((double)a.vec0f / (2 * M_PI));
+  d.nf = 1;
   if (a.epi == PSH_EPI_SURVEYL) {
-    if (a.ell < 0 || a.ell > 6 || a.m < -a.ell || a.m > a.ell) return hipErrorInvalidValue;
-    d.ypoly = make_ylm_poly(a.ell, a.m, a.ynorm);
+    if (a.nf < 1 || a.nf > PSH_MAX_YLM_FIELDS) return hipErrorInvalidValue;
+    d.nf = a.nf;
+    d.fstride_c = a.fstride_floats / 2;
+    for (int f = 0; f < a.nf; f++) {
+      const int ell = a.ells[f], m = a.ms[f];
+      if (ell < 1 || ell > 6 || m < -ell || m > ell) return hipErrorInvalidValue;
+      d.yp[f] = make_ylm_poly(ell, m, a.ynorms[f]);
+    }
   }
   d.vec0f = a.vec0f;
   d.ylive = a.ylive;

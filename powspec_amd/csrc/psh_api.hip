@@ -99,6 +99,7 @@ struct psh_ctx {
   hipStream_t stream = nullptr;
   bool survey = false;
   int ellmax = 0;
+  int nfs = 1;   // survey l > 0 fields the S buffer holds (a pole's group)
 
   float *meshA = nullptr, *meshB = nullptr, *meshS = nullptr;
   int tile_req[3] = {0, 0, 0};   // deposit tiles set by psh_ctx_set_tiles / tuning
@@ -350,6 +351,23 @@ int dev_alloc(psh_ctx *ctx, T **p, size_t count) {
     *p = nullptr;
   }
   return ctx_malloc(ctx, (void **)p, sizeof(T) * count);
+}
+
+// The S buffers of the survey l > 0 fields: one group of up to 2 l_max + 1
+// fields, as many as half of the free device memory holds (at least one).
+int alloc_survey_fields(psh_ctx *ctx) {
+  const size_t field = sizeof(float) * ctx->mstride;
+  int nfs = std::min(2 * ctx->ellmax + 1, PSH_MAX_YLM_FIELDS);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    const size_t fit = free_b / 2 / field;
+    if ((size_t)nfs > fit) nfs = (int)std::max<size_t>(1, fit);
+  } else {
+    (void)hipGetLastError();
+    nfs = 1;
+  }
+  ctx->nfs = nfs;
+  return dev_alloc(ctx, &ctx->meshS, ctx->mstride * (size_t)nfs);
 }
 
 // k bins: src/multipole.c:334-394.
@@ -1023,46 +1041,64 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
     if (lprof && (size_t)q < ctx->tm.lev.size())
       (void)hipEventRecord(ctx->tm.lev[q], ctx->stream);
   };
+  // The (l, m) fields of a pole go in groups of up to ctx->nfs (the S
+  // buffers): each field's z and y passes, then one fused x pass that
+  // transforms every field of the group and bins Re(F0 conj(sum_m Y_lm(k)
+  // F_lm)) once.  A group's sums go to the slot of its first m; the other
+  // slots of the pole stay zero (the host adds a pole's slots).
+  if (nslots > 1)
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_bins + nb, 0, sizeof(double) * (size_t)(nslots - 1) * nb,
+                               ctx->stream));
+  int ngrp = 0;
+  const size_t fstride_c = ctx->mstride / 2;
   for (int n = 1; n < nl; n++) {
     const int ell = ctx->conf.poles[n];
-    for (int m = -ell; m <= ell; m++, slot++) {
+    for (int m0 = -ell; m0 <= ell; m0 += ctx->nfs) {
+      const int nf = std::min(ctx->nfs, ell - m0 + 1);
+      const int gslot = slot;
       const Mark lm("psh:survey_lm");
-      lrec(3 * (slot - 1));
-      PshYlmR y{};
-      y.ell = ell;
-      y.m = m;
-      y.norm = ylm_norm(ell, m);
-      // r-space position of cell i: (i + smin_g) L with smin_g = bmin Ng / L
-      // (src/genr_mesh.c:913-914, src/mp_template.c:71-83), i L + bmin Ng
-      for (int ax = 0; ax < 3; ax++) {
-        y.org[ax] = (float)(ctx->bmin[ax] * N);
-        y.step[ax] = (float)ctx->L[ax];
-      }
-      HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, S, 1, ctx->mstride, ctx->tw, y, ctx->stream,
-                                ctx->kzlim));
-      HIPCHK(ctx, psh_fft_y(N, S, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
-                            ctx->ysched));
-      lrec(3 * (slot - 1) + 1);
+      lrec(3 * ngrp);
       PshXBinArgs b = a;
       b.epi = PSH_EPI_SURVEYL;
       b.grid = gridl;
       b.has_dc = 0;
-      b.ylive = ctx->d_ylive;   // S is read by this pass only
+      b.ylive = ctx->d_ylive;   // the S fields are read by this pass only
       b.A = S;
       b.B = nullptr;
       b.store = nullptr;
       b.F0 = A;
-      b.ell = ell;
-      b.m = m;
-      b.ynorm = y.norm;
+      b.nf = nf;
+      b.fstride_floats = ctx->mstride;
+      for (int f = 0; f < nf; f++, slot++) {
+        PshYlmR y{};
+        y.ell = ell;
+        y.m = m0 + f;
+        y.norm = ylm_norm(ell, y.m);
+        // r-space position of cell i: (i + smin_g) L with smin_g = bmin Ng / L
+        // (src/genr_mesh.c:913-914, src/mp_template.c:71-83), i L + bmin Ng
+        for (int ax = 0; ax < 3; ax++) {
+          y.org[ax] = (float)(ctx->bmin[ax] * N);
+          y.step[ax] = (float)ctx->L[ax];
+        }
+        float2 *Sf = S + (size_t)f * fstride_c;
+        HIPCHK(ctx, psh_fft_z_r2c(N, ctx->meshB, Sf, 1, ctx->mstride, ctx->tw, y, ctx->stream,
+                                  ctx->kzlim));
+        HIPCHK(ctx, psh_fft_y(N, Sf, 1, ctx->mstride, ctx->tw, 0, ctx->stream, ctx->d_ylive,
+                              ctx->ysched));
+        b.ells[f] = ell;
+        b.ms[f] = y.m;
+        b.ynorms[f] = y.norm;
+      }
+      lrec(3 * ngrp + 1);
       for (int kb0 = 0; kb0 < nb; kb0 += ctx->xwin_survey) {
         b.kb0 = kb0;
         b.nbw = std::min(ctx->xwin_survey, nb - kb0);
         HIPCHK(ctx, psh_fft_x_bin(N, b, ctx->tw, ctx->stream));
       }
-      lrec(3 * (slot - 1) + 2);
+      lrec(3 * ngrp + 2);
+      ngrp++;
       HIPCHK(ctx, psh_reduce_partials(ctx->partials, 1, gridl, nb,
-                                      ctx->d_bins + (size_t)slot * nb, ctx->stream));
+                                      ctx->d_bins + (size_t)gslot * nb, ctx->stream));
     }
   }
   ev_record(ctx, E_END);
@@ -1075,8 +1111,8 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
     // surveys: [5] the l > 0 fields' fused x passes, [6] their z and y passes
     double tx = 0.0, tzy = 0.0;
     const size_t nev = ctx->tm.lev.size();
-    bool ok = lprof && nev >= (size_t)3 * (nslots - 1);
-    for (int q = 0; ok && q < nslots - 1; q++) {
+    bool ok = lprof && nev >= (size_t)3 * ngrp;
+    for (int q = 0; ok && q < ngrp; q++) {
       float a0, a1;
       ok = hipEventElapsedTime(&a0, ctx->tm.lev[3 * q], ctx->tm.lev[3 * q + 1]) == hipSuccess &&
            hipEventElapsedTime(&a1, ctx->tm.lev[3 * q + 1], ctx->tm.lev[3 * q + 2]) == hipSuccess;
@@ -1345,7 +1381,7 @@ int ctx_init(psh_ctx *ctx, const psh_conf *conf, int nmock_max) {
   const size_t nm = (size_t)ctx->nmock_max;
   if ((rc = dev_alloc(ctx, &ctx->meshA, ctx->mstride * nm)) ||
       ((conf->intlace || ctx->survey) && (rc = dev_alloc(ctx, &ctx->meshB, ctx->mstride * nm))) ||
-      (ctx->survey && ctx->ellmax > 0 && (rc = dev_alloc(ctx, &ctx->meshS, ctx->mstride))))
+      (ctx->survey && ctx->ellmax > 0 && (rc = alloc_survey_fields(ctx))))
     return rc;
   if ((rc = dev_alloc(ctx, &ctx->tw, (size_t)ctx->Ng))) return rc;
   if (psh_fft_twiddles_fill(ctx->Ng, ctx->tw) != hipSuccess)

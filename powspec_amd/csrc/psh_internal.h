@@ -16,6 +16,8 @@
 
 #define PSH_PAD 16          // complex padding per row (one 16-column tile)
 #define PSH_MAX_NL 7
+// fields of one survey pole l > 0 that one x pass bins together (2 l + 1 <= 13)
+#define PSH_MAX_YLM_FIELDS 13
 
 // Per-axis FP64 tables used by the binning epilogues, device resident.
 // For axis a and index i (signed frequency i' for x,y; i for z):
@@ -184,8 +186,13 @@ struct PshXBinArgs {
   const float2 *B;         // shifted-grid field or nullptr
   float2 *store;           // F store target for SURVEY0 (usually == A)
   const float2 *F0;        // SURVEYL: the l=0 field
-  int ell, m;              // SURVEYL: Y_lm(k) weight
-  float ynorm;             // SURVEYL: normalisation of the real-form Y_lm
+  // SURVEYL: nf fields of Y_lm-weighted densities (A + f fstride_floats),
+  // field f binned with the weight Y_{ells[f] ms[f]}(k), normalisation
+  // ynorms[f]; their sum is binned once
+  int nf;
+  size_t fstride_floats;
+  int ells[PSH_MAX_YLM_FIELDS], ms[PSH_MAX_YLM_FIELDS];
+  float ynorms[PSH_MAX_YLM_FIELDS];
   float vec0f;             // 2 pi / L_x (FP32)
   int nmock;
   size_t mstride_floats;

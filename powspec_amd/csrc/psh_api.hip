@@ -355,9 +355,14 @@ int dev_alloc(psh_ctx *ctx, T **p, size_t count) {
 
 // The S buffers of the survey l > 0 fields: one group of up to 2 l_max + 1
 // fields, as many as half of the free device memory holds (at least one).
+// PSH_SURVEY_FIELDS caps the group (tests of poles split over groups).
 int alloc_survey_fields(psh_ctx *ctx) {
   const size_t field = sizeof(float) * ctx->mstride;
   int nfs = std::min(2 * ctx->ellmax + 1, PSH_MAX_YLM_FIELDS);
+  if (const char *e = getenv("PSH_SURVEY_FIELDS")) {
+    const int cap = atoi(e);
+    if (cap >= 1 && cap < nfs) nfs = cap;
+  }
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
     const size_t fit = free_b / 2 / field;

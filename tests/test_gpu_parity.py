@@ -351,6 +351,29 @@ def test_survey_parity(psh, oracle, scheme, intlace, Ng, poles):
     print(f"survey scheme={scheme} intlace={intlace} Ng={Ng} worst={w:.2e}")
 
 
+@pytest.mark.parametrize("cap", [1, 2, 4])
+def test_survey_poles_split_over_groups(psh, oracle, monkeypatch, cap):
+    """The (l, m) fields of a pole binned by several x passes when the field
+    buffers hold fewer than 2l + 1 (PSH_SURVEY_FIELDS caps them): every
+    split gives the oracle's multipoles, and one group's to FP32 rounding."""
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(120000, 47)
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    d = np.column_stack([dp, wd])
+    r = np.column_stack([rp, wr])
+    kw = dict(gsize=32, assign=2, intlace=True, issim=False, kbin=0.01, poles=[0, 1, 2, 4])
+    monkeypatch.delenv("PSH_SURVEY_FIELDS", raising=False)
+    whole = psh.Context(psh.make_conf(**kw)).run_host([d], [r], [sums])
+    monkeypatch.setenv("PSH_SURVEY_FIELDS", str(cap))
+    got = psh.Context(psh.make_conf(**kw)).run_host([d], [r], [sums])
+    ref = oracle.run(oracle.make_conf(**kw), [d], [r], [sums])
+    _check(got, ref, [0, 1, 2, 4], sums["shot"], sums["norm"])
+    # a group sums its fields per mode in FP32 before binning, so a split
+    # differs from one group by FP32 rounding only
+    for li in range(4):
+        s = np.abs(whole["pl"][0][li]) + sums["shot"]
+        assert np.all(np.abs(got["pl"][0][li] - whole["pl"][0][li]) <= 1e-5 * s)
+
+
 def _check_cross(got, ref, poles, scale, tol=TOL):
     has = ref["cnt"] > 0
     worst = 0.0

@@ -151,9 +151,13 @@ __device__ inline void infer_layer(const short *A, const short *F, const float *
   }
 }
 
+// blockIdx.y: network of an ensemble (parameters P + y pstride, outputs
+// Y + y B); every network reads the same rows X.
 __global__ void __launch_bounds__(64 * kInferWaves)
-mlp_infer_kernel(const float *__restrict__ P, const float *__restrict__ X, int B,
-                 float *__restrict__ Y) {
+mlp_infer_kernel(const float *__restrict__ P0, size_t pstride, const float *__restrict__ X,
+                 int B, float *__restrict__ Y0) {
+  const float *__restrict__ P = P0 + (size_t)blockIdx.y * pstride;
+  float *__restrict__ Y = Y0 + (size_t)blockIdx.y * B;
   extern __shared__ __align__(16) unsigned char infer_raw[];
   InferLds &S = *reinterpret_cast<InferLds *>(infer_raw);
   pack_frags<IN, H>(P + oW1, S.w1);
@@ -488,8 +492,8 @@ hipError_t psh_alias_apply(const float *Y, int nnet, const float *X, const doubl
 }
 
 hipError_t psh_mlp_forward_async(const float *d_params, const float *d_X, int B, float *d_Y,
-                                 hipStream_t s) {
-  if (B <= 0) return hipSuccess;
+                                 hipStream_t s, int nnet) {
+  if (B <= 0 || nnet <= 0) return hipSuccess;
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -499,8 +503,8 @@ hipError_t psh_mlp_forward_async(const float *d_params, const float *d_X, int B,
   const size_t lds = sizeof(InferLds);
   (void)hipFuncSetAttribute((const void *)mlp_infer_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, s, d_params,
-                     d_X, B, d_Y);
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid, nnet), dim3(64 * kInferWaves), lds, s,
+                     d_params, (size_t)psh_mlp_param_count(), d_X, B, d_Y);
   return hipGetLastError();
 }
 
@@ -522,7 +526,7 @@ extern "C" int psh_mlp_forward(const float *d_params, const float *d_X, int B, f
     attr = true;
   }
   hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(64 * kInferWaves), lds, nullptr,
-                     d_params, d_X, B, d_Y);
+                     d_params, (size_t)0, d_X, B, d_Y);
   return hipDeviceSynchronize() == hipSuccess ? PSH_OK : PSH_ERR_DEVICE;
 }
 

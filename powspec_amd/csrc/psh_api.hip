@@ -1623,14 +1623,13 @@ int sim_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<double>
   if (mlp && (rc = ensure_alias_rows(ctx))) return rc;
   if (mlp) {
     const int rows = nb * nmu;
-    const int P = psh_mlp_param_count();
     HIPCHK(ctx, psh_alias_features(ctx->d_bins, ctx->d_gcnt, nb, nmu, norm, shot,
                                    M_PI * ctx->Ng / std::max(ctx->L[0], std::max(ctx->L[1], ctx->L[2])),
                                    ctx->bins.kedge0, ctx->bins.dk, ctx->bins.logscale,
                                    ctx->conf.assign, ctx->d_ax, ctx->d_afeat, ctx->stream));
-    for (int n = 0; n < ctx->mlp_nnet; n++)
-      HIPCHK(ctx, psh_mlp_forward_async(ctx->d_mlp + (size_t)n * P, ctx->d_ax, rows,
-                                        ctx->d_ay + (size_t)n * rows, ctx->stream));
+    // the ensemble in one launch (the networks' rows side by side in d_ay)
+    HIPCHK(ctx, psh_mlp_forward_async(ctx->d_mlp, ctx->d_ax, rows, ctx->d_ay, ctx->stream,
+                                      ctx->mlp_nnet));
     HIPCHK(ctx, psh_alias_apply(ctx->d_ay, ctx->mlp_nnet, ctx->d_ax, ctx->d_bins, nl, nb, nmu,
                                 ctx->d_acorr, ctx->stream));
   }

@@ -274,6 +274,8 @@ size_t psh_alias_scratch_doubles(int nbin);
 // out[l][b] = sum_c exp(mean_n Y[n][b nmu + c]) cells[l][b][c] (device).
 hipError_t psh_alias_apply(const float *Y, int nnet, const float *X, const double *cells, int nl,
     int nbin, int nmu, double *out, hipStream_t s);
-// Network outputs Y[B] of rows X[B][32] on stream s (no synchronisation).
+// Network outputs Y[n][B] of rows X[B][32] for the nnet networks whose
+// parameters follow one another from d_params (psh_mlp_param_count() each),
+// one launch on stream s (no synchronisation).
 hipError_t psh_mlp_forward_async(const float *d_params, const float *d_X, int B, float *d_Y,
-    hipStream_t s);
+    hipStream_t s, int nnet = 1);

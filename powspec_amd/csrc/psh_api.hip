@@ -1293,7 +1293,8 @@ int upload_host(psh_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (bytes == 0) return PSH_OK;
   std::lock_guard<std::mutex> lk(g_up_mu);
   for (int b = 0; b < 2; b++)
-    if (!g_up_buf[b] && hipHostMalloc(&g_up_buf[b], kUpChunk, hipHostMallocDefault) != hipSuccess) {
+    if (!g_up_buf[b] &&
+        hipHostMalloc(&g_up_buf[b], kUpChunk, hipHostMallocPortable) != hipSuccess) {
       g_up_buf[b] = nullptr;
       (void)hipGetLastError();
       // no pinned staging: the runtime's own pageable copy

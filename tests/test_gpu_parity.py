@@ -592,6 +592,38 @@ def test_survey_device_entry(psh, oracle):
     assert ms[2] + ms[5] + ms[6] < ms[3]
 
 
+def test_host_upload_over_staging_chunks(psh, oracle):
+    """Host catalogues larger than the upload's pinned chunks (32 MiB, two
+    reused) and not a multiple of them: the host entry point gives the same
+    bits as the device entry point on the same records, a survey's data and
+    randoms (two uploads through the same chunks) included."""
+    n = 3 * (1 << 20) + 12345          # 96.4 MiB of records: 3 chunks + a part
+    d = _uniform(n, [1000.0] * 3, 91, weights=True)
+    kw = dict(gsize=64, assign=2, intlace=True, bsize=1000.0, kbin=0.02, poles=[0, 2])
+    host = psh.Context(psh.make_conf(**kw)).run_host([d])
+    dev = psh.Context(psh.make_conf(**kw)).run_device(psh.DeviceArray.from_numpy(d), n,
+                                                      float(d[:, 3].sum()))
+    assert np.array_equal(host["cnt"], dev["cnt"])
+    for li in range(2):
+        assert np.array_equal(host["pl"][0][li], dev["pl"][0][li])
+    (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(400000, 92)
+    # randoms repeated to 1.6e6 records (51 MB) so that they span two chunks
+    rp, rwc, rfkp, rnz = (np.concatenate([a] * 4) for a in (rp, rwc, rfkp, rnz))
+    wd, wr, sums = oracle.survey_sums(dwc, dfkp, dnz, rwc, rfkp, rnz)
+    sd, sr = np.column_stack([dp, wd]), np.column_stack([rp, wr])
+    kw = dict(gsize=32, assign=1, intlace=True, issim=False, kbin=0.01, poles=[0, 2])
+    h = psh.Context(psh.make_conf(**kw)).run_host([sd], [sr], [sums])
+    g = psh.Context(psh.make_conf(**kw)).run_survey_device(
+        psh.DeviceArray.from_numpy(sd), len(sd), psh.DeviceArray.from_numpy(sr), len(sr),
+        sums["alpha"], sums["shot"], sums["norm"])
+    assert np.array_equal(h["cnt"], g["cnt"])
+    for li in range(2):
+        s = np.abs(g["pl"][0][li]) + sums["shot"]
+        # the box and weight sums come from the host scan on one side and
+        # the device bounds pass on the other: equal to FP64 rounding
+        assert np.all(np.abs(h["pl"][0][li] - g["pl"][0][li]) <= 1e-9 * s)
+
+
 @pytest.mark.parametrize("intlace", [False, True])
 def test_sim_ng512_streaming_parity(psh, oracle, intlace):
     """N_g = 512 (the learned-correction size): meshes of 537 MB per grid take

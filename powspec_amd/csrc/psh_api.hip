@@ -1118,10 +1118,13 @@ int survey_raw(psh_ctx *ctx, const std::vector<PshDevPart> &hp, std::vector<doub
     const size_t nev = ctx->tm.lev.size();
     bool ok = lprof && nev >= (size_t)3 * ngrp;
     for (int q = 0; ok && q < ngrp; q++) {
-      float a0, a1;
+      float a0 = 0.f, a1 = 0.f;
       ok = hipEventElapsedTime(&a0, ctx->tm.lev[3 * q], ctx->tm.lev[3 * q + 1]) == hipSuccess &&
            hipEventElapsedTime(&a1, ctx->tm.lev[3 * q + 1], ctx->tm.lev[3 * q + 2]) == hipSuccess;
-      if (!ok) (void)hipGetLastError();
+      if (!ok) {
+        (void)hipGetLastError();
+        break;
+      }
       tzy += a0;
       tx += a1;
     }

@@ -562,8 +562,8 @@ int psh_ingest_host(psh_ingest *ing, uint64_t n, int dtype, unsigned cols,
  * %lf %le %lg %lE %lG %lF (values), %d %i %ld %li %u %lu (integer values),
  * %*<conversion> (skipped; numeric ones must parse) and %*s %*c %*[...]
  * (any token).  Columns are 1-based among the conversions that are not
- * suppressed, like `$n` in the reference's expressions; only plain column
- * references are supported (no libast arithmetic, no selection).  Numbers
+ * suppressed, like `$n` in the reference's expressions (psh_ingest_ascii_expr
+ * below takes whole expressions and a selection).  Numbers
  * are converted exactly (the correctly rounded double, as strtod); a line
  * that does not match stops the ingest with PSH_ERR_FILE naming its line. */
 typedef struct {
@@ -576,6 +576,30 @@ typedef struct {
 
 int psh_ingest_ascii(psh_ingest *ing, const char *path, const psh_ascii_fmt *fmt,
     int sim, psh_data *d_out, size_t cap);
+
+/* The same with the reference's libast expressions in place of column
+ * numbers (CONF keys DATA_POSITION, DATA_WT_COMP, DATA_WT_FKP, DATA_NZ and
+ * DATA_SELECTION; io/read_ascii.c:656-735 builds them, :857-903 evaluates
+ * them per line; the column numbers of `fmt` are ignored).  Value
+ * expressions are evaluated in double: numbers, $1..$9 and ${n} (1-based
+ * among the value conversions), + - * / % (fmod) ** (pow, left-associative
+ * like every binary operator), unary -, abs( sqrt( ln( log( (base 10) and
+ * parentheses.  The selection (NULL: keep every line) is a boolean
+ * expression: comparisons < <= > >= == != of numbers, ! && || and
+ * isfinite(; %d columns and numbers written with digits only are integers
+ * there.  Refused with PSH_ERR_ARG when compiled: string literals, the
+ * bitwise operators, integer / % ** (libast's integer arithmetic), a
+ * boolean compared with a number, more than 16 distinct columns, 64
+ * operations or 16 nested values per expression.  A line the selection
+ * rejects is dropped. */
+typedef struct {
+  const char *pos[3];            /* x, y, z (RA, Dec, z for surveys)       */
+  const char *wcomp, *wfkp, *nz; /* NULL: absent                           */
+  const char *sel;               /* NULL: no selection                     */
+} psh_ascii_expr;
+
+int psh_ingest_ascii_expr(psh_ingest *ing, const char *path, const psh_ascii_fmt *fmt,
+    const psh_ascii_expr *ex, int sim, psh_data *d_out, size_t cap);
 
 /* The ASCII ingest's decimal conversion on the host: the correctly rounded
  * double of the whole token s[0..len) (PSH_ERR_ARG when it is not a

@@ -75,6 +75,7 @@ EXPORTS = [
     "psh_mlp_forward", "psh_mlp_grad", "psh_mlp_train", "psh_ingest_create",
     "psh_ingest_destroy", "psh_cat_write", "psh_ingest_file", "psh_ingest_host",
     "psh_ingest_wait", "psh_ingest_error", "psh_ingest_ascii", "psh_parse_double",
+    "psh_ingest_ascii_expr",
     "psh_fault_inject", "psh_run_device_static", "psh_ctx_set_mu_cells", "psh_ctx_cells",
     "psh_ctx_alias_rows", "psh_ctx_set_alias_mlp", "psh_ctx_set_fft_schedule",
     "psh_ctx_get_fft_schedule", "psh_autotune_fft", "psh_fft_plan", "psh_fused_tile",
@@ -240,6 +241,8 @@ def load():
     L.psh_ingest_error.argtypes = [vp]
     L.psh_ingest_error.restype = C.c_char_p
     L.psh_ingest_ascii.argtypes = [vp, C.c_char_p, C.POINTER(AsciiFmt), i, vp, sz]
+    L.psh_ingest_ascii_expr.argtypes = [vp, C.c_char_p, C.POINTER(AsciiFmt),
+                                        C.POINTER(AsciiExpr), i, vp, sz]
     L.psh_parse_double.argtypes = [C.c_char_p, sz, C.POINTER(d)]
     L.psh_fault_inject.argtypes = [i, C.c_long]
     L.psh_run_device_static.argtypes = [C.POINTER(Conf), vp, sz, d, C.POINTER(Pk)]
@@ -719,6 +722,14 @@ class AsciiFmt(C.Structure):
                 ("pos", C.c_int * 3), ("wcomp", C.c_int), ("wfkp", C.c_int), ("nz", C.c_int)]
 
 
+class AsciiExpr(C.Structure):
+    """psh_ascii_expr: the libast expressions of DATA_POSITION, DATA_WT_COMP,
+    DATA_WT_FKP, DATA_NZ and DATA_SELECTION."""
+
+    _fields_ = [("pos", C.c_char_p * 3), ("wcomp", C.c_char_p), ("wfkp", C.c_char_p),
+                ("nz", C.c_char_p), ("sel", C.c_char_p)]
+
+
 def _column(v) -> int:
     """Column number from an int or a plain libast reference "$n" / "${n}"."""
     if v is None:
@@ -785,6 +796,26 @@ class Ingest:
         f.wcomp, f.wfkp, f.nz = _column(wcomp), _column(wfkp), _column(nz)
         self._check(self.L.psh_ingest_ascii(self.h, str(path).encode(), C.byref(f),
                                             int(bool(sim)), out.ptr, C.c_size_t(cap)))
+
+    def start_ascii_expr(self, path, out: "DeviceArray", cap, fmtr="%lf %lf %lf",
+                         pos=("$1", "$2", "$3"), wcomp=None, wfkp=None, nz=None, sel=None,
+                         skip=0, comment="#", sim=True):
+        """ASCII catalogue with libast expressions for the positions, weights
+        and n(z) and an optional selection (psh_ingest_ascii_expr)."""
+        f = AsciiFmt()
+        f.skip = int(skip)
+        f.comment = (comment or "\0").encode()[:1]
+        enc = [fmtr.encode()] + [None if e is None else str(e).encode()
+                                 for e in (*pos, wcomp, wfkp, nz, sel)]
+        self._keep = enc
+        f.fmtr = enc[0]
+        e = AsciiExpr()
+        for a in range(3):
+            e.pos[a] = enc[1 + a]
+        e.wcomp, e.wfkp, e.nz, e.sel = enc[4], enc[5], enc[6], enc[7]
+        self._check(self.L.psh_ingest_ascii_expr(self.h, str(path).encode(), C.byref(f),
+                                                 C.byref(e), int(bool(sim)), out.ptr,
+                                                 C.c_size_t(cap)))
 
     def wait(self):
         r = IngestResult()

@@ -30,10 +30,12 @@
 #include <string>
 #include <thread>
 #include <vector>
+#include <vector>
 
 #include "powspec_hip.h"
 #include "psh_decimal.h"
 #include "psh_internal.h"
+#include "psh_expr.h"
 
 namespace {
 
@@ -116,8 +118,13 @@ struct AsciiArgs {
   int nf;                       // fields parsed per line
   int comment;                  // comment character, 0: none
   int has;                      // output columns present (bit c)
-  signed char map[kMaxFields];  // output column 0..5, or an F_* kind
+  // output column 0..5 (expressions: the value slot 0..15), or an F_* kind
+  signed char map[kMaxFields];
   unsigned char integer[kMaxFields];   // %d-type: the token must be an integer
+  // expressions: program c of the instruction buffer (outputs 0..5, the
+  // selection 6) is [pstart[c], pstart[c] + plen[c]); plen 0: absent
+  int expr;
+  short pstart[7], plen[7];
 };
 
 __device__ __forceinline__ bool is_space(char c) {
@@ -154,8 +161,8 @@ line_start_kernel(const char *__restrict__ buf, size_t nb, const uint32_t *__res
 // marks the line bad (the reference's sscanf failure, :799-805).
 __global__ void __launch_bounds__(kIngThreads)
 parse_kernel(const char *__restrict__ buf, const uint32_t *__restrict__ starts,
-             uint32_t nlines, AsciiArgs a, double *__restrict__ cols,
-             uint32_t *__restrict__ valid, int *__restrict__ err) {
+             uint32_t nlines, AsciiArgs a, const pshexpr::Ins *__restrict__ prog,
+             double *__restrict__ cols, uint32_t *__restrict__ valid, int *__restrict__ err) {
   const uint32_t o = blockIdx.x * kIngThreads + threadIdx.x;
   if (o >= nlines) return;
   const char *p = buf + starts[o];
@@ -166,6 +173,7 @@ parse_kernel(const char *__restrict__ buf, const uint32_t *__restrict__ starts,
     return;
   }
   double v[6] = {0, 0, 0, 0, 0, 0};
+  double slot[pshexpr::kMaxSlots];   // expressions: the columns they name
   bool ok = true;
   for (int f = 0; f < a.nf && ok; f++) {
     while (p < e && is_space(*p)) p++;
@@ -189,12 +197,25 @@ parse_kernel(const char *__restrict__ buf, const uint32_t *__restrict__ starts,
       ok = false;
       break;
     }
-    if (m >= 0) v[m] = x;
+    if (m >= 0) {
+      if (a.expr) slot[m] = x;
+      else v[m] = x;
+    }
   }
   if (!ok) {
     atomicMin(err, (int)o);
     valid[o] = 0;
     return;
+  }
+  if (a.expr) {
+    // the selection first (io/read_ascii.c:857-866): a line it rejects is
+    // dropped, not an error
+    if (a.plen[6] && pshexpr::eval(prog + a.pstart[6], a.plen[6], slot) == 0.0) {
+      valid[o] = 0;
+      return;
+    }
+    for (int c = 0; c < 6; c++)
+      if (a.plen[c]) v[c] = pshexpr::eval(prog + a.pstart[c], a.plen[c], slot);
   }
   valid[o] = 1;
 #pragma unroll
@@ -242,6 +263,7 @@ struct psh_ingest {
          scan_cap = 0, acols_cap = 0, aerr_cap = 0;
   double *acols = nullptr;
   int *aerr = nullptr;
+  pshexpr::Ins *dprog = nullptr;    // expression programs of the ASCII ingest
 };
 
 namespace {
@@ -433,6 +455,7 @@ struct AsciiJob {
   long skip = 0;
   AsciiArgs args{};
   std::string path;
+  std::vector<pshexpr::Ins> prog;   // expression programs (empty: plain columns)
 };
 
 // Reads [off, off + len) of the file into dst with the read threads.
@@ -480,6 +503,12 @@ int run_ascii(psh_ingest *g, AsciiJob job, int sim, double *d_out, size_t cap) {
   r = psh_ingest_result{};
   std::vector<double> hp;
   int rc;
+  if (!job.prog.empty()) {
+    if (!g->dprog && hipMalloc((void **)&g->dprog, sizeof(pshexpr::Ins) * 7 * pshexpr::kMaxIns))
+      return ing_err(g, PSH_ERR_MEMORY, "cannot allocate the expression programs");
+    ICHK(g, hipMemcpyAsync(g->dprog, job.prog.data(), sizeof(pshexpr::Ins) * job.prog.size(),
+                           hipMemcpyHostToDevice, g->stream));
+  }
   for (int k = 0; !eof; k++) {
     const int slot = k & 1;
     ICHK(g, hipEventSynchronize(g->done[slot]));
@@ -548,7 +577,7 @@ int run_ascii(psh_ingest *g, AsciiJob job, int sim, double *d_out, size_t cap) {
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, g->stream, g->aerr, 0x7fffffff);
     const unsigned lb = (unsigned)((nlines + kIngThreads - 1) / kIngThreads);
     hipLaunchKernelGGL(parse_kernel, dim3(lb), dim3(kIngThreads), 0, g->stream, dbuf, g->starts,
-                       nlines, job.args, g->acols, g->valid, g->aerr);
+                       nlines, job.args, g->dprog, g->acols, g->valid, g->aerr);
     ICHK(g, psh_exclusive_scan_u32(g->valid, g->idx, g->scan_tmp, nlines, g->stream));
     uint32_t nvalid = 0;
     int bad = 0;
@@ -651,6 +680,7 @@ extern "C" int psh_ingest_destroy(psh_ingest *g) {
     if (g->done[s]) (void)hipEventDestroy(g->done[s]);
   }
   if (g->partial) (void)hipFree(g->partial);
+  if (g->dprog) (void)hipFree(g->dprog);
   for (void *p : {(void *)g->seg, (void *)g->segoff, (void *)g->starts, (void *)g->valid,
                   (void *)g->idx, (void *)g->scan_tmp, (void *)g->acols, (void *)g->aerr})
     if (p) (void)hipFree(p);
@@ -743,11 +773,59 @@ extern "C" const char *psh_ingest_error(const psh_ingest *g) {
   return g ? g->err.c_str() : "null ingest";
 }
 
+// The expressions of DATA_POSITION, DATA_WT_COMP, DATA_WT_FKP, DATA_NZ and
+// DATA_SELECTION (io/read_ascii.c:656-735 builds them as libast trees) as
+// postfix programs over the formatter's value columns; the fields after
+// the last column they name are not read.
+static int compile_exprs(psh_ingest *g, const std::vector<int> &kind,
+                         const std::vector<int> &vfield, const psh_ascii_expr *ex,
+                         AsciiArgs &a, std::vector<pshexpr::Ins> &prog) {
+  std::vector<char> integer(vfield.size());
+  for (size_t i = 0; i < vfield.size(); i++) integer[i] = kind[vfield[i]] == 1;
+  std::vector<int> slots;   // value column of each slot
+  const char *src[7] = {ex->pos[0], ex->pos[1], ex->pos[2], ex->wcomp, ex->wfkp, ex->nz,
+                        ex->sel};
+  static const char *name[7] = {"position 1", "position 2", "position 3", "completeness weight",
+                                "FKP weight", "number density", "selection"};
+  a.has = 0;
+  prog.clear();
+  for (int c = 0; c < 7; c++) {
+    a.pstart[c] = (short)prog.size();
+    a.plen[c] = 0;
+    if (!src[c]) {
+      if (c < 3) return ing_err(g, PSH_ERR_ARG, "the %s expression is not given", name[c]);
+      continue;
+    }
+    pshexpr::Compiler cp;
+    cp.boolean = c == 6;
+    cp.integer = &integer;
+    cp.slots = &slots;
+    if (!cp.compile(src[c]))
+      return ing_err(g, PSH_ERR_ARG, "%s expression: %s", name[c], cp.err.c_str());
+    prog.insert(prog.end(), cp.out.begin(), cp.out.end());
+    a.plen[c] = (short)cp.out.size();
+    if (c < 6) a.has |= 1 << c;
+  }
+  int last = -1;
+  for (int col : slots) last = std::max(last, vfield[col]);
+  a.nf = last + 1;
+  if (a.nf > kMaxFields) return ing_err(g, PSH_ERR_ARG, "more than %d formatter fields", kMaxFields);
+  for (int i = 0; i < a.nf; i++) {
+    a.map[i] = kind[i] == 3 ? F_SKIP_ANY : (kind[i] == 2 ? F_SKIP_NUM : F_UNUSED);
+    a.integer[i] = kind[i] == 1;
+  }
+  for (size_t sl = 0; sl < slots.size(); sl++) a.map[vfield[slots[sl]]] = (signed char)sl;
+  a.expr = 1;
+  return PSH_OK;
+}
+
 // Formatter of io/ascii_fmtr.c, restricted to whitespace-separated
 // conversions: %lf %le %lg %lE %lG %lF (double columns), %d %i %ld %li %u
 // %lu (integer columns, read as doubles), %*<conversion> (skipped; a numeric
 // conversion must still parse) and %*s / %*c / %*[..] (any token).
-static int parse_fmtr(psh_ingest *g, const psh_ascii_fmt *f, AsciiArgs &a) {
+static int parse_fmtr(psh_ingest *g, const psh_ascii_fmt *f, AsciiArgs &a,
+                      const psh_ascii_expr *ex = nullptr,
+                      std::vector<pshexpr::Ins> *prog = nullptr) {
   const char *p = f->fmtr;
   std::vector<int> kind;   // per field: 0 double, 1 integer, 2 skip numeric, 3 skip any
   while (*p) {
@@ -778,6 +856,8 @@ static int parse_fmtr(psh_ingest *g, const psh_ascii_fmt *f, AsciiArgs &a) {
   std::vector<int> vfield;
   for (size_t i = 0; i < kind.size(); i++)
     if (kind[i] < 2) vfield.push_back((int)i);
+  a.comment = (unsigned char)f->comment;
+  if (ex) return compile_exprs(g, kind, vfield, ex, a, *prog);
   const int want[6] = {f->pos[0], f->pos[1], f->pos[2], f->wcomp, f->wfkp, f->nz};
   int last = -1;
   a.has = 0;
@@ -805,16 +885,17 @@ static int parse_fmtr(psh_ingest *g, const psh_ascii_fmt *f, AsciiArgs &a) {
         return ing_err(g, PSH_ERR_ARG, "column $%d is used for two quantities", want[c]);
       a.map[fi] = (signed char)c;
     }
-  a.comment = (unsigned char)f->comment;
+  a.expr = 0;
   return PSH_OK;
 }
 
-extern "C" int psh_ingest_ascii(psh_ingest *g, const char *path, const psh_ascii_fmt *fmt,
-                                int sim, psh_data *d_out, size_t cap) {
-  if (!g || !path || !fmt || !fmt->fmtr || !d_out) return ing_err(g, PSH_ERR_ARG, "null argument");
+static int ingest_ascii(psh_ingest *g, const char *path, const psh_ascii_fmt *fmt,
+                        const psh_ascii_expr *ex, int sim, psh_data *d_out, size_t cap) {
+  if (!g) return PSH_ERR_ARG;
+  if (!path || !fmt || !fmt->fmtr || !d_out) return ing_err(g, PSH_ERR_ARG, "null argument");
   if (g->busy) return ing_err(g, PSH_ERR_ARG, "an ingest is already in flight on this object");
   AsciiJob job;
-  if (parse_fmtr(g, fmt, job.args)) return g->rc;
+  if (parse_fmtr(g, fmt, job.args, ex, &job.prog)) return g->rc;
   if (fmt->skip < 0) return ing_err(g, PSH_ERR_ARG, "negative number of header lines");
   job.skip = fmt->skip;
   job.path = path;
@@ -831,6 +912,19 @@ extern "C" int psh_ingest_ascii(psh_ingest *g, const char *path, const psh_ascii
     close(job.fd);
   });
   return PSH_OK;
+}
+
+extern "C" int psh_ingest_ascii(psh_ingest *g, const char *path, const psh_ascii_fmt *fmt,
+                                int sim, psh_data *d_out, size_t cap) {
+  return ingest_ascii(g, path, fmt, nullptr, sim, d_out, cap);
+}
+
+extern "C" int psh_ingest_ascii_expr(psh_ingest *g, const char *path, const psh_ascii_fmt *fmt,
+                                     const psh_ascii_expr *ex, int sim, psh_data *d_out,
+                                     size_t cap) {
+  if (!g) return PSH_ERR_ARG;
+  if (!ex) return ing_err(g, PSH_ERR_ARG, "null argument");
+  return ingest_ascii(g, path, fmt, ex, sim, d_out, cap);
 }
 
 extern "C" int psh_parse_double(const char *s, size_t len, double *out) {

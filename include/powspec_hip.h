@@ -503,8 +503,9 @@ int psh_autotune(const psh_conf *conf, const psh_data *d_data, const size_t *off
  * conversion are pipelined through two pinned buffers on the ingest's own
  * stream and host thread: psh_ingest_file / psh_ingest_host return at once
  * and psh_ingest_wait completes, so the caller's context can work on the
- * previous catalogue meanwhile.  ASCII parsing, libast column expressions
- * and selection criteria are not part of this path. */
+ * previous catalogue meanwhile.  ASCII catalogues, with column numbers or
+ * with libast expressions and a selection, go through psh_ingest_ascii and
+ * psh_ingest_ascii_expr below. */
 #define PSH_COL_X 1u
 #define PSH_COL_Y 2u
 #define PSH_COL_Z 4u

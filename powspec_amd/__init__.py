@@ -742,7 +742,8 @@ def _column(v) -> int:
     elif t.startswith("$"):
         t = t[1:]
     if not t.isdigit():
-        raise ValueError(f"only plain column references are supported, got {v!r}")
+        raise ValueError(f"start_ascii takes plain column references (start_ascii_expr "
+                         f"takes expressions), got {v!r}")
     return int(t)
 
 

@@ -554,11 +554,22 @@ def run_ingest(P, data, tile, reps=3):
             ra = ing.wait()
             t_ascii.append(time.perf_counter() - t0)
         assert ra["n"] == na
+        # the same file through libast-style expressions and a selection that
+        # keeps every line (positions lie in [0, L))
+        t_expr = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ing.start_ascii_expr(apath, out[0], n, pos=("$1*1", "$2*1", "$3*1"),
+                                 sel="$1 >= 0 && $2 >= 0 && $3 >= 0", comment="#")
+            re_ = ing.wait()
+            t_expr.append(time.perf_counter() - t0)
+        assert re_["n"] == na
     ing.close()
     tf, th = min(t_file), min(t_host)
-    ta = min(t_ascii)
+    ta, te = min(t_ascii), min(t_expr)
     log(f"ingest ASCII: {na} lines, {abytes / 1e6:.0f} MB in {ta * 1e3:.1f} ms "
-        f"({abytes / ta / 1e9:.2f} GB/s, {na / ta / 1e6:.1f} M lines/s)")
+        f"({abytes / ta / 1e9:.2f} GB/s, {na / ta / 1e6:.1f} M lines/s); with position "
+        f"expressions and a selection {te * 1e3:.1f} ms ({na / te / 1e6:.1f} M lines/s)")
     log(f"ingest: file {nbytes / tf / 1e9:.1f} GB/s, host columns {nbytes / th / 1e9:.1f} GB/s; "
         f"{ncat} catalogues ingest+process: sequential {t_seq * 1e3:.1f} ms, "
         f"overlapped {t_ovl * 1e3:.1f} ms")
@@ -569,7 +580,9 @@ def run_ingest(P, data, tile, reps=3):
             "overlapped_meshes_per_s": round(ncat / t_ovl, 2),
             "ascii": {"lines": na, "bytes": abytes, "ms": round(ta * 1e3, 2),
                       "GBps": round(abytes / ta / 1e9, 3),
-                      "lines_per_s": round(na / ta, 1)}}
+                      "lines_per_s": round(na / ta, 1),
+                      "expr_sel_ms": round(te * 1e3, 2),
+                      "expr_sel_lines_per_s": round(na / te, 1)}}
 
 
 def parity_vs_oracle(gpu, ref, poles=(0, 2, 4), per_bin_path=None):

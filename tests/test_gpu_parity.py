@@ -412,6 +412,35 @@ def test_cross_sim(psh, oracle, scheme, intlace, isauto):
     print(f"cross sim scheme={scheme} il={intlace} worst={w:.2e}")
 
 
+@pytest.mark.parametrize("share", [0.9, 0.0])
+def test_cross_sim_256(psh, oracle, share):
+    """The cross spectrum from the sum field, (S[0+1] - S[0] - S[1]) / 2 with
+    FP32 meshes, at N_g = 256 (TSC, interlaced): tracers that share 90 % of
+    their objects, and independent ones whose cross power is near zero
+    against their autos (the cancellation case), both within the bar of
+    sqrt(S_0 S_1) per bin."""
+    Ng, L, n = 256, 1000.0, 300000
+    rng = np.random.default_rng(4242)
+    c = rng.uniform(0, L, (400, 3))
+    p0 = (c[rng.integers(0, 400, n)] + rng.normal(0, 20.0, (n, 3))) % L
+    m = int(share * n)
+    q = (c[rng.integers(0, 400, n - m)] + rng.normal(0, 20.0, (n - m, 3))) % L
+    if share == 0.0:
+        q = rng.uniform(0, L, (n, 3))
+    p1 = np.concatenate([p0[:m], q])
+    d0 = np.column_stack([p0, np.ones(n)])
+    d1 = np.column_stack([p1, rng.uniform(0.5, 1.5, len(p1))])
+    kw = dict(gsize=Ng, assign=psh.TSC, intlace=True, bsize=L, kbin=0.01, poles=[0, 2, 4],
+              ndata=2, isauto=[True, True], iscross=True)
+    got = psh.Context(psh.make_conf(**kw)).run_host([d0, d1])
+    ref = oracle.run(oracle.make_conf(**kw), [d0, d1])
+    assert np.array_equal(got["cnt"], ref["cnt"])
+    s = [np.abs(ref["pl"][c][0]) + ref["shot"][c] for c in range(2)]
+    w = _check_cross(got, ref, [0, 2, 4], np.sqrt(s[0] * s[1]))
+    x0 = np.abs(ref["xpl"][0][ref["cnt"] > 0]) / np.sqrt(s[0] * s[1])[ref["cnt"] > 0]
+    print(f"cross N_g=256 share={share}: worst {w:.2e}, median |P_x|/scale {np.median(x0):.2e}")
+
+
 @pytest.mark.parametrize("scheme,intlace", [(1, True), (2, False)])
 def test_cross_survey(psh, oracle, scheme, intlace):
     (dp, dwc, dfkp, dnz), (rp, rwc, rfkp, rnz) = _survey_cats(100000, 80 + scheme)

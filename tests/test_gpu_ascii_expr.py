@@ -121,3 +121,33 @@ def test_refused(psh, tmp_path):
     finally:
         ing.close()
         out.free()
+
+
+def test_braced_columns_and_skipped_fields(psh, tmp_path):
+    """${n} past 9 and value columns counted among the conversions that are
+    not suppressed: a 13-field formatter with %*s and %*lf fields, the
+    positions read from ${10}, ${11} and $2 with a selection on ${12}."""
+    rng = np.random.default_rng(7)
+    n = 500
+    vals = rng.uniform(0.0, 1.0, (n, 12))
+    lines = []
+    for r in vals:
+        t = [repr(float(x)) for x in r]
+        lines.append(" ".join(t[:3] + ["tag"] + t[3:6] + ["9.75"] + t[6:]))
+    p = tmp_path / "wide.txt"
+    p.write_text("\n".join(lines) + "\n")
+    fmtr = "%lf %lf %lf %*s %lf %lf %lf %*lf %lf %lf %lf %lf %lf %lf"
+    out = psh.DeviceArray.empty((n, 4))
+    ing = psh.Ingest()
+    try:
+        ing.start_ascii_expr(p, out, n, fmtr=fmtr, pos=("${10}", "${11}", "$2"),
+                             wcomp="${12}", sel="${12} >= 0.25")
+        res = ing.wait()
+        keep = vals[:, 11] >= 0.25
+        got = out.to_numpy((n, 4))[:res["n"]]
+        assert res["n"] == keep.sum()
+        assert np.array_equal(got, vals[keep][:, [9, 10, 1, 11]])
+        assert res["sumw"] == pytest.approx(vals[keep, 11].sum(), rel=1e-14)
+    finally:
+        ing.close()
+        out.free()

@@ -1,0 +1,105 @@
+"""Random expressions through the ASCII ingest's expression compiler and
+evaluator (a host build of powspec_amd/csrc/psh_expr.h) against the
+reference's own libast (oracle/_ref/ast_ref, built from lib/libast.c by
+`make -C oracle ref`), on the rows of the golden vectors.
+
+Every expression accepted here must give libast's value bit for bit (both
+sides use the host C library's math here).  libast converts a long variable
+to double in place when it meets a double operand, after which the next
+row's integer no longer fits the variable and libast fails from the second
+row on: there only the first row is compared.  Expressions libast accepts
+and this build refuses must be the documented integer forms (/ % ** of two
+integers).  Skipped where the reference build or hipcc is absent."""
+import json
+import os
+import random
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+AST_REF = os.path.join(ROOT, "oracle", "_ref", "ast_ref")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+V = json.load(open(os.path.join(ROOT, "tests", "golden", "ascii_expr_vectors.json")))
+ROWS = V["rows"][:40]
+
+pytestmark = pytest.mark.skipif(not (os.path.exists(AST_REF) and os.path.exists(HIPCC)),
+                                reason="needs oracle/_ref/ast_ref and hipcc")
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("expr") / "host_check")
+    subprocess.run([HIPCC, "-O2", "-std=c++17", "-w", "-o", exe,
+                    os.path.join(ROOT, "tests", "expr", "host_check.cpp")], check=True,
+                   capture_output=True)
+    return exe
+
+
+def _gen(rng, n):
+    nums = ["2", "3", "0.5", "1e-2", ".25", "10", "7", "1.5e1"]
+    vars_ = ["$1", "$2", "$3", "$4", "$5", "$6", "${1}", "${4}"]
+
+    def val(d=0):
+        r = rng.random()
+        if d > 3 or r < 0.3:
+            return rng.choice(nums + vars_)
+        if r < 0.45:
+            f = rng.choice(["-", "abs(", "sqrt(", "ln(", "log(", "("])
+            return "-" + val(d + 1) if f == "-" else f + val(d + 1) + ")"
+        sp = rng.choice(["", " "])
+        return val(d + 1) + sp + rng.choice(["+", "-", "*", "/", "**", "%"]) + sp + val(d + 1)
+
+    def sel(d=0):
+        r = rng.random()
+        if d > 2 or r < 0.5:
+            return val(2) + " " + rng.choice(["<", "<=", ">", ">=", "==", "!="]) + " " + val(2)
+        if r < 0.6:
+            return "!(" + sel(d + 1) + ")"
+        if r < 0.65:
+            return "isfinite(" + val(2) + ")"
+        return sel(d + 1) + rng.choice([" && ", " || "]) + sel(d + 1)
+
+    return [("D", val()) for _ in range(n)] + [("B", sel()) for _ in range(n)]
+
+
+def _libast(kind, expr):
+    inp = ["6", "d i d d i d", "1", f"{kind} {expr}", str(len(ROWS))] + [" ".join(r) for r in ROWS]
+    p = subprocess.run([AST_REF], input="\n".join(inp) + "\n", capture_output=True, text=True)
+    out = p.stdout.split("\n")
+    if p.returncode != 0 or len(out) < 1 + len(ROWS):
+        return None, None          # libast crashed (integer division by zero)
+    return out[0] == "OK", out[1:1 + len(ROWS)]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_fuzz_against_libast(harness, seed):
+    rng = random.Random(seed)
+    cases = _gen(rng, 120)
+    inp = [str(len(cases))] + [f"{k} {e}" for k, e in cases]
+    inp += [str(len(ROWS))] + [" ".join(r) for r in ROWS]
+    mine = subprocess.run([harness], input="\n".join(inp) + "\n", capture_output=True,
+                          text=True, check=True).stdout.strip().split("\n")
+    assert len(mine) == len(cases)
+    same = quirk = refused = 0
+    for (kind, expr), m in zip(cases, mine):
+        built, col = _libast(kind, expr)
+        ok_all = built and all(c not in ("E", "-") for c in col)
+        if m.startswith("REFUSED"):
+            if ok_all:
+                assert "integer division, remainder and power" in m, (expr, m)
+                refused += 1
+            continue
+        assert built is not None and built, (kind, expr)
+        got = m.split()[1:]
+        if not ok_all:
+            # libast's in-place conversion: the first row only
+            assert col[0] not in ("E", "-") and got[0] == col[0], (expr, got[0], col[0])
+            quirk += 1
+            continue
+        assert got == col, expr
+        same += 1
+    print(f"seed {seed}: {same} identical on every row, {quirk} on the first row, "
+          f"{refused} integer forms refused")
+    assert same > len(cases) // 2

@@ -17,11 +17,13 @@
 // Variables are $1 .. $9 and ${n}, 1-based among the formatter's value
 // conversions.
 //
-// Evaluated on the device in double arithmetic.  Long-typed / % and **
-// (integer division, remainder and power in libast), the bitwise operators
-// (<< >> & ^ | ~), string literals and comparisons mixing booleans with
-// numbers are refused when the expression is compiled; long + - * agree
-// with libast while the values stay below 2^53.
+// Evaluated on the device in double arithmetic; two longs divide with
+// truncation, take C's remainder and libast's integer power.  The bitwise
+// operators (<< >> & ^ | ~), string literals and comparisons mixing
+// booleans with numbers are refused when the expression is compiled; long
+// arithmetic agrees with libast while the values stay below 2^53 (an
+// integer division by zero, which stops libast with SIGFPE, gives inf or
+// nan here).
 #pragma once
 
 #include <cmath>
@@ -39,6 +41,7 @@ enum Op : int {
   OP_IMM,       // push imm
   OP_NEG, OP_ABS, OP_SQRT, OP_LN, OP_LOG10, OP_ISFIN, OP_LNOT,
   OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_POW, OP_FMOD,
+  OP_IDIV, OP_IMOD, OP_IPOW,   // two integers (libast's long arithmetic)
   OP_LT, OP_LE, OP_GT, OP_GE, OP_EQ, OP_NE, OP_AND, OP_OR
 };
 
@@ -51,6 +54,19 @@ struct Ins {
 constexpr int kMaxIns = 64;     // instructions per expression
 constexpr int kStack = 16;      // evaluation stack depth
 constexpr int kMaxSlots = 16;   // distinct columns over all expressions of a file
+
+// libast's power of two longs (lib/libast.c:1223-1290 and its caller,
+// :1778-1795): exponents above 255 give 1, +-1 or 0; others are taken
+// modulo 256 (a uint8_t) and from 63 on give the same overflow values;
+// below, square-and-multiply in 64-bit two's complement.
+__host__ __device__ inline long long ipow_long(long long base, long long e) {
+  if (e <= 255) e = (unsigned char)e;
+  if (e >= 63) return base == 1 ? 1 : (base == -1 ? 1 - 2 * (e & 1) : 0);
+  unsigned long long r = 1, b = (unsigned long long)base;
+  for (; e; e >>= 1, b *= b)
+    if (e & 1) r *= b;
+  return (long long)r;
+}
 
 // One program: n instructions from `ins`; the stack depth never exceeds
 // kStack (checked when compiled).
@@ -79,6 +95,10 @@ __host__ __device__ inline double eval(const Ins *__restrict__ ins, int n, const
           case OP_DIV: r = a / b; break;
           case OP_POW: r = pow(a, b); break;
           case OP_FMOD: r = fmod(a, b); break;
+          // integers below 2^53: the rounded quotient truncates to C's
+          case OP_IDIV: r = trunc(a / b); break;
+          case OP_IMOD: r = fmod(a, b); break;   // exact, sign of a as C's %
+          case OP_IPOW: r = (double)ipow_long((long long)a, (long long)b); break;
           case OP_LT: r = a < b; break;
           case OP_LE: r = a <= b; break;
           case OP_GT: r = a > b; break;
@@ -273,8 +293,11 @@ struct Compiler {
         if (cmp) {
           t = T_BOOL;
         } else {
-          if (t == T_LONG && r == T_LONG && (op == OP_DIV || op == OP_FMOD || op == OP_POW))
-            return fail("integer division, remainder and power are not supported", at);
+          if (t == T_LONG && r == T_LONG) {
+            if (op == OP_DIV) op = OP_IDIV;
+            else if (op == OP_FMOD) op = OP_IMOD;
+            else if (op == OP_POW) op = OP_IPOW;
+          }
           t = (t == T_LONG && r == T_LONG) ? T_LONG : T_DBL;
         }
       }

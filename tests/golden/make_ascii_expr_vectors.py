@@ -39,9 +39,13 @@ EXPRS = [
     ("B", "$4 > 0 && $4 < 0.5 || $1 > 0.9"), ("B", "!!($3 < 0)"), ("B", "$1 > 0"),
     ("B", "$1"), ("B", "$1 + 1"), ("B", "($1 > 0.5) + 1 > 0"), ("B", "$1 > 0.5 && $2"),
     ("B", "$1 > (0.5"),
+    # libast's long arithmetic: truncating division, C remainder, integer power
+    ("B", "$2 / 2 > 1"), ("B", "$2 % 2 == 0"), ("B", "-$5 % 3 == -1"), ("B", "$5 / 2 == -1"),
+    ("B", "$2 ** 2 > 10"), ("B", "$5 ** 3 < -8"), ("B", "2 ** $2 >= 64"),
+    ("B", "$5 ** 300 == 0"), ("B", "(-1) ** ($2 - 9) == -1"), ("B", "$2 ** 63 != 0"),
 ]
-# accepted by libast, refused here (integer arithmetic and bitwise operators)
-REFUSED_HERE = ["$2 / 2 > 1", "$2 % 2 == 0", "($2 & 1) == 1"]
+# accepted by libast, refused here (bitwise operators)
+REFUSED_HERE = ["($2 & 1) == 1", "$2 << 1 > 2", "~$2 < 0"]
 
 
 def rows(n=160, seed=20261203):

@@ -52,6 +52,13 @@ def test_selection_closed_forms():
     # comparisons bind tighter than ==
     assert np.array_equal(keep("$1 > 0.3 == $3 > 0.3"), (C[:, 0] > 0.3) == (C[:, 2] > 0.3))
     assert np.array_equal(keep("isfinite(ln($6))"), C[:, 5] > 0)
+    # long arithmetic: C's truncating division and remainder, integer power
+    assert np.array_equal(keep("$2 % 2 == 0"), C[:, 1] % 2 == 0)
+    assert np.array_equal(keep("$5 / 2 == -1"), np.trunc(C[:, 4] / 2) == -1)
+    assert np.array_equal(keep("-$5 % 3 == -1"), np.fmod(-C[:, 4], 3) == -1)
+    assert np.array_equal(keep("2 ** $2 >= 64"), C[:, 1] >= 6)
+    assert np.array_equal(keep("$2 ** 63 != 0"), C[:, 1] == 1)   # overflow: only 1 ** n
+    assert np.array_equal(keep("(-1) ** ($2 - 9) == -1"), C[:, 1] % 2 == 0)
 
 
 def test_refusals_recorded():

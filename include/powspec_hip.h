@@ -588,10 +588,11 @@ int psh_ingest_ascii(psh_ingest *ing, const char *path, const psh_ascii_fmt *fmt
  * parentheses.  The selection (NULL: keep every line) is a boolean
  * expression: comparisons < <= > >= == != of numbers, ! && || and
  * isfinite(; %d columns and numbers written with digits only are integers
- * there, and two integers divide with truncation, take C's remainder and
- * libast's integer power (exact below 2^53).  Refused with PSH_ERR_ARG when
- * compiled: string literals, the bitwise operators, a boolean compared with
- * a number, more than 16 distinct columns, 64 operations or 16 nested
+ * there: two integers divide with truncation, take C's remainder and
+ * libast's integer power, and the bitwise operators << >> & ^ | ~ take
+ * integers (exact below 2^53).  Refused with PSH_ERR_ARG when compiled:
+ * string literals, a boolean compared with a number, bitwise operators on
+ * non-integers, more than 16 distinct columns, 64 operations or 16 nested
  * values per expression.  A line the selection rejects is dropped. */
 typedef struct {
   const char *pos[3];            /* x, y, z (RA, Dec, z for surveys)       */

@@ -18,12 +18,12 @@
 // conversions.
 //
 // Evaluated on the device in double arithmetic; two longs divide with
-// truncation, take C's remainder and libast's integer power.  The bitwise
-// operators (<< >> & ^ | ~), string literals and comparisons mixing
-// booleans with numbers are refused when the expression is compiled; long
-// arithmetic agrees with libast while the values stay below 2^53 (an
-// integer division by zero, which stops libast with SIGFPE, gives inf or
-// nan here).
+// truncation, take C's remainder and libast's integer power, and the
+// bitwise operators (<< >> & ^ | ~) take longs only.  String literals and
+// comparisons mixing booleans with numbers are refused when the expression
+// is compiled; long arithmetic agrees with libast while the values stay
+// below 2^53 (an integer division by zero, which stops libast with SIGFPE,
+// gives inf or nan here).
 #pragma once
 
 #include <cmath>
@@ -39,9 +39,10 @@ namespace pshexpr {
 enum Op : int {
   OP_COL = 0,   // push value slot `arg`
   OP_IMM,       // push imm
-  OP_NEG, OP_ABS, OP_SQRT, OP_LN, OP_LOG10, OP_ISFIN, OP_LNOT,
+  OP_NEG, OP_ABS, OP_SQRT, OP_LN, OP_LOG10, OP_ISFIN, OP_LNOT, OP_BNOT,
   OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_POW, OP_FMOD,
   OP_IDIV, OP_IMOD, OP_IPOW,   // two integers (libast's long arithmetic)
+  OP_BAND, OP_BOR, OP_BXOR, OP_SHL, OP_SHR,
   OP_LT, OP_LE, OP_GT, OP_GE, OP_EQ, OP_NE, OP_AND, OP_OR
 };
 
@@ -85,6 +86,7 @@ __host__ __device__ inline double eval(const Ins *__restrict__ ins, int n, const
       case OP_LOG10: st[sp - 1] = log10(st[sp - 1]); break;
       case OP_ISFIN: st[sp - 1] = __builtin_isfinite(st[sp - 1]) ? 1.0 : 0.0; break;
       case OP_LNOT: st[sp - 1] = st[sp - 1] == 0.0 ? 1.0 : 0.0; break;
+      case OP_BNOT: st[sp - 1] = (double)~(long long)st[sp - 1]; break;
       default: {
         const double b = st[--sp], a = st[sp - 1];
         double r;
@@ -99,6 +101,15 @@ __host__ __device__ inline double eval(const Ins *__restrict__ ins, int n, const
           case OP_IDIV: r = trunc(a / b); break;
           case OP_IMOD: r = fmod(a, b); break;   // exact, sign of a as C's %
           case OP_IPOW: r = (double)ipow_long((long long)a, (long long)b); break;
+          // the bitwise operators of two longs; shift counts taken modulo 64
+          // as the x86-64 shift instructions take them
+          case OP_BAND: r = (double)((long long)a & (long long)b); break;
+          case OP_BOR: r = (double)((long long)a | (long long)b); break;
+          case OP_BXOR: r = (double)((long long)a ^ (long long)b); break;
+          case OP_SHL:
+            r = (double)(long long)((unsigned long long)(long long)a << ((long long)b & 63));
+            break;
+          case OP_SHR: r = (double)((long long)a >> ((long long)b & 63)); break;
           case OP_LT: r = a < b; break;
           case OP_LE: r = a <= b; break;
           case OP_GT: r = a > b; break;
@@ -151,10 +162,10 @@ struct Compiler {
     struct B { const char *s; int op, prec; };
     static const B tab[] = {
         {"**", OP_POW, 11}, {"*", OP_MUL, 10}, {"/", OP_DIV, 10}, {"%", OP_FMOD, 10},
-        {"+", OP_ADD, 9},   {"-", OP_SUB, 9},  {"<<", -1, 8},     {">>", -1, 8},
+        {"+", OP_ADD, 9},   {"-", OP_SUB, 9},  {"<<", OP_SHL, 8}, {">>", OP_SHR, 8},
         {"<=", OP_LE, 7},   {">=", OP_GE, 7},  {"<", OP_LT, 7},   {">", OP_GT, 7},
-        {"==", OP_EQ, 6},   {"!=", OP_NE, 6},  {"&&", OP_AND, 2}, {"&", -1, 5},
-        {"^", -1, 4},       {"||", OP_OR, 1},  {"|", -1, 3}};
+        {"==", OP_EQ, 6},   {"!=", OP_NE, 6},  {"&&", OP_AND, 2}, {"&", OP_BAND, 5},
+        {"^", OP_BXOR, 4},  {"||", OP_OR, 1},  {"|", OP_BOR, 3}};
     for (const B &b : tab)
       if (starts(b.s)) {
         op = b.op;
@@ -168,24 +179,27 @@ struct Compiler {
   bool unary(int &t) {
     skip();
     const char *at = p;
-    if (*p == '-' || *p == '!') {
-      const bool neg = *p == '-';
-      if (!neg && p[1] == '=') return fail("missing value", at);
-      if (!neg && !boolean) return fail("invalid token for a value expression", at);
+    if (*p == '-' || *p == '!' || *p == '~') {
+      const char u = *p;
+      if (u == '!' && p[1] == '=') return fail("missing value", at);
+      if (u != '-' && !boolean) return fail("invalid token for a value expression", at);
       p++;
       int a;
       if (!unary(a)) return false;
-      if (neg) {
+      if (u == '-') {
         if (a == T_BOOL) return fail("unexpected data type for operator", at);
         emit(OP_NEG);
         t = a;
+      } else if (u == '~') {
+        if (a != T_LONG) return fail("bitwise operator on a non-integer value", at);
+        emit(OP_BNOT);
+        t = T_LONG;
       } else {
         emit(OP_LNOT);
         t = T_BOOL;
       }
       return true;
     }
-    if (*p == '~') return fail("bitwise operators are not supported", at);
     struct F { const char *s; int op; };
     static const F fn[] = {{"abs(", OP_ABS}, {"sqrt(", OP_SQRT}, {"ln(", OP_LN},
                            {"log(", OP_LOG10}, {"isfinite(", OP_ISFIN}};
@@ -274,9 +288,9 @@ struct Compiler {
       const char *at = p;
       const int prec = binop(op, len);
       if (prec == 0 || prec < minprec) return true;
-      if (op < 0) return fail("bitwise operators are not supported", at);
       const bool cmp = op >= OP_LT && op <= OP_NE, logic = op == OP_AND || op == OP_OR;
-      if (!boolean && (cmp || logic))
+      const bool bits = op >= OP_BAND && op <= OP_SHR;
+      if (!boolean && (cmp || logic || bits))
         return fail("invalid token for a value expression", at);
       p += len;
       int r;
@@ -284,6 +298,9 @@ struct Compiler {
       if (logic) {
         if (t != T_BOOL || r != T_BOOL) return fail("unexpected data type for operator", at);
         t = T_BOOL;
+      } else if (bits) {
+        if (t != T_LONG || r != T_LONG)
+          return fail("bitwise operator on a non-integer value", at);
       } else if (op == OP_EQ || op == OP_NE) {
         if ((t == T_BOOL) != (r == T_BOOL))
           return fail("comparison of a boolean with a number", at);

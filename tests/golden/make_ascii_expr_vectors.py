@@ -43,9 +43,13 @@ EXPRS = [
     ("B", "$2 / 2 > 1"), ("B", "$2 % 2 == 0"), ("B", "-$5 % 3 == -1"), ("B", "$5 / 2 == -1"),
     ("B", "$2 ** 2 > 10"), ("B", "$5 ** 3 < -8"), ("B", "2 ** $2 >= 64"),
     ("B", "$5 ** 300 == 0"), ("B", "(-1) ** ($2 - 9) == -1"), ("B", "$2 ** 63 != 0"),
+    # bitwise operators of longs (x86-64 shifts: counts modulo 64)
+    ("B", "($2 & 1) == 1"), ("B", "$2 << 1 > 2"), ("B", "~$2 < -5"), ("B", "($2 | $5) > 3"),
+    ("B", "($2 ^ 5) == 3"), ("B", "$5 >> 1 == -1"), ("B", "1 << $2 >= 32"),
+    ("B", "$2 << 62 < 0"), ("B", "$2 & 1 == 1"), ("B", "($1 & 1) == 1"),
 ]
-# accepted by libast, refused here (bitwise operators)
-REFUSED_HERE = ["($2 & 1) == 1", "$2 << 1 > 2", "~$2 < 0"]
+# accepted by libast, refused here
+REFUSED_HERE = []
 
 
 def rows(n=160, seed=20261203):

@@ -7,13 +7,15 @@ Every expression accepted here must give libast's value bit for bit (both
 sides use the host C library's math here).  libast converts a long variable
 to double in place when it meets a double operand, after which the next
 row's integer no longer fits the variable and libast fails from the second
-row on: there only the first row is compared.  Nothing libast evaluates
+row on (or within the first, when that column then meets a bitwise
+operator): there only the first row is compared.  Nothing libast evaluates
 may be refused; expressions on which libast dies (an integer division by a
 zero column) are not compared.  Skipped where the reference build or hipcc
 is absent."""
 import json
 import os
 import random
+import re
 import shutil
 import subprocess
 
@@ -39,8 +41,8 @@ def harness(tmp_path_factory):
 
 
 def _gen(rng, n):
-    nums = ["2", "3", "0.5", "1e-2", ".25", "10", "7", "1.5e1"]
-    vars_ = ["$1", "$2", "$3", "$4", "$5", "$6", "${1}", "${4}"]
+    nums = ["2", "3", "0.5", "1e-2", ".25", "10", "7", "1.5e1", "1", "5"]
+    vars_ = ["$1", "$2", "$3", "$4", "$5", "$6", "${1}", "${4}", "$2", "${5}"]
 
     def val(d=0):
         r = rng.random()
@@ -50,7 +52,8 @@ def _gen(rng, n):
             f = rng.choice(["-", "abs(", "sqrt(", "ln(", "log(", "("])
             return "-" + val(d + 1) if f == "-" else f + val(d + 1) + ")"
         sp = rng.choice(["", " "])
-        return val(d + 1) + sp + rng.choice(["+", "-", "*", "/", "**", "%"]) + sp + val(d + 1)
+        ops = ["+", "-", "*", "/", "**", "%"] * 3 + ["&", "|", "^", "<<", ">>"]
+        return val(d + 1) + sp + rng.choice(ops) + sp + val(d + 1)
 
     def sel(d=0):
         r = rng.random()
@@ -96,12 +99,17 @@ def test_fuzz_against_libast(harness, seed):
         assert built, (kind, expr)
         got = m.split()[1:]
         if not ok_all:
-            # libast's in-place conversion: the first row only
-            assert col[0] not in ("E", "-") and got[0] == col[0], (expr, got[0], col[0])
+            # libast's in-place conversion: the first row only.  The
+            # conversion can also strike within the first row, when the
+            # converted long column meets a bitwise operator later on.
+            if col[0] in ("E", "-"):
+                assert re.search(r"\$\{?[25]\}?", expr) and re.search(r"[&|^]|<<|>>", expr), expr
+            else:
+                assert got[0] == col[0], (expr, got[0], col[0])
             quirk += 1
             continue
         assert got == col, expr
         same += 1
     print(f"seed {seed}: {same} identical on every row, {quirk} on the first row, "
           f"{crashed} stop libast (integer division by zero)")
-    assert same > len(cases) // 2
+    assert same > len(cases) // 3

@@ -59,9 +59,18 @@ def test_selection_closed_forms():
     assert np.array_equal(keep("2 ** $2 >= 64"), C[:, 1] >= 6)
     assert np.array_equal(keep("$2 ** 63 != 0"), C[:, 1] == 1)   # overflow: only 1 ** n
     assert np.array_equal(keep("(-1) ** ($2 - 9) == -1"), C[:, 1] % 2 == 0)
+    # bitwise operators of longs; == binds tighter than &
+    i2, i5 = C[:, 1].astype(np.int64), C[:, 4].astype(np.int64)
+    assert np.array_equal(keep("($2 & 1) == 1"), (i2 & 1) == 1)
+    assert np.array_equal(keep("~$2 < -5"), ~i2 < -5)
+    assert np.array_equal(keep("$5 >> 1 == -1"), (i5 >> 1) == -1)
+    wrapped = (i2.astype(np.uint64) << np.uint64(62)).astype(np.int64)   # two's complement
+    assert np.array_equal(keep("$2 << 62 < 0"), wrapped < 0)
+    assert not BY["B", "$2 & 1 == 1"]["builds"]     # $2 & (1 == 1): a long and a bool
+    assert BY["B", "($1 & 1) == 1"]["eval_error"]   # a double column
 
 
 def test_refusals_recorded():
     assert not BY["D", "$1 < 2"]["builds"] and not BY["D", "$12"]["builds"]
     assert not BY["B", "$1"]["builds"] and BY["B", "$1 > 0.5 && $2"]["eval_error"]
-    assert V["refused_here"]
+    assert V["refused_here"] == []

@@ -96,8 +96,8 @@ def test_weights_sums_and_chunks(psh, tmp_path):
 
 
 def test_refused(psh, tmp_path):
-    """Expressions libast does not build (or cannot evaluate), and the
-    bitwise forms this build refuses: PSH_ERR_ARG at start."""
+    """Expressions libast does not build or cannot evaluate: PSH_ERR_ARG at
+    start."""
     p = _file(tmp_path)
     out = psh.DeviceArray.empty((N, 4))
     ing = psh.Ingest()
